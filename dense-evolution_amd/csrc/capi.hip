@@ -1,0 +1,834 @@
+// C ABI of the engine (declared in include/desv.h): state handles, compiled
+// programs, Pauli sums and the adjoint gradient driver.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/desv.h"
+#include "desv_internal.h"
+#include "kernels.h"
+
+using namespace desv;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return fail(e_ == hipErrorOutOfMemory ? DESV_ENOMEM : DESV_EDEVICE,                 \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                    \
+    } while (0)
+
+inline size_t amp_bytes(int dtype) { return dtype == DESV_C64 ? 8 : 16; }
+
+// Device-side copy of a compiled plan (ops + matrices rounded to the state
+// precision).
+struct DevPlan {
+    int device = -1;
+    int dtype = -1;
+    void *buf = nullptr;
+    size_t cap = 0;
+    DevOp *ops = nullptr;
+    void *mats = nullptr;
+
+    void release() {
+        if (buf) (void)hipFree(buf);
+        buf = nullptr;
+        cap = 0;
+    }
+};
+
+int upload_plan(const Plan &p, int dtype, int device, hipStream_t s, DevPlan &d) {
+    const size_t ob = std::max<size_t>(p.ops.size(), 1) * sizeof(DevOp);
+    const size_t oba = (ob + 255) & ~size_t(255);
+    const size_t mb = std::max<size_t>(p.mats.size(), 1) * amp_bytes(dtype);
+    const size_t need = oba + mb;
+    HIP_TRY(hipStreamSynchronize(s));  // the previous upload may still be read
+    if (d.cap < need || d.device != device) {
+        d.release();
+        HIP_TRY(hipMalloc(&d.buf, need));
+        d.cap = need;
+    }
+    std::vector<unsigned char> host(need);
+    std::memcpy(host.data(), p.ops.data(), p.ops.size() * sizeof(DevOp));
+    if (dtype == DESV_C64) {
+        float *f = reinterpret_cast<float *>(host.data() + oba);
+        for (size_t i = 0; i < p.mats.size(); ++i) {
+            f[2 * i] = (float)p.mats[i].real();
+            f[2 * i + 1] = (float)p.mats[i].imag();
+        }
+    } else {
+        double *f = reinterpret_cast<double *>(host.data() + oba);
+        for (size_t i = 0; i < p.mats.size(); ++i) {
+            f[2 * i] = p.mats[i].real();
+            f[2 * i + 1] = p.mats[i].imag();
+        }
+    }
+    HIP_TRY(hipMemcpyAsync(d.buf, host.data(), need, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    d.ops = reinterpret_cast<DevOp *>(d.buf);
+    d.mats = reinterpret_cast<unsigned char *>(d.buf) + oba;
+    d.device = device;
+    d.dtype = dtype;
+    return DESV_OK;
+}
+
+CompileOpts default_opts(int dtype, int device, int nv) {
+    CompileOpts o;
+    o.fuse_bits = 2;
+    o.low_bits = 5;
+    o.nv = nv;
+    const size_t lim = stage_lds_limit(device);
+    int m = 14;
+    while (m > 6 && (size_t)nv * ((size_t)1 << m) * amp_bytes(dtype) > lim) --m;
+    // 32 KB per vector keeps several workgroups resident per CU.
+    const int pref = dtype == DESV_C64 ? 12 : 11;
+    o.tile_bits = std::min(m, pref);
+    o.per_gate = 0;
+    o.use_mfma = 0;
+    return o;
+}
+
+}  // namespace
+
+struct desv_state {
+    int n = 0;
+    int dtype = 0;
+    int device = 0;
+    int64_t dim = 0;
+    size_t bytes = 0;
+    void *sv = nullptr;
+    void *aux[2] = {nullptr, nullptr};
+    hipStream_t own = nullptr;
+    hipStream_t stream = nullptr;
+    double *d_part = nullptr;
+    size_t part_cap = 0;  // doubles
+    double *d_out = nullptr;
+    size_t out_cap = 0;   // doubles
+    DevPlan scratch_plan;
+    void *d_small = nullptr;  // term tables etc.
+    size_t small_cap = 0;
+};
+
+struct desv_prog {
+    int n = 0;
+    std::vector<Gate> gates;
+    int64_t n_logical = 0;
+    bool compiled = false;
+    CompileOpts opts;
+    Plan plan;
+    DevPlan dev;
+};
+
+namespace {
+
+int check_handle(desv_handle h) {
+    if (!h) return fail(DESV_EINVAL, "null handle");
+    return DESV_OK;
+}
+
+int ensure_part(desv_handle h, size_t doubles) {
+    if (h->part_cap >= doubles) return DESV_OK;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (h->d_part) HIP_TRY(hipFree(h->d_part));
+    h->d_part = nullptr;
+    HIP_TRY(hipMalloc(&h->d_part, doubles * sizeof(double)));
+    h->part_cap = doubles;
+    return DESV_OK;
+}
+
+int ensure_out(desv_handle h, size_t doubles) {
+    if (h->out_cap >= doubles) return DESV_OK;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (h->d_out) HIP_TRY(hipFree(h->d_out));
+    h->d_out = nullptr;
+    HIP_TRY(hipMalloc(&h->d_out, doubles * sizeof(double)));
+    h->out_cap = doubles;
+    return DESV_OK;
+}
+
+int ensure_small(desv_handle h, size_t bytes) {
+    if (h->small_cap >= bytes) return DESV_OK;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (h->d_small) HIP_TRY(hipFree(h->d_small));
+    h->d_small = nullptr;
+    HIP_TRY(hipMalloc(&h->d_small, bytes));
+    h->small_cap = bytes;
+    return DESV_OK;
+}
+
+int ensure_aux(desv_handle h, int i) {
+    if (h->aux[i]) return DESV_OK;
+    HIP_TRY(hipMalloc(&h->aux[i], h->bytes));
+    return DESV_OK;
+}
+
+int reduce_blocks(int64_t dim) {
+    int64_t b = (dim + kThreads * 4 - 1) / (kThreads * 4);
+    return (int)std::max<int64_t>(1, std::min<int64_t>(b, 1024));
+}
+
+// Read `nvals` sums of partials back to the host (synchronous).
+int sum_to_host(desv_handle h, int nvals, int nblk, double *out) {
+    int rc = ensure_out(h, nvals);
+    if (rc) return rc;
+    HIP_TRY(launch_sum_partials(h->d_part, nvals, nblk, h->d_out, h->stream));
+    HIP_TRY(hipMemcpyAsync(out, h->d_out, nvals * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return DESV_OK;
+}
+
+int run_plan_on(desv_handle h, const Plan &plan, const DevPlan &dp, int nv, void *sv0, void *sv1,
+                int grad_wg) {
+    for (const DevStage &st : plan.stages) {
+        StageArgs a;
+        std::memset(&a, 0, sizeof(a));
+        a.st = st;
+        a.n = plan.n;
+        a.ops = dp.ops;
+        a.mats = dp.mats;
+        a.partials = h->d_part;
+        int grid;
+        if (st.n_grad > 0) grid = grad_wg;
+        else grid = (int)std::min<int64_t>(st.n_tiles, 1 << 16);
+        a.n_wg = grid;
+        hipError_t e = h->dtype == DESV_C64 ? launch_stage<float>(nv, sv0, sv1, a, grid, h->stream)
+                                            : launch_stage<double>(nv, sv0, sv1, a, grid, h->stream);
+        if (e != hipSuccess) return fail(DESV_EDEVICE, std::string("stage launch: ") + hipGetErrorString(e));
+    }
+    return DESV_OK;
+}
+
+int decode_ops(const double *ops, int64_t n_ops, int n, std::vector<Gate> &gates) {
+    std::string err;
+    for (int64_t i = 0; i < n_ops; ++i) {
+        size_t before = gates.size();
+        if (!decode_row(ops + 4 * i, n, gates, err)) return fail(DESV_EINVAL, err);
+        for (size_t j = before; j < gates.size(); ++j) gates[j].src = (int)i;
+    }
+    return DESV_OK;
+}
+
+int apply_gates(desv_handle h, const std::vector<Gate> &gates) {
+    if (gates.empty()) return DESV_OK;
+    CompileOpts o = default_opts(h->dtype, h->device, 1);
+    Plan plan;
+    std::string err;
+    if (!build_plan(gates, h->n, o, plan, err)) return fail(DESV_EINVAL, err);
+    int rc = upload_plan(plan, h->dtype, h->device, h->stream, h->scratch_plan);
+    if (rc) return rc;
+    return run_plan_on(h, plan, h->scratch_plan, 1, h->sv, nullptr, 1);
+}
+
+// Pauli-sum matvec into `out` (device buffer) from `in`.
+int pauli_matvec_raw(desv_handle h, void *out, const void *in, const uint64_t *xmask, const uint64_t *zmask,
+                     const double *cre, const double *cim, int64_t T) {
+    // group terms by flip mask; fold i^{#Y} into the coefficient
+    std::vector<int64_t> order(T);
+    for (int64_t i = 0; i < T; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return xmask[a] < xmask[b]; });
+    std::vector<MatvecGroup> groups;
+    std::vector<uint64_t> z(T);
+    std::vector<double2> cc(T);
+    for (int64_t r = 0; r < T; ++r) {
+        const int64_t t = order[r];
+        z[r] = zmask[t];
+        const int ny = __builtin_popcountll(xmask[t] & zmask[t]) & 3;
+        double re = cre[t], im = cim ? cim[t] : 0.0, a = re, b = im;
+        // multiply by i^ny
+        for (int k = 0; k < ny; ++k) {
+            double t2 = a;
+            a = -b;
+            b = t2;
+        }
+        cc[r] = make_double2(a, b);
+        if (groups.empty() || groups.back().x != xmask[t]) groups.push_back({xmask[t], (int32_t)r, (int32_t)r + 1});
+        else groups.back().end = (int32_t)r + 1;
+    }
+    const size_t gb = groups.size() * sizeof(MatvecGroup), zb = T * sizeof(uint64_t), cb = T * sizeof(double2);
+    const size_t ga = (gb + 255) & ~size_t(255), za = (zb + 255) & ~size_t(255);
+    int rc = ensure_small(h, ga + za + cb + 256);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    std::vector<unsigned char> host(ga + za + cb);
+    std::memcpy(host.data(), groups.data(), gb);
+    std::memcpy(host.data() + ga, z.data(), zb);
+    std::memcpy(host.data() + ga + za, cc.data(), cb);
+    HIP_TRY(hipMemcpyAsync(h->d_small, host.data(), host.size(), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    unsigned char *base = reinterpret_cast<unsigned char *>(h->d_small);
+    hipError_t e = h->dtype == DESV_C64
+                       ? launch_pauli_matvec<float>(out, in, h->n, (const MatvecGroup *)base, (int)groups.size(),
+                                                    (const uint64_t *)(base + ga), (const double2 *)(base + ga + za),
+                                                    h->stream)
+                       : launch_pauli_matvec<double>(out, in, h->n, (const MatvecGroup *)base, (int)groups.size(),
+                                                     (const uint64_t *)(base + ga), (const double2 *)(base + ga + za),
+                                                     h->stream);
+    if (e != hipSuccess) return fail(DESV_EDEVICE, hipGetErrorString(e));
+    return DESV_OK;
+}
+
+int check_masks(desv_handle h, const uint64_t *xmask, const uint64_t *zmask, int64_t T) {
+    const uint64_t all = (h->n >= 64) ? ~0ull : ((1ull << h->n) - 1);
+    for (int64_t t = 0; t < T; ++t)
+        if ((xmask[t] | zmask[t]) & ~all) return fail(DESV_EINVAL, "Pauli term references a qubit out of range");
+    return DESV_OK;
+}
+
+int dense_matvec_raw(desv_handle h, void *out, const void *in, const double *H) {
+    if (h->n > 14) return fail(DESV_EINVAL, "dense Hamiltonians are limited to n <= 14");
+    const size_t bytes = (size_t)h->dim * h->dim * 16;
+    int rc = ensure_small(h, bytes);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    HIP_TRY(hipMemcpyAsync(h->d_small, H, bytes, hipMemcpyHostToDevice, h->stream));
+    hipError_t e = h->dtype == DESV_C64
+                       ? launch_dense_matvec<float>(out, in, h->n, (const double2 *)h->d_small, h->stream)
+                       : launch_dense_matvec<double>(out, in, h->n, (const double2 *)h->d_small, h->stream);
+    if (e != hipSuccess) return fail(DESV_EDEVICE, hipGetErrorString(e));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return DESV_OK;
+}
+
+int inner_raw(desv_handle h, const void *a, const void *b, double *re, double *im) {
+    const int nblk = reduce_blocks(h->dim);
+    int rc = ensure_part(h, 2 * (size_t)nblk);
+    if (rc) return rc;
+    hipError_t e = h->dtype == DESV_C64 ? launch_inner<float>(a, b, h->dim, h->d_part, nblk, h->stream)
+                                        : launch_inner<double>(a, b, h->dim, h->d_part, nblk, h->stream);
+    if (e != hipSuccess) return fail(DESV_EDEVICE, hipGetErrorString(e));
+    double out[2];
+    rc = sum_to_host(h, 2, nblk, out);
+    if (rc) return rc;
+    if (re) *re = out[0];
+    if (im) *im = out[1];
+    return DESV_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *desv_last_error(void) { return g_err.c_str(); }
+
+int desv_version(void) { return 1; }
+
+int desv_device_count(int *out) {
+    int c = 0;
+    hipError_t e = hipGetDeviceCount(&c);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        c = 0;
+    }
+    if (out) *out = c;
+    return DESV_OK;
+}
+
+int desv_create(int n, int dtype, int device, desv_handle *out) {
+    if (!out) return fail(DESV_EINVAL, "null output pointer");
+    *out = nullptr;
+    if (n < 1 || n > 34) return fail(DESV_EINVAL, "n_qubits must be in [1, 34], got " + std::to_string(n));
+    if (dtype != DESV_C64 && dtype != DESV_C128) return fail(DESV_EINVAL, "dtype must be DESV_C64 or DESV_C128");
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+        (void)hipGetLastError();
+        return fail(DESV_EDEVICE, "no HIP device available");
+    }
+    if (device < 0 || device >= count) return fail(DESV_EINVAL, "device index out of range");
+    HIP_TRY(hipSetDevice(device));
+    std::unique_ptr<desv_state> h(new (std::nothrow) desv_state());
+    if (!h) return fail(DESV_ENOMEM, "host allocation failed");
+    h->n = n;
+    h->dtype = dtype;
+    h->device = device;
+    h->dim = 1ll << n;
+    h->bytes = (size_t)h->dim * amp_bytes(dtype);
+    HIP_TRY(hipStreamCreateWithFlags(&h->own, hipStreamNonBlocking));
+    h->stream = h->own;
+    hipError_t e = hipMalloc(&h->sv, h->bytes);
+    if (e != hipSuccess) {
+        (void)hipStreamDestroy(h->own);
+        return fail(DESV_ENOMEM, "cannot allocate " + std::to_string(h->bytes) + " bytes of device memory");
+    }
+    (void)stage_lds_limit(device);
+    e = dtype == DESV_C64 ? launch_fill_basis<float>(h->sv, h->dim, 0, h->stream)
+                          : launch_fill_basis<double>(h->sv, h->dim, 0, h->stream);
+    if (e != hipSuccess) {
+        (void)hipFree(h->sv);
+        (void)hipStreamDestroy(h->own);
+        return fail(DESV_EDEVICE, hipGetErrorString(e));
+    }
+    *out = h.release();
+    return DESV_OK;
+}
+
+int desv_destroy(desv_handle h) {
+    if (!h) return DESV_OK;
+    (void)hipSetDevice(h->device);
+    (void)hipStreamSynchronize(h->stream);
+    if (h->sv) (void)hipFree(h->sv);
+    for (void *a : h->aux)
+        if (a) (void)hipFree(a);
+    if (h->d_part) (void)hipFree(h->d_part);
+    if (h->d_out) (void)hipFree(h->d_out);
+    if (h->d_small) (void)hipFree(h->d_small);
+    h->scratch_plan.release();
+    if (h->own) (void)hipStreamDestroy(h->own);
+    delete h;
+    return DESV_OK;
+}
+
+int desv_set_stream(desv_handle h, void *stream) {
+    if (int rc = check_handle(h)) return rc;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->stream = stream ? (hipStream_t)stream : h->own;
+    return DESV_OK;
+}
+
+int desv_synchronize(desv_handle h) {
+    if (int rc = check_handle(h)) return rc;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return DESV_OK;
+}
+
+int desv_reset(desv_handle h) {
+    if (int rc = check_handle(h)) return rc;
+    HIP_TRY(hipSetDevice(h->device));
+    HIP_TRY(h->dtype == DESV_C64 ? launch_fill_basis<float>(h->sv, h->dim, 0, h->stream)
+                                 : launch_fill_basis<double>(h->sv, h->dim, 0, h->stream));
+    return DESV_OK;
+}
+
+int desv_set_state(desv_handle h, const void *host) {
+    if (int rc = check_handle(h)) return rc;
+    if (!host) return fail(DESV_EINVAL, "null host buffer");
+    HIP_TRY(hipSetDevice(h->device));
+    HIP_TRY(hipMemcpyAsync(h->sv, host, h->bytes, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return DESV_OK;
+}
+
+int desv_get_state(desv_handle h, void *host) {
+    if (int rc = check_handle(h)) return rc;
+    if (!host) return fail(DESV_EINVAL, "null host buffer");
+    HIP_TRY(hipSetDevice(h->device));
+    HIP_TRY(hipMemcpyAsync(host, h->sv, h->bytes, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return DESV_OK;
+}
+
+int desv_copy(desv_handle dst, desv_handle src) {
+    if (int rc = check_handle(dst)) return rc;
+    if (int rc = check_handle(src)) return rc;
+    if (dst->n != src->n || dst->dtype != src->dtype) return fail(DESV_EINVAL, "copy between different shapes");
+    HIP_TRY(hipStreamSynchronize(src->stream));
+    HIP_TRY(hipMemcpyAsync(dst->sv, src->sv, dst->bytes, hipMemcpyDeviceToDevice, dst->stream));
+    return DESV_OK;
+}
+
+int desv_scale(desv_handle h, double re, double im) {
+    if (int rc = check_handle(h)) return rc;
+    HIP_TRY(h->dtype == DESV_C64 ? launch_scale<float>(h->sv, h->dim, re, im, h->stream)
+                                 : launch_scale<double>(h->sv, h->dim, re, im, h->stream));
+    return DESV_OK;
+}
+
+int desv_norm2(desv_handle h, double *out) {
+    if (int rc = check_handle(h)) return rc;
+    const int nblk = reduce_blocks(h->dim);
+    int rc = ensure_part(h, nblk);
+    if (rc) return rc;
+    HIP_TRY(h->dtype == DESV_C64 ? launch_norm2<float>(h->sv, h->dim, h->d_part, nblk, h->stream)
+                                 : launch_norm2<double>(h->sv, h->dim, h->d_part, nblk, h->stream));
+    return sum_to_host(h, 1, nblk, out);
+}
+
+int desv_inner(desv_handle a, desv_handle b, double *re, double *im) {
+    if (int rc = check_handle(a)) return rc;
+    if (int rc = check_handle(b)) return rc;
+    if (a->n != b->n || a->dtype != b->dtype) return fail(DESV_EINVAL, "inner product of different shapes");
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return inner_raw(a, a->sv, b->sv, re, im);
+}
+
+int desv_probabilities(desv_handle h, double *out) {
+    if (int rc = check_handle(h)) return rc;
+    if (!out) return fail(DESV_EINVAL, "null output buffer");
+    double *d = nullptr;
+    HIP_TRY(hipMalloc(&d, h->dim * sizeof(double)));
+    hipError_t e = h->dtype == DESV_C64 ? launch_probs<float>(h->sv, h->dim, d, h->stream)
+                                        : launch_probs<double>(h->sv, h->dim, d, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d, h->dim * sizeof(double), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(DESV_EDEVICE, hipGetErrorString(e));
+    return DESV_OK;
+}
+
+int desv_qubit_probs(desv_handle h, int qubit, double *p0, double *p1) {
+    if (int rc = check_handle(h)) return rc;
+    if (qubit < 0 || qubit >= h->n) return fail(DESV_EINVAL, "Qubit " + std::to_string(qubit) + " out of range");
+    const int nblk = reduce_blocks(h->dim);
+    int rc = ensure_part(h, 2 * (size_t)nblk);
+    if (rc) return rc;
+    const int bit = h->n - 1 - qubit;
+    HIP_TRY(h->dtype == DESV_C64 ? launch_qubit_probs<float>(h->sv, h->dim, bit, h->d_part, nblk, h->stream)
+                                 : launch_qubit_probs<double>(h->sv, h->dim, bit, h->d_part, nblk, h->stream));
+    double out[2];
+    rc = sum_to_host(h, 2, nblk, out);
+    if (rc) return rc;
+    if (p0) *p0 = out[0];
+    if (p1) *p1 = out[1];
+    return DESV_OK;
+}
+
+int desv_collapse(desv_handle h, int qubit, int outcome, double scale) {
+    if (int rc = check_handle(h)) return rc;
+    if (qubit < 0 || qubit >= h->n) return fail(DESV_EINVAL, "Qubit " + std::to_string(qubit) + " out of range");
+    if (outcome != 0 && outcome != 1) return fail(DESV_EINVAL, "outcome must be 0 or 1");
+    const int bit = h->n - 1 - qubit;
+    HIP_TRY(h->dtype == DESV_C64 ? launch_collapse<float>(h->sv, h->dim, bit, outcome, scale, h->stream)
+                                 : launch_collapse<double>(h->sv, h->dim, bit, outcome, scale, h->stream));
+    return DESV_OK;
+}
+
+int desv_run_ops(desv_handle h, const double *ops, int64_t n_ops) {
+    if (int rc = check_handle(h)) return rc;
+    if (n_ops < 0 || (n_ops > 0 && !ops)) return fail(DESV_EINVAL, "bad op table");
+    HIP_TRY(hipSetDevice(h->device));
+    std::vector<Gate> gates;
+    int rc = decode_ops(ops, n_ops, h->n, gates);
+    if (rc) return rc;
+    return apply_gates(h, gates);
+}
+
+int desv_apply_matrix(desv_handle h, const double *mat, const int *qubits, int k) {
+    if (int rc = check_handle(h)) return rc;
+    if (!mat || !qubits) return fail(DESV_EINVAL, "null matrix or qubit list");
+    HIP_TRY(hipSetDevice(h->device));
+    Gate g;
+    std::string err;
+    if (!decode_matrix(mat, qubits, k, h->n, g, err)) return fail(DESV_EINVAL, err);
+    std::vector<Gate> gs{g};
+    return apply_gates(h, gs);
+}
+
+int desv_default_opts(int dtype, desv_compile_opts *opts) {
+    if (!opts) return fail(DESV_EINVAL, "null opts");
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    CompileOpts o = default_opts(dtype, dev, 1);
+    opts->fuse_bits = o.fuse_bits;
+    opts->tile_bits = o.tile_bits;
+    opts->low_bits = o.low_bits;
+    opts->per_gate = o.per_gate;
+    opts->use_mfma = o.use_mfma;
+    opts->dtype = dtype;
+    return DESV_OK;
+}
+
+int desv_program_create(int n, desv_program *out) {
+    if (!out) return fail(DESV_EINVAL, "null output pointer");
+    if (n < 1 || n > 34) return fail(DESV_EINVAL, "n_qubits must be in [1, 34], got " + std::to_string(n));
+    desv_prog *p = new (std::nothrow) desv_prog();
+    if (!p) return fail(DESV_ENOMEM, "host allocation failed");
+    p->n = n;
+    *out = p;
+    return DESV_OK;
+}
+
+int desv_program_destroy(desv_program p) {
+    if (!p) return DESV_OK;
+    p->dev.release();
+    delete p;
+    return DESV_OK;
+}
+
+int desv_program_add_ops(desv_program p, const double *ops, int64_t n_ops) {
+    if (!p) return fail(DESV_EINVAL, "null program");
+    if (n_ops < 0 || (n_ops > 0 && !ops)) return fail(DESV_EINVAL, "bad op table");
+    int rc = decode_ops(ops, n_ops, p->n, p->gates);
+    if (rc) return rc;
+    p->n_logical += n_ops;
+    p->compiled = false;
+    return DESV_OK;
+}
+
+int desv_program_add_matrix(desv_program p, const double *mat, const int *qubits, int k) {
+    if (!p) return fail(DESV_EINVAL, "null program");
+    if (!mat || !qubits) return fail(DESV_EINVAL, "null matrix or qubit list");
+    Gate g;
+    std::string err;
+    if (!decode_matrix(mat, qubits, k, p->n, g, err)) return fail(DESV_EINVAL, err);
+    p->gates.push_back(g);
+    p->n_logical += 1;
+    p->compiled = false;
+    return DESV_OK;
+}
+
+int desv_program_compile(desv_program p, const desv_compile_opts *opts) {
+    if (!p) return fail(DESV_EINVAL, "null program");
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    const int dtype = opts ? opts->dtype : DESV_C64;
+    CompileOpts o = default_opts(dtype, dev, 1);
+    if (opts) {
+        if (opts->fuse_bits) o.fuse_bits = opts->fuse_bits;
+        if (opts->tile_bits) o.tile_bits = opts->tile_bits;
+        if (opts->low_bits) o.low_bits = opts->low_bits;
+        o.per_gate = opts->per_gate;
+        o.use_mfma = opts->use_mfma;
+    }
+    if (o.fuse_bits < 1 || o.fuse_bits > 5) return fail(DESV_EINVAL, "fuse_bits must be in [1, 5]");
+    const size_t lim = stage_lds_limit(dev);
+    if (((size_t)1 << std::min(o.tile_bits, p->n)) * amp_bytes(dtype) > lim)
+        return fail(DESV_EINVAL, "tile_bits exceeds the LDS of this device");
+    std::string err;
+    if (!build_plan(p->gates, p->n, o, p->plan, err)) return fail(DESV_EINVAL, err);
+    p->plan.n_gates = p->n_logical;
+    p->opts = o;
+    p->compiled = true;
+    p->dev.dtype = -1;  // force re-upload
+    return DESV_OK;
+}
+
+int desv_program_stats(desv_program p, int64_t *gates, int64_t *passes, int64_t *blocks, int64_t *mfma) {
+    if (!p) return fail(DESV_EINVAL, "null program");
+    if (!p->compiled) return fail(DESV_EINVAL, "program is not compiled");
+    if (gates) *gates = p->n_logical;
+    if (passes) *passes = (int64_t)p->plan.stages.size();
+    if (blocks) *blocks = (int64_t)p->plan.ops.size();
+    if (mfma) *mfma = p->plan.mfma_blocks;
+    return DESV_OK;
+}
+
+int desv_program_plan_sizes(desv_program p, int64_t *n_stages, int64_t *n_ops, int64_t *n_mats) {
+    if (!p) return fail(DESV_EINVAL, "null program");
+    if (!p->compiled) return fail(DESV_EINVAL, "program is not compiled");
+    if (n_stages) *n_stages = (int64_t)p->plan.stages.size();
+    if (n_ops) *n_ops = (int64_t)p->plan.ops.size();
+    if (n_mats) *n_mats = (int64_t)p->plan.mats.size();
+    return DESV_OK;
+}
+
+int desv_program_plan_dump(desv_program p, uint64_t *tile_masks, int64_t *stage_ops, int64_t *ops, double *mats) {
+    if (!p) return fail(DESV_EINVAL, "null program");
+    if (!p->compiled) return fail(DESV_EINVAL, "program is not compiled");
+    const Plan &pl = p->plan;
+    for (size_t s = 0; s < pl.stages.size(); ++s) {
+        if (tile_masks) tile_masks[s] = pl.stages[s].tile_mask;
+        if (stage_ops) {
+            stage_ops[2 * s] = pl.stages[s].op_begin;
+            stage_ops[2 * s + 1] = pl.stages[s].op_end;
+        }
+    }
+    if (ops)
+        for (size_t i = 0; i < pl.ops.size(); ++i) {
+            const DevOp &d = pl.ops[i];
+            int64_t *o = ops + 11 * i;
+            o[0] = d.kind; o[1] = d.k; o[2] = d.mat; o[3] = d.slot;
+            for (int j = 0; j < 5; ++j) o[4 + j] = d.pos[j];
+            o[9] = d.cloc;
+            o[10] = (int64_t)d.cglob;
+        }
+    if (mats)
+        for (size_t i = 0; i < pl.mats.size(); ++i) {
+            mats[2 * i] = pl.mats[i].real();
+            mats[2 * i + 1] = pl.mats[i].imag();
+        }
+    return DESV_OK;
+}
+
+int desv_run_program(desv_handle h, desv_program p) {
+    if (int rc = check_handle(h)) return rc;
+    if (!p) return fail(DESV_EINVAL, "null program");
+    if (p->n != h->n) return fail(DESV_EINVAL, "program and state have different qubit counts");
+    HIP_TRY(hipSetDevice(h->device));
+    if (!p->compiled) {
+        desv_compile_opts o;
+        desv_default_opts(h->dtype, &o);
+        int rc = desv_program_compile(p, &o);
+        if (rc) return rc;
+    }
+    if (p->plan.mfma_blocks > 0)
+        return fail(DESV_EUNSUPPORTED, "MFMA block kernels are not built into this library yet");
+    if (p->dev.dtype != h->dtype || p->dev.device != h->device) {
+        int rc = upload_plan(p->plan, h->dtype, h->device, h->stream, p->dev);
+        if (rc) return rc;
+    }
+    return run_plan_on(h, p->plan, p->dev, 1, h->sv, nullptr, 1);
+}
+
+int desv_pauli_expect(desv_handle h, const uint64_t *xmask, const uint64_t *zmask, int64_t T, double *out) {
+    if (int rc = check_handle(h)) return rc;
+    if (T == 0) return DESV_OK;
+    if (T < 0 || !xmask || !zmask || !out) return fail(DESV_EINVAL, "bad Pauli term arrays");
+    if (int rc = check_masks(h, xmask, zmask, T)) return rc;
+    HIP_TRY(hipSetDevice(h->device));
+    std::vector<int64_t> order(T);
+    for (int64_t i = 0; i < T; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return xmask[a] < xmask[b]; });
+    std::vector<uint64_t> z(T);
+    std::vector<PauliItem> items;
+    for (int64_t r = 0; r < T; ++r) {
+        const int64_t t = order[r];
+        z[r] = zmask[t];
+        if (items.empty() || items.back().x != xmask[t] || items.back().count == kTermsPerItem)
+            items.push_back({xmask[t], (int32_t)r, 1});
+        else
+            items.back().count++;
+    }
+    const size_t ib = items.size() * sizeof(PauliItem), ia = (ib + 255) & ~size_t(255);
+    int rc = ensure_small(h, ia + T * sizeof(uint64_t));
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    std::vector<unsigned char> host(ia + T * sizeof(uint64_t));
+    std::memcpy(host.data(), items.data(), ib);
+    std::memcpy(host.data() + ia, z.data(), T * sizeof(uint64_t));
+    HIP_TRY(hipMemcpyAsync(h->d_small, host.data(), host.size(), hipMemcpyHostToDevice, h->stream));
+    int nblk = (int)std::max<int64_t>(1, std::min<int64_t>(h->dim / (kThreads * 8), 256));
+    rc = ensure_part(h, 2 * (size_t)T * nblk);
+    if (rc) return rc;
+    unsigned char *base = reinterpret_cast<unsigned char *>(h->d_small);
+    hipError_t e = h->dtype == DESV_C64
+                       ? launch_pauli_expect<float>(h->sv, h->n, (const PauliItem *)base, (int)items.size(),
+                                                    (const uint64_t *)(base + ia), h->d_part, nblk, h->stream)
+                       : launch_pauli_expect<double>(h->sv, h->n, (const PauliItem *)base, (int)items.size(),
+                                                     (const uint64_t *)(base + ia), h->d_part, nblk, h->stream);
+    if (e != hipSuccess) return fail(DESV_EDEVICE, hipGetErrorString(e));
+    std::vector<double> sums(2 * T);
+    rc = sum_to_host(h, (int)(2 * T), nblk, sums.data());
+    if (rc) return rc;
+    for (int64_t r = 0; r < T; ++r) {
+        const int64_t t = order[r];
+        const int ny = __builtin_popcountll(xmask[t] & zmask[t]) & 3;
+        const double re = sums[2 * r], im = sums[2 * r + 1];
+        // Re(i^ny (re + i im))
+        double v = ny == 0 ? re : ny == 1 ? -im : ny == 2 ? -re : im;
+        out[t] = v;
+    }
+    return DESV_OK;
+}
+
+int desv_pauli_matvec(desv_handle out, desv_handle in, const uint64_t *xmask, const uint64_t *zmask,
+                      const double *cre, const double *cim, int64_t T) {
+    if (int rc = check_handle(out)) return rc;
+    if (int rc = check_handle(in)) return rc;
+    if (out == in) return fail(DESV_EINVAL, "pauli_matvec needs distinct input and output states");
+    if (out->n != in->n || out->dtype != in->dtype) return fail(DESV_EINVAL, "matvec between different shapes");
+    if (T < 0 || (T > 0 && (!xmask || !zmask || !cre))) return fail(DESV_EINVAL, "bad Pauli term arrays");
+    if (int rc = check_masks(in, xmask, zmask, T)) return rc;
+    HIP_TRY(hipSetDevice(out->device));
+    HIP_TRY(hipStreamSynchronize(in->stream));
+    if (T == 0) {
+        HIP_TRY(out->dtype == DESV_C64 ? launch_scale<float>(out->sv, out->dim, 0.0, 0.0, out->stream)
+                                       : launch_scale<double>(out->sv, out->dim, 0.0, 0.0, out->stream));
+        return DESV_OK;
+    }
+    return pauli_matvec_raw(out, out->sv, in->sv, xmask, zmask, cre, cim, T);
+}
+
+int desv_dense_matvec(desv_handle out, desv_handle in, const double *H) {
+    if (int rc = check_handle(out)) return rc;
+    if (int rc = check_handle(in)) return rc;
+    if (out == in) return fail(DESV_EINVAL, "dense_matvec needs distinct input and output states");
+    if (out->n != in->n || out->dtype != in->dtype) return fail(DESV_EINVAL, "matvec between different shapes");
+    if (!H) return fail(DESV_EINVAL, "null matrix");
+    HIP_TRY(hipSetDevice(out->device));
+    HIP_TRY(hipStreamSynchronize(in->stream));
+    return dense_matvec_raw(out, out->sv, in->sv, H);
+}
+
+int desv_adjoint(desv_handle h, const double *ops, int64_t n_ops, const int64_t *param_of_op, int64_t n_params,
+                 const uint64_t *xmask, const uint64_t *zmask, const double *coeff, int64_t T,
+                 const double *dense_h, double *energy, double *grad) {
+    if (int rc = check_handle(h)) return rc;
+    if (n_ops < 0 || (n_ops > 0 && (!ops || !param_of_op))) return fail(DESV_EINVAL, "bad op table");
+    if (n_params < 0 || (n_params > 0 && !grad)) return fail(DESV_EINVAL, "bad gradient buffer");
+    if (!xmask && !dense_h) return fail(DESV_EINVAL, "no Hamiltonian given");
+    if (xmask && (T < 0 || (T > 0 && (!zmask || !coeff)))) return fail(DESV_EINVAL, "bad Pauli term arrays");
+    if (xmask)
+        if (int rc = check_masks(h, xmask, zmask, T)) return rc;
+    HIP_TRY(hipSetDevice(h->device));
+    // forward gates + reverse list with gradient probes
+    std::vector<Gate> fwd;
+    std::vector<Gate> rev;
+    std::vector<int64_t> slot_param;
+    {
+        std::string err;
+        std::vector<std::vector<Gate>> per_row(n_ops);
+        for (int64_t i = 0; i < n_ops; ++i) {
+            if (!decode_row(ops + 4 * i, h->n, per_row[i], err)) return fail(DESV_EINVAL, err);
+            for (auto &g : per_row[i]) fwd.push_back(g);
+        }
+        for (int64_t i = n_ops - 1; i >= 0; --i) {
+            const int64_t pi = param_of_op[i];
+            if (pi >= n_params) return fail(DESV_EINVAL, "param_of_op index out of range");
+            if (pi >= 0) {
+                Gate gg;
+                if (!grad_gate_for_row(ops + 4 * i, h->n, (int)slot_param.size(), gg, err))
+                    return fail(DESV_EINVAL, err);
+                slot_param.push_back(pi);
+                rev.push_back(gg);
+            }
+            for (auto it = per_row[i].rbegin(); it != per_row[i].rend(); ++it) rev.push_back(inverse_gate(*it));
+        }
+    }
+    int rc = ensure_aux(h, 0);
+    if (rc) return rc;
+    rc = ensure_aux(h, 1);
+    if (rc) return rc;
+    // forward
+    rc = apply_gates(h, fwd);
+    if (rc) return rc;
+    // lambda = H psi
+    if (xmask) {
+        std::vector<double> zero(T, 0.0);
+        rc = pauli_matvec_raw(h, h->aux[1], h->sv, xmask, zmask, coeff, zero.data(), T);
+    } else {
+        rc = dense_matvec_raw(h, h->aux[1], h->sv, dense_h);
+    }
+    if (rc) return rc;
+    double e_re = 0;
+    rc = inner_raw(h, h->sv, h->aux[1], &e_re, nullptr);
+    if (rc) return rc;
+    if (energy) *energy = e_re;
+    for (int64_t k = 0; k < n_params; ++k) grad[k] = 0.0;
+    if (slot_param.empty()) return DESV_OK;
+    // reverse sweep on a copy of psi (the handle keeps the final state)
+    HIP_TRY(hipMemcpyAsync(h->aux[0], h->sv, h->bytes, hipMemcpyDeviceToDevice, h->stream));
+    CompileOpts o = default_opts(h->dtype, h->device, 2);
+    Plan plan;
+    std::string err;
+    if (!build_plan(rev, h->n, o, plan, err)) return fail(DESV_EINVAL, err);
+    const int grad_wg = (int)std::max<int64_t>(1, std::min<int64_t>(1ll << std::max(0, h->n - o.tile_bits), 1024));
+    rc = ensure_part(h, (size_t)plan.n_grad_slots * grad_wg);
+    if (rc) return rc;
+    rc = upload_plan(plan, h->dtype, h->device, h->stream, h->scratch_plan);
+    if (rc) return rc;
+    rc = run_plan_on(h, plan, h->scratch_plan, 2, h->aux[0], h->aux[1], grad_wg);
+    if (rc) return rc;
+    std::vector<double> slots(plan.n_grad_slots);
+    rc = sum_to_host(h, plan.n_grad_slots, grad_wg, slots.data());
+    if (rc) return rc;
+    for (size_t s = 0; s < slot_param.size(); ++s) grad[slot_param[s]] += slots[s];
+    return DESV_OK;
+}
+
+}  // extern "C"

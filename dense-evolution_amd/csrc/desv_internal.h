@@ -1,0 +1,126 @@
+// Internal structures shared by the host compiler (plan.cpp) and the HIP
+// engine (engine.hip).  Not part of the public ABI.
+#pragma once
+
+#include <complex>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace desv {
+
+using cd = std::complex<double>;
+
+// ---------------------------------------------------------------------------
+// Host gate representation
+// ---------------------------------------------------------------------------
+// A gate acts on k target bits (global basis-index bit positions, LSB-first
+// matrix index: matrix index bit j <-> bits[j]) and is conditioned on every
+// bit of `cmask` being 1.
+enum GateKind : int {
+    G_DENSE = 0,   // 2^k x 2^k row-major matrix
+    G_DIAG = 1,    // 2^k diagonal entries
+    G_SCALAR = 2,  // one complex scalar on the whole state (k = 0)
+    G_GRAD = 3,    // adjoint inner product Re<lam| M |psi> (M dense or diag)
+};
+
+struct Gate {
+    int kind = G_DENSE;
+    int k = 0;
+    int bits[5] = {0, 0, 0, 0, 0};
+    uint64_t cmask = 0;
+    std::vector<cd> m;   // dense: 4^k entries, diag: 2^k entries, scalar: 1
+    bool grad_diag = false;  // for G_GRAD: m holds a diagonal
+    int grad_slot = -1;      // for G_GRAD: index into the gradient slots
+    int src = -1;            // index of the logical gate it came from
+
+    uint64_t target_mask() const {
+        uint64_t t = 0;
+        for (int j = 0; j < k; ++j) t |= 1ull << bits[j];
+        return t;
+    }
+    // Bits that must sit inside an LDS tile (non-diagonal action).
+    uint64_t tile_mask() const {
+        // Gradient probes read their target bits from the tile, diagonal or not.
+        if (kind == G_DENSE || kind == G_GRAD) return target_mask();
+        return 0;
+    }
+    uint64_t all_mask() const { return target_mask() | cmask; }
+    bool is_diag() const { return kind == G_DIAG || kind == G_SCALAR || (kind == G_GRAD && grad_diag); }
+};
+
+// Decode one reference op-table row [g_id, q1, q2, param] into a gate on an
+// n-qubit state.  Returns false (with `err`) for unknown ids / bad qubits.
+// Identity rows produce no gate (ok = true, out empty).
+bool decode_row(const double *row, int n, std::vector<Gate> &out, std::string &err);
+// Arbitrary k-qubit matrix (qubits[0] = most significant index bit).
+bool decode_matrix(const double *mat, const int *qubits, int k, int n, Gate &g, std::string &err);
+// Inverse (conjugate transpose) of a gate.
+Gate inverse_gate(const Gate &g);
+// Generator gate for the adjoint sweep of a theta-carrying row:
+// grad contribution = Re<lam| M |psi> with M = 2 * dU/dtheta * U^dag.
+bool grad_gate_for_row(const double *row, int n, int slot, Gate &g, std::string &err);
+
+// ---------------------------------------------------------------------------
+// Device-side op encoding (one LDS-tile micro-op)
+// ---------------------------------------------------------------------------
+enum OpKind : int32_t {
+    OPK_DENSE1 = 0,   // 2x2 on one tile bit
+    OPK_DENSE2 = 1,   // 4x4 on two tile bits
+    OPK_DENSEK = 2,   // 2^k x 2^k (k = 3..5) on VALU
+    OPK_DIAG = 3,     // diagonal over k bits (tile-local or global)
+    OPK_SCALAR = 4,   // whole-tile scalar
+    OPK_GRAD = 5,     // adjoint inner product (dense k<=2 on tile bits)
+    OPK_GRADDIAG = 6, // adjoint inner product, diagonal generator
+    OPK_MFMA = 7,     // 2^k x 2^k (k = 4, 5) real-split complex GEMM on MFMA
+};
+
+struct DevOp {
+    int32_t kind;
+    int32_t k;
+    int32_t mat;       // offset (complex elements) into the matrix pool
+    int32_t slot;      // gradient slot (program-wide index)
+    int32_t pos[5];    // tile-local bit of matrix index bit j; for OPK_DIAG a
+                       // negative value -1-b means global bit b (outside tile)
+    uint32_t cloc;     // controls on tile-local bits
+    uint64_t cglob;    // controls on global bits outside the tile
+};
+
+constexpr int MAX_HBITS = 24;
+
+struct DevStage {
+    int32_t m;         // tile bits (amplitudes per tile = 2^m)
+    int32_t L;         // low contiguous global bits inside the tile
+    int32_t nh;        // high tile bits (m - L)
+    int32_t hbits[MAX_HBITS];
+    uint64_t tile_mask;
+    int64_t n_tiles;
+    int32_t op_begin, op_end;
+    int32_t n_grad;    // OPK_GRAD* ops in this stage
+    int32_t grad_base; // first gradient slot of this stage
+};
+
+struct Plan {
+    int n = 0;
+    int64_t n_gates = 0;
+    int64_t mfma_blocks = 0;
+    int n_grad_slots = 0;
+    std::vector<DevStage> stages;
+    std::vector<DevOp> ops;
+    std::vector<cd> mats;  // pooled matrices (complex128), rounded at upload
+};
+
+struct CompileOpts {
+    int fuse_bits = 2;
+    int tile_bits = 12;
+    int low_bits = 5;
+    int per_gate = 0;
+    int use_mfma = 0;
+    int nv = 1;            // vectors per tile (2 for the adjoint reverse sweep)
+};
+
+// Build the device plan for a gate list (in application order).
+bool build_plan(const std::vector<Gate> &gates, int n, const CompileOpts &o, Plan &plan,
+                std::string &err);
+
+}  // namespace desv

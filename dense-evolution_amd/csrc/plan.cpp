@@ -1,0 +1,511 @@
+// Host-side circuit compiler: op-table decoding, gate fusion and
+// cache-blocked stage scheduling for the LDS-tile engine.
+//
+// The reference applies one gate per full-vector pass (compiler.py:136-315,
+// scanned at :333).  Here a circuit is cut into *stages*: each stage picks a
+// set of m basis-index bits (always including the low bits, so global loads
+// are contiguous runs), and every gate whose non-diagonal action stays inside
+// that set is applied while a tile of 2^m amplitudes sits in LDS.  One stage
+// costs one HBM read + write of the state no matter how many gates it holds.
+// Diagonal gates and controls may involve bits outside the tile: those bits
+// are constant over a tile.
+#include "desv_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace desv {
+
+namespace {
+
+const double kInv2 = 0.70710678118654752440;
+
+inline int popc(uint64_t x) { return __builtin_popcountll(x); }
+
+// 2x2 matrix of 1-qubit gate ids 0..14 (reference compiler.py:52-134).
+void mat_1q(int g, double p, cd m[4]) {
+    const cd I(0, 1);
+    double c = std::cos(0.5 * p), s = std::sin(0.5 * p);
+    cd e = std::exp(I * p);
+    m[0] = 1; m[1] = 0; m[2] = 0; m[3] = 1;
+    switch (g) {
+        case 1: m[0] = kInv2; m[1] = kInv2; m[2] = kInv2; m[3] = -kInv2; break;
+        case 2: m[0] = 0; m[1] = 1; m[2] = 1; m[3] = 0; break;
+        case 3: m[0] = 0; m[1] = -I; m[2] = I; m[3] = 0; break;
+        case 4: m[3] = -1; break;
+        case 5: m[3] = I; break;
+        case 6: m[3] = -I; break;
+        case 7: m[3] = std::exp(I * (M_PI / 4)); break;
+        case 8: m[3] = std::exp(-I * (M_PI / 4)); break;
+        case 9: m[0] = c; m[1] = -I * s; m[2] = -I * s; m[3] = c; break;
+        case 10: m[0] = c; m[1] = -s; m[2] = s; m[3] = c; break;
+        case 11: m[0] = std::exp(-0.5 * I * p); m[3] = std::exp(0.5 * I * p); break;
+        case 12: m[3] = e; break;
+        case 13: m[0] = cd(0.5, 0.5); m[1] = cd(0.5, -0.5); m[2] = cd(0.5, -0.5); m[3] = cd(0.5, 0.5); break;
+        case 14: m[0] = e; m[3] = e; break;
+        default: break;
+    }
+}
+
+bool is_diag_1q(int g) {
+    return g == 0 || g == 4 || g == 5 || g == 6 || g == 7 || g == 8 || g == 11 || g == 12 || g == 14;
+}
+
+}  // namespace
+
+bool decode_row(const double *row, int n, std::vector<Gate> &out, std::string &err) {
+    double gd = row[0];
+    int g = (int)gd;
+    if ((double)g != gd || !((g >= 0 && g <= 14) || (g >= 20 && g <= 25))) {
+        err = "unknown gate id " + std::to_string(gd);
+        return false;
+    }
+    double fq1 = row[1], fq2 = row[2], p = row[3];
+    int q1 = (int)fq1;
+    if ((double)q1 != fq1 || q1 < 0 || q1 >= n) {
+        err = "Qubit index " + std::to_string((long long)fq1) + " out of range [0, " + std::to_string(n) + ")";
+        return false;
+    }
+    const cd I(0, 1);
+    if (g <= 14) {
+        if (g == 0) return true;
+        cd m[4];
+        mat_1q(g, p, m);
+        Gate gt;
+        gt.k = 1;
+        gt.bits[0] = n - 1 - q1;
+        if (g == 14) {
+            gt.kind = G_SCALAR;
+            gt.k = 0;
+            gt.m = {m[0]};
+        } else if (is_diag_1q(g)) {
+            gt.kind = G_DIAG;
+            gt.m = {m[0], m[3]};
+        } else {
+            gt.kind = G_DENSE;
+            gt.m.assign(m, m + 4);
+        }
+        out.push_back(gt);
+        return true;
+    }
+    int q2 = (int)fq2;
+    if ((double)q2 != fq2 || q2 < 0 || q2 >= n) {
+        err = "Qubit index " + std::to_string((long long)fq2) + " out of range [0, " + std::to_string(n) + ")";
+        return false;
+    }
+    if (q1 == q2) {
+        err = "Control and target qubits must differ";
+        return false;
+    }
+    int cb = n - 1 - q1, tb = n - 1 - q2;
+    Gate gt;
+    gt.k = 1;
+    gt.bits[0] = tb;
+    gt.cmask = 1ull << cb;
+    switch (g) {
+        case 20: gt.kind = G_DENSE; gt.m = {0, 1, 1, 0}; break;
+        case 21: gt.kind = G_DIAG; gt.m = {1, -1}; break;
+        case 22: gt.kind = G_DIAG; gt.m = {1, std::exp(I * p)}; break;
+        case 23:
+            gt.kind = G_DENSE;
+            gt.k = 2;
+            gt.cmask = 0;
+            gt.bits[0] = tb;
+            gt.bits[1] = cb;
+            gt.m.assign(16, 0);
+            gt.m[0] = 1; gt.m[1 * 4 + 2] = 1; gt.m[2 * 4 + 1] = 1; gt.m[15] = 1;
+            break;
+        case 24: gt.kind = G_DENSE; gt.m = {0, -I, I, 0}; break;
+        case 25: gt.kind = G_DIAG; gt.m = {std::exp(-0.5 * I * p), std::exp(0.5 * I * p)}; break;
+        default: err = "unknown gate id " + std::to_string(g); return false;
+    }
+    out.push_back(gt);
+    return true;
+}
+
+bool decode_matrix(const double *mat, const int *qubits, int k, int n, Gate &g, std::string &err) {
+    if (k < 1 || k > 5) {
+        err = "matrix gates act on 1..5 qubits";
+        return false;
+    }
+    uint64_t seen = 0;
+    for (int j = 0; j < k; ++j) {
+        if (qubits[j] < 0 || qubits[j] >= n) {
+            err = "Qubit index " + std::to_string(qubits[j]) + " out of range [0, " + std::to_string(n) + ")";
+            return false;
+        }
+        uint64_t b = 1ull << (n - 1 - qubits[j]);
+        if (seen & b) {
+            err = "Control and target qubits must differ";
+            return false;
+        }
+        seen |= b;
+    }
+    g = Gate();
+    g.kind = G_DENSE;
+    g.k = k;
+    for (int i = 0; i < k; ++i) g.bits[i] = n - 1 - qubits[k - 1 - i];
+    int d = 1 << k;
+    g.m.resize((size_t)d * d);
+    for (int i = 0; i < d * d; ++i) g.m[i] = cd(mat[2 * i], mat[2 * i + 1]);
+    return true;
+}
+
+Gate inverse_gate(const Gate &g) {
+    Gate r = g;
+    if (g.kind == G_DENSE) {
+        int d = 1 << g.k;
+        for (int i = 0; i < d; ++i)
+            for (int j = 0; j < d; ++j) r.m[(size_t)i * d + j] = std::conj(g.m[(size_t)j * d + i]);
+    } else {
+        for (auto &x : r.m) x = std::conj(x);
+    }
+    return r;
+}
+
+bool grad_gate_for_row(const double *row, int n, int slot, Gate &g, std::string &err) {
+    int gid = (int)row[0];
+    int q1 = (int)row[1], q2 = (int)row[2];
+    const cd I(0, 1);
+    g = Gate();
+    g.kind = G_GRAD;
+    g.k = 1;
+    g.grad_slot = slot;
+    g.bits[0] = n - 1 - q1;
+    // M = 2 * G with dU/dtheta = G U, so grad += Re<lam| M |psi>.
+    switch (gid) {
+        case 9: g.grad_diag = false; g.m = {0, -I, -I, 0}; break;         // -i X
+        case 10: g.grad_diag = false; g.m = {0, -1, 1, 0}; break;         // -i Y
+        case 11: g.grad_diag = true; g.m = {-I, I}; break;                // -i Z
+        case 12: g.grad_diag = true; g.m = {0, 2.0 * I}; break;           // 2i |1><1|
+        case 22:
+            g.grad_diag = true; g.m = {0, 2.0 * I};
+            g.bits[0] = n - 1 - q2; g.cmask = 1ull << (n - 1 - q1); break;
+        case 25:
+            g.grad_diag = true; g.m = {-I, I};
+            g.bits[0] = n - 1 - q2; g.cmask = 1ull << (n - 1 - q1); break;
+        default:
+            err = "gate id " + std::to_string(gid) + " does not carry a differentiable parameter";
+            return false;
+    }
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// Fusion
+// ---------------------------------------------------------------------------
+namespace {
+
+// Apply gate g (global bits) to every column of the dense block matrix B that
+// acts on `ub` (LSB-first list of K global bits): B <- G B.
+void left_apply(std::vector<cd> &B, const std::vector<int> &ub, const Gate &g) {
+    int K = (int)ub.size();
+    int D = 1 << K;
+    auto local = [&](int b) {
+        for (int i = 0; i < K; ++i)
+            if (ub[i] == b) return i;
+        return -1;
+    };
+    uint32_t cl = 0;
+    for (int b = 0; b < 64; ++b)
+        if (g.cmask >> b & 1) cl |= 1u << local(b);
+    if (g.kind == G_SCALAR) {
+        for (auto &x : B) x *= g.m[0];
+        return;
+    }
+    int lp[5];
+    for (int j = 0; j < g.k; ++j) lp[j] = local(g.bits[j]);
+    uint32_t tm = 0;
+    for (int j = 0; j < g.k; ++j) tm |= 1u << lp[j];
+    int gd = 1 << g.k;
+    std::vector<cd> tmp(gd);
+    for (int col = 0; col < D; ++col) {
+        for (int r0 = 0; r0 < D; ++r0) {
+            if (r0 & tm) continue;
+            if ((r0 & cl) != cl) continue;
+            int idx[32];
+            for (int s = 0; s < gd; ++s) {
+                int r = r0;
+                for (int j = 0; j < g.k; ++j)
+                    if (s >> j & 1) r |= 1 << lp[j];
+                idx[s] = r;
+            }
+            if (g.kind == G_DIAG) {
+                for (int s = 0; s < gd; ++s) B[(size_t)idx[s] * D + col] *= g.m[s];
+            } else {
+                for (int s = 0; s < gd; ++s) {
+                    cd acc = 0;
+                    for (int t = 0; t < gd; ++t) acc += g.m[(size_t)s * gd + t] * B[(size_t)idx[t] * D + col];
+                    tmp[s] = acc;
+                }
+                for (int s = 0; s < gd; ++s) B[(size_t)idx[s] * D + col] = tmp[s];
+            }
+        }
+    }
+}
+
+struct Block {
+    uint64_t bits = 0;
+    std::vector<int> gates;  // indices into the stage gate list, in order
+};
+
+// Turn a block into one gate: a lone gate stays as it is; otherwise the
+// product is formed over the union bits (all-diagonal products stay diagonal).
+Gate materialise(const Block &b, const std::vector<Gate> &gs) {
+    if (b.gates.size() == 1) return gs[b.gates[0]];
+    std::vector<int> ub;
+    for (int i = 0; i < 64; ++i)
+        if (b.bits >> i & 1) ub.push_back(i);
+    int K = (int)ub.size();
+    int D = 1 << K;
+    bool all_diag = true;
+    for (int gi : b.gates) all_diag &= gs[gi].kind == G_DIAG || gs[gi].kind == G_SCALAR;
+    Gate out;
+    out.k = K;
+    for (int i = 0; i < K; ++i) out.bits[i] = ub[i];
+    out.src = gs[b.gates[0]].src;
+    if (all_diag) {
+        out.kind = G_DIAG;
+        out.m.assign(D, 1.0);
+        for (int gi : b.gates) {
+            const Gate &g = gs[gi];
+            for (int s = 0; s < D; ++s) {
+                bool on = true;
+                int sub = 0;
+                for (int i = 0; i < K; ++i) {
+                    bool v = s >> i & 1;
+                    if ((g.cmask >> ub[i] & 1) && !v) on = false;
+                    for (int j = 0; j < g.k; ++j)
+                        if (g.kind == G_DIAG && g.bits[j] == ub[i] && v) sub |= 1 << j;
+                }
+                if (!on) continue;
+                out.m[s] *= (g.kind == G_SCALAR) ? g.m[0] : g.m[sub];
+            }
+        }
+        return out;
+    }
+    out.kind = G_DENSE;
+    out.m.assign((size_t)D * D, 0.0);
+    for (int i = 0; i < D; ++i) out.m[(size_t)i * D + i] = 1.0;
+    for (int gi : b.gates) left_apply(out.m, ub, gs[gi]);
+    return out;
+}
+
+// Greedy fusion of a gate sequence into blocks of at most K bits.  A gate is
+// fusable when it is not a gradient probe, and all its bits lie in `allowed`.
+// Open blocks are always the latest writer of their bits, so merging open
+// blocks that touch a new gate keeps a valid order.
+std::vector<Gate> fuse(const std::vector<Gate> &gs, int K, uint64_t allowed) {
+    std::vector<Gate> out;
+    std::vector<Block> open;
+    auto emit_touching = [&](uint64_t mask) {
+        std::vector<Block> keep;
+        for (auto &b : open) {
+            if (b.bits & mask) out.push_back(materialise(b, gs));
+            else keep.push_back(std::move(b));
+        }
+        open.swap(keep);
+    };
+    for (int i = 0; i < (int)gs.size(); ++i) {
+        const Gate &g = gs[i];
+        uint64_t am = g.all_mask();
+        bool fusable = g.kind != G_GRAD && (am & ~allowed) == 0 && popc(am) <= K;
+        if (g.kind == G_SCALAR) fusable = false;
+        if (!fusable) {
+            emit_touching(am);
+            out.push_back(g);
+            continue;
+        }
+        uint64_t u = am;
+        for (auto &b : open)
+            if (b.bits & am) u |= b.bits;
+        if (popc(u) <= K) {
+            Block nb;
+            nb.bits = u;
+            std::vector<Block> keep;
+            for (auto &b : open) {
+                if (b.bits & am) nb.gates.insert(nb.gates.end(), b.gates.begin(), b.gates.end());
+                else keep.push_back(std::move(b));
+            }
+            nb.gates.push_back(i);
+            keep.push_back(std::move(nb));
+            open.swap(keep);
+        } else {
+            emit_touching(am);
+            Block nb;
+            nb.bits = am;
+            nb.gates.push_back(i);
+            open.push_back(std::move(nb));
+        }
+    }
+    for (auto &b : open) out.push_back(materialise(b, gs));
+    return out;
+}
+
+uint64_t low_mask(int L) { return L >= 64 ? ~0ull : ((1ull << L) - 1); }
+
+// Fill a tile mask up to m bits with the lowest free bits.
+uint64_t fill_tile(uint64_t tile, int m, int n) {
+    for (int b = 0; b < n && popc(tile) < m; ++b) tile |= 1ull << b;
+    return tile;
+}
+
+const int kMaxGradPerStage = 64;
+
+bool lower_stage(const std::vector<Gate> &ops, uint64_t tile, int n, const CompileOpts &o,
+                 Plan &plan, std::string &err) {
+    DevStage st;
+    std::memset(&st, 0, sizeof(st));
+    st.m = popc(tile);
+    int L = 0;
+    while (L < n && (tile >> L & 1)) ++L;
+    st.L = L;
+    st.nh = 0;
+    for (int b = L; b < n; ++b)
+        if (tile >> b & 1) {
+            if (st.nh >= MAX_HBITS) {
+                err = "tile has too many high bits";
+                return false;
+            }
+            st.hbits[st.nh++] = b;
+        }
+    st.tile_mask = tile;
+    st.n_tiles = 1ll << (n - st.m);
+    st.op_begin = (int)plan.ops.size();
+    int lpos[64];
+    for (int b = 0, c = 0; b < 64; ++b) lpos[b] = (b < n && (tile >> b & 1)) ? c++ : -1;
+    int ngrad = 0;
+    for (const Gate &g : ops) {
+        DevOp d;
+        std::memset(&d, 0, sizeof(d));
+        d.k = g.k;
+        d.mat = (int)plan.mats.size();
+        d.slot = g.grad_slot;
+        for (int b = 0; b < n; ++b)
+            if (g.cmask >> b & 1) {
+                if (lpos[b] >= 0) d.cloc |= 1u << lpos[b];
+                else d.cglob |= 1ull << b;
+            }
+        for (int j = 0; j < g.k; ++j) {
+            int b = g.bits[j];
+            d.pos[j] = lpos[b] >= 0 ? lpos[b] : -1 - b;
+        }
+        bool need_local = g.kind == G_DENSE || (g.kind == G_GRAD && !g.grad_diag);
+        if (need_local)
+            for (int j = 0; j < g.k; ++j)
+                if (d.pos[j] < 0) {
+                    err = "internal: non-diagonal target outside the tile";
+                    return false;
+                }
+        switch (g.kind) {
+            case G_SCALAR: d.kind = OPK_SCALAR; break;
+            case G_DIAG: d.kind = OPK_DIAG; break;
+            case G_DENSE:
+                if (g.k == 1) d.kind = OPK_DENSE1;
+                else if (g.k == 2) d.kind = OPK_DENSE2;
+                else if (o.use_mfma && g.k >= 4) { d.kind = OPK_MFMA; plan.mfma_blocks++; }
+                else d.kind = OPK_DENSEK;
+                break;
+            case G_GRAD:
+                if (g.grad_diag) d.kind = OPK_GRADDIAG;
+                else d.kind = OPK_GRAD;
+                if (d.kind == OPK_GRADDIAG)
+                    for (int j = 0; j < g.k; ++j)
+                        if (d.pos[j] < 0) {
+                            err = "internal: gradient probe outside the tile";
+                            return false;
+                        }
+                ++ngrad;
+                break;
+        }
+        plan.mats.insert(plan.mats.end(), g.m.begin(), g.m.end());
+        plan.ops.push_back(d);
+    }
+    st.op_end = (int)plan.ops.size();
+    st.n_grad = ngrad;
+    plan.stages.push_back(st);
+    return true;
+}
+
+}  // namespace
+
+bool build_plan(const std::vector<Gate> &gates, int n, const CompileOpts &o, Plan &plan,
+                std::string &err) {
+    plan = Plan();
+    plan.n = n;
+    for (const Gate &g : gates)
+        if (g.kind == G_GRAD) plan.n_grad_slots = std::max(plan.n_grad_slots, g.grad_slot + 1);
+    int K = std::max(1, std::min(5, o.fuse_bits));
+    int m = std::min(n, std::max(o.tile_bits, 1));
+    int L = std::min(std::max(o.low_bits, 1), m);
+    uint64_t all = low_mask(n);
+    if (o.per_gate && m < n && m < K + L) {
+        err = "tile_bits too small for the fusion width";
+        return false;
+    }
+    for (const Gate &g : gates)
+        if (popc(g.tile_mask() | low_mask(L)) > m) {
+            err = "gate wider than a tile";
+            return false;
+        }
+
+    if (o.per_gate) {
+        // One HBM pass per fused block.
+        std::vector<Gate> fused = fuse(gates, K, all);
+        for (const Gate &g : fused) {
+            uint64_t tile = fill_tile(low_mask(L) | g.tile_mask(), m, n);
+            if (!lower_stage({g}, tile, n, o, plan, err)) return false;
+        }
+        return true;
+    }
+
+    // Cache-blocked scheduling.
+    std::vector<int> remaining(gates.size());
+    for (size_t i = 0; i < gates.size(); ++i) remaining[i] = (int)i;
+    while (!remaining.empty()) {
+        uint64_t tile = low_mask(L);
+        uint64_t blocked = 0;
+        int cnt = popc(tile);
+        int ngrad = 0;
+        std::vector<Gate> take;
+        std::vector<int> defer;
+        defer.reserve(remaining.size());
+        for (size_t r = 0; r < remaining.size(); ++r) {
+            int gi = remaining[r];
+            const Gate &g = gates[gi];
+            if (blocked == all) {
+                defer.insert(defer.end(), remaining.begin() + r, remaining.end());
+                break;
+            }
+            uint64_t am = g.all_mask();
+            if (am & blocked) {
+                blocked |= am;
+                defer.push_back(gi);
+                continue;
+            }
+            uint64_t need = g.tile_mask() & ~tile;
+            bool grad_full = g.kind == G_GRAD && ngrad >= kMaxGradPerStage;
+            if (!grad_full && cnt + popc(need) <= m) {
+                tile |= need;
+                cnt += popc(need);
+                if (g.kind == G_GRAD) ++ngrad;
+                take.push_back(g);
+            } else {
+                blocked |= am;
+                defer.push_back(gi);
+            }
+        }
+        if (take.empty()) {
+            err = "internal: scheduler made no progress";
+            return false;
+        }
+        tile = fill_tile(tile, m, n);
+        std::vector<Gate> ops = K > 1 ? fuse(take, K, tile) : take;
+        if (!lower_stage(ops, tile, n, o, plan, err)) return false;
+        remaining.swap(defer);
+    }
+    return true;
+}
+
+}  // namespace desv

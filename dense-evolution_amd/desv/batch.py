@@ -1,0 +1,81 @@
+"""Heterogeneous VQE batches: many instances of uneven qubit counts, one call.
+
+Generalises ``run_batch_jit`` (reference ``statevector.py:496-584``, one
+circuit vmapped over parameter rows) to independent circuits, Hamiltonians
+and qubit counts.  Each instance is an ansatz circuit (QASMCircuit), its
+theta and a Pauli-sum Hamiltonian; :meth:`Batch.run` returns every energy
+and adjoint gradient.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import List, Tuple
+
+import numpy as np
+
+from . import _capi
+from ._capi import check, lib
+from .autodiff import _build_template
+from .observables import encode_pauli_terms
+
+
+class Batch:
+    def __init__(self, dtype="complex64", device: int = 0):
+        self.code = _capi.C64 if np.dtype(dtype) == np.complex64 else _capi.C128
+        p = C.c_void_p()
+        check(lib.desv_batch_create(self.code, device, C.byref(p)))
+        self.ptr = p
+        self.n_params: List[int] = []
+        self.slots: List[np.ndarray] = []
+
+    def close(self):
+        if getattr(self, "ptr", None) and self.ptr.value:
+            lib.desv_batch_destroy(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001
+            pass
+
+    def add(self, circuit, n_qubits: int, theta, terms) -> int:
+        tmpl = _build_template(circuit, n_qubits)
+        slots = np.nonzero(tmpl[:, 3] == -1.0)[0]
+        pmap = np.full(len(tmpl), -1, dtype=np.int64)
+        pmap[slots] = np.arange(len(slots))
+        ops = tmpl.copy()
+        th = np.asarray(theta, dtype=np.float64).reshape(-1)[:len(slots)]
+        ops[slots, 3] = th
+        xm, zm, cs = encode_pauli_terms(terms, n_qubits)
+        if np.any(np.imag(cs) != 0):
+            raise ValueError("batch Hamiltonians must have real coefficients")
+        coeff = np.ascontiguousarray(np.real(cs))
+        ops = np.ascontiguousarray(ops)
+        idx = C.c_int64()
+        check(lib.desv_batch_add(self.ptr, n_qubits, _capi.dptr(ops), len(ops), _capi.i64ptr(pmap), len(slots),
+                                 _capi.u64ptr(xm), _capi.u64ptr(zm), _capi.dptr(coeff), len(xm), C.byref(idx)))
+        self.n_params.append(len(slots))
+        return idx.value
+
+    def set_params(self, index: int, theta) -> None:
+        th = np.ascontiguousarray(np.asarray(theta, dtype=np.float64).reshape(-1)[:self.n_params[index]])
+        check(lib.desv_batch_set_params(self.ptr, index, _capi.dptr(th)))
+
+    def configure(self, tile_bits: int = 0, use_graph: bool = True) -> None:
+        check(lib.desv_batch_configure(self.ptr, int(tile_bits), int(use_graph)))
+
+    def run(self) -> Tuple[np.ndarray, List[np.ndarray]]:
+        e = np.zeros(len(self.n_params), dtype=np.float64)
+        g = np.zeros(max(1, sum(self.n_params)), dtype=np.float64)
+        check(lib.desv_batch_run(self.ptr, _capi.dptr(e), _capi.dptr(g)))
+        out, off = [], 0
+        for k in self.n_params:
+            out.append(g[off:off + k].copy())
+            off += k
+        return e, out
+
+    def last_time_us(self) -> float:
+        v = C.c_double()
+        check(lib.desv_batch_last_time_us(self.ptr, C.byref(v)))
+        return v.value

@@ -1,0 +1,191 @@
+/*
+ * desv.h -- C ABI of the MI355X-native dense statevector engine.
+ *
+ * Plain pointers and sizes only.  Every entry point returns an int status
+ * (DESV_OK = 0); on failure a thread-local message is available from
+ * desv_last_error().  Host buffers are owned by the caller; device memory is
+ * owned by the handle.  Calls on one handle are serialised by the caller and
+ * are asynchronous on the handle's HIP stream unless they return host data.
+ *
+ * Conventions (same as the reference engine): the state is a flat array of
+ * 2^n interleaved (re, im) complex numbers, complex64 or complex128; qubit q
+ * is bit (n-1-q) of the basis index (MSB-first).  An op table row is
+ * [g_id, q1, q2, param] as float64 with the reference's gate ids:
+ *   0 id, 1 h, 2 x, 3 y, 4 z, 5 s, 6 sdg, 7 t, 8 tdg, 9 rx, 10 ry, 11 rz,
+ *   12 p/u1/phase, 13 sx, 14 gphase, 20 cx, 21 cz, 22 cp, 23 swap, 24 cy, 25 crz
+ * (reference: dense_evolution/circuits/compiler.py:24-48, gates.py:53-63).
+ * Pauli strings are passed as bit masks over basis-index bits:
+ *   xmask = bits where the letter is X or Y, zmask = bits where it is Z or Y.
+ */
+#ifndef DESV_H
+#define DESV_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    DESV_OK = 0,
+    DESV_EINVAL = 1,      /* bad argument: maps to ValueError             */
+    DESV_ENOMEM = 2,      /* device or host allocation failed: MemoryError */
+    DESV_EDEVICE = 3,     /* HIP runtime error: RuntimeError               */
+    DESV_EUNSUPPORTED = 4 /* valid request this build does not implement   */
+};
+
+enum { DESV_C64 = 0, DESV_C128 = 1 };
+
+typedef struct desv_state *desv_handle;
+typedef struct desv_prog *desv_program;
+typedef struct desv_batch_s *desv_batch;
+
+/* ---- library ---------------------------------------------------------- */
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *desv_last_error(void);
+/* ABI version number of this library. */
+int desv_version(void);
+/* Number of visible HIP devices. */
+int desv_device_count(int *out);
+
+/* ---- state handles ------------------------------------------------------
+ * Replace DenseSVSimulator's state allocation and host transfers:
+ * __init__/_reset_sv (statevector.py:86-107), set_initial_state/set_state
+ * (:109-137), normalize (:141-147), get_statevector (:609-611). */
+
+/* Allocate a 2^n state (1 <= n <= 34) of dtype DESV_C64/DESV_C128 on
+ * `device`, initialised to |0...0>. */
+int desv_create(int n_qubits, int dtype, int device, desv_handle *out);
+int desv_destroy(desv_handle h);
+/* Use an existing HIP stream (e.g. torch's current stream) for all work on
+ * this handle; NULL restores the handle's own stream. */
+int desv_set_stream(desv_handle h, void *hip_stream);
+int desv_synchronize(desv_handle h);
+/* |0...0> */
+int desv_reset(desv_handle h);
+/* Copy 2^n host amplitudes in the handle's dtype (no normalisation). */
+int desv_set_state(desv_handle h, const void *host_amps);
+int desv_get_state(desv_handle h, void *host_amps);
+/* dst <- src (same n and dtype). */
+int desv_copy(desv_handle dst, desv_handle src);
+/* state <- (re + i im) * state */
+int desv_scale(desv_handle h, double re, double im);
+
+/* ---- reductions (FP64 accumulation, deterministic order) ----------------
+ * Replace get_probabilities (statevector.py:599-607), the marginal of
+ * measure (:318-334) and the vdot of statevector_fidelity
+ * (utils/measurement.py:71-94). */
+int desv_norm2(desv_handle h, double *out);
+int desv_inner(desv_handle a, desv_handle b, double *re, double *im);
+/* |a_i|^2 as float64 into a host buffer of 2^n doubles (raw, not clipped). */
+int desv_probabilities(desv_handle h, double *out_host);
+/* Marginal probabilities of one qubit (raw sums, not normalised). */
+int desv_qubit_probs(desv_handle h, int qubit, double *p0, double *p1);
+/* Zero the amplitudes whose `qubit` differs from `outcome`, scale the rest. */
+int desv_collapse(desv_handle h, int qubit, int outcome, double scale);
+
+/* ---- gate application ----------------------------------------------------
+ * desv_run_ops replaces _compile_and_run_circuit_jit(sv, ops) and its
+ * donated twin (compiler.py:318-355) as called by run_circuit_jit
+ * (statevector.py:414-471): the whole table is compiled (fusion + cache
+ * blocking) and applied in place.  Qubits are range-checked; an unknown gate
+ * id or a 2-qubit row with q1 == q2 is DESV_EINVAL. */
+int desv_run_ops(desv_handle h, const double *ops, int64_t n_ops);
+/* Arbitrary k-qubit unitary (1 <= k <= 5), row-major 2^k x 2^k complex128
+ * as interleaved doubles; qubits[0] is the most significant index bit.
+ * Replaces apply_gate_1q / apply_gate_2q (statevector.py:167-203). */
+int desv_apply_matrix(desv_handle h, const double *mat, const int *qubits, int k);
+
+/* ---- compiled programs ---------------------------------------------------
+ * A program is a gate list compiled once (host) and replayed many times.
+ * Compilation options: */
+typedef struct {
+    int fuse_bits;  /* max qubits of a fused block: 1 (none), 2..5          */
+    int tile_bits;  /* amplitudes per LDS tile = 2^tile_bits; 0 = default    */
+    int low_bits;   /* low index bits always in a tile (coalescing); 0 = dflt*/
+    int per_gate;   /* 1: one HBM pass per fused block (no cache blocking)   */
+    int use_mfma;   /* 1: 4/5-qubit blocks run as MFMA complex GEMMs         */
+    int dtype;      /* DESV_C64 / DESV_C128: matrix precision of the upload */
+} desv_compile_opts;
+
+/* Fill `opts` with the defaults for a dtype. */
+int desv_default_opts(int dtype, desv_compile_opts *opts);
+int desv_program_create(int n_qubits, desv_program *out);
+int desv_program_destroy(desv_program p);
+int desv_program_add_ops(desv_program p, const double *ops, int64_t n_ops);
+int desv_program_add_matrix(desv_program p, const double *mat, const int *qubits, int k);
+int desv_program_compile(desv_program p, const desv_compile_opts *opts);
+/* gates = logical gates added, passes = HBM sweeps per run, blocks = device
+ * ops after fusion, mfma_blocks = ops run on MFMA. */
+int desv_program_stats(desv_program p, int64_t *gates, int64_t *passes,
+                       int64_t *blocks, int64_t *mfma_blocks);
+int desv_run_program(desv_handle h, desv_program p);
+/* Introspection of a compiled plan (no device needed): sizes, then per stage
+ * the tile bit mask and [op_begin, op_end); per op 11 int64 fields
+ * (kind, k, mat, slot, pos[0..4], cloc, cglob) and the matrix pool as
+ * interleaved complex128.  Used by the host-compiler tests. */
+int desv_program_plan_sizes(desv_program p, int64_t *n_stages, int64_t *n_ops, int64_t *n_mats);
+int desv_program_plan_dump(desv_program p, uint64_t *stage_tile_mask, int64_t *stage_ops,
+                           int64_t *ops, double *mats);
+
+/* ---- Pauli sums ----------------------------------------------------------
+ * desv_pauli_expect replaces pauli_expectation / pauli_sum_expectation
+ * (physics/observables.py:97-186): out[t] = <psi|P_t|psi> (real), FP64
+ * accumulation, one state sweep per distinct xmask.
+ * desv_pauli_matvec replaces pauli_sum_matvec(_jax) and
+ * PauliSumOperator.__matmul__ (observables.py:250-308, :344-450):
+ * out <- sum_t (coeff_re[t] + i coeff_im[t]) P_t in. */
+int desv_pauli_expect(desv_handle h, const uint64_t *xmask, const uint64_t *zmask,
+                      int64_t n_terms, double *out);
+int desv_pauli_matvec(desv_handle out, desv_handle in, const uint64_t *xmask,
+                      const uint64_t *zmask, const double *coeff_re,
+                      const double *coeff_im, int64_t n_terms);
+/* out <- H in for a dense row-major complex128 H (2^n x 2^n, n <= 14); the
+ * `h_matrix @ sv` of energy_fn (solvers/autodiff.py:166). */
+int desv_dense_matvec(desv_handle out, desv_handle in, const double *h_matrix);
+
+/* ---- adjoint gradient ----------------------------------------------------
+ * Energy and the full gradient of E(theta) = Re<psi|H|psi>, psi = U(ops)|h>,
+ * in one forward + one reverse sweep.  Replaces jax.value_and_grad over
+ * energy_fn (solvers/autodiff.py:92-169, caller test_autodiff.py:56).
+ * `ops` are the patched rows (angles filled in); param_of_op[i] is the theta
+ * index the row's angle came from, or -1.  Supported theta-carrying ids:
+ * rx, ry, rz, p, cp, crz.  The handle keeps the final state psi.
+ * H is a Pauli sum (coeff real) or, if xmask == NULL, the dense complex128
+ * matrix passed in `dense_h`. */
+int desv_adjoint(desv_handle h, const double *ops, int64_t n_ops,
+                 const int64_t *param_of_op, int64_t n_params,
+                 const uint64_t *xmask, const uint64_t *zmask, const double *coeff,
+                 int64_t n_terms, const double *dense_h, double *energy, double *grad);
+
+/* ---- heterogeneous batches ------------------------------------------------
+ * Many independent (circuit, Hamiltonian, theta) instances of uneven qubit
+ * counts evaluated together; generalises run_batch_jit
+ * (statevector.py:496-584), which vmaps one circuit over parameter rows.
+ * Instances are added once; desv_batch_run evaluates energy and adjoint
+ * gradient of every instance, replaying a captured hipGraph after the first
+ * call. */
+int desv_batch_create(int dtype, int device, desv_batch *out);
+int desv_batch_destroy(desv_batch b);
+/* Returns the instance index in *index. */
+int desv_batch_add(desv_batch b, int n_qubits, const double *ops, int64_t n_ops,
+                   const int64_t *param_of_op, int64_t n_params, const uint64_t *xmask,
+                   const uint64_t *zmask, const double *coeff, int64_t n_terms,
+                   int64_t *index);
+/* Update the theta-carrying angles of instance i (n_params values). */
+int desv_batch_set_params(desv_batch b, int64_t index, const double *theta);
+/* tile_bits/threads: 0 = autotuned/default. use_graph: capture + replay. */
+int desv_batch_configure(desv_batch b, int tile_bits, int use_graph);
+/* Evaluate everything; energies[i], grads[offset_i ...] in instance order
+ * (grads packed by instance, n_params_i each). */
+int desv_batch_run(desv_batch b, double *energies, double *grads);
+/* Per-instance device time of the last run is not separable; this returns
+ * the wall time of the last desv_batch_run in microseconds. */
+int desv_batch_last_time_us(desv_batch b, double *us);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DESV_H */

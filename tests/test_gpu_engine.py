@@ -1,0 +1,272 @@
+"""GPU parity of the gate engine and observables against the CPU oracle.
+
+Bars: complex128 results within 1e-10 of the oracle; complex64 results at
+state fidelity >= 1 - 1e-6 against the complex128 oracle and expectations
+within 1e-6 (the precision policy: FP32 gate arithmetic, FP64 reductions).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import desv
+from desv import workloads
+from oracle import sv_oracle as O
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def rand_state(n, rng):
+    v = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    return v / np.linalg.norm(v)
+
+
+def random_ops(n, count, rng):
+    rows = []
+    for _ in range(count):
+        if n > 1 and rng.random() < 0.4:
+            g = int(rng.choice([20, 21, 22, 23, 24, 25]))
+            a, b = rng.choice(n, 2, replace=False)
+            rows.append([g, a, b, rng.uniform(-np.pi, np.pi)])
+        else:
+            rows.append([int(rng.integers(0, 15)), int(rng.integers(0, n)), 0, rng.uniform(-np.pi, np.pi)])
+    return np.array(rows, dtype=np.float64)
+
+
+def fid(a, b):
+    return O.fidelity(a, b)
+
+
+@pytest.mark.parametrize("use_float32", [False, True])
+def test_known_answers(use_float32):
+    with open(os.path.join(GOLD, "known_answers.json")) as f:
+        cases = json.load(f)
+    for case in cases:
+        sim = desv.DenseSVSimulator(case["n"], use_float32=use_float32)
+        sim.run_circuit([tuple(c) for c in case["circuit"]])
+        p = sim.get_probabilities()
+        for idx, val in case.get("probs", {}).items():
+            assert abs(p[int(idx)] - val) < 1e-6, case["name"]
+        for term, val in case.get("expect", {}).items():
+            assert abs(desv.pauli_expectation(sim.sv, term) - val) < 1e-6, case["name"]
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 8, 12, 15])
+def test_random_op_tables_c128(n, rng):
+    ops = random_ops(n, 200, rng)
+    sv0 = rand_state(n, rng)
+    sim = desv.DenseSVSimulator(n)
+    sim.sv = sv0
+    sim.handle.run_ops(ops)
+    want = O.run_ops_fast(sv0.astype(np.complex128), ops)
+    assert np.allclose(sim.get_statevector(), want, atol=1e-10)
+
+
+@pytest.mark.parametrize("n", [3, 9, 14])
+def test_random_op_tables_c64(n, rng):
+    ops = random_ops(n, 200, rng)
+    sv0 = rand_state(n, rng)
+    sim = desv.DenseSVSimulator(n, use_float32=True)
+    sim.sv = sv0.astype(np.complex64)
+    sim.handle.run_ops(ops)
+    want = O.run_ops_fast(sv0, ops)
+    assert fid(sim.get_statevector(), want) >= 1 - 1e-6
+
+
+@pytest.mark.parametrize("opts", [
+    dict(fuse_bits=1, tile_bits=6, low_bits=2),
+    dict(fuse_bits=2, tile_bits=7, low_bits=3),
+    dict(fuse_bits=3, tile_bits=8, low_bits=4),
+    dict(fuse_bits=5, tile_bits=9, low_bits=2),
+    dict(fuse_bits=2, tile_bits=8, low_bits=5, per_gate=True),
+])
+@pytest.mark.parametrize("dtype", ["complex128", "complex64"])
+def test_programs_multi_stage(opts, dtype, rng):
+    n = 13
+    ops = random_ops(n, 300, rng)
+    sv0 = rand_state(n, rng)
+    prog = desv.Program(n).add_ops(ops).compile(dtype, **opts)
+    assert prog.stats()["passes"] > 1
+    h = desv.StateHandle(n, dtype == "complex128")
+    h.upload(sv0)
+    prog.run(h)
+    want = O.run_ops_fast(sv0, ops)
+    got = h.download()
+    if dtype == "complex128":
+        assert np.allclose(got, want, atol=1e-10)
+    else:
+        assert fid(got, want) >= 1 - 1e-6
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("use_float32", [False, True])
+def test_apply_arbitrary_unitaries(k, use_float32, rng):
+    n = 11
+    sv = rand_state(n, rng)
+    sim = desv.DenseSVSimulator(n, use_float32=use_float32)
+    sim.sv = sv.astype(sim.dtype)
+    want = sv.copy()
+    for _ in range(6):
+        q = [int(x) for x in rng.choice(n, k, replace=False)]
+        z = rng.normal(size=(1 << k, 1 << k)) + 1j * rng.normal(size=(1 << k, 1 << k))
+        u, _ = np.linalg.qr(z)
+        if k == 1:
+            sim.apply_gate_1q(u, q[0])
+        elif k == 2:
+            sim.apply_gate_2q(u, q[0], q[1])
+        else:
+            sim.apply_gate(u, q)
+        want = O.apply_matrix(want, u, q)
+    got = sim.get_statevector()
+    if use_float32:
+        assert fid(got, want) >= 1 - 1e-6
+    else:
+        assert np.allclose(got, want, atol=1e-10)
+
+
+@pytest.mark.parametrize("fuse,mfma", [(2, False), (4, False), (5, False)])
+@pytest.mark.parametrize("dtype", ["complex64", "complex128"])
+def test_brickwork_su4_small(fuse, mfma, dtype):
+    n, depth = 12, 8
+    gates = workloads.brickwork_su4(n, depth, seed=5)
+    prog = desv.Program(n)
+    want = O.zero_state(n)
+    for u, a, b in gates:
+        prog.add_matrix(u, [a, b])
+        want = O.apply_matrix(want, u, [a, b])
+    prog.compile(dtype, fuse_bits=fuse, use_mfma=mfma)
+    h = desv.StateHandle(n, dtype == "complex128")
+    prog.run(h)
+    got = h.download()
+    if dtype == "complex128":
+        assert np.allclose(got, want, atol=1e-10)
+    else:
+        assert fid(got, want) >= 1 - 1e-6
+
+
+def test_config_plain_20q_c64_parity():
+    """20-qubit random circuit, 20 layers of rx/ry/rz + cx ladder, complex64,
+    vs the complex128 oracle: fidelity and every <Z_i>."""
+    n = 20
+    circ = workloads.random_layers_circuit(n, 20, seed=0)
+    sim = desv.DenseSVSimulator(n, use_float32=True)
+    sim.run_circuit(circ)
+    want = O.run_ops_fast(O.zero_state(n), O.encode_ops(circ, n))
+    got = sim.get_statevector()
+    assert got.dtype == np.complex64
+    assert fid(got, want) >= 1 - 1e-6
+    zs = [desv.pauli_expectation(sim.sv, {q: 'Z'}) for q in range(n)]
+    want_z = [O.pauli_expectation(want, {q: 'Z'}) for q in range(n)]
+    assert np.max(np.abs(np.array(zs) - want_z)) < 1e-6
+
+
+@pytest.mark.parametrize("use_float32", [False, True])
+def test_pauli_expectations_and_matvec(use_float32, rng):
+    n = 10
+    sv = rand_state(n, rng)
+    terms = workloads.random_pauli_hamiltonian(rng, n, 60)
+    terms += [(0.3, {}), (1.2, {0: 'Y'}), (-0.4, {n - 1: 'X', 0: 'Y'})]
+    dt = np.complex64 if use_float32 else np.complex128
+    sim = desv.DenseSVSimulator(n, use_float32=use_float32)
+    sim.sv = sv.astype(dt)
+    ref_sv = sv.astype(dt).astype(np.complex128)
+    tol = 1e-6 if use_float32 else 1e-11
+    for c, t in terms[:20]:
+        assert abs(desv.pauli_expectation(sim.sv, t) - O.pauli_expectation(ref_sv, t)) < tol
+    e = desv.pauli_sum_expectation(sim.sv, terms)
+    assert abs(e - O.pauli_sum_expectation(ref_sv, terms)) < 10 * tol
+    mv = desv.pauli_sum_matvec(sim.sv, terms)
+    assert np.allclose(mv, O.pauli_sum_matvec(ref_sv, terms), atol=10 * tol)
+    op = desv.PauliSumOperator(terms, n)
+    assert np.allclose(op @ ref_sv, O.pauli_sum_matvec(ref_sv, terms), atol=1e-10)
+    # host array input (uploaded) gives the same value
+    assert abs(desv.pauli_expectation(ref_sv, 'XY' + 'I' * (n - 2)) -
+               O.pauli_expectation(ref_sv, 'XY' + 'I' * (n - 2))) < 1e-11
+
+
+def test_observable_validation():
+    sim = desv.DenseSVSimulator(3)
+    with pytest.raises(ValueError):
+        desv.pauli_expectation(sim.sv, 'XZ', n_qubits=3)
+    with pytest.raises(ValueError):
+        desv.pauli_expectation(sim.sv, {5: 'X'})
+    with pytest.raises(ValueError):
+        desv.pauli_expectation(sim.sv, {0: 'Q'})
+
+
+def test_readout_measure_and_normalisation(rng):
+    n = 6
+    sim = desv.DenseSVSimulator(n)
+    sv = rand_state(n, rng)
+    sim.set_initial_state(3.0 * sv)
+    assert np.allclose(sim.get_statevector(), sv, atol=1e-12)
+    assert np.allclose(sim.get_probabilities(), O.probabilities(sv), atol=1e-12)
+    r = sim.measure(2, rng=np.random.default_rng(0))
+    after = sim.get_statevector()
+    assert abs(np.linalg.norm(after) - 1) < 1e-12
+    bit = n - 1 - 2
+    idx = np.arange(1 << n)
+    assert np.all(np.abs(after[((idx >> bit) & 1) != r]) == 0)
+    with pytest.raises(ValueError):
+        sim.set_initial_state(np.zeros(1 << n))
+    with pytest.raises(ValueError):
+        sim.set_initial_state(np.ones(3))
+    assert sim.memory_mb() == (1 << n) * 16 / 1e6
+
+
+def test_errors_match_reference_messages():
+    sim = desv.DenseSVSimulator(3)
+    with pytest.raises(ValueError, match="unknown gate"):
+        sim.run_circuit([('foo', 0)])
+    with pytest.raises(ValueError, match="out of range"):
+        sim.run_circuit_jit([('h', 5)])
+    with pytest.raises(ValueError, match="must differ"):
+        sim.apply_gate_2q(np.eye(4), 1, 1)
+    with pytest.raises(ValueError, match="parameter_batch"):
+        sim.run_batch_jit([('rx', 0, 0.1), ('h', 1)], np.zeros((2, 3)))
+
+
+def test_eager_and_table_paths_agree(rng):
+    n = 5
+    circ = [('h', 0), ('u3', 1, 0.3, -0.7, 1.1), ('iswap', 0, 2), ('ecr', 3, 4), ('ccx', 0, 1, 4),
+            ('u2', 2, 0.5, 0.25), ('swap', 1, 3), ('crz', 2, 0, 0.8), ('cp', 4, 1, -0.4), ('cy', 0, 3)]
+    a = desv.DenseSVSimulator(n)
+    a.run_circuit(circ)
+    want = O.run_circuit(circ, n)
+    assert np.allclose(a.get_statevector(), want, atol=1e-10)
+    b = desv.DenseSVSimulator(n)
+    b.run_circuit([('iswap', 0, 2)], transpile=False)  # eager matrix path
+    assert np.allclose(b.get_statevector(), O.apply_matrix(O.zero_state(n), O.GATES['iswap'], [0, 2]))
+
+
+def test_run_batch_jit(rng):
+    n = 4
+    circ = [('ry', q, 0.0) for q in range(n)] + [('cx', 0, 1), ('rz', 2, 0.0), ('cp', 1, 3, 0.0)]
+    P = rng.uniform(-np.pi, np.pi, (5, n + 2))
+    sim = desv.DenseSVSimulator(n)
+    out = sim.run_batch_jit(circ, P)
+    for b in range(5):
+        c = [('ry', q, P[b, q]) for q in range(n)] + [('cx', 0, 1), ('rz', 2, P[b, n]), ('cp', 1, 3, P[b, n + 1])]
+        assert np.allclose(out[b], O.run_circuit(c, n), atol=1e-10)
+
+
+def test_qasm_to_state_integration():
+    qasm = ('OPENQASM 2.0; include "qelib1.inc"; qreg q[6]; h q[0]; cx q[0],q[1]; rx(pi/3) q[2]; '
+            'ry(0.4) q[3]; cz q[2],q[3]; crz(0.7) q[1],q[4]; ccx q[0],q[4],q[5]; u3(0.1,0.2,0.3) q[5]; '
+            'swap q[2],q[5];')
+    circ = desv.QASMParser().parse(qasm)
+    sim = desv.DenseSVSimulator(circ.n_qubits)
+    sim.run_circuit(circ)
+    assert np.allclose(sim.get_statevector(), O.run_circuit(list(circ), circ.n_qubits), atol=1e-10)
+
+
+def test_fidelity_and_sampling():
+    sim = desv.DenseSVSimulator(3)
+    sim.run_circuit([('h', 0), ('cx', 0, 1), ('cx', 1, 2)])
+    ghz = np.zeros(8)
+    ghz[[0, 7]] = 1 / np.sqrt(2)
+    assert abs(desv.statevector_fidelity(sim.sv, ghz) - 1) < 1e-12
+    counts = desv.sample_counts(sim.sv, 2000, rng=np.random.default_rng(1))
+    assert set(counts) == {'000', '111'} and sum(counts.values()) == 2000
