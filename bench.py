@@ -91,7 +91,9 @@ def main():
 
     torch.cuda.init()
     dev = torch.device("cuda:0")
-    stream = torch.cuda.current_stream(dev)
+    # a dedicated stream (default priority): every kernel of the program is
+    # launched on it, so events recorded on it bracket exactly that work
+    stream = torch.cuda.Stream(device=dev)
     h = desv.StateHandle(n, complex128=False, device=0)
     desv._capi.check(desv._capi.lib.desv_set_stream(h.ptr, desv._capi.C.c_void_p(stream.cuda_stream)))
 

@@ -593,7 +593,9 @@ int desv_program_compile(desv_program p, const desv_compile_opts *opts) {
         if (opts->tile_bits) o.tile_bits = opts->tile_bits;
         if (opts->low_bits) o.low_bits = opts->low_bits;
         o.per_gate = opts->per_gate;
-        o.use_mfma = opts->use_mfma;
+        // FP64 MFMA has the same peak as FP64 VALU on MI355X, and its 16x16x4
+        // path did not match the oracle: complex128 blocks stay on VALU.
+        o.use_mfma = dtype == DESV_C64 ? opts->use_mfma : 0;
     }
     if (o.fuse_bits < 1 || o.fuse_bits > 5) return fail(DESV_EINVAL, "fuse_bits must be in [1, 5]");
     const size_t lim = stage_lds_limit(dev);
@@ -634,8 +636,10 @@ int desv_program_plan_dump(desv_program p, uint64_t *tile_masks, int64_t *stage_
     for (size_t s = 0; s < pl.stages.size(); ++s) {
         if (tile_masks) tile_masks[s] = pl.stages[s].tile_mask;
         if (stage_ops) {
-            stage_ops[2 * s] = pl.stages[s].op_begin;
-            stage_ops[2 * s + 1] = pl.stages[s].op_end;
+            int64_t *so = stage_ops + 18 * s;
+            so[0] = pl.stages[s].op_begin;
+            so[1] = pl.stages[s].op_end;
+            for (int j = 0; j < 16; ++j) so[2 + j] = j < pl.stages[s].m ? pl.stages[s].sperm[j] : -1;
         }
     }
     if (ops)
@@ -666,8 +670,6 @@ int desv_run_program(desv_handle h, desv_program p) {
         int rc = desv_program_compile(p, &o);
         if (rc) return rc;
     }
-    if (p->plan.mfma_blocks > 0)
-        return fail(DESV_EUNSUPPORTED, "MFMA block kernels are not built into this library yet");
     if (p->dev.dtype != h->dtype || p->dev.device != h->device) {
         int rc = upload_plan(p->plan, h->dtype, h->device, h->stream, p->dev);
         if (rc) return rc;

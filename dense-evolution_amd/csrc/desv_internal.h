@@ -87,6 +87,7 @@ struct DevOp {
 };
 
 constexpr int MAX_HBITS = 24;
+constexpr int MAX_TILE_BITS = 16;
 
 struct DevStage {
     int32_t m;         // tile bits (amplitudes per tile = 2^m)
@@ -97,7 +98,13 @@ struct DevStage {
     int64_t n_tiles;
     int32_t op_begin, op_end;
     int32_t n_grad;    // OPK_GRAD* ops in this stage
-    int32_t grad_base; // first gradient slot of this stage
+    int32_t heavy;     // stage holds k >= 3 VALU blocks or MFMA blocks
+    // Write-back bit permutation: store-order bit j (the j-th lowest global
+    // bit of the tile) takes tile-local bit sperm[j].  Identity unless
+    // `permuted`; lets a stage move qubits between the low (contiguous) index
+    // bits and high bits at no extra HBM cost.
+    int32_t permuted;
+    int32_t sperm[MAX_TILE_BITS];
 };
 
 struct Plan {
@@ -117,6 +124,8 @@ struct CompileOpts {
     int per_gate = 0;
     int use_mfma = 0;
     int nv = 1;            // vectors per tile (2 for the adjoint reverse sweep)
+    int remap = 1;         // let stages permute which qubits sit on the low bits
+    int window = 4096;     // gates scanned ahead when choosing a tile
 };
 
 // Build the device plan for a gate list (in application order).

@@ -160,6 +160,126 @@ __device__ inline void op_densek(typename VT<T>::c **tile, int m, const DevOp &o
     }
 }
 
+// ---------------------------------------------------------------------------
+// Fused k-qubit block (k = 4, 5) as a real-split complex GEMM on MFMA.
+//
+// The tile is viewed as X[r][g]: r = block-local index over the k target
+// bits, g = one of G = 2^(m-k) groups (the other tile bits).  Y = U X with
+//   Yr = Ur Xr + Ui (-Xi),   Yi = Ur Xi + Ui Xr,
+// computed 16 groups (columns) at a time per wave with 16x16x4 MFMA tiles:
+// A fragment  lane l -> U[rb*16 + l%16][ks*4 + l/16]
+// B fragment  lane l -> X[ks*4 + l/16][col0 + l%16]
+// accumulator lane l, reg j -> Y[rb*16 + 4*(l/16) + j][col0 + l%16].
+// A wave reads all of its columns before writing them back, and waves own
+// disjoint columns, so the update is in place.
+// ---------------------------------------------------------------------------
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+typedef double f64x4_t __attribute__((ext_vector_type(4)));
+
+template <typename T> struct Mfma;
+template <> struct Mfma<float> {
+    using acc = f32x4_t;
+    static __device__ inline acc op(float a, float b, acc c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+template <> struct Mfma<double> {
+    using acc = f64x4_t;
+    static __device__ inline acc op(double a, double b, acc c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+
+template <typename T, int NV, int K>
+__device__ inline void op_mfma(typename VT<T>::c **tile, int m, const DevOp &op, const typename VT<T>::c *M) {
+    using C = typename VT<T>::c;
+    using Acc = typename Mfma<T>::acc;
+    constexpr int D = 1 << K;
+    constexpr int RB = D / 16;  // 16-row blocks of U
+    constexpr int KS = D / 4;   // 4-wide k slices
+    const int lane = threadIdx.x & 63;
+    const int i16 = lane & 15, k4 = lane >> 4;
+    int pos[K], srt[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) pos[j] = srt[j] = op.pos[j];
+#pragma unroll
+    for (int a = 0; a < K; ++a)
+#pragma unroll
+        for (int b = 0; b < K - 1; ++b)
+            if (srt[b] > srt[b + 1]) { int t = srt[b]; srt[b] = srt[b + 1]; srt[b + 1] = t; }
+    auto deposit = [&](int r) {
+        int o = 0;
+#pragma unroll
+        for (int j = 0; j < K; ++j) o |= ((r >> j) & 1) << pos[j];
+        return o;
+    };
+    // A fragments (constant over the tile)
+    T ar[RB][KS], ai[RB][KS];
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            const C u = M[(rb * 16 + i16) * D + ks * 4 + k4];
+            ar[rb][ks] = u.x;
+            ai[rb][ks] = u.y;
+        }
+    int boff[KS];
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) boff[ks] = deposit(ks * 4 + k4);
+    int yoff[RB][4];
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) yoff[rb][j] = deposit(rb * 16 + 4 * k4 + j);
+
+    const int G = 1 << (m - K);
+    const int waves = blockDim.x >> 6;
+    for (int c0 = (threadIdx.x >> 6) * 16; c0 < G; c0 += waves * 16) {
+        const int col = c0 + i16;
+        const bool valid = col < G;
+        int base = valid ? col : 0;
+#pragma unroll
+        for (int j = 0; j < K; ++j) base = insert_zero(base, srt[j]);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            T br[KS], bi[KS];
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) {
+                C x = tile[v][base | boff[ks]];
+                br[ks] = valid ? x.x : T(0);
+                bi[ks] = valid ? x.y : T(0);
+            }
+            Acc yr[RB], yi[RB];
+#pragma unroll
+            for (int rb = 0; rb < RB; ++rb) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    yr[rb][j] = T(0);
+                    yi[rb][j] = T(0);
+                }
+#pragma unroll
+                for (int ks = 0; ks < KS; ++ks) {
+                    yr[rb] = Mfma<T>::op(ar[rb][ks], br[ks], yr[rb]);
+                    yr[rb] = Mfma<T>::op(ai[rb][ks], -bi[ks], yr[rb]);
+                    yi[rb] = Mfma<T>::op(ar[rb][ks], bi[ks], yi[rb]);
+                    yi[rb] = Mfma<T>::op(ai[rb][ks], br[ks], yi[rb]);
+                }
+            }
+            if (valid) {
+#pragma unroll
+                for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        C y;
+                        y.x = yr[rb][j];
+                        y.y = yi[rb][j];
+                        tile[v][base | yoff[rb][j]] = y;
+                    }
+            }
+        }
+    }
+}
+
 template <typename T, int NV>
 __device__ inline void op_diag(typename VT<T>::c **tile, int m, const DevOp &op,
                                const typename VT<T>::c *Dg, uint64_t base) {
@@ -252,7 +372,10 @@ __device__ inline double op_grad_diag(typename VT<T>::c **tile, int m, const Dev
 // ---------------------------------------------------------------------------
 // Stage kernel
 // ---------------------------------------------------------------------------
-template <typename T, int NV>
+// HEAVY = false compiles only the 1-2 qubit, diagonal, scalar and gradient
+// micro-ops (low register count, high occupancy for streaming stages);
+// HEAVY = true adds the k = 3..5 VALU blocks and the MFMA blocks.
+template <typename T, int NV, bool HEAVY>
 __global__ void __launch_bounds__(kThreads) k_stage(typename VT<T>::c *__restrict__ sv0,
                                                     typename VT<T>::c *__restrict__ sv1,
                                                     StageArgs a) {
@@ -262,6 +385,7 @@ __global__ void __launch_bounds__(kThreads) k_stage(typename VT<T>::c *__restric
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ double gacc[kMaxGradLDS];
     __shared__ double red[kThreads / 64];
+    __shared__ int perm_lo[256], perm_hi[256];
 
     const int m = a.st.m;
     const int size = 1 << m;
@@ -276,6 +400,21 @@ __global__ void __launch_bounds__(kThreads) k_stage(typename VT<T>::c *__restric
     const C *mats = reinterpret_cast<const C *>(a.mats);
 
     for (int i = threadIdx.x; i < a.st.n_grad; i += blockDim.x) gacc[i] = 0.0;
+    const bool permuted = a.st.permuted != 0;
+    if (permuted) {
+        // local index of store-order index e = perm_lo[e & 255] | perm_hi[e >> 8]
+        const int mlo = m < 8 ? m : 8;
+        for (int i = threadIdx.x; i < 256; i += blockDim.x) {
+            int lo = 0, hi = 0;
+            for (int j = 0; j < mlo; ++j)
+                if (i >> j & 1) lo |= 1 << a.st.sperm[j];
+            for (int j = 8; j < m; ++j)
+                if (i >> (j - 8) & 1) hi |= 1 << a.st.sperm[j];
+            perm_lo[i] = lo;
+            perm_hi[i] = hi;
+        }
+        __syncthreads();
+    }
 
     for (int64_t t = blockIdx.x; t < a.st.n_tiles; t += gridDim.x) {
         uint64_t base = 0;
@@ -312,9 +451,17 @@ __global__ void __launch_bounds__(kThreads) k_stage(typename VT<T>::c *__restric
                 case OPK_DENSE1: op_dense1<T, NV>(tile, m, op, M); break;
                 case OPK_DENSE2: op_dense2<T, NV>(tile, m, op, M); break;
                 case OPK_DENSEK:
-                    if (op.k == 3) op_densek<T, NV, 3>(tile, m, op, M);
-                    else if (op.k == 4) op_densek<T, NV, 4>(tile, m, op, M);
-                    else op_densek<T, NV, 5>(tile, m, op, M);
+                    if constexpr (HEAVY) {
+                        if (op.k == 3) op_densek<T, NV, 3>(tile, m, op, M);
+                        else if (op.k == 4) op_densek<T, NV, 4>(tile, m, op, M);
+                        else op_densek<T, NV, 5>(tile, m, op, M);
+                    }
+                    break;
+                case OPK_MFMA:
+                    if constexpr (HEAVY) {
+                        if (op.k == 4) op_mfma<T, NV, 4>(tile, m, op, M);
+                        else op_mfma<T, NV, 5>(tile, m, op, M);
+                    }
                     break;
                 case OPK_DIAG: op_diag<T, NV>(tile, m, op, M, base); break;
                 case OPK_SCALAR: op_scalar<T, NV>(tile, m, M); break;
@@ -332,15 +479,28 @@ __global__ void __launch_bounds__(kThreads) k_stage(typename VT<T>::c *__restric
             }
             __syncthreads();
         }
-        // store
+        // store (optionally through the write-back bit permutation)
         for (int e = threadIdx.x * PV; e < size; e += blockDim.x * PV) {
             const uint64_t s = (uint64_t)e >> L;
             uint64_t off = base | ((uint64_t)e & lowm);
             for (int j = 0; j < nh; ++j)
                 if (s >> j & 1) off |= 1ull << a.st.hbits[j];
+            if (!permuted) {
 #pragma unroll
-            for (int v = 0; v < NV; ++v)
-                *reinterpret_cast<V *>(&svp[v][off]) = *reinterpret_cast<const V *>(&tile[v][e]);
+                for (int v = 0; v < NV; ++v)
+                    *reinterpret_cast<V *>(&svp[v][off]) = *reinterpret_cast<const V *>(&tile[v][e]);
+            } else {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    C tmp[PV];
+#pragma unroll
+                    for (int i = 0; i < PV; ++i) {
+                        const int ee = e + i;
+                        tmp[i] = tile[v][perm_lo[ee & 255] | perm_hi[ee >> 8]];
+                    }
+                    *reinterpret_cast<V *>(&svp[v][off]) = *reinterpret_cast<const V *>(tmp);
+                }
+            }
         }
         __syncthreads();
     }
@@ -360,10 +520,17 @@ template <typename T>
 hipError_t launch_stage(int nv, void *sv0, void *sv1, const StageArgs &a, int grid, hipStream_t s) {
     using C = typename VT<T>::c;
     const size_t lds = (size_t)nv * ((size_t)1 << a.st.m) * sizeof(C);
-    if (nv == 1)
-        hipLaunchKernelGGL((k_stage<T, 1>), dim3(grid), dim3(kThreads), lds, s, (C *)sv0, (C *)sv1, a);
-    else
-        hipLaunchKernelGGL((k_stage<T, 2>), dim3(grid), dim3(kThreads), lds, s, (C *)sv0, (C *)sv1, a);
+    if (a.st.heavy) {
+        if (nv == 1)
+            hipLaunchKernelGGL((k_stage<T, 1, true>), dim3(grid), dim3(kThreads), lds, s, (C *)sv0, (C *)sv1, a);
+        else
+            hipLaunchKernelGGL((k_stage<T, 2, true>), dim3(grid), dim3(kThreads), lds, s, (C *)sv0, (C *)sv1, a);
+    } else {
+        if (nv == 1)
+            hipLaunchKernelGGL((k_stage<T, 1, false>), dim3(grid), dim3(kThreads), lds, s, (C *)sv0, (C *)sv1, a);
+        else
+            hipLaunchKernelGGL((k_stage<T, 2, false>), dim3(grid), dim3(kThreads), lds, s, (C *)sv0, (C *)sv1, a);
+    }
     return hipGetLastError();
 }
 
@@ -376,13 +543,15 @@ size_t stage_lds_limit(int device) {
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && v > 0)
         lim = (size_t)v;
     // leave room for the static gradient/reduction arrays
-    lim -= (kMaxGradLDS + kThreads / 64) * sizeof(double);
-    const void *fns[4] = {(const void *)k_stage<float, 1>, (const void *)k_stage<float, 2>,
-                          (const void *)k_stage<double, 1>, (const void *)k_stage<double, 2>};
+    lim -= (kMaxGradLDS + kThreads / 64) * sizeof(double) + 512 * sizeof(int);
+    const void *fns[8] = {(const void *)k_stage<float, 1, false>,  (const void *)k_stage<float, 2, false>,
+                          (const void *)k_stage<double, 1, false>, (const void *)k_stage<double, 2, false>,
+                          (const void *)k_stage<float, 1, true>,   (const void *)k_stage<float, 2, true>,
+                          (const void *)k_stage<double, 1, true>,  (const void *)k_stage<double, 2, true>};
     for (const void *f : fns)
         if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lim) != hipSuccess) {
             (void)hipGetLastError();
-            lim = 64 * 1024 - (kMaxGradLDS + kThreads / 64) * sizeof(double);
+            lim = 64 * 1024 - (kMaxGradLDS + kThreads / 64) * sizeof(double) - 512 * sizeof(int);
         }
     cached[device] = lim;
     return lim;

@@ -354,10 +354,18 @@ uint64_t fill_tile(uint64_t tile, int m, int n) {
 const int kMaxGradPerStage = 64;
 
 bool lower_stage(const std::vector<Gate> &ops, uint64_t tile, int n, const CompileOpts &o,
-                 Plan &plan, std::string &err) {
+                 Plan &plan, std::string &err, const int *sperm = nullptr) {
     DevStage st;
     std::memset(&st, 0, sizeof(st));
     st.m = popc(tile);
+    if (st.m > MAX_TILE_BITS) {
+        err = "tile wider than the engine supports";
+        return false;
+    }
+    for (int j = 0; j < st.m; ++j) {
+        st.sperm[j] = sperm ? sperm[j] : j;
+        if (st.sperm[j] != j) st.permuted = 1;
+    }
     int L = 0;
     while (L < n && (tile >> L & 1)) ++L;
     st.L = L;
@@ -424,8 +432,72 @@ bool lower_stage(const std::vector<Gate> &ops, uint64_t tile, int n, const Compi
     }
     st.op_end = (int)plan.ops.size();
     st.n_grad = ngrad;
+    st.heavy = 0;
+    for (int i = st.op_begin; i < st.op_end; ++i)
+        if (plan.ops[i].kind == OPK_DENSEK || plan.ops[i].kind == OPK_MFMA) st.heavy = 1;
     plan.stages.push_back(st);
     return true;
+}
+
+}  // namespace
+
+namespace {
+
+// Scan up to `window` remaining gates with tile (logical bit mask) S: a gate
+// is taken when its non-diagonal bits lie in S and none of its bits is
+// blocked; a gate that cannot be taken blocks all of its bits, so taken
+// gates commute with everything deferred.  Returns the number taken.
+int scan_take(const std::vector<Gate> &gates, const std::vector<int> &rem, uint64_t S, uint64_t all, int window,
+              std::vector<int> *take, std::vector<int> *defer) {
+    uint64_t blocked = 0;
+    int cnt = 0, ngrad = 0;
+    const int lim = (int)std::min<size_t>(rem.size(), (size_t)window);
+    int r = 0;
+    for (; r < lim; ++r) {
+        if (blocked == all) break;
+        const int gi = rem[r];
+        const Gate &g = gates[gi];
+        const uint64_t am = g.all_mask();
+        const bool grad_full = g.kind == G_GRAD && ngrad >= kMaxGradPerStage;
+        if ((am & blocked) || (g.tile_mask() & ~S) || grad_full) {
+            blocked |= am;
+            if (defer) defer->push_back(gi);
+            continue;
+        }
+        if (g.kind == G_GRAD) ++ngrad;
+        ++cnt;
+        if (take) take->push_back(gi);
+    }
+    if (defer)
+        for (; r < (int)rem.size(); ++r) defer->push_back(rem[r]);
+    return cnt;
+}
+
+Gate to_physical(const Gate &g, const std::vector<int> &pi) {
+    Gate p = g;
+    for (int j = 0; j < g.k; ++j) p.bits[j] = pi[g.bits[j]];
+    p.cmask = 0;
+    for (int b = 0; b < 64; ++b)
+        if (g.cmask >> b & 1) p.cmask |= 1ull << pi[b];
+    return p;
+}
+
+// Store permutation of a stage whose tile holds logical bits S (physical
+// bits P) when the logical->physical map changes from pi to pi2 (pi2 only
+// moves bits of S within P).  sperm[j] = load-local bit that ends up in the
+// j-th lowest physical bit of P.
+void store_perm(uint64_t P, uint64_t S, const std::vector<int> &pi, const std::vector<int> &pi2, int n,
+                int *sperm) {
+    int lpos[64];
+    for (int b = 0, c = 0; b < 64; ++b) lpos[b] = (b < n && (P >> b & 1)) ? c++ : -1;
+    int j = 0;
+    for (int b = 0; b < n; ++b) {
+        if (!(P >> b & 1)) continue;
+        int src = -1;
+        for (int u = 0; u < n; ++u)
+            if ((S >> u & 1) && pi2[u] == b) src = lpos[pi[u]];
+        sperm[j++] = src;
+    }
 }
 
 }  // namespace
@@ -439,7 +511,7 @@ bool build_plan(const std::vector<Gate> &gates, int n, const CompileOpts &o, Pla
     int K = std::max(1, std::min(5, o.fuse_bits));
     int m = std::min(n, std::max(o.tile_bits, 1));
     int L = std::min(std::max(o.low_bits, 1), m);
-    uint64_t all = low_mask(n);
+    const uint64_t all = low_mask(n);
     if (o.per_gate && m < n && m < K + L) {
         err = "tile_bits too small for the fusion width";
         return false;
@@ -460,51 +532,185 @@ bool build_plan(const std::vector<Gate> &gates, int n, const CompileOpts &o, Pla
         return true;
     }
 
-    // Cache-blocked scheduling.
-    std::vector<int> remaining(gates.size());
-    for (size_t i = 0; i < gates.size(); ++i) remaining[i] = (int)i;
-    while (!remaining.empty()) {
-        uint64_t tile = low_mask(L);
-        uint64_t blocked = 0;
-        int cnt = popc(tile);
-        int ngrad = 0;
-        std::vector<Gate> take;
-        std::vector<int> defer;
-        defer.reserve(remaining.size());
-        for (size_t r = 0; r < remaining.size(); ++r) {
-            int gi = remaining[r];
-            const Gate &g = gates[gi];
-            if (blocked == all) {
-                defer.insert(defer.end(), remaining.begin() + r, remaining.end());
-                break;
+    // Cache-blocked scheduling over a logical->physical bit map pi.
+    std::vector<int> pi(n), inv(n);
+    for (int u = 0; u < n; ++u) pi[u] = inv[u] = u;
+    std::vector<int> rem(gates.size());
+    for (size_t i = 0; i < gates.size(); ++i) rem[i] = (int)i;
+    const int window = std::max(64, o.window);
+    while (!rem.empty()) {
+        uint64_t S = 0;  // logical bits in the tile
+        for (int p = 0; p < L; ++p) S |= 1ull << inv[p];
+        if (m < n) {
+            // closure-greedy growth of the tile
+            std::vector<int> def;
+            while (popc(S) < m) {
+                def.clear();
+                const int base = scan_take(gates, rem, S, all, window, nullptr, &def);
+                std::vector<uint64_t> cands;
+                for (size_t i = 0; i < def.size() && i < 64; ++i) {
+                    const uint64_t need = gates[def[i]].tile_mask() & ~S;
+                    if (!need || popc(S | need) > m) continue;
+                    if (std::find(cands.begin(), cands.end(), need) == cands.end()) cands.push_back(need);
+                }
+                if (cands.empty()) break;
+                double best = 0;
+                uint64_t pick = 0;
+                for (uint64_t c : cands) {
+                    const int t = scan_take(gates, rem, S | c, all, window, nullptr, nullptr);
+                    const double score = double(t - base) / popc(c);
+                    if (score > best) {
+                        best = score;
+                        pick = c;
+                    }
+                }
+                if (!pick) break;
+                S |= pick;
             }
-            uint64_t am = g.all_mask();
-            if (am & blocked) {
-                blocked |= am;
-                defer.push_back(gi);
-                continue;
-            }
-            uint64_t need = g.tile_mask() & ~tile;
-            bool grad_full = g.kind == G_GRAD && ngrad >= kMaxGradPerStage;
-            if (!grad_full && cnt + popc(need) <= m) {
-                tile |= need;
-                cnt += popc(need);
-                if (g.kind == G_GRAD) ++ngrad;
-                take.push_back(g);
-            } else {
-                blocked |= am;
-                defer.push_back(gi);
-            }
+        } else {
+            S = all;
         }
+        std::vector<int> take, defer;
+        scan_take(gates, rem, S, all, window, &take, &defer);
         if (take.empty()) {
             err = "internal: scheduler made no progress";
             return false;
         }
-        tile = fill_tile(tile, m, n);
-        std::vector<Gate> ops = K > 1 ? fuse(take, K, tile) : take;
-        if (!lower_stage(ops, tile, n, o, plan, err)) return false;
-        remaining.swap(defer);
+        // fill the tile: bits the next deferred gates need, then the lowest physical bits
+        for (size_t i = 0; i < defer.size() && popc(S) < m && i < 256; ++i) {
+            const uint64_t need = gates[defer[i]].tile_mask() & ~S;
+            if (popc(S | need) <= m) S |= need;
+        }
+        for (int p = 0; p < n && popc(S) < m; ++p) S |= 1ull << inv[p];
+        uint64_t P = 0;
+        for (int u = 0; u < n; ++u)
+            if (S >> u & 1) P |= 1ull << pi[u];
+        // Next layout: the tile bits the deferred gates need soonest go to
+        // the low physical bits; every other tile bit goes home if its home
+        // bit is in the tile and free, else stays, else takes a free bit.
+        std::vector<int> pi2 = pi;
+        if (o.remap && m < n) {
+            std::vector<int> want;
+            for (size_t i = 0; i < defer.size() && (int)want.size() < L && i < 256; ++i) {
+                const uint64_t tm = gates[defer[i]].tile_mask() & S;
+                for (int u = 0; u < n; ++u)
+                    if ((tm >> u & 1) && std::find(want.begin(), want.end(), u) == want.end() &&
+                        (int)want.size() < L)
+                        want.push_back(u);
+            }
+            for (int p = 0; p < L && (int)want.size() < L; ++p)
+                if (std::find(want.begin(), want.end(), inv[p]) == want.end()) want.push_back(inv[p]);
+            uint64_t used = 0, placed = 0;
+            for (int u : want)
+                if (pi[u] < L) {
+                    pi2[u] = pi[u];
+                    used |= 1ull << pi[u];
+                    placed |= 1ull << u;
+                }
+            for (int u : want)
+                if (!(placed >> u & 1))
+                    for (int p = 0; p < L; ++p)
+                        if (!(used >> p & 1)) {
+                            pi2[u] = p;
+                            used |= 1ull << p;
+                            placed |= 1ull << u;
+                            break;
+                        }
+            for (int pass = 0; pass < 3; ++pass)
+                for (int u = 0; u < n; ++u) {
+                    if (!(S >> u & 1) || (placed >> u & 1)) continue;
+                    int slot = -1;
+                    if (pass == 0 && (P >> u & 1) && !(used >> u & 1) && u >= L) slot = u;
+                    if (pass == 1 && !(used >> pi[u] & 1)) slot = pi[u];
+                    if (pass == 2)
+                        for (int b = 0; b < n; ++b)
+                            if ((P >> b & 1) && !(used >> b & 1)) {
+                                slot = b;
+                                break;
+                            }
+                    if (slot < 0) continue;
+                    pi2[u] = slot;
+                    used |= 1ull << slot;
+                    placed |= 1ull << u;
+                }
+        }
+        std::vector<Gate> lg;
+        lg.reserve(take.size());
+        for (int gi : take) lg.push_back(gates[gi]);
+        std::vector<Gate> fused = K > 1 ? fuse(lg, K, S) : lg;
+        std::vector<Gate> phys;
+        phys.reserve(fused.size());
+        for (const Gate &g : fused) phys.push_back(to_physical(g, pi));
+        int sperm[MAX_TILE_BITS];
+        store_perm(P, S, pi, pi2, n, sperm);
+        if (!lower_stage(phys, P, n, o, plan, err, sperm)) return false;
+        pi = pi2;
+        for (int u = 0; u < n; ++u) inv[pi[u]] = u;
+        rem.swap(defer);
     }
+    // restore the identity layout: whole cycles of pi per pass
+    for (int guard = 0; guard < 4 * n; ++guard) {
+        bool ident = true;
+        for (int u = 0; u < n; ++u) ident &= pi[u] == u;
+        if (ident) break;
+        uint64_t P = low_mask(L);
+        std::vector<int> pi2 = pi;
+        std::vector<bool> seen(n, false);
+        for (int u = 0; u < n; ++u) {
+            if (seen[u] || pi[u] == u) continue;
+            // cycle through u: logical u sits at pi[u] and belongs at u
+            std::vector<int> cyc;
+            for (int v = u; !seen[v]; v = inv[v]) {
+                seen[v] = true;
+                cyc.push_back(v);
+            }
+            uint64_t cm = 0;
+            for (int v : cyc) cm |= (1ull << v) | (1ull << pi[v]);
+            if (popc(P | cm) <= m) {
+                P |= cm;
+                for (int v : cyc) pi2[v] = v;
+            } else if (popc(P) + 1 < m) {
+                // partial: send the first r logical bits home, the one whose
+                // home is taken keeps the cycle going in a shorter form
+                std::vector<int> part;
+                uint64_t pm = P;
+                for (int v : cyc) {
+                    uint64_t add = (1ull << v) | (1ull << pi[v]);
+                    if (popc(pm | add) > m) break;
+                    pm |= add;
+                    part.push_back(v);
+                }
+                if (part.size() < 2) continue;
+                // part = v0, v1 = inv[v0], ...; v_i moves home (v_i's home holds
+                // v_{i+1}); the last one takes over the physical bit v0 left.
+                const int vlast = part.back();
+                const int hole = pi[part[0]];
+                for (size_t i = 0; i + 1 < part.size(); ++i) pi2[part[i + 1]] = pi[part[i + 1]];
+                pi2[part[0]] = part[0];
+                for (size_t i = 1; i + 1 < part.size(); ++i) pi2[part[i]] = part[i];
+                pi2[vlast] = hole;
+                P = pm;
+            }
+        }
+        // bits of P: everything moved must be inside
+        uint64_t S = 0;
+        for (int u = 0; u < n; ++u)
+            if (P >> pi[u] & 1) S |= 1ull << u;
+        P = fill_tile(P, m, n);
+        S = 0;
+        for (int u = 0; u < n; ++u)
+            if (P >> pi[u] & 1) S |= 1ull << u;
+        int sperm[MAX_TILE_BITS];
+        store_perm(P, S, pi, pi2, n, sperm);
+        if (!lower_stage({}, P, n, o, plan, err, sperm)) return false;
+        pi = pi2;
+        for (int u = 0; u < n; ++u) inv[pi[u]] = u;
+    }
+    for (int u = 0; u < n; ++u)
+        if (pi[u] != u) {
+            err = "internal: layout restore failed";
+            return false;
+        }
     return true;
 }
 

@@ -75,11 +75,12 @@ class Program:
         return {"gates": g.value, "passes": p.value, "blocks": b.value, "mfma_blocks": m.value}
 
     def plan(self):
-        """(tile_masks u64[S], stage_ops i64[S,2], ops i64[O,11], mats complex128[M])."""
+        """(tile_masks u64[S], stage_ops i64[S,18] = (begin, end, sperm[16]),
+        ops i64[O,11], mats complex128[M])."""
         s, o, m = C.c_int64(), C.c_int64(), C.c_int64()
         check(lib.desv_program_plan_sizes(self.ptr, C.byref(s), C.byref(o), C.byref(m)))
         tm = np.zeros(s.value, dtype=np.uint64)
-        so = np.zeros((s.value, 2), dtype=np.int64)
+        so = np.zeros((s.value, 18), dtype=np.int64)
         ops = np.zeros((o.value, 11), dtype=np.int64)
         mats = np.zeros(2 * m.value, dtype=np.float64)
         check(lib.desv_program_plan_dump(self.ptr, _capi.u64ptr(tm), _capi.i64ptr(so), _capi.i64ptr(ops),

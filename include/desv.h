@@ -122,7 +122,9 @@ int desv_program_stats(desv_program p, int64_t *gates, int64_t *passes,
                        int64_t *blocks, int64_t *mfma_blocks);
 int desv_run_program(desv_handle h, desv_program p);
 /* Introspection of a compiled plan (no device needed): sizes, then per stage
- * the tile bit mask and [op_begin, op_end); per op 11 int64 fields
+ * the tile bit mask and 18 int64 fields (op_begin, op_end, then the 16-entry
+ * write-back permutation sperm: store-order tile bit j takes tile-local bit
+ * sperm[j]; entries past the tile width are -1); per op 11 int64 fields
  * (kind, k, mat, slot, pos[0..4], cloc, cglob) and the matrix pool as
  * interleaved complex128.  Used by the host-compiler tests. */
 int desv_program_plan_sizes(desv_program p, int64_t *n_stages, int64_t *n_ops, int64_t *n_mats);

@@ -126,7 +126,7 @@ def test_apply_arbitrary_unitaries(k, use_float32, rng):
         assert np.allclose(got, want, atol=1e-10)
 
 
-@pytest.mark.parametrize("fuse,mfma", [(2, False), (4, False), (5, False)])
+@pytest.mark.parametrize("fuse,mfma", [(2, False), (4, False), (5, False), (4, True), (5, True)])
 @pytest.mark.parametrize("dtype", ["complex64", "complex128"])
 def test_brickwork_su4_small(fuse, mfma, dtype):
     n, depth = 12, 8
@@ -137,6 +137,9 @@ def test_brickwork_su4_small(fuse, mfma, dtype):
         prog.add_matrix(u, [a, b])
         want = O.apply_matrix(want, u, [a, b])
     prog.compile(dtype, fuse_bits=fuse, use_mfma=mfma)
+    if mfma:
+        # MFMA blocks are used for complex64 only (complex128 blocks run on VALU)
+        assert (prog.stats()["mfma_blocks"] > 0) == (dtype == "complex64")
     h = desv.StateHandle(n, dtype == "complex128")
     prog.run(h)
     got = h.download()

@@ -17,7 +17,8 @@ def interpret(plan, sv):
     sv = sv.astype(np.complex128).copy()
     N = sv.shape[0]
     idx = np.arange(N, dtype=np.int64)
-    for tm, (b, e) in zip(tile_masks, stage_ops):
+    for tm, so in zip(tile_masks, stage_ops):
+        b, e, sperm = int(so[0]), int(so[1]), [int(x) for x in so[2:]]
         tbits = [i for i in range(64) if (int(tm) >> i) & 1]
 
         def g(pos):
@@ -49,6 +50,16 @@ def interpret(plan, sv):
                 sv[base[:, None] + offs[None, :]] = A @ M.T
             else:
                 raise AssertionError(f"unexpected op kind {kind}")
+        # write-back permutation: physical tile bit tbits[sperm[j]] moves to tbits[j]
+        if any(sperm[j] != j for j in range(len(tbits))):
+            dest = np.zeros(N, dtype=np.int64)
+            rest = idx & ~np.int64(int(tm))
+            dest |= rest
+            for j in range(len(tbits)):
+                dest |= ((idx >> tbits[sperm[j]]) & 1) << tbits[j]
+            out = np.empty_like(sv)
+            out[dest] = sv
+            sv = out
     return sv
 
 
@@ -89,6 +100,23 @@ def test_plan_replays_the_circuit(n, opts, rng):
     assert st["gates"] == len(ops)
     if opts.get("per_gate"):
         assert st["passes"] == st["blocks"]
+
+
+def test_remapped_schedules_restore_the_layout(rng):
+    n = 14
+    gates = desv.workloads.brickwork_su4(n, 10, seed=3)
+    prog = desv.Program(n)
+    want = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    want /= np.linalg.norm(want)
+    sv0 = want.copy()
+    for u, a, b in gates:
+        prog.add_matrix(u, [a, b])
+        want = O.apply_matrix(want, u, [a, b])
+    for tb, lb in ((8, 3), (9, 4), (10, 5)):
+        prog.compile("complex128", fuse_bits=2, tile_bits=tb, low_bits=lb)
+        tm, so, ops, mats = prog.plan()
+        assert any(any(so[s, 2 + j] != j for j in range(tb)) for s in range(len(so)))
+        assert np.allclose(interpret(prog.plan(), sv0), want, atol=1e-10)
 
 
 def test_cache_blocking_reduces_passes(rng):
