@@ -107,6 +107,8 @@ CompileOpts default_opts(int dtype, int device, int nv) {
 
 }  // namespace
 
+int desv::set_last_error(int code, const std::string &msg) { return fail(code, msg); }
+
 struct desv_state {
     int n = 0;
     int dtype = 0;
@@ -194,22 +196,54 @@ int sum_to_host(desv_handle h, int nvals, int nblk, double *out) {
     return DESV_OK;
 }
 
+// Launch every stage of a plan on one state (one tile per workgroup).
+// Gradient probes write partials[slot * pstride + tile], pstride = 2^(n - m).
+// A stage that is exactly one dense 1-/2-qubit or diagonal (k <= 2) op with
+// no write-back permutation runs on the streaming direct kernel.
+bool direct_op(const Plan &plan, const DevStage &st, DirectOp &d) {
+    if (st.op_end - st.op_begin != 1 || st.permuted) return false;
+    const DevOp &op = plan.ops[st.op_begin];
+    if (op.kind != OPK_DENSE1 && op.kind != OPK_DENSE2 && op.kind != OPK_DIAG) return false;
+    if (op.k < 1 || op.k > 2) return false;
+    int gb[64], c = 0;
+    for (int b = 0; b < plan.n; ++b)
+        if (st.tile_mask >> b & 1) gb[c++] = b;
+    std::memset(&d, 0, sizeof(d));
+    d.k = op.k;
+    d.diag = op.kind == OPK_DIAG;
+    for (int j = 0; j < op.k; ++j) d.pos[j] = op.pos[j] >= 0 ? gb[op.pos[j]] : -1 - op.pos[j];
+    d.cmask = op.cglob;
+    for (int j = 0; j < 32; ++j)
+        if (op.cloc >> j & 1) d.cmask |= 1ull << gb[j];
+    return true;
+}
+
 int run_plan_on(desv_handle h, const Plan &plan, const DevPlan &dp, int nv, void *sv0, void *sv1,
-                int grad_wg) {
+                int64_t pstride) {
     for (const DevStage &st : plan.stages) {
-        StageArgs a;
-        std::memset(&a, 0, sizeof(a));
-        a.st = st;
-        a.n = plan.n;
-        a.ops = dp.ops;
-        a.mats = dp.mats;
-        a.partials = h->d_part;
-        int grid;
-        if (st.n_grad > 0) grid = grad_wg;
-        else grid = (int)std::min<int64_t>(st.n_tiles, 1 << 16);
-        a.n_wg = grid;
-        hipError_t e = h->dtype == DESV_C64 ? launch_stage<float>(nv, sv0, sv1, a, grid, h->stream)
-                                            : launch_stage<double>(nv, sv0, sv1, a, grid, h->stream);
+        DirectOp dop;
+        if (nv == 1 && direct_op(plan, st, dop)) {
+            const size_t eb = h->dtype == DESV_C64 ? 8 : 16;
+            const void *M = reinterpret_cast<const char *>(dp.mats) + (size_t)plan.ops[st.op_begin].mat * eb;
+            hipError_t e = h->dtype == DESV_C64 ? launch_direct<float>(sv0, plan.n, dop, M, h->stream)
+                                                : launch_direct<double>(sv0, plan.n, dop, M, h->stream);
+            if (e != hipSuccess) return fail(DESV_EDEVICE, std::string("direct launch: ") + hipGetErrorString(e));
+            continue;
+        }
+        StageTask t;
+        std::memset(&t, 0, sizeof(t));
+        t.st = st;
+        t.n = plan.n;
+        t.sv0 = sv0;
+        t.sv1 = sv1;
+        t.ops = dp.ops;
+        t.mats = dp.mats;
+        t.partials = h->d_part;
+        t.pstride = pstride;
+        t.wg_begin = 0;
+        hipError_t e = h->dtype == DESV_C64
+                           ? launch_stage<float>(nv, st.heavy != 0, nullptr, nullptr, t, st.n_tiles, st.m, h->stream)
+                           : launch_stage<double>(nv, st.heavy != 0, nullptr, nullptr, t, st.n_tiles, st.m, h->stream);
         if (e != hipSuccess) return fail(DESV_EDEVICE, std::string("stage launch: ") + hipGetErrorString(e));
     }
     return DESV_OK;
@@ -233,7 +267,7 @@ int apply_gates(desv_handle h, const std::vector<Gate> &gates) {
     if (!build_plan(gates, h->n, o, plan, err)) return fail(DESV_EINVAL, err);
     int rc = upload_plan(plan, h->dtype, h->device, h->stream, h->scratch_plan);
     if (rc) return rc;
-    return run_plan_on(h, plan, h->scratch_plan, 1, h->sv, nullptr, 1);
+    return run_plan_on(h, plan, h->scratch_plan, 1, h->sv, nullptr, 0);
 }
 
 // Pauli-sum matvec into `out` (device buffer) from `in`.
@@ -674,7 +708,7 @@ int desv_run_program(desv_handle h, desv_program p) {
         int rc = upload_plan(p->plan, h->dtype, h->device, h->stream, p->dev);
         if (rc) return rc;
     }
-    return run_plan_on(h, p->plan, p->dev, 1, h->sv, nullptr, 1);
+    return run_plan_on(h, p->plan, p->dev, 1, h->sv, nullptr, 0);
 }
 
 int desv_pauli_expect(desv_handle h, const uint64_t *xmask, const uint64_t *zmask, int64_t T, double *out) {
@@ -819,15 +853,15 @@ int desv_adjoint(desv_handle h, const double *ops, int64_t n_ops, const int64_t 
     Plan plan;
     std::string err;
     if (!build_plan(rev, h->n, o, plan, err)) return fail(DESV_EINVAL, err);
-    const int grad_wg = (int)std::max<int64_t>(1, std::min<int64_t>(1ll << std::max(0, h->n - o.tile_bits), 1024));
-    rc = ensure_part(h, (size_t)plan.n_grad_slots * grad_wg);
+    const int64_t pstride = 1ll << (h->n - std::min(h->n, o.tile_bits));
+    rc = ensure_part(h, (size_t)plan.n_grad_slots * pstride);
     if (rc) return rc;
     rc = upload_plan(plan, h->dtype, h->device, h->stream, h->scratch_plan);
     if (rc) return rc;
-    rc = run_plan_on(h, plan, h->scratch_plan, 2, h->aux[0], h->aux[1], grad_wg);
+    rc = run_plan_on(h, plan, h->scratch_plan, 2, h->aux[0], h->aux[1], pstride);
     if (rc) return rc;
     std::vector<double> slots(plan.n_grad_slots);
-    rc = sum_to_host(h, plan.n_grad_slots, grad_wg, slots.data());
+    rc = sum_to_host(h, plan.n_grad_slots, (int)pstride, slots.data());
     if (rc) return rc;
     for (size_t s = 0; s < slot_param.size(); ++s) grad[slot_param[s]] += slots[s];
     return DESV_OK;

@@ -107,8 +107,18 @@ struct DevStage {
     int32_t sperm[MAX_TILE_BITS];
 };
 
+// How a device op's matrix is formed from the input gate list: a lone gate
+// (gates.size() == 1, bits == 0) or the product of `gates` (in order) over the
+// logical bit set `bits`.  Lets a compiled plan refresh its matrices for new
+// angles without rescheduling.
+struct Recipe {
+    std::vector<int> gates;
+    uint64_t bits = 0;
+};
+
 struct Plan {
     int n = 0;
+    std::vector<Recipe> recipes;  // one per op (empty for restore-only stages)
     int64_t n_gates = 0;
     int64_t mfma_blocks = 0;
     int n_grad_slots = 0;
@@ -131,5 +141,11 @@ struct CompileOpts {
 // Build the device plan for a gate list (in application order).
 bool build_plan(const std::vector<Gate> &gates, int n, const CompileOpts &o, Plan &plan,
                 std::string &err);
+// Record the thread-local message returned by desv_last_error(); returns code.
+int set_last_error(int code, const std::string &msg);
+
+// Recompute plan.mats from a gate list with the same structure (same kinds,
+// bits and controls; only matrix entries differ) as the one compiled.
+void refresh_plan_mats(Plan &plan, const std::vector<Gate> &gates);
 
 }  // namespace desv

@@ -9,6 +9,8 @@
 // stage on the tile in LDS, and writes it back: one HBM round trip per stage.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "kernels.h"
 
 namespace desv {
@@ -284,35 +286,31 @@ template <typename T, int NV>
 __device__ inline void op_diag(typename VT<T>::c **tile, int m, const DevOp &op,
                                const typename VT<T>::c *Dg, uint64_t base) {
     using C = typename VT<T>::c;
+    // Out-of-tile bits are constant over the tile; in-tile bit j of the
+    // diagonal index comes from tile bit p[j] (p[j] = 31 marks "not local", a
+    // bit that is always 0 for tile indices).  Loops are fully unrolled over
+    // the 5 possible bits so nothing is indexed at run time (no scratch).
     int gsub = 0;
-    int lp[5];
-    int nl = 0;
-    int lj[5];
-    for (int j = 0; j < op.k; ++j) {
-        if (op.pos[j] < 0) {
-            const int b = -1 - op.pos[j];
-            if (base >> b & 1) gsub |= 1 << j;
-        } else {
-            lp[nl] = op.pos[j];
-            lj[nl] = j;
-            ++nl;
+    int p[5];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+        p[j] = 31;
+        if (j < op.k) {
+            const int pj = op.pos[j];
+            if (pj < 0) {
+                if (base >> (-1 - pj) & 1) gsub |= 1 << j;
+            } else {
+                p[j] = pj;
+            }
         }
     }
     const int size = 1 << m;
     const int cl = (int)op.cloc;
-    if (nl == 0) {
-        const C f = Dg[gsub];
-        for (int i = threadIdx.x; i < size; i += blockDim.x) {
-            if ((i & cl) != cl) continue;
-#pragma unroll
-            for (int v = 0; v < NV; ++v) tile[v][i] = cmul(f, tile[v][i]);
-        }
-        return;
-    }
     for (int i = threadIdx.x; i < size; i += blockDim.x) {
         if ((i & cl) != cl) continue;
         int d = gsub;
-        for (int j = 0; j < nl; ++j) d |= ((i >> lp[j]) & 1) << lj[j];
+#pragma unroll
+        for (int j = 0; j < 5; ++j) d |= ((i >> p[j]) & 1) << j;
         const C f = Dg[d];
 #pragma unroll
         for (int v = 0; v < NV; ++v) tile[v][i] = cmul(f, tile[v][i]);
@@ -376,161 +374,405 @@ __device__ inline double op_grad_diag(typename VT<T>::c **tile, int m, const Dev
 // micro-ops (low register count, high occupancy for streaming stages);
 // HEAVY = true adds the k = 3..5 VALU blocks and the MFMA blocks.
 template <typename T, int NV, bool HEAVY>
-__global__ void __launch_bounds__(kThreads) k_stage(typename VT<T>::c *__restrict__ sv0,
-                                                    typename VT<T>::c *__restrict__ sv1,
-                                                    StageArgs a) {
+__global__ void __launch_bounds__(kThreads) k_stage(const StageTask *__restrict__ tasks,
+                                                    const int32_t *__restrict__ wg_task, StageTask single) {
     using C = typename VT<T>::c;
     using V = typename VT<T>::v;
     constexpr int PV = VT<T>::per_vec;
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ double gacc[kMaxGradLDS];
-    __shared__ double red[kThreads / 64];
+    __shared__ double gacc[kMaxGradLDS * (kThreads / 64)];
     __shared__ int perm_lo[256], perm_hi[256];
+    __shared__ uint64_t hoff[256];
+    __shared__ StageTask tk;
 
-    const int m = a.st.m;
+    {
+        // copy the task into LDS in 4-byte words, one word per thread
+        constexpr int W = (int)(sizeof(StageTask) / 4);
+        const uint32_t *src = tasks ? reinterpret_cast<const uint32_t *>(tasks + wg_task[blockIdx.x])
+                                    : reinterpret_cast<const uint32_t *>(&single);
+        for (int i = threadIdx.x; i < W; i += blockDim.x) reinterpret_cast<uint32_t *>(&tk)[i] = src[i];
+    }
+    __syncthreads();
+    const int64_t t = (int64_t)blockIdx.x - tk.wg_begin;
+    if (t < 0 || t >= tk.st.n_tiles) return;  // uniform over the workgroup
+
+    const int m = tk.st.m;
     const int size = 1 << m;
     C *tile[2];
     tile[0] = reinterpret_cast<C *>(smem);
     tile[1] = tile[0] + size;
-    C *svp[2] = {sv0, sv1};
-    const int L = a.st.L, nh = a.st.nh;
+    C *svp[2] = {reinterpret_cast<C *>(tk.sv0), reinterpret_cast<C *>(tk.sv1)};
+    const int L = tk.st.L, nh = tk.st.nh;
     const uint64_t lowm = (1ull << L) - 1;
-    const uint64_t all = (a.n >= 64) ? ~0ull : ((1ull << a.n) - 1);
-    const uint64_t free_mask = ~a.st.tile_mask & all;
-    const C *mats = reinterpret_cast<const C *>(a.mats);
+    const uint64_t all = (tk.n >= 64) ? ~0ull : ((1ull << tk.n) - 1);
+    const uint64_t free_mask = ~tk.st.tile_mask & all;
+    const C *mats = reinterpret_cast<const C *>(tk.mats);
+    const DevOp *ops = tk.ops;
 
-    for (int i = threadIdx.x; i < a.st.n_grad; i += blockDim.x) gacc[i] = 0.0;
-    const bool permuted = a.st.permuted != 0;
+    for (int i = threadIdx.x; i < tk.st.n_grad * (kThreads / 64); i += blockDim.x) gacc[i] = 0.0;
+    for (int i = threadIdx.x; i < (1 << tk.st.nh) && tk.st.nh <= 8; i += blockDim.x) {
+        uint64_t o = 0;
+        for (int j = 0; j < tk.st.nh; ++j)
+            if (i >> j & 1) o |= 1ull << tk.st.hbits[j];
+        hoff[i] = o;
+    }
+    const bool permuted = tk.st.permuted != 0;
     if (permuted) {
         // local index of store-order index e = perm_lo[e & 255] | perm_hi[e >> 8]
         const int mlo = m < 8 ? m : 8;
         for (int i = threadIdx.x; i < 256; i += blockDim.x) {
             int lo = 0, hi = 0;
             for (int j = 0; j < mlo; ++j)
-                if (i >> j & 1) lo |= 1 << a.st.sperm[j];
+                if (i >> j & 1) lo |= 1 << tk.st.sperm[j];
             for (int j = 8; j < m; ++j)
-                if (i >> (j - 8) & 1) hi |= 1 << a.st.sperm[j];
+                if (i >> (j - 8) & 1) hi |= 1 << tk.st.sperm[j];
             perm_lo[i] = lo;
             perm_hi[i] = hi;
         }
-        __syncthreads();
     }
 
-    for (int64_t t = blockIdx.x; t < a.st.n_tiles; t += gridDim.x) {
-        uint64_t base = 0;
-        {
-            uint64_t tt = (uint64_t)t, fm = free_mask;
-            while (fm) {
-                const uint64_t lb = fm & (~fm + 1);
-                if (tt & 1) base |= lb;
-                tt >>= 1;
-                fm ^= lb;
-            }
+    uint64_t base = 0;
+    {
+        uint64_t tt = (uint64_t)t, fm = free_mask;
+        while (fm) {
+            const uint64_t lb = fm & (~fm + 1);
+            if (tt & 1) base |= lb;
+            tt >>= 1;
+            fm ^= lb;
         }
-        // load
-        for (int e = threadIdx.x * PV; e < size; e += blockDim.x * PV) {
-            const uint64_t s = (uint64_t)e >> L;
-            uint64_t off = base | ((uint64_t)e & lowm);
-            for (int j = 0; j < nh; ++j)
-                if (s >> j & 1) off |= 1ull << a.st.hbits[j];
+    }
+    __syncthreads();  // hoff / perm tables ready
+    const bool small_nh = nh <= 8;
+    auto seg_off = [&](uint64_t s) -> uint64_t {
+        if (small_nh) return hoff[s];
+        uint64_t o = 0;
+        for (int j = 0; j < nh; ++j)
+            if (s >> j & 1) o |= 1ull << tk.st.hbits[j];
+        return o;
+    };
+    // load
+    for (int e = threadIdx.x * PV; e < size; e += blockDim.x * PV) {
+        const uint64_t s = (uint64_t)e >> L;
+        const uint64_t off = base | ((uint64_t)e & lowm) | seg_off(s);
 #pragma unroll
-            for (int v = 0; v < NV; ++v)
-                *reinterpret_cast<V *>(&tile[v][e]) = *reinterpret_cast<const V *>(&svp[v][off]);
+        for (int v = 0; v < NV; ++v)
+            *reinterpret_cast<V *>(&tile[v][e]) = *reinterpret_cast<const V *>(&svp[v][off]);
+    }
+    __syncthreads();
+    int gi = 0;
+    for (int o = tk.st.op_begin; o < tk.st.op_end; ++o) {
+        const DevOp op = ops[o];
+        const bool is_grad = op.kind == OPK_GRAD || op.kind == OPK_GRADDIAG;
+        if ((base & op.cglob) != op.cglob) {
+            gi += is_grad;
+            continue;
         }
-        __syncthreads();
-        int gi = 0;
-        for (int o = a.st.op_begin; o < a.st.op_end; ++o) {
-            const DevOp op = a.ops[o];
-            const bool is_grad = op.kind == OPK_GRAD || op.kind == OPK_GRADDIAG;
-            if ((base & op.cglob) != op.cglob) {
-                gi += is_grad;
-                continue;
-            }
-            const C *M = mats + op.mat;
-            switch (op.kind) {
-                case OPK_DENSE1: op_dense1<T, NV>(tile, m, op, M); break;
-                case OPK_DENSE2: op_dense2<T, NV>(tile, m, op, M); break;
-                case OPK_DENSEK:
-                    if constexpr (HEAVY) {
-                        if (op.k == 3) op_densek<T, NV, 3>(tile, m, op, M);
-                        else if (op.k == 4) op_densek<T, NV, 4>(tile, m, op, M);
-                        else op_densek<T, NV, 5>(tile, m, op, M);
-                    }
-                    break;
-                case OPK_MFMA:
-                    if constexpr (HEAVY) {
-                        if (op.k == 4) op_mfma<T, NV, 4>(tile, m, op, M);
-                        else op_mfma<T, NV, 5>(tile, m, op, M);
-                    }
-                    break;
-                case OPK_DIAG: op_diag<T, NV>(tile, m, op, M, base); break;
-                case OPK_SCALAR: op_scalar<T, NV>(tile, m, M); break;
-                case OPK_GRAD:
-                case OPK_GRADDIAG:
-                    if constexpr (NV == 2) {
-                        const double part = op.kind == OPK_GRAD ? op_grad_dense<T>(tile, m, op, M)
-                                                                : op_grad_diag<T>(tile, m, op, M);
-                        const double s = block_sum(part, red);
-                        if (threadIdx.x == 0) gacc[gi] += s;
-                    }
-                    ++gi;
-                    break;
-                default: break;
-            }
-            __syncthreads();
-        }
-        // store (optionally through the write-back bit permutation)
-        for (int e = threadIdx.x * PV; e < size; e += blockDim.x * PV) {
-            const uint64_t s = (uint64_t)e >> L;
-            uint64_t off = base | ((uint64_t)e & lowm);
-            for (int j = 0; j < nh; ++j)
-                if (s >> j & 1) off |= 1ull << a.st.hbits[j];
-            if (!permuted) {
-#pragma unroll
-                for (int v = 0; v < NV; ++v)
-                    *reinterpret_cast<V *>(&svp[v][off]) = *reinterpret_cast<const V *>(&tile[v][e]);
-            } else {
-#pragma unroll
-                for (int v = 0; v < NV; ++v) {
-                    C tmp[PV];
-#pragma unroll
-                    for (int i = 0; i < PV; ++i) {
-                        const int ee = e + i;
-                        tmp[i] = tile[v][perm_lo[ee & 255] | perm_hi[ee >> 8]];
-                    }
-                    *reinterpret_cast<V *>(&svp[v][off]) = *reinterpret_cast<const V *>(tmp);
+        const C *M = mats + op.mat;
+        switch (op.kind) {
+            case OPK_DENSE1: op_dense1<T, NV>(tile, m, op, M); break;
+            case OPK_DENSE2: op_dense2<T, NV>(tile, m, op, M); break;
+            case OPK_DENSEK:
+                if constexpr (HEAVY) {
+                    if (op.k == 3) op_densek<T, NV, 3>(tile, m, op, M);
+                    else if (op.k == 4) op_densek<T, NV, 4>(tile, m, op, M);
+                    else op_densek<T, NV, 5>(tile, m, op, M);
                 }
-            }
+                break;
+            case OPK_MFMA:
+                if constexpr (HEAVY) {
+                    if (op.k == 4) op_mfma<T, NV, 4>(tile, m, op, M);
+                    else op_mfma<T, NV, 5>(tile, m, op, M);
+                }
+                break;
+            case OPK_DIAG: op_diag<T, NV>(tile, m, op, M, base); break;
+            case OPK_SCALAR: op_scalar<T, NV>(tile, m, M); break;
+            case OPK_GRAD:
+            case OPK_GRADDIAG:
+                if constexpr (NV == 2) {
+                    const double part = op.kind == OPK_GRAD ? op_grad_dense<T>(tile, m, op, M)
+                                                            : op_grad_diag<T>(tile, m, op, M);
+                    // one wave-level reduction; each wave owns its own slot
+                    const double ws = wave_sum(part);
+                    if ((threadIdx.x & 63) == 0) gacc[gi * (kThreads / 64) + (threadIdx.x >> 6)] = ws;
+                }
+                ++gi;
+                break;
+            default: break;
         }
         __syncthreads();
     }
-    if (a.st.n_grad > 0 && threadIdx.x == 0) {
-        int gi = 0;
-        for (int o = a.st.op_begin; o < a.st.op_end; ++o) {
-            const DevOp op = a.ops[o];
-            if (op.kind == OPK_GRAD || op.kind == OPK_GRADDIAG) {
-                a.partials[(int64_t)op.slot * a.n_wg + blockIdx.x] = gacc[gi];
-                ++gi;
+    // store (optionally through the write-back bit permutation)
+    for (int e = threadIdx.x * PV; e < size; e += blockDim.x * PV) {
+        const uint64_t s = (uint64_t)e >> L;
+        const uint64_t off = base | ((uint64_t)e & lowm) | seg_off(s);
+        if (!permuted) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v)
+                *reinterpret_cast<V *>(&svp[v][off]) = *reinterpret_cast<const V *>(&tile[v][e]);
+        } else {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                C tmp[PV];
+#pragma unroll
+                for (int i = 0; i < PV; ++i) {
+                    const int ee = e + i;
+                    tmp[i] = tile[v][perm_lo[ee & 255] | perm_hi[ee >> 8]];
+                }
+                *reinterpret_cast<V *>(&svp[v][off]) = *reinterpret_cast<const V *>(tmp);
+            }
+        }
+    }
+    if (tk.st.n_grad > 0) {
+        __syncthreads();
+        int g2 = 0;
+        for (int o = tk.st.op_begin; o < tk.st.op_end; ++o) {
+            const int kind = ops[o].kind;
+            if (kind == OPK_GRAD || kind == OPK_GRADDIAG) {
+                if (threadIdx.x == 0) {
+                    double sum = 0;
+                    for (int w = 0; w < kThreads / 64; ++w) sum += gacc[g2 * (kThreads / 64) + w];
+                    tk.partials[(int64_t)ops[o].slot * tk.pstride + t] = sum;
+                }
+                ++g2;
             }
         }
     }
 }
 
 template <typename T>
-hipError_t launch_stage(int nv, void *sv0, void *sv1, const StageArgs &a, int grid, hipStream_t s) {
+hipError_t launch_stage(int nv, bool heavy, const StageTask *tasks, const int32_t *wg_task, const StageTask &single,
+                        int64_t grid, int max_m, hipStream_t s) {
     using C = typename VT<T>::c;
-    const size_t lds = (size_t)nv * ((size_t)1 << a.st.m) * sizeof(C);
-    if (a.st.heavy) {
-        if (nv == 1)
-            hipLaunchKernelGGL((k_stage<T, 1, true>), dim3(grid), dim3(kThreads), lds, s, (C *)sv0, (C *)sv1, a);
-        else
-            hipLaunchKernelGGL((k_stage<T, 2, true>), dim3(grid), dim3(kThreads), lds, s, (C *)sv0, (C *)sv1, a);
+    const size_t lds = (size_t)nv * ((size_t)1 << max_m) * sizeof(C);
+    if (grid <= 0) return hipSuccess;
+    if (grid > 0x7fffffffll) return hipErrorInvalidValue;
+    const dim3 g((unsigned)grid);
+    if (heavy) {
+        if (nv == 1) hipLaunchKernelGGL((k_stage<T, 1, true>), g, dim3(kThreads), lds, s, tasks, wg_task, single);
+        else hipLaunchKernelGGL((k_stage<T, 2, true>), g, dim3(kThreads), lds, s, tasks, wg_task, single);
     } else {
-        if (nv == 1)
-            hipLaunchKernelGGL((k_stage<T, 1, false>), dim3(grid), dim3(kThreads), lds, s, (C *)sv0, (C *)sv1, a);
-        else
-            hipLaunchKernelGGL((k_stage<T, 2, false>), dim3(grid), dim3(kThreads), lds, s, (C *)sv0, (C *)sv1, a);
+        if (nv == 1) hipLaunchKernelGGL((k_stage<T, 1, false>), g, dim3(kThreads), lds, s, tasks, wg_task, single);
+        else hipLaunchKernelGGL((k_stage<T, 2, false>), g, dim3(kThreads), lds, s, tasks, wg_task, single);
     }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Multi-instance helpers for batches
+// ---------------------------------------------------------------------------
+template <typename Task>
+__device__ inline int find_task(const Task *tasks, int ntasks) {
+    int lo = 0, hi = ntasks - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tasks[mid].wg_begin <= (int64_t)blockIdx.x) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_matvec_tasks(const MatvecTask *__restrict__ tasks, int ntasks) {
+    using C = typename VT<T>::c;
+    __shared__ double red[kThreads / 64];
+    const MatvecTask tk = tasks[find_task(tasks, ntasks)];
+    const int64_t w = (int64_t)blockIdx.x - tk.wg_begin;
+    const int64_t dim = 1ll << tk.n;
+    const int64_t i0 = w * tk.chunk;
+    const int64_t i1 = i0 + tk.chunk < dim ? i0 + tk.chunk : dim;
+    const C *psi = reinterpret_cast<const C *>(tk.psi);
+    C *lam = reinterpret_cast<C *>(tk.lam);
+    C *psi2 = reinterpret_cast<C *>(tk.psi2);
+    const MatvecGroup *groups = reinterpret_cast<const MatvecGroup *>(tk.groups);
+    double e = 0;
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+        double2 acc = make_double2(0.0, 0.0);
+        for (int g = 0; g < tk.ngroups; ++g) {
+            const MatvecGroup gr = groups[g];
+            const uint64_t s = (uint64_t)i ^ gr.x;
+            double fr = 0, fi = 0;
+            for (int q = gr.begin; q < gr.end; ++q) {
+                const double2 c = tk.cc[q];
+                if (__popcll(s & tk.z[q]) & 1) {
+                    fr -= c.x;
+                    fi -= c.y;
+                } else {
+                    fr += c.x;
+                    fi += c.y;
+                }
+            }
+            const double2 a = to_d(psi[s]);
+            acc.x += fr * a.x - fi * a.y;
+            acc.y += fr * a.y + fi * a.x;
+        }
+        const C p = psi[i];
+        const double2 pd = to_d(p);
+        e += pd.x * acc.x + pd.y * acc.y;
+        C r;
+        r.x = (T)acc.x;
+        r.y = (T)acc.y;
+        lam[i] = r;
+        if (psi2) psi2[i] = p;
+    }
+    const double sum = block_sum(e, red);
+    if (threadIdx.x == 0) tk.epart[w] = sum;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_fill_tasks(const FillTask *__restrict__ tasks, int ntasks) {
+    using C = typename VT<T>::c;
+    const FillTask tk = tasks[find_task(tasks, ntasks)];
+    const int64_t w = (int64_t)blockIdx.x - tk.wg_begin;
+    const int64_t i0 = w * tk.chunk;
+    const int64_t i1 = i0 + tk.chunk < tk.dim ? i0 + tk.chunk : tk.dim;
+    C *dst = reinterpret_cast<C *>(tk.dst);
+    const C *src = reinterpret_cast<const C *>(tk.src);
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+        C v;
+        if (src) {
+            v = src[i];
+        } else {
+            v.x = i == 0 ? T(1) : T(0);
+            v.y = T(0);
+        }
+        dst[i] = v;
+    }
+}
+
+__global__ void __launch_bounds__(kThreads) k_sum_tasks(const SumTask *__restrict__ tasks) {
+    __shared__ double red[kThreads / 64];
+    const SumTask tk = tasks[blockIdx.x];
+    double acc = 0;
+    for (int64_t i = threadIdx.x; i < tk.count; i += blockDim.x) acc += tk.src[i];
+    const double s = block_sum(acc, red);
+    if (threadIdx.x == 0) *tk.dst = s;
+}
+
+template <typename T>
+hipError_t launch_matvec_tasks(const MatvecTask *tasks, int ntasks, int64_t grid, hipStream_t s) {
+    if (grid <= 0) return hipSuccess;
+    hipLaunchKernelGGL((k_matvec_tasks<T>), dim3((unsigned)grid), dim3(kThreads), 0, s, tasks, ntasks);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_fill_tasks(const FillTask *tasks, int ntasks, int64_t grid, hipStream_t s) {
+    if (grid <= 0) return hipSuccess;
+    hipLaunchKernelGGL((k_fill_tasks<T>), dim3((unsigned)grid), dim3(kThreads), 0, s, tasks, ntasks);
+    return hipGetLastError();
+}
+hipError_t launch_sum_tasks(const SumTask *tasks, int ntasks, hipStream_t s) {
+    if (ntasks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sum_tasks, dim3(ntasks), dim3(kThreads), 0, s, tasks);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Direct single-op kernel (no LDS): one dense 1-/2-qubit or diagonal op over
+// the whole state, streaming.  Thread g owns VEC consecutive groups; when bit
+// 0 is neither a target nor a control, VEC = 2 amplitudes of c64 share one
+// 16-byte access, so every load/store of a wave is a contiguous run.
+// `pos` are global bit positions (matrix index bit j <-> pos[j]).
+// ---------------------------------------------------------------------------
+template <typename T, int K, int VEC, bool DIAG>
+__global__ void __launch_bounds__(kThreads) k_direct(typename VT<T>::c *__restrict__ sv, int64_t groups, DirectOp op,
+                                                    const typename VT<T>::c *__restrict__ M) {
+    using C = typename VT<T>::c;
+    constexpr int D = 1 << K;
+    int srt[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) srt[j] = op.pos[j];
+#pragma unroll
+    for (int a = 0; a < K; ++a)
+#pragma unroll
+        for (int b = 0; b < K - 1; ++b)
+            if (srt[b] > srt[b + 1]) { int t = srt[b]; srt[b] = srt[b + 1]; srt[b + 1] = t; }
+    uint64_t off[D];
+#pragma unroll
+    for (int s = 0; s < D; ++s) {
+        uint64_t o = 0;
+#pragma unroll
+        for (int j = 0; j < K; ++j)
+            if (s >> j & 1) o |= 1ull << op.pos[j];
+        off[s] = o;
+    }
+    C mm[DIAG ? D : D * D];
+#pragma unroll
+    for (int i = 0; i < (DIAG ? D : D * D); ++i) mm[i] = M[i];
+    const uint64_t cm = op.cmask;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x * VEC;
+    for (int64_t g0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * VEC; g0 < groups; g0 += stride) {
+        uint64_t b = (uint64_t)g0;
+#pragma unroll
+        for (int j = 0; j < K; ++j) b = ((b >> srt[j]) << (srt[j] + 1)) | (b & ((1ull << srt[j]) - 1));
+        if ((b & cm) != cm) continue;
+        C a[D][VEC];
+#pragma unroll
+        for (int s = 0; s < D; ++s) {
+            if constexpr (VEC == 2) {
+                const float4 v = *reinterpret_cast<const float4 *>(&sv[b | off[s]]);
+                a[s][0] = make_float2(v.x, v.y);
+                a[s][1] = make_float2(v.z, v.w);
+            } else {
+                a[s][0] = sv[b | off[s]];
+            }
+        }
+        C r[D][VEC];
+#pragma unroll
+        for (int u = 0; u < VEC; ++u)
+#pragma unroll
+            for (int q = 0; q < D; ++q) {
+                if constexpr (DIAG) {
+                    r[q][u] = cmul(mm[q], a[q][u]);
+                } else {
+                    C acc = cmul(mm[q * D], a[0][u]);
+#pragma unroll
+                    for (int c = 1; c < D; ++c) acc = cfma(mm[q * D + c], a[c][u], acc);
+                    r[q][u] = acc;
+                }
+            }
+#pragma unroll
+        for (int s = 0; s < D; ++s) {
+            if constexpr (VEC == 2) {
+                *reinterpret_cast<float4 *>(&sv[b | off[s]]) = make_float4(r[s][0].x, r[s][0].y, r[s][1].x, r[s][1].y);
+            } else {
+                sv[b | off[s]] = r[s][0];
+            }
+        }
+    }
+}
+
+template <typename T>
+hipError_t launch_direct(void *sv, int n, const DirectOp &op, const void *M, hipStream_t s) {
+    using C = typename VT<T>::c;
+    const int K = op.k;
+    const int64_t groups = 1ll << (n - K);
+    bool bit0 = (op.cmask & 1) != 0;
+    for (int j = 0; j < K; ++j) bit0 |= op.pos[j] == 0;
+    const bool vec = std::is_same<T, float>::value && !bit0 && groups >= 2;
+    const int64_t per_block = (int64_t)kThreads * (vec ? 2 : 1) * 4;  // 4 groups-vectors per thread
+    int64_t grid = (groups + per_block - 1) / per_block;
+    if (grid < 1) grid = 1;
+    if (grid > (1 << 20)) grid = 1 << 20;
+    C *p = reinterpret_cast<C *>(sv);
+    const C *m = reinterpret_cast<const C *>(M);
+#define DESV_DIRECT(KK, VV, DD) \
+    hipLaunchKernelGGL((k_direct<T, KK, VV, DD>), dim3((unsigned)grid), dim3(kThreads), 0, s, p, groups, op, m)
+    if constexpr (std::is_same<T, float>::value) {
+        if (K == 1) {
+            if (op.diag) { if (vec) DESV_DIRECT(1, 2, true); else DESV_DIRECT(1, 1, true); }
+            else { if (vec) DESV_DIRECT(1, 2, false); else DESV_DIRECT(1, 1, false); }
+        } else {
+            if (op.diag) { if (vec) DESV_DIRECT(2, 2, true); else DESV_DIRECT(2, 1, true); }
+            else { if (vec) DESV_DIRECT(2, 2, false); else DESV_DIRECT(2, 1, false); }
+        }
+    } else {
+        if (K == 1) {
+            if (op.diag) DESV_DIRECT(1, 1, true); else DESV_DIRECT(1, 1, false);
+        } else {
+            if (op.diag) DESV_DIRECT(2, 1, true); else DESV_DIRECT(2, 1, false);
+        }
+    }
+#undef DESV_DIRECT
     return hipGetLastError();
 }
 
@@ -543,7 +785,7 @@ size_t stage_lds_limit(int device) {
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && v > 0)
         lim = (size_t)v;
     // leave room for the static gradient/reduction arrays
-    lim -= (kMaxGradLDS + kThreads / 64) * sizeof(double) + 512 * sizeof(int);
+    lim -= kMaxGradLDS * (kThreads / 64) * sizeof(double) + 512 * sizeof(int) + 256 * 8 + sizeof(StageTask);
     const void *fns[8] = {(const void *)k_stage<float, 1, false>,  (const void *)k_stage<float, 2, false>,
                           (const void *)k_stage<double, 1, false>, (const void *)k_stage<double, 2, false>,
                           (const void *)k_stage<float, 1, true>,   (const void *)k_stage<float, 2, true>,
@@ -551,7 +793,7 @@ size_t stage_lds_limit(int device) {
     for (const void *f : fns)
         if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lim) != hipSuccess) {
             (void)hipGetLastError();
-            lim = 64 * 1024 - (kMaxGradLDS + kThreads / 64) * sizeof(double) - 512 * sizeof(int);
+            lim = 64 * 1024 - kMaxGradLDS * (kThreads / 64) * sizeof(double) - 512 * sizeof(int) - 256 * 8 - sizeof(StageTask);
         }
     cached[device] = lim;
     return lim;
@@ -848,7 +1090,11 @@ hipError_t launch_dense_matvec(void *out, const void *in, int n, const double2 *
 }
 
 #define DESV_INST(T)                                                                                            \
-    template hipError_t launch_stage<T>(int, void *, void *, const StageArgs &, int, hipStream_t);             \
+    template hipError_t launch_direct<T>(void *, int, const DirectOp &, const void *, hipStream_t);            \
+    template hipError_t launch_stage<T>(int, bool, const StageTask *, const int32_t *, const StageTask &, int64_t, int, \
+                                        hipStream_t);                                                          \
+    template hipError_t launch_matvec_tasks<T>(const MatvecTask *, int, int64_t, hipStream_t);                 \
+    template hipError_t launch_fill_tasks<T>(const FillTask *, int, int64_t, hipStream_t);                     \
     template hipError_t launch_fill_basis<T>(void *, int64_t, int64_t, hipStream_t);                          \
     template hipError_t launch_scale<T>(void *, int64_t, double, double, hipStream_t);                        \
     template hipError_t launch_norm2<T>(const void *, int64_t, double *, int, hipStream_t);                   \

@@ -27,13 +27,53 @@ template <> struct VT<double> {
 constexpr int kThreads = 256;
 constexpr int kMaxGradLDS = 64;
 
-struct StageArgs {
+// One stage of one state: which tiles, which ops, which buffers.  A launch
+// runs one task (passed by value) or an array of tasks (one stage of many
+// batch instances); workgroup b serves tile (b - wg_begin) of the task with
+// the largest wg_begin <= b.
+struct StageTask {
     DevStage st;
-    int n;
+    int32_t n;
+    int32_t pad0;
+    void *sv0;           // psi
+    void *sv1;           // lambda (adjoint reverse sweep), else unused
     const DevOp *ops;
     const void *mats;    // complex pool in the state precision
-    double *partials;    // gradient partials [slot][n_wg]
-    int n_wg;
+    double *partials;    // gradient partials [slot][pstride]
+    int64_t pstride;     // tiles per gradient slot row
+    int64_t wg_begin;
+};
+
+// lambda = H psi, psi2 = psi and FP64 partials of Re<psi|lambda>, for a chunk of
+// one instance's amplitudes per workgroup.
+struct MatvecTask {
+    int32_t n;
+    int32_t ngroups;
+    const void *psi;
+    void *lam;
+    void *psi2;          // may be null
+    const void *groups;  // MatvecGroup[ngroups]
+    const uint64_t *z;
+    const double2 *cc;
+    double *epart;       // one double per workgroup of this task
+    int64_t wg_begin;
+    int64_t chunk;       // amplitudes per workgroup
+};
+
+// dst[0..dim) = src, or |0...0> when src is null.
+struct FillTask {
+    void *dst;
+    const void *src;
+    int64_t dim;
+    int64_t wg_begin;
+    int64_t chunk;
+};
+
+// *dst = sum of src[0..count)
+struct SumTask {
+    const double *src;
+    int64_t count;
+    double *dst;
 };
 
 // Pauli-expectation work item: one flip mask, up to kTermsPerItem terms.
@@ -51,9 +91,28 @@ struct MatvecGroup {
     int32_t end;
 };
 
+// One dense (k = 1, 2) or diagonal op over a whole state, global bit
+// positions; applied by the streaming direct kernel.
+struct DirectOp {
+    int32_t k;
+    int32_t diag;
+    int32_t pos[2];
+    uint64_t cmask;
+};
+
 // ---- launchers (engine_kernels.hip) --------------------------------------
 template <typename T>
-hipError_t launch_stage(int nv, void *sv0, void *sv1, const StageArgs &a, int grid, hipStream_t s);
+hipError_t launch_direct(void *sv, int n, const DirectOp &op, const void *M, hipStream_t s);
+// Stage launch: `tasks` (device array) with wg_task[b] = task of workgroup b,
+// or, if tasks is null, `single`.
+template <typename T>
+hipError_t launch_stage(int nv, bool heavy, const StageTask *tasks, const int32_t *wg_task, const StageTask &single,
+                        int64_t grid, int max_m, hipStream_t s);
+template <typename T>
+hipError_t launch_matvec_tasks(const MatvecTask *tasks, int ntasks, int64_t grid, hipStream_t s);
+template <typename T>
+hipError_t launch_fill_tasks(const FillTask *tasks, int ntasks, int64_t grid, hipStream_t s);
+hipError_t launch_sum_tasks(const SumTask *tasks, int ntasks, hipStream_t s);
 template <typename T>
 hipError_t launch_fill_basis(void *sv, int64_t dim, int64_t index, hipStream_t s);
 template <typename T>

@@ -296,13 +296,22 @@ Gate materialise(const Block &b, const std::vector<Gate> &gs) {
 // fusable when it is not a gradient probe, and all its bits lie in `allowed`.
 // Open blocks are always the latest writer of their bits, so merging open
 // blocks that touch a new gate keeps a valid order.
-std::vector<Gate> fuse(const std::vector<Gate> &gs, int K, uint64_t allowed) {
+std::vector<Gate> fuse(const std::vector<Gate> &gs, int K, uint64_t allowed, std::vector<Recipe> *rec) {
     std::vector<Gate> out;
     std::vector<Block> open;
+    auto emit = [&](const Block &b) {
+        out.push_back(materialise(b, gs));
+        if (rec) {
+            Recipe r;
+            r.gates = b.gates;
+            r.bits = b.gates.size() == 1 ? 0 : b.bits;
+            rec->push_back(r);
+        }
+    };
     auto emit_touching = [&](uint64_t mask) {
         std::vector<Block> keep;
         for (auto &b : open) {
-            if (b.bits & mask) out.push_back(materialise(b, gs));
+            if (b.bits & mask) emit(b);
             else keep.push_back(std::move(b));
         }
         open.swap(keep);
@@ -315,6 +324,7 @@ std::vector<Gate> fuse(const std::vector<Gate> &gs, int K, uint64_t allowed) {
         if (!fusable) {
             emit_touching(am);
             out.push_back(g);
+            if (rec) rec->push_back(Recipe{{i}, 0});
             continue;
         }
         uint64_t u = am;
@@ -339,7 +349,7 @@ std::vector<Gate> fuse(const std::vector<Gate> &gs, int K, uint64_t allowed) {
             open.push_back(std::move(nb));
         }
     }
-    for (auto &b : open) out.push_back(materialise(b, gs));
+    for (auto &b : open) emit(b);
     return out;
 }
 
@@ -524,10 +534,13 @@ bool build_plan(const std::vector<Gate> &gates, int n, const CompileOpts &o, Pla
 
     if (o.per_gate) {
         // One HBM pass per fused block.
-        std::vector<Gate> fused = fuse(gates, K, all);
-        for (const Gate &g : fused) {
+        std::vector<Recipe> rec;
+        std::vector<Gate> fused = fuse(gates, K, all, &rec);
+        for (size_t i = 0; i < fused.size(); ++i) {
+            const Gate &g = fused[i];
             uint64_t tile = fill_tile(low_mask(L) | g.tile_mask(), m, n);
             if (!lower_stage({g}, tile, n, o, plan, err)) return false;
+            plan.recipes.push_back(rec[i]);
         }
         return true;
     }
@@ -637,7 +650,17 @@ bool build_plan(const std::vector<Gate> &gates, int n, const CompileOpts &o, Pla
         std::vector<Gate> lg;
         lg.reserve(take.size());
         for (int gi : take) lg.push_back(gates[gi]);
-        std::vector<Gate> fused = K > 1 ? fuse(lg, K, S) : lg;
+        std::vector<Recipe> rec;
+        std::vector<Gate> fused;
+        if (K > 1) {
+            fused = fuse(lg, K, S, &rec);
+        } else {
+            fused = lg;
+            for (size_t i = 0; i < lg.size(); ++i) rec.push_back(Recipe{{(int)i}, 0});
+        }
+        for (Recipe &r : rec)
+            for (int &gi : r.gates) gi = take[gi];
+        plan.recipes.insert(plan.recipes.end(), rec.begin(), rec.end());
         std::vector<Gate> phys;
         phys.reserve(fused.size());
         for (const Gate &g : fused) phys.push_back(to_physical(g, pi));
@@ -712,6 +735,23 @@ bool build_plan(const std::vector<Gate> &gates, int n, const CompileOpts &o, Pla
             return false;
         }
     return true;
+}
+
+void refresh_plan_mats(Plan &plan, const std::vector<Gate> &gates) {
+    for (size_t i = 0; i < plan.ops.size(); ++i) {
+        const Recipe &r = plan.recipes[i];
+        const DevOp &d = plan.ops[i];
+        if (r.gates.size() == 1) {
+            const Gate &g = gates[r.gates[0]];
+            std::copy(g.m.begin(), g.m.end(), plan.mats.begin() + d.mat);
+        } else {
+            Block b;
+            b.bits = r.bits;
+            b.gates = r.gates;
+            const Gate g = materialise(b, gates);
+            std::copy(g.m.begin(), g.m.end(), plan.mats.begin() + d.mat);
+        }
+    }
 }
 
 }  // namespace desv

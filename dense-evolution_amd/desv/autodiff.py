@@ -91,7 +91,43 @@ class AdjointProgram:
         ops[self.template[:, 3] == 0.0, 3] = 0.0
         return ops
 
+    # -- Pauli-sum Hamiltonians: a one-instance batch (compiled once, angles
+    #    refreshed per call, launches replayed from a captured hipGraph) --------
+    def _batch_for(self, hamiltonian):
+        from .batch import Batch
+        key = id(hamiltonian)
+        if getattr(self, "_bkey", None) != key:
+            xm, zm, cs = hamiltonian.encoded()
+            if np.any(np.imag(cs) != 0):
+                raise ValueError("Pauli-sum Hamiltonians must have real coefficients")
+            b = Batch("complex128" if self.handle.code == _capi.C128 else "complex64", self.handle.device)
+            b.add_ops(self.n, self.ops_for(np.zeros(self.n_params)), self.param_of_op, self.n_params,
+                      xm, zm, np.real(cs))
+            self._batch, self._bkey, self._bham = b, key, hamiltonian
+        return self._batch
+
+    def _prep_batch(self, theta, hamiltonian, initial_state):
+        b = self._batch_for(hamiltonian)
+        b.set_params(0, np.asarray(theta, dtype=np.float64).reshape(-1))
+        if initial_state is None:
+            b.set_initial_state(0, None)
+        else:
+            b.set_initial_state(0, initial_state.get() if isinstance(initial_state, DeviceState)
+                                else np.asarray(initial_state))
+        self._last_state = _BatchState(b, self.n, self.handle.dtype)
+        return b
+
+    def state(self):
+        """DeviceState view of the state after the last evaluation."""
+        return DeviceState(getattr(self, "_last_state", None) or self.handle)
+
     def value_and_grad(self, theta, hamiltonian, initial_state=None) -> Tuple[float, np.ndarray]:
+        if isinstance(hamiltonian, PauliSumOperator):
+            self.ops_for(theta)  # validates the length
+            b = self._prep_batch(theta, hamiltonian, initial_state)
+            e, g = b.run()
+            return float(e[0]), g[0]
+        self._last_state = None
         ops = np.ascontiguousarray(self.ops_for(theta))
         h = self.handle
         if initial_state is None:
@@ -120,6 +156,11 @@ class AdjointProgram:
 
     def energy(self, theta, hamiltonian, initial_state=None) -> float:
         """Forward only (no gradient sweep)."""
+        if isinstance(hamiltonian, PauliSumOperator):
+            self.ops_for(theta)
+            b = self._prep_batch(theta, hamiltonian, initial_state)
+            return float(b.energies()[0])
+        self._last_state = None
         ops = self.ops_for(theta)
         h = self.handle
         if initial_state is None:
@@ -131,6 +172,23 @@ class AdjointProgram:
 
     def close(self):
         self.handle.close()
+        if getattr(self, "_batch", None) is not None:
+            self._batch.close()
+            self._batch = None
+
+
+class _BatchState:
+    """Host-transfer view of a batch instance's final state (for DeviceState)."""
+
+    def __init__(self, batch, n, dtype):
+        self.batch, self.n, self.dtype = batch, n, np.dtype(dtype)
+
+    @property
+    def dim(self):
+        return 1 << self.n
+
+    def download(self):
+        return self.batch.get_state(0, self.n)
 
 
 def _dense_h(hamiltonian, n):
@@ -235,9 +293,10 @@ def circuit_to_energy_fn(circuit, n_qubits: int, dtype=None, device: int = 0) ->
             else:
                 e = torch.as_tensor(prog.energy(th.detach().cpu().numpy(), h_matrix, stato_zero),
                                     dtype=theta.dtype, device=theta.device)
-            return e, DeviceState(prog.handle)
+            return e, prog.state()
         th = _theta_for_prog(np.asarray(theta, dtype=np.float64).reshape(-1))
-        return prog.energy(th, h_matrix, stato_zero), DeviceState(prog.handle)
+        e = prog.energy(th, h_matrix, stato_zero)
+        return e, prog.state()
 
     def value_and_grad(theta, h_matrix, stato_zero=None):
         th = np.asarray(theta, dtype=np.float64).reshape(-1)

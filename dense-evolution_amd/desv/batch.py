@@ -62,8 +62,40 @@ class Batch:
         th = np.ascontiguousarray(np.asarray(theta, dtype=np.float64).reshape(-1)[:self.n_params[index]])
         check(lib.desv_batch_set_params(self.ptr, index, _capi.dptr(th)))
 
+    def set_initial_state(self, index: int, state=None) -> None:
+        """Initial state of one instance (host array) or None for |0...0>."""
+        if state is None:
+            check(lib.desv_batch_set_initial_state(self.ptr, index, None))
+            return
+        a = np.ascontiguousarray(state, dtype=np.complex64 if self.code == _capi.C64 else np.complex128)
+        check(lib.desv_batch_set_initial_state(self.ptr, index, _capi.vptr(a)))
+
+    def get_state(self, index: int, n_qubits: int) -> np.ndarray:
+        out = np.empty(1 << n_qubits, dtype=np.complex64 if self.code == _capi.C64 else np.complex128)
+        check(lib.desv_batch_get_state(self.ptr, index, _capi.vptr(out)))
+        return out
+
     def configure(self, tile_bits: int = 0, use_graph: bool = True) -> None:
         check(lib.desv_batch_configure(self.ptr, int(tile_bits), int(use_graph)))
+
+    def add_ops(self, n_qubits: int, ops, param_of_op, n_params: int, xmask, zmask, coeff) -> int:
+        """Add an instance from an op table and encoded Pauli terms."""
+        ops = np.ascontiguousarray(ops, dtype=np.float64).reshape(-1, 4)
+        pmap = np.ascontiguousarray(param_of_op, dtype=np.int64)
+        xm = np.ascontiguousarray(xmask, dtype=np.uint64)
+        zm = np.ascontiguousarray(zmask, dtype=np.uint64)
+        cf = np.ascontiguousarray(coeff, dtype=np.float64)
+        idx = C.c_int64()
+        check(lib.desv_batch_add(self.ptr, n_qubits, _capi.dptr(ops), len(ops), _capi.i64ptr(pmap), n_params,
+                                 _capi.u64ptr(xm), _capi.u64ptr(zm), _capi.dptr(cf), len(xm), C.byref(idx)))
+        self.n_params.append(int(n_params))
+        return idx.value
+
+    def energies(self) -> np.ndarray:
+        """Forward pass and energies only."""
+        e = np.zeros(len(self.n_params), dtype=np.float64)
+        check(lib.desv_batch_run(self.ptr, _capi.dptr(e), None))
+        return e
 
     def run(self) -> Tuple[np.ndarray, List[np.ndarray]]:
         e = np.zeros(len(self.n_params), dtype=np.float64)

@@ -14,7 +14,7 @@ from .simulator import DeviceState, StateHandle
 
 
 def _device(statevector):
-    if isinstance(statevector, DeviceState):
+    if isinstance(statevector, DeviceState) and isinstance(statevector.handle, StateHandle):
         return statevector.handle, False
     a = np.asarray(statevector)
     n = a.shape[0].bit_length() - 1

@@ -88,7 +88,7 @@ def encode_pauli_terms(terms: Iterable, n: int, n_qubits=None, what='statevector
 
 def _as_device(statevector):
     """(handle, owned) for a DeviceState, simulator or host array."""
-    if isinstance(statevector, DeviceState):
+    if isinstance(statevector, DeviceState) and isinstance(statevector.handle, StateHandle):
         return statevector.handle, False
     h = getattr(statevector, 'handle', None)
     if isinstance(h, StateHandle):

@@ -177,10 +177,16 @@ int desv_batch_add(desv_batch b, int n_qubits, const double *ops, int64_t n_ops,
                    int64_t *index);
 /* Update the theta-carrying angles of instance i (n_params values). */
 int desv_batch_set_params(desv_batch b, int64_t index, const double *theta);
+/* Initial state of instance i (2^n amplitudes in the batch dtype), or NULL
+ * for |0...0> (the default). */
+int desv_batch_set_initial_state(desv_batch b, int64_t index, const void *host_amps);
+/* Final state psi of instance i after the last run. */
+int desv_batch_get_state(desv_batch b, int64_t index, void *host_amps);
 /* tile_bits/threads: 0 = autotuned/default. use_graph: capture + replay. */
 int desv_batch_configure(desv_batch b, int tile_bits, int use_graph);
 /* Evaluate everything; energies[i], grads[offset_i ...] in instance order
- * (grads packed by instance, n_params_i each). */
+ * (grads packed by instance, n_params_i each).  grads == NULL runs the
+ * forward pass and energies only. */
 int desv_batch_run(desv_batch b, double *energies, double *grads);
 /* Per-instance device time of the last run is not separable; this returns
  * the wall time of the last desv_batch_run in microseconds. */
