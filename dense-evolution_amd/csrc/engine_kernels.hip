@@ -9,6 +9,8 @@
 // stage on the tile in LDS, and writes it back: one HBM round trip per stage.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cstdlib>
 #include <type_traits>
 
 #include "kernels.h"
@@ -673,7 +675,38 @@ hipError_t launch_sum_tasks(const SumTask *tasks, int ntasks, hipStream_t s) {
 // 16-byte access, so every load/store of a wave is a contiguous run.
 // `pos` are global bit positions (matrix index bit j <-> pos[j]).
 // ---------------------------------------------------------------------------
-template <typename T, int K, int VEC, bool DIAG>
+typedef float nt_f4 __attribute__((ext_vector_type(4)));
+typedef float nt_f2 __attribute__((ext_vector_type(2)));
+typedef double nt_d2 __attribute__((ext_vector_type(2)));
+template <typename V> struct NtVec;
+template <> struct NtVec<float4> { using t = nt_f4; };
+template <> struct NtVec<float2> { using t = nt_f2; };
+template <> struct NtVec<double2> { using t = nt_d2; };
+
+// Plain or non-temporal (streaming) vector access.
+template <typename V, bool NT> __device__ inline V ld(const V *p) {
+    if constexpr (NT) {
+        using N = typename NtVec<V>::t;
+        const N v = __builtin_nontemporal_load(reinterpret_cast<const N *>(p));
+        V r;
+        __builtin_memcpy(&r, &v, sizeof(V));
+        return r;
+    } else {
+        return *p;
+    }
+}
+template <typename V, bool NT> __device__ inline void st(V *p, V v) {
+    if constexpr (NT) {
+        using N = typename NtVec<V>::t;
+        N w;
+        __builtin_memcpy(&w, &v, sizeof(V));
+        __builtin_nontemporal_store(w, reinterpret_cast<N *>(p));
+    } else {
+        *p = v;
+    }
+}
+
+template <typename T, int K, int VEC, bool DIAG, int GPT, bool NT>
 __global__ void __launch_bounds__(kThreads) k_direct(typename VT<T>::c *__restrict__ sv, int64_t groups, DirectOp op,
                                                     const typename VT<T>::c *__restrict__ M) {
     using C = typename VT<T>::c;
@@ -699,46 +732,100 @@ __global__ void __launch_bounds__(kThreads) k_direct(typename VT<T>::c *__restri
 #pragma unroll
     for (int i = 0; i < (DIAG ? D : D * D); ++i) mm[i] = M[i];
     const uint64_t cm = op.cmask;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x * VEC;
-    for (int64_t g0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * VEC; g0 < groups; g0 += stride) {
-        uint64_t b = (uint64_t)g0;
+    // thread t owns group vectors t, t + T, ..., t + (GPT-1) T (T = threads in
+    // the grid): every access instruction of a wave is one contiguous run.
+    const int64_t nthr = (int64_t)gridDim.x * blockDim.x;
+    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t b[GPT];
+    bool on[GPT];
+    C a[GPT][D][VEC];
 #pragma unroll
-        for (int j = 0; j < K; ++j) b = ((b >> srt[j]) << (srt[j] + 1)) | (b & ((1ull << srt[j]) - 1));
-        if ((b & cm) != cm) continue;
-        C a[D][VEC];
+    for (int u = 0; u < GPT; ++u) {
+        const int64_t g0 = (t0 + u * nthr) * VEC;
+        uint64_t x = (uint64_t)g0;
+#pragma unroll
+        for (int j = 0; j < K; ++j) x = ((x >> srt[j]) << (srt[j] + 1)) | (x & ((1ull << srt[j]) - 1));
+        b[u] = x;
+        on[u] = g0 < groups && (x & cm) == cm;
+        if (!on[u]) continue;
 #pragma unroll
         for (int s = 0; s < D; ++s) {
             if constexpr (VEC == 2) {
-                const float4 v = *reinterpret_cast<const float4 *>(&sv[b | off[s]]);
-                a[s][0] = make_float2(v.x, v.y);
-                a[s][1] = make_float2(v.z, v.w);
+                const float4 v = ld<float4, NT>(reinterpret_cast<const float4 *>(&sv[x | off[s]]));
+                a[u][s][0] = make_float2(v.x, v.y);
+                a[u][s][1] = make_float2(v.z, v.w);
             } else {
-                a[s][0] = sv[b | off[s]];
+                a[u][s][0] = ld<C, NT>(&sv[x | off[s]]);
             }
         }
+    }
+#pragma unroll
+    for (int u = 0; u < GPT; ++u) {
+        if (!on[u]) continue;
         C r[D][VEC];
 #pragma unroll
-        for (int u = 0; u < VEC; ++u)
+        for (int w = 0; w < VEC; ++w)
 #pragma unroll
             for (int q = 0; q < D; ++q) {
                 if constexpr (DIAG) {
-                    r[q][u] = cmul(mm[q], a[q][u]);
+                    r[q][w] = cmul(mm[q], a[u][q][w]);
                 } else {
-                    C acc = cmul(mm[q * D], a[0][u]);
+                    C acc = cmul(mm[q * D], a[u][0][w]);
 #pragma unroll
-                    for (int c = 1; c < D; ++c) acc = cfma(mm[q * D + c], a[c][u], acc);
-                    r[q][u] = acc;
+                    for (int c = 1; c < D; ++c) acc = cfma(mm[q * D + c], a[u][c][w], acc);
+                    r[q][w] = acc;
                 }
             }
 #pragma unroll
         for (int s = 0; s < D; ++s) {
             if constexpr (VEC == 2) {
-                *reinterpret_cast<float4 *>(&sv[b | off[s]]) = make_float4(r[s][0].x, r[s][0].y, r[s][1].x, r[s][1].y);
+                st<float4, NT>(reinterpret_cast<float4 *>(&sv[b[u] | off[s]]),
+                               make_float4(r[s][0].x, r[s][0].y, r[s][1].x, r[s][1].y));
             } else {
-                sv[b | off[s]] = r[s][0];
+                st<C, NT>(&sv[b[u] | off[s]], r[s][0]);
             }
         }
     }
+}
+
+namespace {
+// Tuning knobs of the direct kernel (group vectors per thread, non-temporal
+// access), read once from DESV_DIRECT_GPT / DESV_DIRECT_NT.
+int direct_gpt() {
+    static int v = [] {
+        const char *e = std::getenv("DESV_DIRECT_GPT");
+        const int x = e ? std::atoi(e) : 1;
+        return (x == 1 || x == 2 || x == 4) ? x : 1;
+    }();
+    return v;
+}
+bool direct_nt() {
+    static bool v = [] {
+        const char *e = std::getenv("DESV_DIRECT_NT");
+        return e ? std::atoi(e) != 0 : false;
+    }();
+    return v;
+}
+}  // namespace
+
+template <typename T, int K, int VEC, bool DIAG>
+hipError_t launch_direct_k(typename VT<T>::c *p, int64_t groups, const DirectOp &op, const typename VT<T>::c *m,
+                           hipStream_t s) {
+    const int gpt = direct_gpt();
+    const bool nt = direct_nt();
+    const int64_t vecs = (groups + VEC - 1) / VEC;
+    const int64_t threads = (vecs + gpt - 1) / gpt;
+    const int64_t grid = std::max<int64_t>(1, (threads + kThreads - 1) / kThreads);
+    if (grid > 0x7fffffffll) return hipErrorInvalidValue;
+#define DESV_DK(G, N) \
+    hipLaunchKernelGGL((k_direct<T, K, VEC, DIAG, G, N>), dim3((unsigned)grid), dim3(kThreads), 0, s, p, groups, op, m)
+    if (nt) {
+        if (gpt == 1) DESV_DK(1, true); else if (gpt == 2) DESV_DK(2, true); else DESV_DK(4, true);
+    } else {
+        if (gpt == 1) DESV_DK(1, false); else if (gpt == 2) DESV_DK(2, false); else DESV_DK(4, false);
+    }
+#undef DESV_DK
+    return hipGetLastError();
 }
 
 template <typename T>
@@ -749,31 +836,25 @@ hipError_t launch_direct(void *sv, int n, const DirectOp &op, const void *M, hip
     bool bit0 = (op.cmask & 1) != 0;
     for (int j = 0; j < K; ++j) bit0 |= op.pos[j] == 0;
     const bool vec = std::is_same<T, float>::value && !bit0 && groups >= 2;
-    const int64_t per_block = (int64_t)kThreads * (vec ? 2 : 1) * 4;  // 4 groups-vectors per thread
-    int64_t grid = (groups + per_block - 1) / per_block;
-    if (grid < 1) grid = 1;
-    if (grid > (1 << 20)) grid = 1 << 20;
     C *p = reinterpret_cast<C *>(sv);
     const C *m = reinterpret_cast<const C *>(M);
-#define DESV_DIRECT(KK, VV, DD) \
-    hipLaunchKernelGGL((k_direct<T, KK, VV, DD>), dim3((unsigned)grid), dim3(kThreads), 0, s, p, groups, op, m)
     if constexpr (std::is_same<T, float>::value) {
         if (K == 1) {
-            if (op.diag) { if (vec) DESV_DIRECT(1, 2, true); else DESV_DIRECT(1, 1, true); }
-            else { if (vec) DESV_DIRECT(1, 2, false); else DESV_DIRECT(1, 1, false); }
-        } else {
-            if (op.diag) { if (vec) DESV_DIRECT(2, 2, true); else DESV_DIRECT(2, 1, true); }
-            else { if (vec) DESV_DIRECT(2, 2, false); else DESV_DIRECT(2, 1, false); }
+            if (op.diag) return vec ? launch_direct_k<T, 1, 2, true>(p, groups, op, m, s)
+                                    : launch_direct_k<T, 1, 1, true>(p, groups, op, m, s);
+            return vec ? launch_direct_k<T, 1, 2, false>(p, groups, op, m, s)
+                       : launch_direct_k<T, 1, 1, false>(p, groups, op, m, s);
         }
+        if (op.diag) return vec ? launch_direct_k<T, 2, 2, true>(p, groups, op, m, s)
+                                : launch_direct_k<T, 2, 1, true>(p, groups, op, m, s);
+        return vec ? launch_direct_k<T, 2, 2, false>(p, groups, op, m, s)
+                   : launch_direct_k<T, 2, 1, false>(p, groups, op, m, s);
     } else {
-        if (K == 1) {
-            if (op.diag) DESV_DIRECT(1, 1, true); else DESV_DIRECT(1, 1, false);
-        } else {
-            if (op.diag) DESV_DIRECT(2, 1, true); else DESV_DIRECT(2, 1, false);
-        }
+        if (K == 1) return op.diag ? launch_direct_k<T, 1, 1, true>(p, groups, op, m, s)
+                                   : launch_direct_k<T, 1, 1, false>(p, groups, op, m, s);
+        return op.diag ? launch_direct_k<T, 2, 1, true>(p, groups, op, m, s)
+                       : launch_direct_k<T, 2, 1, false>(p, groups, op, m, s);
     }
-#undef DESV_DIRECT
-    return hipGetLastError();
 }
 
 size_t stage_lds_limit(int device) {
