@@ -166,6 +166,8 @@ def main():
     ap.add_argument("--low-bits", type=int, default=0)
     ap.add_argument("--per-gate", action="store_true")
     ap.add_argument("--mfma", action="store_true")
+    ap.add_argument("--in-place", action="store_true",
+                    help="streaming passes update the state in place (no second buffer)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--suite", default="plain20,vqe20,batch,precision",
                     help="extra workloads to measure after the headline ('' for none)")
@@ -202,6 +204,7 @@ def main():
     stream = torch.cuda.Stream(device=dev)
     h = desv.StateHandle(n, complex128=False, device=0)
     desv._capi.check(desv._capi.lib.desv_set_stream(h.ptr, desv._capi.C.c_void_p(stream.cuda_stream)))
+    h.set_double_buffer(not args.in_place)
 
     for i in range(args.warmup):
         h.reset()

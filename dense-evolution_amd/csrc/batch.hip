@@ -62,7 +62,7 @@ struct BInst {
 struct Launch {
     int kind;        // 0 fill, 1 stage, 2 matvec, 3 sum
     int nv = 1;
-    bool heavy = false;
+    int heavy = 0;  // stage kernel variant
     const void *tasks = nullptr;
     const int32_t *wg_task = nullptr;  // stage launches: task of each workgroup
     int ntasks = 0;
@@ -387,13 +387,13 @@ int build(desv_batch b) {
     auto stage_launches = [&](bool rev, std::vector<PendingLaunch> &out) {
         const int maxS = rev ? maxR : maxF;
         for (int k = 0; k < maxS; ++k)
-            for (int heavy = 0; heavy < 2; ++heavy) {
+            for (int heavy = 0; heavy < 3; ++heavy) {
                 std::vector<StageTask> ts;
                 int64_t wg = 0;
                 int max_m = 0;
                 for (auto &in : b->inst) {
                     const Plan &p = rev ? in.rp : in.fp;
-                    if ((int)p.stages.size() <= k || (p.stages[k].heavy != 0) != (heavy != 0)) continue;
+                    if ((int)p.stages.size() <= k || p.stages[k].heavy != heavy) continue;
                     StageTask t;
                     std::memset(&t, 0, sizeof(t));
                     t.st = p.stages[k];
@@ -413,7 +413,7 @@ int build(desv_batch b) {
                 Launch l;
                 l.kind = 1;
                 l.nv = rev ? 2 : 1;
-                l.heavy = heavy != 0;
+                l.heavy = heavy;
                 l.ntasks = (int)ts.size();
                 l.grid = wg;
                 l.max_m = max_m;

@@ -117,6 +117,10 @@ struct desv_state {
     size_t bytes = 0;
     void *sv = nullptr;
     void *aux[2] = {nullptr, nullptr};
+    // Second state buffer for out-of-place streaming stages (read sv, write
+    // alt, swap): allocated on first use unless double buffering is off.
+    void *alt = nullptr;
+    int double_buffer = 1;
     hipStream_t own = nullptr;
     hipStream_t stream = nullptr;
     double *d_part = nullptr;
@@ -196,8 +200,6 @@ int sum_to_host(desv_handle h, int nvals, int nblk, double *out) {
     return DESV_OK;
 }
 
-// Launch every stage of a plan on one state (one tile per workgroup).
-// Gradient probes write partials[slot * pstride + tile], pstride = 2^(n - m).
 // A stage that is exactly one dense 1-/2-qubit or diagonal (k <= 2) op with
 // no write-back permutation runs on the streaming direct kernel.
 bool direct_op(const Plan &plan, const DevStage &st, DirectOp &d) {
@@ -218,16 +220,41 @@ bool direct_op(const Plan &plan, const DevStage &st, DirectOp &d) {
     return true;
 }
 
+// The second buffer for out-of-place streaming stages, or nullptr when double
+// buffering is off or the allocation fails (then the stages run in place).
+void *ensure_alt(desv_handle h) {
+    if (!h->double_buffer) return nullptr;
+    if (!h->alt && hipMalloc(&h->alt, h->bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        h->alt = nullptr;
+        h->double_buffer = 0;
+    }
+    return h->alt;
+}
+
+// Launch every stage of a plan (one tile per workgroup for LDS stages).
+// Gradient probes write partials[slot * pstride + tile], pstride = 2^(n - m).
+// When sv0 is the handle's own state, a
+// streaming stage may run out of place into the second buffer and swap it
+// with the state (the handle's state pointer changes; callers re-read it).
 int run_plan_on(desv_handle h, const Plan &plan, const DevPlan &dp, int nv, void *sv0, void *sv1,
                 int64_t pstride) {
+    const bool own_state = nv == 1 && sv0 == h->sv;
     for (const DevStage &st : plan.stages) {
         DirectOp dop;
         if (nv == 1 && direct_op(plan, st, dop)) {
             const size_t eb = h->dtype == DESV_C64 ? 8 : 16;
             const void *M = reinterpret_cast<const char *>(dp.mats) + (size_t)plan.ops[st.op_begin].mat * eb;
-            hipError_t e = h->dtype == DESV_C64 ? launch_direct<float>(sv0, plan.n, dop, M, h->stream)
-                                                : launch_direct<double>(sv0, plan.n, dop, M, h->stream);
+            void *out = own_state ? ensure_alt(h) : nullptr;
+            if (!out) out = sv0;
+            hipError_t e = h->dtype == DESV_C64 ? launch_direct<float>(sv0, out, plan.n, dop, M, h->stream)
+                                                : launch_direct<double>(sv0, out, plan.n, dop, M, h->stream);
             if (e != hipSuccess) return fail(DESV_EDEVICE, std::string("direct launch: ") + hipGetErrorString(e));
+            if (out != sv0) {
+                h->alt = sv0;
+                h->sv = out;
+                sv0 = out;
+            }
             continue;
         }
         StageTask t;
@@ -242,8 +269,8 @@ int run_plan_on(desv_handle h, const Plan &plan, const DevPlan &dp, int nv, void
         t.pstride = pstride;
         t.wg_begin = 0;
         hipError_t e = h->dtype == DESV_C64
-                           ? launch_stage<float>(nv, st.heavy != 0, nullptr, nullptr, t, st.n_tiles, st.m, h->stream)
-                           : launch_stage<double>(nv, st.heavy != 0, nullptr, nullptr, t, st.n_tiles, st.m, h->stream);
+                           ? launch_stage<float>(nv, st.heavy, nullptr, nullptr, t, st.n_tiles, st.m, h->stream)
+                           : launch_stage<double>(nv, st.heavy, nullptr, nullptr, t, st.n_tiles, st.m, h->stream);
         if (e != hipSuccess) return fail(DESV_EDEVICE, std::string("stage launch: ") + hipGetErrorString(e));
     }
     return DESV_OK;
@@ -417,6 +444,7 @@ int desv_destroy(desv_handle h) {
     (void)hipSetDevice(h->device);
     (void)hipStreamSynchronize(h->stream);
     if (h->sv) (void)hipFree(h->sv);
+    if (h->alt) (void)hipFree(h->alt);
     for (void *a : h->aux)
         if (a) (void)hipFree(a);
     if (h->d_part) (void)hipFree(h->d_part);
@@ -432,6 +460,17 @@ int desv_set_stream(desv_handle h, void *stream) {
     if (int rc = check_handle(h)) return rc;
     HIP_TRY(hipStreamSynchronize(h->stream));
     h->stream = stream ? (hipStream_t)stream : h->own;
+    return DESV_OK;
+}
+
+int desv_set_double_buffer(desv_handle h, int enable) {
+    if (int rc = check_handle(h)) return rc;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->double_buffer = enable ? 1 : 0;
+    if (!enable && h->alt) {
+        HIP_TRY(hipFree(h->alt));
+        h->alt = nullptr;
+    }
     return DESV_OK;
 }
 

@@ -23,15 +23,20 @@ template <> __device__ inline double2 make_zero<double2>() { return make_double2
 
 namespace {
 
+// Complex products written as explicit FMA chains: a complex multiply-add is
+// four FMAs (the naive `acc += a*b - c*d` costs a multiply, an FMA and an add
+// per component).
+__device__ inline float fmadd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ inline double fmadd(double a, double b, double c) { return __builtin_fma(a, b, c); }
 template <typename C> __device__ inline C cmul(C a, C b) {
     C r;
-    r.x = a.x * b.x - a.y * b.y;
-    r.y = a.x * b.y + a.y * b.x;
+    r.x = fmadd(-a.y, b.y, a.x * b.x);
+    r.y = fmadd(a.y, b.x, a.x * b.y);
     return r;
 }
 template <typename C> __device__ inline C cfma(C a, C b, C acc) {
-    acc.x += a.x * b.x - a.y * b.y;
-    acc.y += a.x * b.y + a.y * b.x;
+    acc.x = fmadd(-a.y, b.y, fmadd(a.x, b.x, acc.x));
+    acc.y = fmadd(a.y, b.x, fmadd(a.x, b.y, acc.y));
     return acc;
 }
 template <typename C> __device__ inline double2 to_d(C a) { return make_double2((double)a.x, (double)a.y); }
@@ -58,114 +63,197 @@ __device__ inline double block_sum(double v, double *red) {
 __device__ inline int insert_zero(int i, int p) { return ((i >> p) << (p + 1)) | (i & ((1 << p) - 1)); }
 
 // ---------------------------------------------------------------------------
-// Tile micro-ops.  `tile[v]` are NV LDS tiles of 2^m amplitudes.
+// LDS tile layout.  Tile-local amplitude i lives at element slot(i) =
+// i ^ ((i >> 5) & 31): index bits 5..9 are folded onto bits 0..4, so a wave
+// whose lanes step through the tile with any stride 2^s, s <= 5, touches
+// distinct banks.  slot() is linear over XOR (slot(a ^ b) = slot(a) ^ slot(b))
+// and an involution.  Every address below is built from disjoint bit fields,
+// so its slot is the XOR of the fields' slots: an op computes one per-lane
+// slot and combines it with wave-uniform slots (one VALU XOR per access).
+// Offsets are in bytes (slot * sizeof(amplitude)), which is also linear.
 // ---------------------------------------------------------------------------
-template <typename T, int NV>
-__device__ inline void op_dense1(typename VT<T>::c **tile, int m, const DevOp &op,
-                                 const typename VT<T>::c *M) {
-    using C = typename VT<T>::c;
-    const C m00 = M[0], m01 = M[1], m10 = M[2], m11 = M[3];
-    const int t = op.pos[0];
-    const int half = 1 << (m - 1);
-    const int cl = (int)op.cloc;
-    for (int p = threadIdx.x; p < half; p += blockDim.x) {
-        const int i0 = insert_zero(p, t), i1 = i0 | (1 << t);
-        if ((i0 & cl) != cl) continue;
+__device__ inline int slot(int i) { return i ^ ((i >> 5) & 31); }
+
+// Scalar (wave-uniform) copies of values that every lane holds equal.
+__device__ inline int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ inline uint64_t uni64(uint64_t x) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)x);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(x >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+template <typename P> __device__ inline P *uni_ptr(P *p) {
+    return reinterpret_cast<P *>(uni64(reinterpret_cast<uint64_t>(p)));
+}
+// Global-address-space view of a wave-uniform pointer: loads through it are
+// global (not flat) loads, and scalar loads when the address is uniform.
+#define DESV_GLOBAL __attribute__((address_space(1)))
+template <typename P> __device__ inline const DESV_GLOBAL P *gptr(const P *p) {
+    return (const DESV_GLOBAL P *)uni_ptr(p);
+}
+template <typename P> __device__ inline DESV_GLOBAL P *gptr_rw(P *p) { return (DESV_GLOBAL P *)uni_ptr(p); }
+// Loads/stores of HIP vector types through global pointers, as plain vectors.
+template <typename X> struct RawOf;
+template <> struct RawOf<float2> { using t = float __attribute__((ext_vector_type(2))); };
+template <> struct RawOf<float4> { using t = float __attribute__((ext_vector_type(4))); };
+template <> struct RawOf<double2> { using t = double __attribute__((ext_vector_type(2))); };
+template <typename X> __device__ inline X gload(const DESV_GLOBAL X *p) {
+    using R = typename RawOf<X>::t;
+    const R r = *(const DESV_GLOBAL R *)p;
+    X x;
+    __builtin_memcpy(&x, &r, sizeof(X));
+    return x;
+}
+template <typename X> __device__ inline void gstore(DESV_GLOBAL X *p, const X &x) {
+    using R = typename RawOf<X>::t;
+    R r;
+    __builtin_memcpy(&r, &x, sizeof(X));
+    *(DESV_GLOBAL R *)p = r;
+}
+__device__ inline DevOp gload_op(const DESV_GLOBAL DevOp *p) {
+    constexpr int W = (int)(sizeof(DevOp) / 4);
+    const DESV_GLOBAL uint32_t *w = (const DESV_GLOBAL uint32_t *)p;
+    uint32_t buf[W];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            C a0 = tile[v][i0], a1 = tile[v][i1];
-            tile[v][i0] = cfma(m01, a1, cmul(m00, a0));
-            tile[v][i1] = cfma(m11, a1, cmul(m10, a0));
-        }
-    }
+    for (int i = 0; i < W; ++i) buf[i] = w[i];
+    DevOp op;
+    __builtin_memcpy(&op, buf, sizeof(DevOp));
+    return op;
 }
 
-template <typename T, int NV>
-__device__ inline void op_dense2(typename VT<T>::c **tile, int m, const DevOp &op,
-                                 const typename VT<T>::c *M) {
-    using C = typename VT<T>::c;
-    C mm[16];
-#pragma unroll
-    for (int i = 0; i < 16; ++i) mm[i] = M[i];
-    const int p0 = op.pos[0], p1 = op.pos[1];
-    const int lo = p0 < p1 ? p0 : p1, hi = p0 < p1 ? p1 : p0;
-    const int quarter = 1 << (m - 2);
-    const int cl = (int)op.cloc;
-    for (int q = threadIdx.x; q < quarter; q += blockDim.x) {
-        const int b = insert_zero(insert_zero(q, lo), hi);
-        if ((b & cl) != cl) continue;
-        const int idx[4] = {b, b | (1 << p0), b | (1 << p1), b | (1 << p0) | (1 << p1)};
-#pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            C a[4];
-#pragma unroll
-            for (int s = 0; s < 4; ++s) a[s] = tile[v][idx[s]];
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                C acc = cmul(mm[r * 4 + 0], a[0]);
-                acc = cfma(mm[r * 4 + 1], a[1], acc);
-                acc = cfma(mm[r * 4 + 2], a[2], acc);
-                acc = cfma(mm[r * 4 + 3], a[3], acc);
-                tile[v][idx[r]] = acc;
-            }
-        }
-    }
+// Deposit the bits of x, lowest first, into the bit positions not in `fixed`.
+__device__ inline int spread(int x, int fixed) {
+    for (int f = fixed; f; f &= f - 1) x = insert_zero(x, __builtin_ctz(f));
+    return x;
 }
 
-template <typename T, int NV, int K>
-__device__ inline void op_densek(typename VT<T>::c **tile, int m, const DevOp &op,
-                                 const typename VT<T>::c *M) {
+template <typename C> __device__ inline C lds_ld(const char *p, int off) {
+    return *reinterpret_cast<const C *>(p + off);
+}
+template <typename C> __device__ inline void lds_st(char *p, int off, C v) { *reinterpret_cast<C *>(p + off) = v; }
+
+// Work split of an op whose group enumerates the tile bits not in `fixed`:
+// lane `tid` owns groups tid + 256 j, j < J; lanes with tid >= G idle.
+struct Groups {
+    int fixed;  // target and control bits
+    int G;      // number of groups
+    int J;      // groups per lane
+    int P;      // byte slot of this lane's first group
+    __device__ Groups(int m, int fixed_, int eb) : fixed(fixed_) {
+        G = 1 << (m - __builtin_popcount(fixed_));
+        J = G > kThreads ? G / kThreads : 1;
+        P = slot(spread((int)threadIdx.x, fixed_)) * eb;
+    }
+    __device__ bool idle() const { return (int)threadIdx.x >= G; }
+    // byte slot of the uniform part of group tid + 256 j
+    __device__ int U(int j, int eb) const { return slot(spread(j * kThreads, fixed)) * eb; }
+};
+
+// ---------------------------------------------------------------------------
+// Tile micro-ops.  `lds` = tile 0; tile v is at lds + v * vstride.
+// ---------------------------------------------------------------------------
+
+// Dense 2^K x 2^K matrix, K = 1, 2, on tile bits pos[] with local controls.
+template <typename T, int NV, int K, typename MP>
+__device__ inline void op_dense_lo(char *lds, int vstride, int m, const DevOp &op, MP M) {
     using C = typename VT<T>::c;
     constexpr int D = 1 << K;
-    int srt[K];
+    constexpr int EB = sizeof(C);
+    const int cl = (int)op.cloc;
+    int fixed = cl, so[D];
 #pragma unroll
-    for (int j = 0; j < K; ++j) srt[j] = op.pos[j];
-    // sort ascending for zero insertion
-#pragma unroll
-    for (int a = 0; a < K; ++a)
-#pragma unroll
-        for (int b = 0; b < K - 1; ++b)
-            if (srt[b] > srt[b + 1]) { int t = srt[b]; srt[b] = srt[b + 1]; srt[b + 1] = t; }
-    int off[D];
+    for (int j = 0; j < K; ++j) fixed |= 1 << op.pos[j];
 #pragma unroll
     for (int s = 0; s < D; ++s) {
-        int o = 0;
+        int o = cl;
 #pragma unroll
         for (int j = 0; j < K; ++j)
             if (s >> j & 1) o |= 1 << op.pos[j];
-        off[s] = o;
+        so[s] = slot(o) * EB;
     }
-    const int groups = 1 << (m - K);
+    const Groups g(m, fixed, EB);
+    if (g.idle()) return;
+    C mm[D * D];
+#pragma unroll
+    for (int i = 0; i < D * D; ++i) mm[i] = gload(M + i);
+    auto row = [&](const C *a, int r) {
+        C acc = cmul(mm[r * D], a[0]);
+#pragma unroll
+        for (int c = 1; c < D; ++c) acc = cfma(mm[r * D + c], a[c], acc);
+        return acc;
+    };
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        char *tb = lds + v * vstride;
+        int j = 0;
+        for (; j + 2 <= g.J; j += 2) {
+            const int A0 = g.P ^ g.U(j, EB), A1 = g.P ^ g.U(j + 1, EB);
+            C a0[D], a1[D];
+#pragma unroll
+            for (int s = 0; s < D; ++s) {
+                a0[s] = lds_ld<C>(tb, A0 ^ so[s]);
+                a1[s] = lds_ld<C>(tb, A1 ^ so[s]);
+            }
+#pragma unroll
+            for (int r = 0; r < D; ++r) {
+                lds_st<C>(tb, A0 ^ so[r], row(a0, r));
+                lds_st<C>(tb, A1 ^ so[r], row(a1, r));
+            }
+        }
+        if (j < g.J) {
+            const int A0 = g.P ^ g.U(j, EB);
+            C a0[D];
+#pragma unroll
+            for (int s = 0; s < D; ++s) a0[s] = lds_ld<C>(tb, A0 ^ so[s]);
+#pragma unroll
+            for (int r = 0; r < D; ++r) lds_st<C>(tb, A0 ^ so[r], row(a0, r));
+        }
+    }
+}
+
+// Dense 2^K x 2^K block on VALU (K = 3..5): one group's amplitudes in
+// registers, rows formed one at a time from the (scalar) matrix row.
+template <typename T, int NV, int K, typename MP>
+__device__ inline void op_densek(char *lds, int vstride, int m, const DevOp &op, MP M) {
+    using C = typename VT<T>::c;
+    constexpr int D = 1 << K;
+    constexpr int EB = sizeof(C);
     const int cl = (int)op.cloc;
-    for (int g = threadIdx.x; g < groups; g += blockDim.x) {
-        int b = g;
+    int fixed = cl;
 #pragma unroll
-        for (int j = 0; j < K; ++j) b = insert_zero(b, srt[j]);
-        if ((b & cl) != cl) continue;
+    for (int j = 0; j < K; ++j) fixed |= 1 << op.pos[j];
+    // row/column r of the matrix <-> tile offset cl | sum_j bit_j(r) << pos[j]
+    auto roff = [&](int r) {
+        int o = cl;
 #pragma unroll
-        for (int v = 0; v < NV; ++v) {
+        for (int j = 0; j < K; ++j)
+            if (r >> j & 1) o |= 1 << op.pos[j];
+        return slot(o) * EB;
+    };
+    const Groups g(m, fixed, EB);
+    if (g.idle()) return;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        char *tb = lds + v * vstride;
+        for (int j = 0; j < g.J; ++j) {
+            const int A = g.P ^ g.U(j, EB);
             C a[D];
 #pragma unroll
-            for (int s = 0; s < D; ++s) a[s] = tile[v][b | off[s]];
-            // rows are not unrolled (keeps code size and compile time small);
-            // the row offset is rebuilt from r instead of indexing off[].
+            for (int s = 0; s < D; ++s) a[s] = lds_ld<C>(tb, A ^ roff(s));
 #pragma unroll 1
             for (int r = 0; r < D; ++r) {
-                C acc = make_zero<C>();
+                const MP Mr = M + r * D;
+                C acc = cmul(gload(Mr), a[0]);
 #pragma unroll
-                for (int c = 0; c < D; ++c) acc = cfma(M[r * D + c], a[c], acc);
-                int o = 0;
-#pragma unroll
-                for (int j = 0; j < K; ++j)
-                    if (r >> j & 1) o |= 1 << op.pos[j];
-                tile[v][b | o] = acc;
+                for (int c = 1; c < D; ++c) acc = cfma(gload(Mr + c), a[c], acc);
+                lds_st<C>(tb, A ^ roff(r), acc);
             }
         }
     }
 }
 
 // ---------------------------------------------------------------------------
-// Fused k-qubit block (k = 4, 5) as a real-split complex GEMM on MFMA.
+// Fused k-qubit block (k = 4, 5) as a real-split complex GEMM on MFMA
+// (complex64 only).
 //
 // The tile is viewed as X[r][g]: r = block-local index over the k target
 // bits, g = one of G = 2^(m-k) groups (the other tile bits).  Y = U X with
@@ -178,171 +266,162 @@ __device__ inline void op_densek(typename VT<T>::c **tile, int m, const DevOp &o
 // disjoint columns, so the update is in place.
 // ---------------------------------------------------------------------------
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
-typedef double f64x4_t __attribute__((ext_vector_type(4)));
 
-template <typename T> struct Mfma;
-template <> struct Mfma<float> {
-    using acc = f32x4_t;
-    static __device__ inline acc op(float a, float b, acc c) {
-        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-    }
-};
-template <> struct Mfma<double> {
-    using acc = f64x4_t;
-    static __device__ inline acc op(double a, double b, acc c) {
-        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-    }
-};
-
-template <typename T, int NV, int K>
-__device__ inline void op_mfma(typename VT<T>::c **tile, int m, const DevOp &op, const typename VT<T>::c *M) {
-    using C = typename VT<T>::c;
-    using Acc = typename Mfma<T>::acc;
+template <int NV, int K, typename MP>
+__device__ inline void op_mfma(char *lds, int vstride, int m, const DevOp &op, MP M) {
     constexpr int D = 1 << K;
     constexpr int RB = D / 16;  // 16-row blocks of U
     constexpr int KS = D / 4;   // 4-wide k slices
+    constexpr int EB = sizeof(float2);
     const int lane = threadIdx.x & 63;
     const int i16 = lane & 15, k4 = lane >> 4;
-    int pos[K], srt[K];
+    const int cl = (int)op.cloc;
+    int fixed = cl;
 #pragma unroll
-    for (int j = 0; j < K; ++j) pos[j] = srt[j] = op.pos[j];
-#pragma unroll
-    for (int a = 0; a < K; ++a)
-#pragma unroll
-        for (int b = 0; b < K - 1; ++b)
-            if (srt[b] > srt[b + 1]) { int t = srt[b]; srt[b] = srt[b + 1]; srt[b + 1] = t; }
-    auto deposit = [&](int r) {
+    for (int j = 0; j < K; ++j) fixed |= 1 << op.pos[j];
+    auto roff = [&](int r) {
         int o = 0;
 #pragma unroll
-        for (int j = 0; j < K; ++j) o |= ((r >> j) & 1) << pos[j];
+        for (int j = 0; j < K; ++j)
+            if (r >> j & 1) o |= 1 << op.pos[j];
         return o;
     };
     // A fragments (constant over the tile)
-    T ar[RB][KS], ai[RB][KS];
+    float ar[RB][KS], ai[RB][KS];
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks) {
-            const C u = M[(rb * 16 + i16) * D + ks * 4 + k4];
+            const float2 u = gload(M + (rb * 16 + i16) * D + ks * 4 + k4);
             ar[rb][ks] = u.x;
             ai[rb][ks] = u.y;
         }
-    int boff[KS];
+    // per-lane byte slots: this lane's column (within a 16-column block) and
+    // the rows it reads (B) and writes (accumulator)
+    const int pl = spread(i16, fixed) | cl;
+    int pb[KS], py[RB][4];
 #pragma unroll
-    for (int ks = 0; ks < KS; ++ks) boff[ks] = deposit(ks * 4 + k4);
-    int yoff[RB][4];
+    for (int ks = 0; ks < KS; ++ks) pb[ks] = slot(pl | roff(ks * 4 + k4)) * EB;
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) yoff[rb][j] = deposit(rb * 16 + 4 * k4 + j);
+        for (int j = 0; j < 4; ++j) py[rb][j] = slot(pl | roff(rb * 16 + 4 * k4 + j)) * EB;
 
-    const int G = 1 << (m - K);
+    const int G = 1 << (m - __builtin_popcount(fixed));
+    const bool valid = i16 < G;  // tiles narrower than 16 columns
     const int waves = blockDim.x >> 6;
     for (int c0 = (threadIdx.x >> 6) * 16; c0 < G; c0 += waves * 16) {
-        const int col = c0 + i16;
-        const bool valid = col < G;
-        int base = valid ? col : 0;
-#pragma unroll
-        for (int j = 0; j < K; ++j) base = insert_zero(base, srt[j]);
+        const int U = slot(spread(c0, fixed)) * EB;
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            T br[KS], bi[KS];
+            char *tb = lds + v * vstride;
+            float br[KS], bi[KS];
 #pragma unroll
             for (int ks = 0; ks < KS; ++ks) {
-                C x = tile[v][base | boff[ks]];
-                br[ks] = valid ? x.x : T(0);
-                bi[ks] = valid ? x.y : T(0);
+                const float2 x = valid ? lds_ld<float2>(tb, pb[ks] ^ U) : make_float2(0.f, 0.f);
+                br[ks] = x.x;
+                bi[ks] = x.y;
             }
-            Acc yr[RB], yi[RB];
+            f32x4_t yr[RB], yi[RB];
 #pragma unroll
             for (int rb = 0; rb < RB; ++rb) {
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    yr[rb][j] = T(0);
-                    yi[rb][j] = T(0);
-                }
+                yr[rb] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+                yi[rb] = f32x4_t{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
                 for (int ks = 0; ks < KS; ++ks) {
-                    yr[rb] = Mfma<T>::op(ar[rb][ks], br[ks], yr[rb]);
-                    yr[rb] = Mfma<T>::op(ai[rb][ks], -bi[ks], yr[rb]);
-                    yi[rb] = Mfma<T>::op(ar[rb][ks], bi[ks], yi[rb]);
-                    yi[rb] = Mfma<T>::op(ai[rb][ks], br[ks], yi[rb]);
+                    yr[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[rb][ks], br[ks], yr[rb], 0, 0, 0);
+                    yr[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[rb][ks], -bi[ks], yr[rb], 0, 0, 0);
+                    yi[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[rb][ks], bi[ks], yi[rb], 0, 0, 0);
+                    yi[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[rb][ks], br[ks], yi[rb], 0, 0, 0);
                 }
             }
             if (valid) {
 #pragma unroll
                 for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        C y;
-                        y.x = yr[rb][j];
-                        y.y = yi[rb][j];
-                        tile[v][base | yoff[rb][j]] = y;
-                    }
+                    for (int j = 0; j < 4; ++j) lds_st<float2>(tb, py[rb][j] ^ U, make_float2(yr[rb][j], yi[rb][j]));
             }
         }
     }
 }
 
-template <typename T, int NV>
-__device__ inline void op_diag(typename VT<T>::c **tile, int m, const DevOp &op,
-                               const typename VT<T>::c *Dg, uint64_t base) {
+// Diagonal op: matrix-index bit j is tile bit pos[j] (pos >= 0) or global bit
+// -1-pos[j] outside the tile (constant over the tile, taken from `base`).
+// Groups enumerate the tile bits that are neither local targets nor
+// controls; each group holds 2^nl amplitudes with wave-uniform factors.
+template <typename T, int NV, typename MP>
+__device__ inline void op_diag(char *lds, int vstride, int m, const DevOp &op, MP Dg, uint64_t base) {
     using C = typename VT<T>::c;
-    // Out-of-tile bits are constant over the tile; in-tile bit j of the
-    // diagonal index comes from tile bit p[j] (p[j] = 31 marks "not local", a
-    // bit that is always 0 for tile indices).  Loops are fully unrolled over
-    // the 5 possible bits so nothing is indexed at run time (no scratch).
-    int gsub = 0;
-    int p[5];
+    constexpr int EB = sizeof(C);
+    int gsub = 0, gmask = 0, lmask = 0;
+    // fully unrolled: no run-time indexing of op.pos (keeps it out of scratch)
 #pragma unroll
     for (int j = 0; j < 5; ++j) {
-        p[j] = 31;
-        if (j < op.k) {
-            const int pj = op.pos[j];
-            if (pj < 0) {
-                if (base >> (-1 - pj) & 1) gsub |= 1 << j;
-            } else {
-                p[j] = pj;
+        if (j >= op.k) break;
+        const int pj = op.pos[j];
+        if (pj < 0) {
+            gmask |= 1 << j;
+            if (base >> (-1 - pj) & 1) gsub |= 1 << j;
+        } else {
+            lmask |= 1 << pj;
+        }
+    }
+    const int cl = (int)op.cloc;
+    const Groups g(m, lmask | cl, EB);
+    if (g.idle()) return;
+    const int dim = 1 << op.k;
+    for (int j = 0; j < g.J; ++j) {
+        const int A = g.P ^ g.U(j, EB);
+        // every matrix index d whose out-of-tile bits match this tile
+        for (int d = 0; d < dim; ++d) {
+            if ((d & gmask) != gsub) continue;
+            int o = cl;
+#pragma unroll
+            for (int q = 0; q < 5; ++q)
+                if (q < op.k && op.pos[q] >= 0 && (d >> q & 1)) o |= 1 << op.pos[q];
+            o = slot(o) * EB;
+            const C f = gload(Dg + d);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                char *tb = lds + v * vstride;
+                lds_st<C>(tb, A ^ o, cmul(f, lds_ld<C>(tb, A ^ o)));
             }
         }
     }
-    const int size = 1 << m;
-    const int cl = (int)op.cloc;
-    for (int i = threadIdx.x; i < size; i += blockDim.x) {
-        if ((i & cl) != cl) continue;
-        int d = gsub;
-#pragma unroll
-        for (int j = 0; j < 5; ++j) d |= ((i >> p[j]) & 1) << j;
-        const C f = Dg[d];
-#pragma unroll
-        for (int v = 0; v < NV; ++v) tile[v][i] = cmul(f, tile[v][i]);
-    }
 }
 
-template <typename T, int NV>
-__device__ inline void op_scalar(typename VT<T>::c **tile, int m, const typename VT<T>::c *S) {
+// Global phase on the whole tile (order of elements irrelevant).
+template <typename T, int NV, typename MP>
+__device__ inline void op_scalar(char *lds, int vstride, int m, MP S) {
     using C = typename VT<T>::c;
-    const C f = S[0];
+    const C f = gload(S);
     const int size = 1 << m;
     for (int i = threadIdx.x; i < size; i += blockDim.x)
 #pragma unroll
-        for (int v = 0; v < NV; ++v) tile[v][i] = cmul(f, tile[v][i]);
+        for (int v = 0; v < NV; ++v) {
+            C *tb = reinterpret_cast<C *>(lds + v * vstride);
+            tb[i] = cmul(f, tb[i]);
+        }
 }
 
-// Re <lam| M |psi> over the tile; tile[0] = psi, tile[1] = lam.
-template <typename T>
-__device__ inline double op_grad_dense(typename VT<T>::c **tile, int m, const DevOp &op,
-                                       const typename VT<T>::c *M) {
-    const double2 m00 = to_d(M[0]), m01 = to_d(M[1]), m10 = to_d(M[2]), m11 = to_d(M[3]);
-    const int t = op.pos[0];
-    const int half = 1 << (m - 1);
-    const int cl = (int)op.cloc;
+// Re <lam| M |psi> over the tile (tile 0 = psi, tile 1 = lam); M is a dense
+// 2x2 generator on tile bit pos[0] with local controls.
+template <typename T, typename MP>
+__device__ inline double op_grad_dense(char *lds, int vstride, int m, const DevOp &op, MP M) {
+    using C = typename VT<T>::c;
+    constexpr int EB = sizeof(C);
+    const double2 m00 = to_d(gload(M)), m01 = to_d(gload(M + 1)), m10 = to_d(gload(M + 2)),
+                  m11 = to_d(gload(M + 3));
+    const int cl = (int)op.cloc, tb = 1 << op.pos[0];
+    const int o0 = slot(cl) * EB, o1 = slot(cl | tb) * EB;
+    const Groups g(m, cl | tb, EB);
+    if (g.idle()) return 0.0;
+    const char *ls = lds + vstride;
     double acc = 0;
-    for (int p = threadIdx.x; p < half; p += blockDim.x) {
-        const int i0 = insert_zero(p, t), i1 = i0 | (1 << t);
-        if ((i0 & cl) != cl) continue;
-        const double2 s0 = to_d(tile[0][i0]), s1 = to_d(tile[0][i1]);
-        const double2 l0 = to_d(tile[1][i0]), l1 = to_d(tile[1][i1]);
+    for (int j = 0; j < g.J; ++j) {
+        const int A = g.P ^ g.U(j, EB);
+        const double2 s0 = to_d(lds_ld<C>(lds, A ^ o0)), s1 = to_d(lds_ld<C>(lds, A ^ o1));
+        const double2 l0 = to_d(lds_ld<C>(ls, A ^ o0)), l1 = to_d(lds_ld<C>(ls, A ^ o1));
         const double2 g0 = cfma(m01, s1, cmul(m00, s0));
         const double2 g1 = cfma(m11, s1, cmul(m10, s0));
         acc += l0.x * g0.x + l0.y * g0.y + l1.x * g1.x + l1.y * g1.y;
@@ -350,19 +429,24 @@ __device__ inline double op_grad_dense(typename VT<T>::c **tile, int m, const De
     return acc;
 }
 
-template <typename T>
-__device__ inline double op_grad_diag(typename VT<T>::c **tile, int m, const DevOp &op,
-                                      const typename VT<T>::c *Dg) {
-    const double2 d0 = to_d(Dg[0]), d1 = to_d(Dg[1]);
-    const int t = op.pos[0];
-    const int size = 1 << m;
-    const int cl = (int)op.cloc;
+// Same for a diagonal generator diag(d0, d1) on tile bit pos[0].
+template <typename T, typename MP>
+__device__ inline double op_grad_diag(char *lds, int vstride, int m, const DevOp &op, MP Dg) {
+    using C = typename VT<T>::c;
+    constexpr int EB = sizeof(C);
+    const double2 d0 = to_d(gload(Dg)), d1 = to_d(gload(Dg + 1));
+    const int cl = (int)op.cloc, tb = 1 << op.pos[0];
+    const int o0 = slot(cl) * EB, o1 = slot(cl | tb) * EB;
+    const Groups g(m, cl | tb, EB);
+    if (g.idle()) return 0.0;
+    const char *ls = lds + vstride;
     double acc = 0;
-    for (int i = threadIdx.x; i < size; i += blockDim.x) {
-        if ((i & cl) != cl) continue;
-        const double2 s = to_d(tile[0][i]), l = to_d(tile[1][i]);
-        const double2 g = cmul((i >> t & 1) ? d1 : d0, s);
-        acc += l.x * g.x + l.y * g.y;
+    for (int j = 0; j < g.J; ++j) {
+        const int A = g.P ^ g.U(j, EB);
+        const double2 g0 = cmul(d0, to_d(lds_ld<C>(lds, A ^ o0)));
+        const double2 g1 = cmul(d1, to_d(lds_ld<C>(lds, A ^ o1)));
+        const double2 l0 = to_d(lds_ld<C>(ls, A ^ o0)), l1 = to_d(lds_ld<C>(ls, A ^ o1));
+        acc += l0.x * g0.x + l0.y * g0.y + l1.x * g1.x + l1.y * g1.y;
     }
     return acc;
 }
@@ -372,19 +456,22 @@ __device__ inline double op_grad_diag(typename VT<T>::c **tile, int m, const Dev
 // ---------------------------------------------------------------------------
 // Stage kernel
 // ---------------------------------------------------------------------------
-// HEAVY = false compiles only the 1-2 qubit, diagonal, scalar and gradient
-// micro-ops (low register count, high occupancy for streaming stages);
-// HEAVY = true adds the k = 3..5 VALU blocks and the MFMA blocks.
-template <typename T, int NV, bool HEAVY>
+// VAR selects the compiled micro-ops (register budget and occupancy):
+//   0  1-2 qubit dense, diagonal, scalar and gradient ops;
+//   1  + k = 3..5 VALU blocks;
+//   2  + k = 3 VALU blocks and k = 4, 5 MFMA blocks (complex64).
+template <typename T, int NV, int VAR>
 __global__ void __launch_bounds__(kThreads) k_stage(const StageTask *__restrict__ tasks,
                                                     const int32_t *__restrict__ wg_task, StageTask single) {
     using C = typename VT<T>::c;
     using V = typename VT<T>::v;
     constexpr int PV = VT<T>::per_vec;
+    constexpr int EB = sizeof(C);
+    constexpr int CH = kThreads * PV;  // amplitudes per load/store chunk
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ double gacc[kMaxGradLDS * (kThreads / 64)];
-    __shared__ int perm_lo[256], perm_hi[256];
-    __shared__ uint64_t hoff[256];
+    __shared__ uint64_t jtab[256];  // global offset of chunk j
+    __shared__ int qtab[256];       // byte slot of the tile index stored at chunk j
     __shared__ StageTask tk;
 
     {
@@ -395,47 +482,46 @@ __global__ void __launch_bounds__(kThreads) k_stage(const StageTask *__restrict_
         for (int i = threadIdx.x; i < W; i += blockDim.x) reinterpret_cast<uint32_t *>(&tk)[i] = src[i];
     }
     __syncthreads();
-    const int64_t t = (int64_t)blockIdx.x - tk.wg_begin;
-    if (t < 0 || t >= tk.st.n_tiles) return;  // uniform over the workgroup
+    const int64_t t = (int64_t)blockIdx.x - (int64_t)uni64((uint64_t)tk.wg_begin);
+    if (t < 0 || t >= (int64_t)uni64((uint64_t)tk.st.n_tiles)) return;  // uniform over the workgroup
 
-    const int m = tk.st.m;
+    const int m = uni(tk.st.m), L = uni(tk.st.L);
     const int size = 1 << m;
-    C *tile[2];
-    tile[0] = reinterpret_cast<C *>(smem);
-    tile[1] = tile[0] + size;
-    C *svp[2] = {reinterpret_cast<C *>(tk.sv0), reinterpret_cast<C *>(tk.sv1)};
-    const int L = tk.st.L, nh = tk.st.nh;
-    const uint64_t lowm = (1ull << L) - 1;
-    const uint64_t all = (tk.n >= 64) ? ~0ull : ((1ull << tk.n) - 1);
-    const uint64_t free_mask = ~tk.st.tile_mask & all;
-    const C *mats = reinterpret_cast<const C *>(tk.mats);
-    const DevOp *ops = tk.ops;
+    char *lds = reinterpret_cast<char *>(smem);
+    const int vstride = size * EB;
+    DESV_GLOBAL C *svp[2] = {gptr_rw(reinterpret_cast<C *>(tk.sv0)), gptr_rw(reinterpret_cast<C *>(tk.sv1))};
+    const uint64_t tile_mask = uni64(tk.st.tile_mask);
+    const int n = uni(tk.n);
+    const uint64_t all = (n >= 64) ? ~0ull : ((1ull << n) - 1);
+    const DESV_GLOBAL C *mats = gptr(reinterpret_cast<const C *>(tk.mats));
+    const DESV_GLOBAL DevOp *ops = gptr(tk.ops);
+    const bool permuted = uni(tk.st.permuted) != 0;
+    const int n_grad = uni(tk.st.n_grad);
 
-    for (int i = threadIdx.x; i < tk.st.n_grad * (kThreads / 64); i += blockDim.x) gacc[i] = 0.0;
-    for (int i = threadIdx.x; i < (1 << tk.st.nh) && tk.st.nh <= 8; i += blockDim.x) {
-        uint64_t o = 0;
-        for (int j = 0; j < tk.st.nh; ++j)
-            if (i >> j & 1) o |= 1ull << tk.st.hbits[j];
-        hoff[i] = o;
+    // global offset of tile-local index e (bits < L stay, bit L+j -> hbits[j])
+    auto goff = [&](int e) -> uint64_t {
+        uint64_t o = (uint64_t)(e & ((1 << L) - 1));
+        for (int r = e >> L, j = 0; r; r >>= 1, ++j)
+            if (r & 1) o |= 1ull << tk.st.hbits[j];
+        return o;
+    };
+    // tile-local index stored at store-order index e (write-back permutation)
+    auto perm = [&](int e) -> int {
+        int o = 0;
+        for (int j = 0; e; e >>= 1, ++j)
+            if (e & 1) o |= 1 << tk.st.sperm[j];
+        return o;
+    };
+    const int nchunks = size >= CH ? size / CH : 1;
+    for (int i = threadIdx.x; i < n_grad * (kThreads / 64); i += blockDim.x) gacc[i] = 0.0;
+    for (int j = threadIdx.x; j < nchunks; j += blockDim.x) {
+        jtab[j] = goff(j * CH);
+        qtab[j] = slot(permuted ? perm(j * CH) : j * CH) * EB;
     }
-    const bool permuted = tk.st.permuted != 0;
-    if (permuted) {
-        // local index of store-order index e = perm_lo[e & 255] | perm_hi[e >> 8]
-        const int mlo = m < 8 ? m : 8;
-        for (int i = threadIdx.x; i < 256; i += blockDim.x) {
-            int lo = 0, hi = 0;
-            for (int j = 0; j < mlo; ++j)
-                if (i >> j & 1) lo |= 1 << tk.st.sperm[j];
-            for (int j = 8; j < m; ++j)
-                if (i >> (j - 8) & 1) hi |= 1 << tk.st.sperm[j];
-            perm_lo[i] = lo;
-            perm_hi[i] = hi;
-        }
-    }
-
+    // tile base: the tile number's bits deposited into the non-tile bits
     uint64_t base = 0;
     {
-        uint64_t tt = (uint64_t)t, fm = free_mask;
+        uint64_t tt = (uint64_t)t, fm = ~tile_mask & all;
         while (fm) {
             const uint64_t lb = fm & (~fm + 1);
             if (tt & 1) base |= lb;
@@ -443,56 +529,64 @@ __global__ void __launch_bounds__(kThreads) k_stage(const StageTask *__restrict_
             fm ^= lb;
         }
     }
-    __syncthreads();  // hoff / perm tables ready
-    const bool small_nh = nh <= 8;
-    auto seg_off = [&](uint64_t s) -> uint64_t {
-        if (small_nh) return hoff[s];
-        uint64_t o = 0;
-        for (int j = 0; j < nh; ++j)
-            if (s >> j & 1) o |= 1ull << tk.st.hbits[j];
-        return o;
-    };
-    // load
-    for (int e = threadIdx.x * PV; e < size; e += blockDim.x * PV) {
-        const uint64_t s = (uint64_t)e >> L;
-        const uint64_t off = base | ((uint64_t)e & lowm) | seg_off(s);
+    const int le = (int)threadIdx.x * PV;  // this lane's first amplitude of a chunk
+    const bool lane_on = le < size;
+    const uint64_t gl = lane_on ? base | goff(le) : 0;
+    const int sl = slot(le) * EB;
+    int ql[PV];
 #pragma unroll
-        for (int v = 0; v < NV; ++v)
-            *reinterpret_cast<V *>(&tile[v][e]) = *reinterpret_cast<const V *>(&svp[v][off]);
-    }
+    for (int i = 0; i < PV; ++i) ql[i] = slot(permuted ? perm(le + i) : le + i) * EB;
+    __syncthreads();  // jtab / qtab / gacc ready
+
+    // load: chunk j covers amplitudes j*CH .. j*CH + CH - 1 of the tile
+    if (lane_on)
+        for (int j = 0; j < nchunks; ++j) {
+            const uint64_t off = gl | jtab[j];
+            const int S = slot(j * CH) * EB;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                const V x = gload((const DESV_GLOBAL V *)(svp[v] + off));
+                const C *xc = reinterpret_cast<const C *>(&x);
+#pragma unroll
+                for (int i = 0; i < PV; ++i) lds_st<C>(lds + v * vstride, (sl ^ S) ^ (i * EB), xc[i]);
+            }
+        }
     __syncthreads();
     int gi = 0;
-    for (int o = tk.st.op_begin; o < tk.st.op_end; ++o) {
-        const DevOp op = ops[o];
+    const int ob = uni(tk.st.op_begin), oe = uni(tk.st.op_end);
+    for (int o = ob; o < oe; ++o) {
+        const DevOp op = gload_op(ops + o);
         const bool is_grad = op.kind == OPK_GRAD || op.kind == OPK_GRADDIAG;
         if ((base & op.cglob) != op.cglob) {
             gi += is_grad;
             continue;
         }
-        const C *M = mats + op.mat;
+        const DESV_GLOBAL C *M = mats + op.mat;
         switch (op.kind) {
-            case OPK_DENSE1: op_dense1<T, NV>(tile, m, op, M); break;
-            case OPK_DENSE2: op_dense2<T, NV>(tile, m, op, M); break;
+            case OPK_DENSE1: op_dense_lo<T, NV, 1>(lds, vstride, m, op, M); break;
+            case OPK_DENSE2: op_dense_lo<T, NV, 2>(lds, vstride, m, op, M); break;
             case OPK_DENSEK:
-                if constexpr (HEAVY) {
-                    if (op.k == 3) op_densek<T, NV, 3>(tile, m, op, M);
-                    else if (op.k == 4) op_densek<T, NV, 4>(tile, m, op, M);
-                    else op_densek<T, NV, 5>(tile, m, op, M);
+                if constexpr (VAR == 1) {
+                    if (op.k == 3) op_densek<T, NV, 3>(lds, vstride, m, op, M);
+                    else if (op.k == 4) op_densek<T, NV, 4>(lds, vstride, m, op, M);
+                    else op_densek<T, NV, 5>(lds, vstride, m, op, M);
+                } else if constexpr (VAR == 2) {
+                    op_densek<T, NV, 3>(lds, vstride, m, op, M);
                 }
                 break;
             case OPK_MFMA:
-                if constexpr (HEAVY) {
-                    if (op.k == 4) op_mfma<T, NV, 4>(tile, m, op, M);
-                    else op_mfma<T, NV, 5>(tile, m, op, M);
+                if constexpr (VAR == 2 && std::is_same<T, float>::value) {
+                    if (op.k == 4) op_mfma<NV, 4>(lds, vstride, m, op, M);
+                    else op_mfma<NV, 5>(lds, vstride, m, op, M);
                 }
                 break;
-            case OPK_DIAG: op_diag<T, NV>(tile, m, op, M, base); break;
-            case OPK_SCALAR: op_scalar<T, NV>(tile, m, M); break;
+            case OPK_DIAG: op_diag<T, NV>(lds, vstride, m, op, M, base); break;
+            case OPK_SCALAR: op_scalar<T, NV>(lds, vstride, m, M); break;
             case OPK_GRAD:
             case OPK_GRADDIAG:
                 if constexpr (NV == 2) {
-                    const double part = op.kind == OPK_GRAD ? op_grad_dense<T>(tile, m, op, M)
-                                                            : op_grad_diag<T>(tile, m, op, M);
+                    const double part = op.kind == OPK_GRAD ? op_grad_dense<T>(lds, vstride, m, op, M)
+                                                            : op_grad_diag<T>(lds, vstride, m, op, M);
                     // one wave-level reduction; each wave owns its own slot
                     const double ws = wave_sum(part);
                     if ((threadIdx.x & 63) == 0) gacc[gi * (kThreads / 64) + (threadIdx.x >> 6)] = ws;
@@ -503,37 +597,30 @@ __global__ void __launch_bounds__(kThreads) k_stage(const StageTask *__restrict_
         }
         __syncthreads();
     }
-    // store (optionally through the write-back bit permutation)
-    for (int e = threadIdx.x * PV; e < size; e += blockDim.x * PV) {
-        const uint64_t s = (uint64_t)e >> L;
-        const uint64_t off = base | ((uint64_t)e & lowm) | seg_off(s);
-        if (!permuted) {
-#pragma unroll
-            for (int v = 0; v < NV; ++v)
-                *reinterpret_cast<V *>(&svp[v][off]) = *reinterpret_cast<const V *>(&tile[v][e]);
-        } else {
+    // store (through the write-back bit permutation when `permuted`)
+    if (lane_on)
+        for (int j = 0; j < nchunks; ++j) {
+            const uint64_t off = gl | jtab[j];
+            const int Q = qtab[j];
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
-                C tmp[PV];
+                V x;
+                C *xc = reinterpret_cast<C *>(&x);
 #pragma unroll
-                for (int i = 0; i < PV; ++i) {
-                    const int ee = e + i;
-                    tmp[i] = tile[v][perm_lo[ee & 255] | perm_hi[ee >> 8]];
-                }
-                *reinterpret_cast<V *>(&svp[v][off]) = *reinterpret_cast<const V *>(tmp);
+                for (int i = 0; i < PV; ++i) xc[i] = lds_ld<C>(lds + v * vstride, ql[i] ^ Q);
+                gstore((DESV_GLOBAL V *)(svp[v] + off), x);
             }
         }
-    }
-    if (tk.st.n_grad > 0) {
+    if (n_grad > 0) {
         __syncthreads();
         int g2 = 0;
-        for (int o = tk.st.op_begin; o < tk.st.op_end; ++o) {
-            const int kind = ops[o].kind;
+        for (int o = ob; o < oe; ++o) {
+            const int kind = gload_op(ops + o).kind;
             if (kind == OPK_GRAD || kind == OPK_GRADDIAG) {
                 if (threadIdx.x == 0) {
                     double sum = 0;
                     for (int w = 0; w < kThreads / 64; ++w) sum += gacc[g2 * (kThreads / 64) + w];
-                    tk.partials[(int64_t)ops[o].slot * tk.pstride + t] = sum;
+                    tk.partials[(int64_t)gload_op(ops + o).slot * tk.pstride + t] = sum;
                 }
                 ++g2;
             }
@@ -542,20 +629,25 @@ __global__ void __launch_bounds__(kThreads) k_stage(const StageTask *__restrict_
 }
 
 template <typename T>
-hipError_t launch_stage(int nv, bool heavy, const StageTask *tasks, const int32_t *wg_task, const StageTask &single,
+hipError_t launch_stage(int nv, int variant, const StageTask *tasks, const int32_t *wg_task, const StageTask &single,
                         int64_t grid, int max_m, hipStream_t s) {
     using C = typename VT<T>::c;
     const size_t lds = (size_t)nv * ((size_t)1 << max_m) * sizeof(C);
     if (grid <= 0) return hipSuccess;
     if (grid > 0x7fffffffll) return hipErrorInvalidValue;
+    if (variant < 0 || variant > 2) return hipErrorInvalidValue;
+    // complex128 never uses MFMA blocks: its variant 2 would only differ by
+    // omitting k = 4, 5 VALU blocks
+    if (!std::is_same<T, float>::value && variant == 2) variant = 1;
     const dim3 g((unsigned)grid);
-    if (heavy) {
-        if (nv == 1) hipLaunchKernelGGL((k_stage<T, 1, true>), g, dim3(kThreads), lds, s, tasks, wg_task, single);
-        else hipLaunchKernelGGL((k_stage<T, 2, true>), g, dim3(kThreads), lds, s, tasks, wg_task, single);
+#define DESV_ST(NVV, VV) \
+    hipLaunchKernelGGL((k_stage<T, NVV, VV>), g, dim3(kThreads), lds, s, tasks, wg_task, single)
+    if (nv == 1) {
+        if (variant == 0) DESV_ST(1, 0); else if (variant == 1) DESV_ST(1, 1); else DESV_ST(1, 2);
     } else {
-        if (nv == 1) hipLaunchKernelGGL((k_stage<T, 1, false>), g, dim3(kThreads), lds, s, tasks, wg_task, single);
-        else hipLaunchKernelGGL((k_stage<T, 2, false>), g, dim3(kThreads), lds, s, tasks, wg_task, single);
+        if (variant == 0) DESV_ST(2, 0); else if (variant == 1) DESV_ST(2, 1); else DESV_ST(2, 2);
     }
+#undef DESV_ST
     return hipGetLastError();
 }
 
@@ -706,8 +798,12 @@ template <typename V, bool NT> __device__ inline void st(V *p, V v) {
     }
 }
 
-template <typename T, int K, int VEC, bool DIAG, int GPT, bool NT>
-__global__ void __launch_bounds__(kThreads) k_direct(typename VT<T>::c *__restrict__ sv, int64_t groups, DirectOp op,
+// OOP = true reads `src` and writes `dst` (a second buffer of the same size):
+// amplitudes whose controls are not satisfied are copied unchanged.  OOP =
+// false updates `dst` (== src) in place and skips them.
+template <typename T, int K, int VEC, bool DIAG, int GPT, bool NT, bool OOP>
+__global__ void __launch_bounds__(kThreads) k_direct(const typename VT<T>::c *src, typename VT<T>::c *dst,
+                                                    int64_t groups, DirectOp op,
                                                     const typename VT<T>::c *__restrict__ M) {
     using C = typename VT<T>::c;
     constexpr int D = 1 << K;
@@ -737,7 +833,7 @@ __global__ void __launch_bounds__(kThreads) k_direct(typename VT<T>::c *__restri
     const int64_t nthr = (int64_t)gridDim.x * blockDim.x;
     const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     uint64_t b[GPT];
-    bool on[GPT];
+    bool on[GPT], ctl[GPT];
     C a[GPT][D][VEC];
 #pragma unroll
     for (int u = 0; u < GPT; ++u) {
@@ -746,16 +842,17 @@ __global__ void __launch_bounds__(kThreads) k_direct(typename VT<T>::c *__restri
 #pragma unroll
         for (int j = 0; j < K; ++j) x = ((x >> srt[j]) << (srt[j] + 1)) | (x & ((1ull << srt[j]) - 1));
         b[u] = x;
-        on[u] = g0 < groups && (x & cm) == cm;
+        ctl[u] = (x & cm) == cm;
+        on[u] = g0 < groups && (OOP || ctl[u]);
         if (!on[u]) continue;
 #pragma unroll
         for (int s = 0; s < D; ++s) {
             if constexpr (VEC == 2) {
-                const float4 v = ld<float4, NT>(reinterpret_cast<const float4 *>(&sv[x | off[s]]));
+                const float4 v = ld<float4, NT>(reinterpret_cast<const float4 *>(&src[x | off[s]]));
                 a[u][s][0] = make_float2(v.x, v.y);
                 a[u][s][1] = make_float2(v.z, v.w);
             } else {
-                a[u][s][0] = ld<C, NT>(&sv[x | off[s]]);
+                a[u][s][0] = ld<C, NT>(&src[x | off[s]]);
             }
         }
     }
@@ -767,7 +864,9 @@ __global__ void __launch_bounds__(kThreads) k_direct(typename VT<T>::c *__restri
         for (int w = 0; w < VEC; ++w)
 #pragma unroll
             for (int q = 0; q < D; ++q) {
-                if constexpr (DIAG) {
+                if (OOP && !ctl[u]) {
+                    r[q][w] = a[u][q][w];
+                } else if constexpr (DIAG) {
                     r[q][w] = cmul(mm[q], a[u][q][w]);
                 } else {
                     C acc = cmul(mm[q * D], a[u][0][w]);
@@ -779,10 +878,10 @@ __global__ void __launch_bounds__(kThreads) k_direct(typename VT<T>::c *__restri
 #pragma unroll
         for (int s = 0; s < D; ++s) {
             if constexpr (VEC == 2) {
-                st<float4, NT>(reinterpret_cast<float4 *>(&sv[b[u] | off[s]]),
+                st<float4, NT>(reinterpret_cast<float4 *>(&dst[b[u] | off[s]]),
                                make_float4(r[s][0].x, r[s][0].y, r[s][1].x, r[s][1].y));
             } else {
-                st<C, NT>(&sv[b[u] | off[s]], r[s][0]);
+                st<C, NT>(&dst[b[u] | off[s]], r[s][0]);
             }
         }
     }
@@ -809,51 +908,55 @@ bool direct_nt() {
 }  // namespace
 
 template <typename T, int K, int VEC, bool DIAG>
-hipError_t launch_direct_k(typename VT<T>::c *p, int64_t groups, const DirectOp &op, const typename VT<T>::c *m,
-                           hipStream_t s) {
+hipError_t launch_direct_k(const typename VT<T>::c *src, typename VT<T>::c *dst, int64_t groups, const DirectOp &op,
+                           const typename VT<T>::c *m, hipStream_t s) {
     const int gpt = direct_gpt();
     const bool nt = direct_nt();
     const int64_t vecs = (groups + VEC - 1) / VEC;
     const int64_t threads = (vecs + gpt - 1) / gpt;
     const int64_t grid = std::max<int64_t>(1, (threads + kThreads - 1) / kThreads);
     if (grid > 0x7fffffffll) return hipErrorInvalidValue;
-#define DESV_DK(G, N) \
-    hipLaunchKernelGGL((k_direct<T, K, VEC, DIAG, G, N>), dim3((unsigned)grid), dim3(kThreads), 0, s, p, groups, op, m)
-    if (nt) {
-        if (gpt == 1) DESV_DK(1, true); else if (gpt == 2) DESV_DK(2, true); else DESV_DK(4, true);
+#define DESV_DK(G, N, O)                                                                               \
+    hipLaunchKernelGGL((k_direct<T, K, VEC, DIAG, G, N, O>), dim3((unsigned)grid), dim3(kThreads), 0, s, src, dst, \
+                       groups, op, m)
+    if (src != dst) {
+        DESV_DK(1, false, true);
+    } else if (nt) {
+        if (gpt == 1) DESV_DK(1, true, false); else if (gpt == 2) DESV_DK(2, true, false); else DESV_DK(4, true, false);
     } else {
-        if (gpt == 1) DESV_DK(1, false); else if (gpt == 2) DESV_DK(2, false); else DESV_DK(4, false);
+        if (gpt == 1) DESV_DK(1, false, false); else if (gpt == 2) DESV_DK(2, false, false); else DESV_DK(4, false, false);
     }
 #undef DESV_DK
     return hipGetLastError();
 }
 
 template <typename T>
-hipError_t launch_direct(void *sv, int n, const DirectOp &op, const void *M, hipStream_t s) {
+hipError_t launch_direct(const void *in, void *out, int n, const DirectOp &op, const void *M, hipStream_t s) {
     using C = typename VT<T>::c;
     const int K = op.k;
     const int64_t groups = 1ll << (n - K);
     bool bit0 = (op.cmask & 1) != 0;
     for (int j = 0; j < K; ++j) bit0 |= op.pos[j] == 0;
     const bool vec = std::is_same<T, float>::value && !bit0 && groups >= 2;
-    C *p = reinterpret_cast<C *>(sv);
+    const C *p = reinterpret_cast<const C *>(in);
+    C *q = reinterpret_cast<C *>(out);
     const C *m = reinterpret_cast<const C *>(M);
     if constexpr (std::is_same<T, float>::value) {
         if (K == 1) {
-            if (op.diag) return vec ? launch_direct_k<T, 1, 2, true>(p, groups, op, m, s)
-                                    : launch_direct_k<T, 1, 1, true>(p, groups, op, m, s);
-            return vec ? launch_direct_k<T, 1, 2, false>(p, groups, op, m, s)
-                       : launch_direct_k<T, 1, 1, false>(p, groups, op, m, s);
+            if (op.diag) return vec ? launch_direct_k<T, 1, 2, true>(p, q, groups, op, m, s)
+                                    : launch_direct_k<T, 1, 1, true>(p, q, groups, op, m, s);
+            return vec ? launch_direct_k<T, 1, 2, false>(p, q, groups, op, m, s)
+                       : launch_direct_k<T, 1, 1, false>(p, q, groups, op, m, s);
         }
-        if (op.diag) return vec ? launch_direct_k<T, 2, 2, true>(p, groups, op, m, s)
-                                : launch_direct_k<T, 2, 1, true>(p, groups, op, m, s);
-        return vec ? launch_direct_k<T, 2, 2, false>(p, groups, op, m, s)
-                   : launch_direct_k<T, 2, 1, false>(p, groups, op, m, s);
+        if (op.diag) return vec ? launch_direct_k<T, 2, 2, true>(p, q, groups, op, m, s)
+                                : launch_direct_k<T, 2, 1, true>(p, q, groups, op, m, s);
+        return vec ? launch_direct_k<T, 2, 2, false>(p, q, groups, op, m, s)
+                   : launch_direct_k<T, 2, 1, false>(p, q, groups, op, m, s);
     } else {
-        if (K == 1) return op.diag ? launch_direct_k<T, 1, 1, true>(p, groups, op, m, s)
-                                   : launch_direct_k<T, 1, 1, false>(p, groups, op, m, s);
-        return op.diag ? launch_direct_k<T, 2, 1, true>(p, groups, op, m, s)
-                       : launch_direct_k<T, 2, 1, false>(p, groups, op, m, s);
+        if (K == 1) return op.diag ? launch_direct_k<T, 1, 1, true>(p, q, groups, op, m, s)
+                                   : launch_direct_k<T, 1, 1, false>(p, q, groups, op, m, s);
+        return op.diag ? launch_direct_k<T, 2, 1, true>(p, q, groups, op, m, s)
+                       : launch_direct_k<T, 2, 1, false>(p, q, groups, op, m, s);
     }
 }
 
@@ -865,16 +968,26 @@ size_t stage_lds_limit(int device) {
     size_t lim = 64 * 1024;
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && v > 0)
         lim = (size_t)v;
-    // leave room for the static gradient/reduction arrays
-    lim -= kMaxGradLDS * (kThreads / 64) * sizeof(double) + 512 * sizeof(int) + 256 * 8 + sizeof(StageTask);
-    const void *fns[8] = {(const void *)k_stage<float, 1, false>,  (const void *)k_stage<float, 2, false>,
-                          (const void *)k_stage<double, 1, false>, (const void *)k_stage<double, 2, false>,
-                          (const void *)k_stage<float, 1, true>,   (const void *)k_stage<float, 2, true>,
-                          (const void *)k_stage<double, 1, true>,  (const void *)k_stage<double, 2, true>};
+    const void *fns[12] = {
+        (const void *)k_stage<float, 1, 0>,  (const void *)k_stage<float, 2, 0>,  (const void *)k_stage<double, 1, 0>,
+        (const void *)k_stage<double, 2, 0>, (const void *)k_stage<float, 1, 1>,  (const void *)k_stage<float, 2, 1>,
+        (const void *)k_stage<double, 1, 1>, (const void *)k_stage<double, 2, 1>, (const void *)k_stage<float, 1, 2>,
+        (const void *)k_stage<float, 2, 2>,  (const void *)k_stage<double, 1, 2>, (const void *)k_stage<double, 2, 2>};
+    // leave room for the kernels' own (static) LDS arrays, as compiled
+    size_t static_lds = 0;
+    for (const void *f : fns) {
+        hipFuncAttributes a;
+        if (hipFuncGetAttributes(&a, f) == hipSuccess) static_lds = std::max(static_lds, (size_t)a.sharedSizeBytes);
+        else (void)hipGetLastError();
+    }
+    static_lds = std::max(static_lds, kMaxGradLDS * (kThreads / 64) * sizeof(double) + 256 * sizeof(uint64_t) +
+                                          256 * sizeof(int) + sizeof(StageTask));
+    static_lds = (static_lds + 255) & ~(size_t)255;
+    lim -= static_lds;
     for (const void *f : fns)
         if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lim) != hipSuccess) {
             (void)hipGetLastError();
-            lim = 64 * 1024 - kMaxGradLDS * (kThreads / 64) * sizeof(double) - 512 * sizeof(int) - 256 * 8 - sizeof(StageTask);
+            lim = 64 * 1024 - static_lds;
         }
     cached[device] = lim;
     return lim;
@@ -1171,8 +1284,8 @@ hipError_t launch_dense_matvec(void *out, const void *in, int n, const double2 *
 }
 
 #define DESV_INST(T)                                                                                            \
-    template hipError_t launch_direct<T>(void *, int, const DirectOp &, const void *, hipStream_t);            \
-    template hipError_t launch_stage<T>(int, bool, const StageTask *, const int32_t *, const StageTask &, int64_t, int, \
+    template hipError_t launch_direct<T>(const void *, void *, int, const DirectOp &, const void *, hipStream_t);            \
+    template hipError_t launch_stage<T>(int, int, const StageTask *, const int32_t *, const StageTask &, int64_t, int, \
                                         hipStream_t);                                                          \
     template hipError_t launch_matvec_tasks<T>(const MatvecTask *, int, int64_t, hipStream_t);                 \
     template hipError_t launch_fill_tasks<T>(const FillTask *, int, int64_t, hipStream_t);                     \

@@ -102,11 +102,13 @@ struct DirectOp {
 
 // ---- launchers (engine_kernels.hip) --------------------------------------
 template <typename T>
-hipError_t launch_direct(void *sv, int n, const DirectOp &op, const void *M, hipStream_t s);
+// in == out: in place; otherwise out-of-place into a second buffer of the same size.
+hipError_t launch_direct(const void *in, void *out, int n, const DirectOp &op, const void *M, hipStream_t s);
 // Stage launch: `tasks` (device array) with wg_task[b] = task of workgroup b,
 // or, if tasks is null, `single`.
 template <typename T>
-hipError_t launch_stage(int nv, bool heavy, const StageTask *tasks, const int32_t *wg_task, const StageTask &single,
+// `variant`: DevStage::heavy (0 light, 1 VALU blocks k = 3..5, 2 MFMA blocks).
+hipError_t launch_stage(int nv, int variant, const StageTask *tasks, const int32_t *wg_task, const StageTask &single,
                         int64_t grid, int max_m, hipStream_t s);
 template <typename T>
 hipError_t launch_matvec_tasks(const MatvecTask *tasks, int ntasks, int64_t grid, hipStream_t s);

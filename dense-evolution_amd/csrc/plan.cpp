@@ -442,9 +442,13 @@ bool lower_stage(const std::vector<Gate> &ops, uint64_t tile, int n, const Compi
     }
     st.op_end = (int)plan.ops.size();
     st.n_grad = ngrad;
+    // kernel variant: 2 if any MFMA block (then VALU blocks are k = 3 only),
+    // 1 if any k >= 3 VALU block, else 0
     st.heavy = 0;
-    for (int i = st.op_begin; i < st.op_end; ++i)
-        if (plan.ops[i].kind == OPK_DENSEK || plan.ops[i].kind == OPK_MFMA) st.heavy = 1;
+    for (int i = st.op_begin; i < st.op_end; ++i) {
+        if (plan.ops[i].kind == OPK_MFMA) st.heavy = 2;
+        else if (plan.ops[i].kind == OPK_DENSEK && st.heavy == 0) st.heavy = 1;
+    }
     plan.stages.push_back(st);
     return true;
 }

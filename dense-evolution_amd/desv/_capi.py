@@ -50,6 +50,7 @@ _SIGS = {
     "desv_destroy": ([_p], _i),
     "desv_set_stream": ([_p, _p], _i),
     "desv_synchronize": ([_p], _i),
+    "desv_set_double_buffer": ([_p, _i], _i),
     "desv_reset": ([_p], _i),
     "desv_set_state": ([_p, _p], _i),
     "desv_get_state": ([_p, _p], _i),

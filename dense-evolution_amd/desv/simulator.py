@@ -72,6 +72,11 @@ class StateHandle:
     def reset(self) -> None:
         check(lib.desv_reset(self.ptr))
 
+    def set_double_buffer(self, enable: bool) -> None:
+        """Out-of-place streaming passes into a second state-sized buffer
+        (default on); off frees it and updates in place."""
+        check(lib.desv_set_double_buffer(self.ptr, int(bool(enable))))
+
     def run_ops(self, ops: np.ndarray) -> None:
         ops = np.ascontiguousarray(ops, dtype=np.float64).reshape(-1, 4)
         if len(ops):

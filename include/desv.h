@@ -62,6 +62,14 @@ int desv_destroy(desv_handle h);
  * this handle; NULL restores the handle's own stream. */
 int desv_set_stream(desv_handle h, void *hip_stream);
 int desv_synchronize(desv_handle h);
+/* Double buffering (on by default): single-gate streaming passes read the
+ * state and write a second buffer of the same size, which then becomes the
+ * state (the device address of the state changes; the handle stays valid).
+ * This is the counterpart of the reference's donated jit call
+ * (circuits/compiler.py:354, backends/statevector.py:467-471), which also
+ * hands back a new buffer.  enable = 0 frees the second buffer and updates in
+ * place; an allocation failure also falls back to in place. */
+int desv_set_double_buffer(desv_handle h, int enable);
 /* |0...0> */
 int desv_reset(desv_handle h);
 /* Copy 2^n host amplitudes in the handle's dtype (no normalisation). */

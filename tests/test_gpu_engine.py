@@ -100,6 +100,38 @@ def test_programs_multi_stage(opts, dtype, rng):
         assert fid(got, want) >= 1 - 1e-6
 
 
+@pytest.mark.parametrize("dtype", ["complex128", "complex64"])
+def test_streaming_passes_double_buffer(dtype, rng):
+    """Per-gate programs run every gate on the streaming kernel, out of place
+    into the second buffer (controlled gates copy the untouched half) or in
+    place; both match the oracle, also after toggling between the modes and
+    with an odd number of buffer swaps."""
+    n = 14
+    ops = random_ops(n, 121, rng)
+    sv0 = rand_state(n, rng)
+    prog = desv.Program(n).add_ops(ops).compile(dtype, fuse_bits=1, per_gate=True)
+    want = O.run_ops_fast(sv0, ops)
+    want2 = O.run_ops_fast(want, ops)
+    for mode in (True, False):
+        h = desv.StateHandle(n, dtype == "complex128")
+        h.set_double_buffer(mode)
+        h.upload(sv0)
+        prog.run(h)
+        got = h.download()
+        h.set_double_buffer(not mode)
+        prog.run(h)
+        got2 = h.download()
+        nrm = h.norm2()
+        h.close()
+        if dtype == "complex128":
+            assert np.allclose(got, want, atol=1e-10)
+            assert np.allclose(got2, want2, atol=1e-10)
+        else:
+            assert fid(got, want) >= 1 - 1e-6
+            assert fid(got2, want2) >= 1 - 1e-6
+        assert abs(nrm - 1) < 1e-5
+
+
 @pytest.mark.parametrize("k", [1, 2, 3, 4, 5])
 @pytest.mark.parametrize("use_float32", [False, True])
 def test_apply_arbitrary_unitaries(k, use_float32, rng):
