@@ -1,15 +1,19 @@
 """Benchmark: 30-qubit complex64 brickwork gate application on one MI355X.
 
 Workload (one step): depth-24 brickwork of Haar-random SU(4) gates on 30
-qubits (348 gates, seed 0), applied to a resident 8.6 GB complex64 state.
-The circuit is compiled once (fusion + cache-blocked LDS stages); a step is
-one replay of the compiled program.
+qubits (348 gates, seed 0), applied to a resident 8 GiB complex64 state.
+The circuit is compiled once (fusion into blocks of up to 4 qubits, 4-qubit
+blocks on MFMA, cache-blocked LDS stages); a step is one replay of the
+compiled program.
 
-Printed JSON line: gates/s as `value`, plus the whole-job HBM rate, the
-roofline of the stage kernel (algorithmic bytes per launch / measured average
-launch time from HIP events on the launch stream) and the CPU baseline (the
-NumPy oracle applying the same gates at 30 qubits on the host, single BLAS
-thread, a bounded sample of gates).
+Printed JSON line: gates/s as `value`; the whole-job HBM rate; the roofline
+of the dominant kernel (algorithmic FLOPs or bytes per launch / its average
+launch time, from HIP events recorded between stages on the launch stream);
+the CPU baseline (the NumPy oracle applying the same gates at 30 qubits on the
+host, one BLAS thread, a bounded sample of gates); and extra workloads
+(per-gate streaming passes, full-size c64-vs-c128 parity, 32 qubits, the
+20-qubit plain circuit, VQE adjoint vs parameter shift, the 1024-instance
+batch, the 28-qubit XXZ precision study).
 """
 import argparse
 import json
@@ -111,10 +115,14 @@ def suite_vqe20():
             "max_abs_dgrad_param_shift_vs_adjoint_c64": float(np.max(np.abs(gps - g64)))}
 
 
-def suite_batch(count=1024):
+def suite_batch(count=1024, solo_sample=128):
     """1024 heterogeneous VQE instances (n ~ U{10..22}, 4-8 layers, 50 terms):
-    instances/s of energy + adjoint gradient."""
+    instances/s of energy + adjoint gradient in one graph-captured batch, and
+    the per-instance latency: in the batch every instance completes with the
+    batch; run alone (its own compiled program, energy + gradient), the
+    latency distribution over a sample of the instances (p50 / p99)."""
     import numpy as np
+    import desv
     from desv import workloads
     from desv.batch import Batch
     insts = workloads.batch_instances(count=count)
@@ -124,9 +132,25 @@ def suite_batch(count=1024):
     b.configure(0, True)
     b.run()
     dt, (es, gs) = _timed(lambda: b.run(), reps=2)
+    b.close()
+    lat = []
+    for i in range(0, count, max(1, count // solo_sample)):
+        n, circ, theta, terms = insts[i]
+        fn, _ = desv.circuit_to_energy_fn(circ, n, dtype="complex64")
+        H = desv.PauliSumOperator(terms, n)
+        fn.value_and_grad(theta, H)
+        t, _ = _timed(lambda: fn.value_and_grad(theta, H), reps=3)
+        lat.append(t * 1e3)
+    lat = np.array(lat)
     return {"workload": f"{count} VQE instances, n ~ U{{10..22}}, 4-8 HEA layers, 50 terms, c64",
             "instances_per_s": count / dt, "ms_per_batch": dt * 1e3,
-            "mean_latency_ms": dt * 1e3 / count}
+            "batch_latency_ms_p50": dt * 1e3, "batch_latency_ms_p99": dt * 1e3,
+            "amortized_ms_per_instance": dt * 1e3 / count,
+            "solo_latency_ms_p50": float(np.percentile(lat, 50)),
+            "solo_latency_ms_p99": float(np.percentile(lat, 99)),
+            "solo_sample": len(lat),
+            "solo_instances_per_s": len(lat) / (lat.sum() / 1e3),
+            "batch_over_solo": (count / dt) / (len(lat) / (lat.sum() / 1e3))}
 
 
 def suite_precision(n=28, steps=200):
@@ -154,6 +178,125 @@ def suite_precision(n=28, steps=200):
             "max_abs_dZZ_c64_vs_c128": float(np.max(np.abs(z64 - z128)))}
 
 
+MFMA_FP32_PEAK_TFLOPS = 157.3   # MI355X dense FP32 matrix peak
+VALU_FP32_TFLOPS = 78.6          # FP32 VALU without packed-FP32 instructions
+
+
+def _load_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 PMC summary (FETCH_SIZE and WRITE_SIZE, copy-kernel scaled)."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            table = json.load(f)
+    except (OSError, ValueError):
+        return None
+    for e in table.get("entries", []):
+        if e.get("kernel") == kernel and e.get("workload") == workload:
+            return e.get("hbm_bytes_per_launch")
+    return None
+
+
+def suite_streaming30(n=30, depth=24, steps=2):
+    """The same brickwork with one streaming pass per gate (no fusion, no cache
+    blocking): the HBM-bound gate-application path, in GB/s and fraction of
+    the 8 TB/s peak."""
+    import desv
+    from desv import workloads
+    gates = workloads.brickwork_su4(n, depth, seed=0)
+    prog = desv.Program(n)
+    for u, a, b in gates:
+        prog.add_matrix(u, [a, b])
+    prog.compile("complex64", fuse_bits=1, per_gate=True)
+    h = desv.StateHandle(n, complex128=False)
+    prog.run(h)
+    dt, _ = _timed(lambda: prog.run(h), reps=steps)
+    h.close()
+    passes = prog.stats()["passes"]
+    gbps = passes * 2 * (1 << n) * 8 / dt / 1e9
+    return {"workload": f"brickwork {n}q depth {depth}, one streaming pass per gate (k_direct), c64",
+            "gates_per_s": len(gates) / dt, "passes_per_s": passes / dt, "hbm_gbps": gbps,
+            "hbm_frac": gbps / HBM_PEAK_GBPS}
+
+
+def suite_parity30(n=30, depth=24):
+    """Full-size parity: the benchmark circuit in complex64 (as benchmarked)
+    against complex128, fidelity and norms computed on the GPU."""
+    import desv
+    from desv import workloads
+    gates = workloads.brickwork_su4(n, depth, seed=0)
+    prog = desv.Program(n)
+    for u, a, b in gates:
+        prog.add_matrix(u, [a, b])
+    prog.compile("complex64", fuse_bits=4, use_mfma=True)
+    h64 = desv.StateHandle(n, complex128=False)
+    prog.run(h64)
+    n64 = h64.norm2()
+    prog.compile("complex128")
+    h128 = desv.StateHandle(n, complex128=True)
+    prog.run(h128)
+    n128 = h128.norm2()
+    up = desv.StateHandle(n, complex128=True)
+    up.copy_from(h64)
+    h64.close()
+    fid = abs(up.inner(h128)) ** 2 / (n64 * n128)
+    up.close()
+    h128.close()
+    return {"workload": f"brickwork {n}q depth {depth}: c64 (fused MFMA blocks) vs c128",
+            "fidelity_c64_vs_c128": fid, "infidelity": 1 - fid, "norm_c64_minus_1": n64 - 1,
+            "norm_c128_minus_1": n128 - 1}
+
+
+def suite_brickwork32(n=32, depth=24):
+    """The same circuit structure at 32 qubits (32 GiB complex64 state)."""
+    import desv
+    from desv import workloads
+    gates = workloads.brickwork_su4(n, depth, seed=0)
+    prog = desv.Program(n)
+    for u, a, b in gates:
+        prog.add_matrix(u, [a, b])
+    prog.compile("complex64", fuse_bits=4, use_mfma=True)
+    h = desv.StateHandle(n, complex128=False)
+    prog.run(h)
+    h.reset()
+    dt, _ = _timed(lambda: prog.run(h), reps=1)
+    nrm = h.norm2()
+    h.close()
+    st = prog.stats()
+    return {"workload": f"brickwork {n}q depth {depth}, {len(gates)} gates, c64 (32 GiB state)",
+            "gates_per_s": len(gates) / dt, "ms_per_circuit": dt * 1e3, "hbm_passes": st["passes"],
+            "hbm_gbps": st["passes"] * 2 * (1 << n) * 8 / dt / 1e9, "norm_minus_1": nrm - 1}
+
+
+def copy_probe(n, steps):
+    """Copy one 2^n complex64 state into another with the library's streaming
+    copy (desv_copy): read + write bytes per second.  Under rocprofv3 --pmc,
+    FETCH_SIZE and WRITE_SIZE of this kernel against its known 2^n * 8 bytes
+    give the scale factors of the byte counters."""
+    import torch
+    import desv
+    stream = torch.cuda.Stream()
+    a = desv.StateHandle(n, complex128=False)
+    b = desv.StateHandle(n, complex128=False)
+    for h in (a, b):
+        desv._capi.check(desv._capi.lib.desv_set_stream(h.ptr, desv._capi.C.c_void_p(stream.cuda_stream)))
+    b.copy_from(a)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(steps):
+        b.copy_from(a)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / steps
+    nbytes = (1 << n) * 8
+    print(json.dumps({"copy_probe": f"desv_copy of a 2^{n} complex64 state", "bytes_each_way": nbytes,
+                      "ms": ms, "gbps_read_plus_write": 2 * nbytes / ms / 1e6,
+                      "frac_of_peak": 2 * nbytes / ms / 1e6 / HBM_PEAK_GBPS}), flush=True)
+    a.close()
+    b.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -161,15 +304,18 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--qubits", type=int, default=30)
     ap.add_argument("--depth", type=int, default=24)
-    ap.add_argument("--fuse", type=int, default=2)
+    ap.add_argument("--fuse", type=int, default=4)
     ap.add_argument("--tile-bits", type=int, default=0)
     ap.add_argument("--low-bits", type=int, default=0)
     ap.add_argument("--per-gate", action="store_true")
-    ap.add_argument("--mfma", action="store_true")
+    ap.add_argument("--no-mfma", action="store_true", help="fused blocks on VALU instead of MFMA")
     ap.add_argument("--in-place", action="store_true",
                     help="streaming passes update the state in place (no second buffer)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--suite", default="plain20,vqe20,batch,precision",
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only time the library's device copy of a 2^qubits complex64 state "
+                         "(calibration of HBM rate and of the rocprofv3 byte counters)")
+    ap.add_argument("--suite", default="streaming30,parity30,brickwork32,plain20,vqe20,batch,precision",
                     help="extra workloads to measure after the headline ('' for none)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -184,17 +330,22 @@ def main():
     from desv import workloads
 
     n = args.qubits
+    if args.copy_probe:
+        copy_probe(n, args.steps)
+        return
+    mfma = not args.no_mfma
     gates = workloads.brickwork_su4(n, args.depth, seed=0)
     prog = desv.Program(n)
     for u, a, b in gates:
         prog.add_matrix(u, [a, b])
-    kw = dict(fuse_bits=args.fuse, per_gate=args.per_gate, use_mfma=args.mfma)
+    kw = dict(fuse_bits=args.fuse, per_gate=args.per_gate, use_mfma=mfma)
     if args.tile_bits:
         kw["tile_bits"] = args.tile_bits
     if args.low_bits:
         kw["low_bits"] = args.low_bits
     prog.compile("complex64", **kw)
     st = prog.stats()
+    kernels = prog.stage_kernels()
     log(f"compiled: {st}")
 
     torch.cuda.init()
@@ -215,28 +366,57 @@ def main():
     torch.cuda.synchronize()
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
+    stage_ms = np.zeros(len(kernels))
     t0 = time.perf_counter()
     ev0.record(stream)
     for _ in range(args.steps):
-        prog.run(h)
+        # the same launches as prog.run, with an event between stages
+        stage_ms += prog.run_timed(h)
     ev1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     gpu_ms = ev0.elapsed_time(ev1)
-    log(f"timed {args.steps} steps: {gpu_ms:.2f} ms (wall {wall*1e3:.1f} ms)")
+    log(f"timed {args.steps} steps: {gpu_ms:.2f} ms (wall {wall*1e3:.1f} ms, stages {stage_ms.sum():.2f} ms)")
 
     # sanity: the state stays normalised
     nrm = h.norm2()
-    log(f"norm^2 after {args.steps + 0} steps: {nrm:.9f}")
+    log(f"norm^2 after {args.steps} steps: {nrm:.9f}")
 
-    amp_bytes = 8
     N = 1 << n
-    bytes_per_pass = 2 * N * amp_bytes
-    launches = st["passes"] * args.steps
-    avg_launch_s = gpu_ms / 1e3 / launches
-    achieved = bytes_per_pass / avg_launch_s / 1e9
+    bytes_per_pass = 2 * N * 8
     ms_per_step = gpu_ms / args.steps
     gates_per_s = len(gates) * args.steps / (gpu_ms / 1e3)
+
+    # per-kernel attribution from the per-stage events
+    per = {}
+    for (name, flops, mflops), ms in zip(kernels, stage_ms):
+        d = per.setdefault(name, {"launches": 0, "ms": 0.0, "flops": 0.0, "mfma_flops": 0.0})
+        d["launches"] += args.steps
+        d["ms"] += float(ms)
+        d["flops"] += flops * args.steps
+        d["mfma_flops"] += mflops * args.steps
+    dom = max(per, key=lambda k: per[k]["ms"])
+    d = per[dom]
+    avg_ms = d["ms"] / d["launches"]
+    workload_key = (f"brickwork{n}_depth{args.depth}_fuse{args.fuse}{'_mfma' if mfma and args.fuse >= 4 else ''}"
+                    f"{'_pergate' if args.per_gate else ''}")
+    traffic = _load_traffic(dom, workload_key)
+    if dom == "k_direct":
+        achieved = bytes_per_pass / (avg_ms / 1e3) / 1e9
+        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBPS}
+    else:
+        achieved = d["flops"] / d["launches"] / (avg_ms / 1e3) / 1e12
+        roof = {"bound": "mfma", "achieved": achieved, "peak": MFMA_FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / MFMA_FP32_PEAK_TFLOPS}
+        roof["mfma_only_tflops"] = d["mfma_flops"] / (d["ms"] / 1e3) / 1e12
+        roof["valu_fp32_peak_tflops"] = VALU_FP32_TFLOPS
+        roof["hbm_gbps_per_launch"] = bytes_per_pass / (avg_ms / 1e3) / 1e9
+    roof.update({"traffic": traffic, "kernel": dom, "launches": d["launches"],
+                 "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": bytes_per_pass,
+                 "algorithmic_flops_per_launch": d["flops"] / d["launches"]})
+    kernel_table = {k: {"launches": v["launches"], "avg_ms": v["ms"] / v["launches"],
+                        "share": v["ms"] / max(stage_ms.sum(), 1e-9)} for k, v in per.items()}
 
     out = {
         "metric": "gates/s + HBM GB/s at 30q c64",
@@ -254,20 +434,24 @@ def main():
         "config": {"workload": f"brickwork Haar-SU(4), {n} qubits, depth {args.depth}, "
                                f"{len(gates)} gates, complex64 state resident in HBM",
                    "qubits": n, "depth": args.depth, "gates": len(gates),
-                   "fuse_bits": args.fuse, "per_gate": args.per_gate, "mfma": args.mfma,
-                   "hbm_passes_per_step": st["passes"], "device_ops_per_step": st["blocks"]},
+                   "fuse_bits": args.fuse, "per_gate": args.per_gate, "mfma": mfma,
+                   "hbm_passes_per_step": st["passes"], "device_ops_per_step": st["blocks"],
+                   "mfma_blocks_per_step": st["mfma_blocks"]},
+        # whole-job HBM rate: algorithmic bytes of the passes executed / time
         "hbm_gbps": st["passes"] * bytes_per_pass * args.steps / (gpu_ms / 1e3) / 1e9,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
-                     "kernel": "k_stage<float,1>",
-                     "bytes_per_launch": bytes_per_pass, "avg_launch_ms": avg_launch_s * 1e3},
+        # the unfused per-gate equivalent: gates x 2 N b / time
+        "gate_equivalent_gbps": len(gates) * bytes_per_pass * args.steps / (gpu_ms / 1e3) / 1e9,
+        "roofline": roof,
+        "kernels": kernel_table,
     }
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(n, gates)
     del prog, h
+    torch.cuda.synchronize()
     extras = {}
-    fns = {"plain20": suite_plain20, "vqe20": suite_vqe20, "batch": suite_batch,
+    fns = {"streaming30": suite_streaming30, "parity30": suite_parity30, "brickwork32": suite_brickwork32,
+           "plain20": suite_plain20, "vqe20": suite_vqe20, "batch": suite_batch,
            "precision": suite_precision}
     for name in [x for x in args.suite.split(",") if x]:
         log(f"suite {name} ...")

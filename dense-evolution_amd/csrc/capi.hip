@@ -237,10 +237,15 @@ void *ensure_alt(desv_handle h) {
 // When sv0 is the handle's own state, a
 // streaming stage may run out of place into the second buffer and swap it
 // with the state (the handle's state pointer changes; callers re-read it).
+// `evs` (optional, stages + 1 events) are recorded before every stage and
+// after the last one.
 int run_plan_on(desv_handle h, const Plan &plan, const DevPlan &dp, int nv, void *sv0, void *sv1,
-                int64_t pstride) {
+                int64_t pstride, hipEvent_t *evs = nullptr) {
     const bool own_state = nv == 1 && sv0 == h->sv;
+    size_t si = 0;
     for (const DevStage &st : plan.stages) {
+        if (evs) HIP_TRY(hipEventRecord(evs[si], h->stream));
+        ++si;
         DirectOp dop;
         if (nv == 1 && direct_op(plan, st, dop)) {
             const size_t eb = h->dtype == DESV_C64 ? 8 : 16;
@@ -273,6 +278,7 @@ int run_plan_on(desv_handle h, const Plan &plan, const DevPlan &dp, int nv, void
                            : launch_stage<double>(nv, st.heavy, nullptr, nullptr, t, st.n_tiles, st.m, h->stream);
         if (e != hipSuccess) return fail(DESV_EDEVICE, std::string("stage launch: ") + hipGetErrorString(e));
     }
+    if (evs) HIP_TRY(hipEventRecord(evs[si], h->stream));
     return DESV_OK;
 }
 
@@ -509,9 +515,20 @@ int desv_get_state(desv_handle h, void *host) {
 int desv_copy(desv_handle dst, desv_handle src) {
     if (int rc = check_handle(dst)) return rc;
     if (int rc = check_handle(src)) return rc;
-    if (dst->n != src->n || dst->dtype != src->dtype) return fail(DESV_EINVAL, "copy between different shapes");
+    if (dst->n != src->n) return fail(DESV_EINVAL, "copy between different shapes");
+    if (dst->device != src->device) return fail(DESV_EINVAL, "copy between different devices");
+    HIP_TRY(hipSetDevice(dst->device));
     HIP_TRY(hipStreamSynchronize(src->stream));
-    HIP_TRY(hipMemcpyAsync(dst->sv, src->sv, dst->bytes, hipMemcpyDeviceToDevice, dst->stream));
+    if (dst->dtype == src->dtype && dst->bytes % 16 == 0) {
+        HIP_TRY(launch_copy16(dst->sv, src->sv, dst->bytes, dst->stream));
+    } else if (dst->dtype == src->dtype) {
+        HIP_TRY(hipMemcpyAsync(dst->sv, src->sv, dst->bytes, hipMemcpyDeviceToDevice, dst->stream));
+    } else {
+        const hipError_t e = dst->dtype == DESV_C64
+                                 ? launch_convert<float, double>(dst->sv, src->sv, dst->dim, dst->stream)
+                                 : launch_convert<double, float>(dst->sv, src->sv, dst->dim, dst->stream);
+        HIP_TRY(e);
+    }
     return DESV_OK;
 }
 
@@ -732,7 +749,9 @@ int desv_program_plan_dump(desv_program p, uint64_t *tile_masks, int64_t *stage_
     return DESV_OK;
 }
 
-int desv_run_program(desv_handle h, desv_program p) {
+namespace {
+// Compile with defaults if needed and upload the plan for this state.
+int prepare_program(desv_handle h, desv_program p) {
     if (int rc = check_handle(h)) return rc;
     if (!p) return fail(DESV_EINVAL, "null program");
     if (p->n != h->n) return fail(DESV_EINVAL, "program and state have different qubit counts");
@@ -747,7 +766,35 @@ int desv_run_program(desv_handle h, desv_program p) {
         int rc = upload_plan(p->plan, h->dtype, h->device, h->stream, p->dev);
         if (rc) return rc;
     }
+    return DESV_OK;
+}
+}  // namespace
+
+int desv_run_program(desv_handle h, desv_program p) {
+    if (int rc = prepare_program(h, p)) return rc;
     return run_plan_on(h, p->plan, p->dev, 1, h->sv, nullptr, 0);
+}
+
+int desv_run_program_timed(desv_handle h, desv_program p, float *stage_ms) {
+    if (!stage_ms) return fail(DESV_EINVAL, "null stage_ms");
+    if (int rc = prepare_program(h, p)) return rc;
+    const size_t S = p->plan.stages.size();
+    std::vector<hipEvent_t> evs(S + 1, nullptr);
+    int rc = DESV_OK;
+    for (auto &e : evs)
+        if (hipEventCreate(&e) != hipSuccess) {
+            e = nullptr;
+            rc = fail(DESV_EDEVICE, "hipEventCreate failed");
+            break;
+        }
+    if (!rc) rc = run_plan_on(h, p->plan, p->dev, 1, h->sv, nullptr, 0, evs.data());
+    if (!rc && hipEventSynchronize(evs[S]) != hipSuccess) rc = fail(DESV_EDEVICE, "hipEventSynchronize failed");
+    for (size_t i = 0; !rc && i < S; ++i)
+        if (hipEventElapsedTime(&stage_ms[i], evs[i], evs[i + 1]) != hipSuccess)
+            rc = fail(DESV_EDEVICE, "hipEventElapsedTime failed");
+    for (auto &e : evs)
+        if (e) (void)hipEventDestroy(e);
+    return rc;
 }
 
 int desv_pauli_expect(desv_handle h, const uint64_t *xmask, const uint64_t *zmask, int64_t T, double *out) {
@@ -887,7 +934,8 @@ int desv_adjoint(desv_handle h, const double *ops, int64_t n_ops, const int64_t 
     for (int64_t k = 0; k < n_params; ++k) grad[k] = 0.0;
     if (slot_param.empty()) return DESV_OK;
     // reverse sweep on a copy of psi (the handle keeps the final state)
-    HIP_TRY(hipMemcpyAsync(h->aux[0], h->sv, h->bytes, hipMemcpyDeviceToDevice, h->stream));
+    HIP_TRY(h->bytes % 16 == 0 ? launch_copy16(h->aux[0], h->sv, h->bytes, h->stream)
+                               : hipMemcpyAsync(h->aux[0], h->sv, h->bytes, hipMemcpyDeviceToDevice, h->stream));
     CompileOpts o = default_opts(h->dtype, h->device, 2);
     Plan plan;
     std::string err;

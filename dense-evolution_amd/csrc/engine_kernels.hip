@@ -996,6 +996,47 @@ size_t stage_lds_limit(int device) {
 // ---------------------------------------------------------------------------
 // Elementwise kernels
 // ---------------------------------------------------------------------------
+// dst <- src, 16 bytes per lane, one access per lane (the streaming copy
+// that reaches the highest HBM rate on MI355X; hipMemcpy device-to-device is
+// slower).  bytes must be a multiple of 16.
+__global__ void __launch_bounds__(kThreads) k_copy16(const float4 *__restrict__ src, float4 *__restrict__ dst,
+                                                     int64_t n16) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n16) dst[i] = src[i];
+}
+hipError_t launch_copy16(void *dst, const void *src, size_t bytes, hipStream_t s) {
+    const int64_t n16 = (int64_t)(bytes / 16);
+    const int64_t grid = (n16 + kThreads - 1) / kThreads;
+    if (grid <= 0) return hipSuccess;
+    if (grid > 0x7fffffffll) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_copy16, dim3((unsigned)grid), dim3(kThreads), 0, s, reinterpret_cast<const float4 *>(src),
+                       reinterpret_cast<float4 *>(dst), n16);
+    return hipGetLastError();
+}
+
+// dst[i] = src[i] with a precision change (complex64 <-> complex128).
+template <typename TD, typename TS>
+__global__ void k_convert(typename VT<TD>::c *__restrict__ dst, const typename VT<TS>::c *__restrict__ src,
+                          int64_t dim) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < dim; i += (int64_t)gridDim.x * blockDim.x) {
+        const typename VT<TS>::c a = src[i];
+        typename VT<TD>::c b;
+        b.x = (TD)a.x;
+        b.y = (TD)a.y;
+        dst[i] = b;
+    }
+}
+template <typename TD, typename TS>
+hipError_t launch_convert(void *dst, const void *src, int64_t dim, hipStream_t s) {
+    const int64_t grid = std::max<int64_t>(1, std::min<int64_t>((dim + kThreads - 1) / kThreads, 1 << 20));
+    hipLaunchKernelGGL((k_convert<TD, TS>), dim3((unsigned)grid), dim3(kThreads), 0, s,
+                       reinterpret_cast<typename VT<TD>::c *>(dst), reinterpret_cast<const typename VT<TS>::c *>(src),
+                       dim);
+    return hipGetLastError();
+}
+template hipError_t launch_convert<float, double>(void *, const void *, int64_t, hipStream_t);
+template hipError_t launch_convert<double, float>(void *, const void *, int64_t, hipStream_t);
+
 template <typename T>
 __global__ void k_fill_basis(typename VT<T>::c *sv, int64_t dim, int64_t index) {
     using C = typename VT<T>::c;

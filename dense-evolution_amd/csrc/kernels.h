@@ -115,6 +115,11 @@ hipError_t launch_matvec_tasks(const MatvecTask *tasks, int ntasks, int64_t grid
 template <typename T>
 hipError_t launch_fill_tasks(const FillTask *tasks, int ntasks, int64_t grid, hipStream_t s);
 hipError_t launch_sum_tasks(const SumTask *tasks, int ntasks, hipStream_t s);
+// dst <- src on the device (bytes a multiple of 16).
+hipError_t launch_copy16(void *dst, const void *src, size_t bytes, hipStream_t s);
+// dst <- src converted between complex64 (float) and complex128 (double).
+template <typename TD, typename TS>
+hipError_t launch_convert(void *dst, const void *src, int64_t dim, hipStream_t s);
 template <typename T>
 hipError_t launch_fill_basis(void *sv, int64_t dim, int64_t index, hipStream_t s);
 template <typename T>

@@ -71,6 +71,7 @@ _SIGS = {
     "desv_program_compile": ([_p, C.POINTER(CompileOpts)], _i),
     "desv_program_stats": ([_p, _pi64, _pi64, _pi64, _pi64], _i),
     "desv_run_program": ([_p, _p], _i),
+    "desv_run_program_timed": ([_p, _p, C.POINTER(C.c_float)], _i),
     "desv_program_plan_sizes": ([_p, _pi64, _pi64, _pi64], _i),
     "desv_program_plan_dump": ([_p, _pu64, _pi64, _pi64, _pd], _i),
     "desv_pauli_expect": ([_p, _pu64, _pu64, _i64, _pd], _i),

@@ -90,3 +90,52 @@ class Program:
     def run(self, target) -> None:
         h = target.handle if not isinstance(target, StateHandle) else target
         check(lib.desv_run_program(h.ptr, self.ptr))
+
+    def run_timed(self, target) -> np.ndarray:
+        """Run once with an event between stages; per-stage milliseconds."""
+        h = target.handle if not isinstance(target, StateHandle) else target
+        s, o, m = C.c_int64(), C.c_int64(), C.c_int64()
+        check(lib.desv_program_plan_sizes(self.ptr, C.byref(s), C.byref(o), C.byref(m)))
+        out = np.zeros(max(1, s.value), dtype=np.float32)
+        check(lib.desv_run_program_timed(h.ptr, self.ptr, out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out[:s.value].astype(np.float64)
+
+    def stage_kernels(self):
+        """Per stage: (kernel name, algorithmic FLOPs, algorithmic HBM bytes)
+        for a state of this program's width in `dtype` (8 or 16 bytes per
+        amplitude).  FLOPs: N 2^k 8 per dense k-qubit block, 6 N per diagonal
+        or scalar op; bytes: one read and one write of the state per stage."""
+        tm, so, ops, _ = self.plan()
+        return [_stage_kernel(so[i], ops, self.n) for i in range(len(tm))]
+
+
+_OPK_DENSE1, _OPK_DENSE2, _OPK_DENSEK, _OPK_DIAG, _OPK_SCALAR, _OPK_GRAD, _OPK_GRADDIAG, _OPK_MFMA = range(8)
+
+
+def _stage_kernel(stage, ops, n):
+    """Which kernel runs a stage (mirrors the launch choice of the engine) and
+    its algorithmic FLOPs per amplitude-count N = 2^n."""
+    b, e = int(stage[0]), int(stage[1])
+    sperm = [int(x) for x in stage[2:] if x >= 0]
+    permuted = any(sperm[j] != j for j in range(len(sperm)))
+    N = 1 << n
+    flops = 0
+    kinds = set()
+    for o in ops[b:e]:
+        kind, k = int(o[0]), int(o[1])
+        kinds.add(kind)
+        if kind in (_OPK_DENSE1, _OPK_DENSE2, _OPK_DENSEK, _OPK_MFMA):
+            flops += N * (1 << k) * 8
+        else:
+            flops += 6 * N
+    if e - b == 1 and not permuted and int(ops[b][0]) in (_OPK_DENSE1, _OPK_DENSE2, _OPK_DIAG) \
+            and 1 <= int(ops[b][1]) <= 2:
+        name = "k_direct"
+    elif _OPK_MFMA in kinds:
+        name = "k_stage<mfma>"
+    elif _OPK_DENSEK in kinds:
+        name = "k_stage<valu-k>"
+    else:
+        name = "k_stage<light>"
+    mfma_flops = sum(N * (1 << int(o[1])) * 8 for o in ops[b:e] if int(o[0]) == _OPK_MFMA)
+    return name, flops, mfma_flops

@@ -77,6 +77,16 @@ class StateHandle:
         (default on); off frees it and updates in place."""
         check(lib.desv_set_double_buffer(self.ptr, int(bool(enable))))
 
+    def copy_from(self, other: "StateHandle") -> None:
+        """self <- other on the device (converting complex64 <-> complex128)."""
+        check(lib.desv_copy(self.ptr, other.ptr))
+
+    def inner(self, other: "StateHandle") -> complex:
+        """<self|other> with FP64 accumulation (same dtype)."""
+        re, im = C.c_double(), C.c_double()
+        check(lib.desv_inner(self.ptr, other.ptr, C.byref(re), C.byref(im)))
+        return complex(re.value, im.value)
+
     def run_ops(self, ops: np.ndarray) -> None:
         ops = np.ascontiguousarray(ops, dtype=np.float64).reshape(-1, 4)
         if len(ops):

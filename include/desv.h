@@ -75,7 +75,9 @@ int desv_reset(desv_handle h);
 /* Copy 2^n host amplitudes in the handle's dtype (no normalisation). */
 int desv_set_state(desv_handle h, const void *host_amps);
 int desv_get_state(desv_handle h, void *host_amps);
-/* dst <- src (same n and dtype). */
+/* dst <- src (same n and device; a dtype change converts complex64 <->
+ * complex128 on the device, e.g. to compare a complex64 run with a
+ * complex128 one). */
 int desv_copy(desv_handle dst, desv_handle src);
 /* state <- (re + i im) * state */
 int desv_scale(desv_handle h, double re, double im);
@@ -129,6 +131,12 @@ int desv_program_compile(desv_program p, const desv_compile_opts *opts);
 int desv_program_stats(desv_program p, int64_t *gates, int64_t *passes,
                        int64_t *blocks, int64_t *mfma_blocks);
 int desv_run_program(desv_handle h, desv_program p);
+/* Same as desv_run_program, with a HIP event recorded on the handle's stream
+ * before every stage and after the last; waits for the run and writes each
+ * stage's duration (milliseconds) to stage_ms[0 .. stages).  No counterpart
+ * in the reference (its jit call has no per-step timing); used to measure
+ * each stage kernel against its roofline. */
+int desv_run_program_timed(desv_handle h, desv_program p, float *stage_ms);
 /* Introspection of a compiled plan (no device needed): sizes, then per stage
  * the tile bit mask and 18 int64 fields (op_begin, op_end, then the 16-entry
  * write-back permutation sperm: store-order tile bit j takes tile-local bit

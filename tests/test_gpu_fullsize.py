@@ -1,0 +1,84 @@
+"""Full-size checks (30 qubits, the brickwork benchmark circuit) through
+size-independent properties, where the CPU oracle cannot follow:
+
+* the complex64 state (fused MFMA blocks, FP32 arithmetic) against the
+  complex128 run of the same engine: fidelity >= 1 - 1e-6 and norm within
+  1e-5 (the complex64 bar);
+* the circuit followed by its inverse returns |0...0> (|<0|psi>|^2 >= 1 - 1e-5
+  in complex64, >= 1 - 1e-10 in complex128 at 24 qubits);
+* device precision conversion matches the host cast.
+"""
+import numpy as np
+import pytest
+
+import desv
+from desv import workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def brickwork_program(n, depth, inverse=False, seed=0):
+    gates = workloads.brickwork_su4(n, depth, seed=seed)
+    prog = desv.Program(n)
+    for u, a, b in gates:
+        prog.add_matrix(u, [a, b])
+    if inverse:
+        for u, a, b in reversed(gates):
+            prog.add_matrix(np.conj(u).T, [a, b])
+    return prog
+
+
+def test_copy_converts_precision(rng):
+    n = 11
+    v = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    h64 = desv.StateHandle(n, False)
+    h128 = desv.StateHandle(n, True)
+    h128.upload(v)
+    h64.copy_from(h128)
+    assert np.array_equal(h64.download(), v.astype(np.complex64))
+    back = desv.StateHandle(n, True)
+    back.copy_from(h64)
+    assert np.array_equal(back.download(), v.astype(np.complex64).astype(np.complex128))
+    for h in (h64, h128, back):
+        h.close()
+
+
+def test_brickwork_30q_c64_against_c128():
+    n = 30
+    prog = brickwork_program(n, 24)
+    h64 = desv.StateHandle(n, False)
+    prog.compile("complex64", fuse_bits=4, use_mfma=True)
+    prog.run(h64)
+    n64 = h64.norm2()
+    h128 = desv.StateHandle(n, True)
+    prog.compile("complex128")
+    prog.run(h128)
+    n128 = h128.norm2()
+    up = desv.StateHandle(n, True)
+    up.copy_from(h64)
+    h64.close()
+    z = up.inner(h128)
+    fid = abs(z) ** 2 / (n64 * n128)
+    print(f"30q brickwork: fidelity c64 vs c128 = 1 - {1 - fid:.3e}, |norm64 - 1| = {abs(n64 - 1):.2e}")
+    assert abs(n128 - 1) < 1e-10
+    assert abs(n64 - 1) < 1e-5
+    assert fid >= 1 - 1e-6
+    up.close()
+    h128.close()
+
+
+@pytest.mark.parametrize("n,dtype,tol,opts", [
+    (30, "complex64", 1e-5, dict(fuse_bits=4, use_mfma=True)),
+    (30, "complex64", 1e-5, dict(fuse_bits=2)),
+    (24, "complex128", 1e-10, dict(fuse_bits=3)),
+])
+def test_circuit_then_inverse_returns_to_zero(n, dtype, tol, opts):
+    prog = brickwork_program(n, 12, inverse=True, seed=1).compile(dtype, **opts)
+    h = desv.StateHandle(n, dtype == "complex128")
+    prog.run(h)
+    zero = desv.StateHandle(n, dtype == "complex128")
+    p0 = abs(zero.inner(h)) ** 2
+    print(f"{n}q {dtype} {opts}: 1 - |<0|U^-1 U|0>|^2 = {1 - p0:.3e}")
+    assert p0 >= 1 - tol
+    h.close()
+    zero.close()
