@@ -129,7 +129,9 @@ def suite_batch(count=1024, solo_sample=128):
     b = Batch("complex64")
     for n, circ, theta, terms in insts:
         b.add(circ, n, theta, terms)
-    b.configure(0, True)
+    tuned = b.autotune(reps=1, cache=False)   # launch shape measured on this device
+    log(f"batch autotune: best {tuned['best']} "
+        f"{[(c['tile_bits'], c['fuse_bits'], c['use_mfma'], c['ms'] and round(c['ms'], 1)) for c in tuned['candidates']]}")
     b.run()
     dt, (es, gs) = _timed(lambda: b.run(), reps=2)
     b.close()
@@ -144,6 +146,9 @@ def suite_batch(count=1024, solo_sample=128):
     lat = np.array(lat)
     return {"workload": f"{count} VQE instances, n ~ U{{10..22}}, 4-8 HEA layers, 50 terms, c64",
             "instances_per_s": count / dt, "ms_per_batch": dt * 1e3,
+            "autotuned": tuned["best"],
+            "autotune_candidates_ms": {f"t{c['tile_bits']}f{c['fuse_bits']}{'m' if c['use_mfma'] else ''}": c["ms"]
+                                       for c in tuned["candidates"]},
             "batch_latency_ms_p50": dt * 1e3, "batch_latency_ms_p99": dt * 1e3,
             "amortized_ms_per_instance": dt * 1e3 / count,
             "solo_latency_ms_p50": float(np.percentile(lat, 50)),

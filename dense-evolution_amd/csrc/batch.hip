@@ -101,6 +101,8 @@ struct desv_batch_s {
     int dtype = 0;
     int device = 0;
     int tile_bits = 0;
+    int fuse_bits = 2;
+    int use_mfma = 0;
     int use_graph = 1;
     double last_us = 0;
     std::vector<BInst> inst;
@@ -233,7 +235,10 @@ int build(desv_batch b) {
         of.tile_bits = in.mf = choose_m(b, in.n, 1);
         orv.tile_bits = in.mr = choose_m(b, in.n, 2);
         of.low_bits = orv.low_bits = std::min(5, in.n);
-        of.fuse_bits = orv.fuse_bits = 2;
+        of.fuse_bits = orv.fuse_bits = b->fuse_bits;
+        // MFMA blocks in the forward pass only (complex64); the reverse sweep's
+        // gradient probes keep its blocks small
+        of.use_mfma = b->dtype == DESV_C64 ? b->use_mfma : 0;
         orv.nv = 2;
         std::string err;
         if (!build_plan(in.fg, in.n, of, in.fp, err) || !build_plan(in.rg, in.n, orv, in.rp, err)) {
@@ -662,12 +667,18 @@ int desv_batch_set_initial_state(desv_batch b, int64_t index, const void *host_a
     return DESV_OK;
 }
 
-int desv_batch_configure(desv_batch b, int tile_bits, int use_graph) {
+int desv_batch_configure(desv_batch b, int tile_bits, int fuse_bits, int use_mfma, int use_graph) {
     if (!b) return bfail(DESV_EINVAL, "null batch");
     if (tile_bits < 0 || tile_bits > MAX_TILE_BITS) return bfail(DESV_EINVAL, "tile_bits out of range");
-    if (b->tile_bits != tile_bits || b->use_graph != (use_graph != 0)) b->built = false;
+    if (fuse_bits == 0) fuse_bits = 2;
+    if (fuse_bits < 1 || fuse_bits > 5) return bfail(DESV_EINVAL, "fuse_bits must be in [1, 5]");
+    const int mf = use_mfma != 0, ug = use_graph != 0;
+    if (b->tile_bits != tile_bits || b->fuse_bits != fuse_bits || b->use_mfma != mf || b->use_graph != ug)
+        b->built = false;
     b->tile_bits = tile_bits;
-    b->use_graph = use_graph != 0;
+    b->fuse_bits = fuse_bits;
+    b->use_mfma = mf;
+    b->use_graph = ug;
     return DESV_OK;
 }
 

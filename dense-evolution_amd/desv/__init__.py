@@ -17,7 +17,7 @@ from .observables import (PauliSumOperator, multiply_pauli_terms, pauli_expectat
                           pauli_hamiltonian_to_matrix, pauli_sum_expectation,
                           pauli_sum_expectation_jax, pauli_sum_matvec, pauli_sum_matvec_jax)
 from .program import Program
-from . import workloads
+from . import tune, workloads
 from .qasm import QASMCircuit, QASMParser
 from .simulator import DenseSVSimulator, DeviceState, StateHandle
 from .transpiler import QuantumTranspiler

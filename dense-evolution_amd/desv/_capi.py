@@ -84,7 +84,7 @@ _SIGS = {
     "desv_batch_set_params": ([_p, _i64, _pd], _i),
     "desv_batch_set_initial_state": ([_p, _i64, _p], _i),
     "desv_batch_get_state": ([_p, _i64, _p], _i),
-    "desv_batch_configure": ([_p, _i, _i], _i),
+    "desv_batch_configure": ([_p, _i, _i, _i, _i], _i),
     "desv_batch_run": ([_p, _pd, _pd], _i),
     "desv_batch_last_time_us": ([_p, _pd], _i),
 }

@@ -27,6 +27,8 @@ class Batch:
         self.ptr = p
         self.n_params: List[int] = []
         self.slots: List[np.ndarray] = []
+        self._shapes: List[Tuple[int, int, int]] = []
+        self.config = {"tile_bits": 0, "fuse_bits": 2, "use_mfma": False, "use_graph": True}
 
     def close(self):
         if getattr(self, "ptr", None) and self.ptr.value:
@@ -56,6 +58,7 @@ class Batch:
         check(lib.desv_batch_add(self.ptr, n_qubits, _capi.dptr(ops), len(ops), _capi.i64ptr(pmap), len(slots),
                                  _capi.u64ptr(xm), _capi.u64ptr(zm), _capi.dptr(coeff), len(xm), C.byref(idx)))
         self.n_params.append(len(slots))
+        self._shapes.append((int(n_qubits), len(ops), len(xm)))
         return idx.value
 
     def set_params(self, index: int, theta) -> None:
@@ -75,8 +78,22 @@ class Batch:
         check(lib.desv_batch_get_state(self.ptr, index, _capi.vptr(out)))
         return out
 
-    def configure(self, tile_bits: int = 0, use_graph: bool = True) -> None:
-        check(lib.desv_batch_configure(self.ptr, int(tile_bits), int(use_graph)))
+    def configure(self, tile_bits: int = 0, use_graph: bool = True, fuse_bits: int = 0,
+                  use_mfma: bool = False) -> None:
+        """Launch shape for the next run (the instances are recompiled)."""
+        check(lib.desv_batch_configure(self.ptr, int(tile_bits), int(fuse_bits), int(bool(use_mfma)),
+                                       int(use_graph)))
+        self.config = {"tile_bits": int(tile_bits), "fuse_bits": int(fuse_bits) or 2,
+                       "use_mfma": bool(use_mfma), "use_graph": bool(use_graph)}
+
+    def signature(self) -> str:
+        """Workload class of the batch for the autotuner's cache."""
+        return f"c{'64' if self.code == _capi.C64 else '128'}-inst{len(self.n_params)}-" + _shape_sig(self._shapes)
+
+    def autotune(self, candidates=None, reps: int = 2, cache=True):
+        """Time candidate launch shapes and keep the fastest (see desv.tune)."""
+        from .tune import autotune_batch
+        return autotune_batch(self, candidates=candidates, reps=reps, cache=cache)
 
     def add_ops(self, n_qubits: int, ops, param_of_op, n_params: int, xmask, zmask, coeff) -> int:
         """Add an instance from an op table and encoded Pauli terms."""
@@ -89,6 +106,7 @@ class Batch:
         check(lib.desv_batch_add(self.ptr, n_qubits, _capi.dptr(ops), len(ops), _capi.i64ptr(pmap), n_params,
                                  _capi.u64ptr(xm), _capi.u64ptr(zm), _capi.dptr(cf), len(xm), C.byref(idx)))
         self.n_params.append(int(n_params))
+        self._shapes.append((int(n_qubits), len(ops), len(xm)))
         return idx.value
 
     def energies(self) -> np.ndarray:
@@ -111,3 +129,11 @@ class Batch:
         v = C.c_double()
         check(lib.desv_batch_last_time_us(self.ptr, C.byref(v)))
         return v.value
+
+
+def _shape_sig(shapes) -> str:
+    """Compact digest of (qubits, ops, terms) per instance."""
+    import hashlib
+    h = hashlib.sha1(repr(sorted(shapes)).encode()).hexdigest()[:12]
+    ns = [x[0] for x in shapes]
+    return f"n{min(ns) if ns else 0}-{max(ns) if ns else 0}-{h}"

@@ -198,8 +198,11 @@ int desv_batch_set_params(desv_batch b, int64_t index, const double *theta);
 int desv_batch_set_initial_state(desv_batch b, int64_t index, const void *host_amps);
 /* Final state psi of instance i after the last run. */
 int desv_batch_get_state(desv_batch b, int64_t index, void *host_amps);
-/* tile_bits/threads: 0 = autotuned/default. use_graph: capture + replay. */
-int desv_batch_configure(desv_batch b, int tile_bits, int use_graph);
+/* Launch shape of the next build: tile_bits (0 = default: 11 for complex64,
+ * 10 for complex128, capped by LDS), fuse_bits (0 = 2), use_mfma (forward
+ * 4/5-qubit blocks on MFMA, complex64), use_graph (capture + replay).  The
+ * Python autotuner (desv.tune) times candidate shapes and sets the fastest. */
+int desv_batch_configure(desv_batch b, int tile_bits, int fuse_bits, int use_mfma, int use_graph);
 /* Evaluate everything; energies[i], grads[offset_i ...] in instance order
  * (grads packed by instance, n_params_i each).  grads == NULL runs the
  * forward pass and energies only. */

@@ -164,3 +164,44 @@ def test_batch_refresh_graph_and_states(dtype, tol):
         assert abs(es2[0] - np.real(np.vdot(sv, O.pauli_sum_matvec(sv, t0)))) < tol
         assert O.fidelity(b.get_state(0, n0), sv) > 1 - 10 * tol
         b.close()
+
+
+@pytest.mark.parametrize("dtype,tol,shape", [
+    ("complex128", 1e-9, dict(tile_bits=8, fuse_bits=3)),
+    ("complex64", 2e-4, dict(tile_bits=8, fuse_bits=4, use_mfma=True)),
+    ("complex64", 2e-4, dict(tile_bits=10, fuse_bits=5, use_mfma=True)),
+])
+def test_batch_launch_shapes(dtype, tol, shape):
+    """Wider fusion (VALU k = 3 blocks, forward MFMA blocks) in the batched
+    launches keeps energies and gradients on the oracle."""
+    from desv.batch import Batch
+    insts = workloads.batch_instances(count=5, n_lo=6, n_hi=11, l_lo=2, l_hi=3, n_terms=12)
+    b = Batch(dtype)
+    b.configure(**shape)
+    for n, circ, theta, terms in insts:
+        b.add(circ, n, theta, terms)
+    es, gs = b.run()
+    for (n, circ, theta, terms), e, g in zip(insts, es, gs):
+        e_ref, g_ref = O.adjoint_gradient(O.build_template(circ.ops, n), theta, terms, n)
+        assert abs(e - e_ref) < tol
+        assert np.max(np.abs(g - g_ref)) < tol
+    b.close()
+
+
+def test_batch_autotune_keeps_parity(tmp_path, monkeypatch):
+    from desv.batch import Batch
+    monkeypatch.setenv("DESV_TUNE_CACHE", str(tmp_path / "tune.json"))
+    insts = workloads.batch_instances(count=6, n_lo=5, n_hi=12, l_lo=1, l_hi=3, n_terms=10)
+    b = Batch("complex64")
+    for n, circ, theta, terms in insts:
+        b.add(circ, n, theta, terms)
+    res = b.autotune(reps=1)
+    assert res["best"] in [{k: c[k] for k in ("tile_bits", "fuse_bits", "use_mfma")}
+                           for c in res["candidates"] if c["ms"] is not None]
+    es, gs = b.run()
+    for (n, circ, theta, terms), e, g in zip(insts, es, gs):
+        e_ref, g_ref = O.adjoint_gradient(O.build_template(circ.ops, n), theta, terms, n)
+        assert abs(e - e_ref) < 2e-4
+        assert np.max(np.abs(g - g_ref)) < 2e-4
+    assert b.autotune()["cached"]
+    b.close()

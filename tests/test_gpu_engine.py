@@ -305,3 +305,24 @@ def test_fidelity_and_sampling():
     assert abs(desv.statevector_fidelity(sim.sv, ghz) - 1) < 1e-12
     counts = desv.sample_counts(sim.sv, 2000, rng=np.random.default_rng(1))
     assert set(counts) == {'000', '111'} and sum(counts.values()) == 2000
+
+
+def test_program_autotune_keeps_parity(rng, tmp_path, monkeypatch):
+    monkeypatch.setenv("DESV_TUNE_CACHE", str(tmp_path / "tune.json"))
+    n = 14
+    prog = desv.Program(n)
+    want = rand_state(n, rng)
+    sv0 = want.copy()
+    for u, a, b in workloads.brickwork_su4(n, 8, seed=5):
+        prog.add_matrix(u, [a, b])
+        want = O.apply_matrix(want, u, [a, b])
+    cands = [dict(fuse_bits=2, tile_bits=11, use_mfma=False), dict(fuse_bits=4, tile_bits=12, use_mfma=True),
+             dict(fuse_bits=5, tile_bits=12, use_mfma=True), dict(fuse_bits=3, tile_bits=10, use_mfma=False)]
+    res = desv.tune.autotune_program(prog, "complex64", candidates=cands, signature="brick14")
+    assert not res["cached"] and all(c["ms"] is not None for c in res["candidates"])
+    h = desv.StateHandle(n, False)
+    h.upload(sv0)
+    prog.run(h)
+    assert fid(h.download(), want) >= 1 - 1e-6
+    h.close()
+    assert desv.tune.autotune_program(prog, "complex64", candidates=cands, signature="brick14")["cached"]
