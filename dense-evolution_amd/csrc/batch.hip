@@ -175,11 +175,7 @@ int decode_instance(BInst &in, bool structure) {
 
 void write_mats(const Plan &p, int dtype, void *host, size_t off) {
     if (dtype == DESV_C64) {
-        float *f = reinterpret_cast<float *>(host) + 2 * off;
-        for (size_t i = 0; i < p.mats.size(); ++i) {
-            f[2 * i] = (float)p.mats[i].real();
-            f[2 * i + 1] = (float)p.mats[i].imag();
-        }
+        round_mats_f32(p, reinterpret_cast<float *>(host) + 2 * off);
     } else {
         double *f = reinterpret_cast<double *>(host) + 2 * off;
         for (size_t i = 0; i < p.mats.size(); ++i) {

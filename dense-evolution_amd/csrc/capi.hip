@@ -68,11 +68,7 @@ int upload_plan(const Plan &p, int dtype, int device, hipStream_t s, DevPlan &d)
     std::vector<unsigned char> host(need);
     std::memcpy(host.data(), p.ops.data(), p.ops.size() * sizeof(DevOp));
     if (dtype == DESV_C64) {
-        float *f = reinterpret_cast<float *>(host.data() + oba);
-        for (size_t i = 0; i < p.mats.size(); ++i) {
-            f[2 * i] = (float)p.mats[i].real();
-            f[2 * i + 1] = (float)p.mats[i].imag();
-        }
+        round_mats_f32(p, reinterpret_cast<float *>(host.data() + oba));
     } else {
         double *f = reinterpret_cast<double *>(host.data() + oba);
         for (size_t i = 0; i < p.mats.size(); ++i) {

@@ -72,7 +72,7 @@ enum OpKind : int32_t {
     OPK_SCALAR = 4,   // whole-tile scalar
     OPK_GRAD = 5,     // adjoint inner product (dense k<=2 on tile bits)
     OPK_GRADDIAG = 6, // adjoint inner product, diagonal generator
-    OPK_MFMA = 7,     // 2^k x 2^k (k = 4, 5) real-split complex GEMM on MFMA
+    OPK_MFMA = 7,     // 2^k x 2^k (k = 3..5) complex GEMM on MFMA (complex64)
 };
 
 struct DevOp {
@@ -147,5 +147,9 @@ int set_last_error(int code, const std::string &msg);
 // Recompute plan.mats from a gate list with the same structure (same kinds,
 // bits and controls; only matrix entries differ) as the one compiled.
 void refresh_plan_mats(Plan &plan, const std::vector<Gate> &gates);
+
+// plan.mats rounded to complex64 (interleaved floats, 2 per entry), round to
+// nearest (products were formed in double).
+void round_mats_f32(const Plan &p, float *out);
 
 }  // namespace desv

@@ -345,6 +345,61 @@ __device__ inline void op_mfma(char *lds, int vstride, int m, const DevOp &op, M
     }
 }
 
+// Fused 3-qubit block on MFMA in the stacked real form: the 16x16 real
+// matrix R whose rows interleave the real and imaginary part of each output,
+//   R[2k][c] = Ur[k][c],  R[2k][8+c] = -Ui[k][c],
+//   R[2k+1][c] = Ui[k][c], R[2k+1][8+c] = Ur[k][c],
+// times the real 16-vector [Xr; Xi] of each column: four 16x16x4 MFMAs per
+// 16 columns (the same FLOPs as the complex product).  B fragment of step ks:
+// lane l -> [Xr; Xi][4 ks + l/16], i.e. rows l/16 and 4 + l/16 of X, real
+// parts for ks < 2 and imaginary parts for ks >= 2; accumulator lane l, reg j
+// -> R row 4 (l/16) + j, so lane group g holds outputs 2g and 2g+1 as
+// (re, im) register pairs.
+template <int NV, typename MP>
+__device__ inline void op_mfma3(char *lds, int vstride, int m, const DevOp &op, MP M) {
+    constexpr int EB = sizeof(float2);
+    const int lane = threadIdx.x & 63;
+    const int i16 = lane & 15, g4 = lane >> 4;
+    const int cl = (int)op.cloc;
+    const int p0 = op.pos[0], p1 = op.pos[1], p2 = op.pos[2];
+    const int fixed = cl | (1 << p0) | (1 << p1) | (1 << p2);
+    auto roff = [&](int r) { return ((r & 1) << p0) | ((r >> 1 & 1) << p1) | ((r >> 2 & 1) << p2); };
+    float a[4];
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+        const int kk = 4 * ks + g4;
+        const float2 u = gload(M + (i16 >> 1) * 8 + (kk & 7));
+        a[ks] = kk < 8 ? ((i16 & 1) ? u.y : u.x) : ((i16 & 1) ? u.x : -u.y);
+    }
+    const int pl = spread(i16, fixed) | cl;
+    const int pb0 = slot(pl | roff(g4)) * EB, pb1 = slot(pl | roff(4 + g4)) * EB;
+    const int py0 = slot(pl | roff(2 * g4)) * EB, py1 = slot(pl | roff(2 * g4 + 1)) * EB;
+    const int G = 1 << (m - __builtin_popcount(fixed));
+    const bool valid = i16 < G;  // tiles narrower than 16 columns
+    const int waves = blockDim.x >> 6;
+    for (int c0 = (threadIdx.x >> 6) * 16; c0 < G; c0 += waves * 16) {
+        const int U = slot(spread(c0, fixed)) * EB;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            char *tb = lds + v * vstride;
+            float2 x0 = make_float2(0.f, 0.f), x1 = x0;
+            if (valid) {
+                x0 = lds_ld<float2>(tb, pb0 ^ U);
+                x1 = lds_ld<float2>(tb, pb1 ^ U);
+            }
+            f32x4_t y = f32x4_t{0.f, 0.f, 0.f, 0.f};
+            y = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0], x0.x, y, 0, 0, 0);
+            y = __builtin_amdgcn_mfma_f32_16x16x4f32(a[1], x1.x, y, 0, 0, 0);
+            y = __builtin_amdgcn_mfma_f32_16x16x4f32(a[2], x0.y, y, 0, 0, 0);
+            y = __builtin_amdgcn_mfma_f32_16x16x4f32(a[3], x1.y, y, 0, 0, 0);
+            if (valid) {
+                lds_st<float2>(tb, py0 ^ U, make_float2(y[0], y[1]));
+                lds_st<float2>(tb, py1 ^ U, make_float2(y[2], y[3]));
+            }
+        }
+    }
+}
+
 // Diagonal op: matrix-index bit j is tile bit pos[j] (pos >= 0) or global bit
 // -1-pos[j] outside the tile (constant over the tile, taken from `base`).
 // Groups enumerate the tile bits that are neither local targets nor
@@ -459,7 +514,7 @@ __device__ inline double op_grad_diag(char *lds, int vstride, int m, const DevOp
 // VAR selects the compiled micro-ops (register budget and occupancy):
 //   0  1-2 qubit dense, diagonal, scalar and gradient ops;
 //   1  + k = 3..5 VALU blocks;
-//   2  + k = 3 VALU blocks and k = 4, 5 MFMA blocks (complex64).
+//   2  + k = 3..5 MFMA blocks (complex64; k = 3 VALU blocks kept as well).
 template <typename T, int NV, int VAR>
 __global__ void __launch_bounds__(kThreads) k_stage(const StageTask *__restrict__ tasks,
                                                     const int32_t *__restrict__ wg_task, StageTask single) {
@@ -576,7 +631,8 @@ __global__ void __launch_bounds__(kThreads) k_stage(const StageTask *__restrict_
                 break;
             case OPK_MFMA:
                 if constexpr (VAR == 2 && std::is_same<T, float>::value) {
-                    if (op.k == 4) op_mfma<NV, 4>(lds, vstride, m, op, M);
+                    if (op.k == 3) op_mfma3<NV>(lds, vstride, m, op, M);
+                    else if (op.k == 4) op_mfma<NV, 4>(lds, vstride, m, op, M);
                     else op_mfma<NV, 5>(lds, vstride, m, op, M);
                 }
                 break;

@@ -422,7 +422,7 @@ bool lower_stage(const std::vector<Gate> &ops, uint64_t tile, int n, const Compi
             case G_DENSE:
                 if (g.k == 1) d.kind = OPK_DENSE1;
                 else if (g.k == 2) d.kind = OPK_DENSE2;
-                else if (o.use_mfma && g.k >= 4) { d.kind = OPK_MFMA; plan.mfma_blocks++; }
+                else if (o.use_mfma && g.k >= 3) { d.kind = OPK_MFMA; plan.mfma_blocks++; }
                 else d.kind = OPK_DENSEK;
                 break;
             case G_GRAD:
@@ -442,7 +442,7 @@ bool lower_stage(const std::vector<Gate> &ops, uint64_t tile, int n, const Compi
     }
     st.op_end = (int)plan.ops.size();
     st.n_grad = ngrad;
-    // kernel variant: 2 if any MFMA block (then VALU blocks are k = 3 only),
+    // kernel variant: 2 if any MFMA block (then no k >= 3 VALU blocks),
     // 1 if any k >= 3 VALU block, else 0
     st.heavy = 0;
     for (int i = st.op_begin; i < st.op_end; ++i) {
@@ -755,6 +755,13 @@ void refresh_plan_mats(Plan &plan, const std::vector<Gate> &gates) {
             const Gate g = materialise(b, gates);
             std::copy(g.m.begin(), g.m.end(), plan.mats.begin() + d.mat);
         }
+    }
+}
+
+void round_mats_f32(const Plan &p, float *out) {
+    for (size_t i = 0; i < p.mats.size(); ++i) {
+        out[2 * i] = (float)p.mats[i].real();
+        out[2 * i + 1] = (float)p.mats[i].imag();
     }
 }
 

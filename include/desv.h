@@ -115,7 +115,7 @@ typedef struct {
     int tile_bits;  /* amplitudes per LDS tile = 2^tile_bits; 0 = default    */
     int low_bits;   /* low index bits always in a tile (coalescing); 0 = dflt*/
     int per_gate;   /* 1: one HBM pass per fused block (no cache blocking)   */
-    int use_mfma;   /* 1: 4/5-qubit blocks run as MFMA complex GEMMs         */
+    int use_mfma;   /* 1: 3-5-qubit blocks run as MFMA GEMMs (complex64)     */
     int dtype;      /* DESV_C64 / DESV_C128: matrix precision of the upload */
 } desv_compile_opts;
 
@@ -200,7 +200,7 @@ int desv_batch_set_initial_state(desv_batch b, int64_t index, const void *host_a
 int desv_batch_get_state(desv_batch b, int64_t index, void *host_amps);
 /* Launch shape of the next build: tile_bits (0 = default: 11 for complex64,
  * 10 for complex128, capped by LDS), fuse_bits (0 = 2), use_mfma (forward
- * 4/5-qubit blocks on MFMA, complex64), use_graph (capture + replay).  The
+ * 3-5-qubit blocks on MFMA, complex64), use_graph (capture + replay).  The
  * Python autotuner (desv.tune) times candidate shapes and sets the fastest. */
 int desv_batch_configure(desv_batch b, int tile_bits, int fuse_bits, int use_mfma, int use_graph);
 /* Evaluate everything; energies[i], grads[offset_i ...] in instance order
