@@ -309,38 +309,58 @@ __device__ inline void op_mfma(char *lds, int vstride, int m, const DevOp &op, M
 
     const int G = 1 << (m - __builtin_popcount(fixed));
     const bool valid = i16 < G;  // tiles narrower than 16 columns
-    const int waves = blockDim.x >> 6;
-    for (int c0 = (threadIdx.x >> 6) * 16; c0 < G; c0 += waves * 16) {
-        const int U = slot(spread(c0, fixed)) * EB;
+    const int step = (blockDim.x >> 6) * 16;
+    // UN column blocks per iteration (independent accumulator chains with
+    // their LDS loads in flight together); 2 measured no faster than 1
+    constexpr int UN = 1;
+    for (int c0 = (threadIdx.x >> 6) * 16; c0 < G; c0 += UN * step) {
+        int U[UN];
+        bool on[UN];
+#pragma unroll
+        for (int u = 0; u < UN; ++u) {
+            on[u] = c0 + u * step < G;  // uniform over the wave
+            U[u] = on[u] ? slot(spread(c0 + u * step, fixed)) * EB : 0;
+        }
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             char *tb = lds + v * vstride;
-            float br[KS], bi[KS];
+            float br[UN][KS], bi[UN][KS];
 #pragma unroll
-            for (int ks = 0; ks < KS; ++ks) {
-                const float2 x = valid ? lds_ld<float2>(tb, pb[ks] ^ U) : make_float2(0.f, 0.f);
-                br[ks] = x.x;
-                bi[ks] = x.y;
-            }
-            f32x4_t yr[RB], yi[RB];
-#pragma unroll
-            for (int rb = 0; rb < RB; ++rb) {
-                yr[rb] = f32x4_t{0.f, 0.f, 0.f, 0.f};
-                yi[rb] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+            for (int u = 0; u < UN; ++u)
 #pragma unroll
                 for (int ks = 0; ks < KS; ++ks) {
-                    yr[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[rb][ks], br[ks], yr[rb], 0, 0, 0);
-                    yr[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[rb][ks], -bi[ks], yr[rb], 0, 0, 0);
-                    yi[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[rb][ks], bi[ks], yi[rb], 0, 0, 0);
-                    yi[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[rb][ks], br[ks], yi[rb], 0, 0, 0);
+                    const float2 x =
+                        (valid && on[u]) ? lds_ld<float2>(tb, pb[ks] ^ U[u]) : make_float2(0.f, 0.f);
+                    br[u][ks] = x.x;
+                    bi[u][ks] = x.y;
                 }
-            }
-            if (valid) {
+            f32x4_t yr[UN][RB], yi[UN][RB];
 #pragma unroll
-                for (int rb = 0; rb < RB; ++rb)
+            for (int rb = 0; rb < RB; ++rb) {
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) lds_st<float2>(tb, py[rb][j] ^ U, make_float2(yr[rb][j], yi[rb][j]));
+                for (int u = 0; u < UN; ++u) {
+                    yr[u][rb] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+                    yi[u][rb] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+                }
+#pragma unroll
+                for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+                    for (int u = 0; u < UN; ++u) {
+                        yr[u][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[rb][ks], br[u][ks], yr[u][rb], 0, 0, 0);
+                        yi[u][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[rb][ks], bi[u][ks], yi[u][rb], 0, 0, 0);
+                        yr[u][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[rb][ks], -bi[u][ks], yr[u][rb], 0, 0, 0);
+                        yi[u][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[rb][ks], br[u][ks], yi[u][rb], 0, 0, 0);
+                    }
             }
+#pragma unroll
+            for (int u = 0; u < UN; ++u)
+                if (valid && on[u]) {
+#pragma unroll
+                    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+                        for (int j = 0; j < 4; ++j)
+                            lds_st<float2>(tb, py[rb][j] ^ U[u], make_float2(yr[u][rb][j], yi[u][rb][j]));
+                }
         }
     }
 }
@@ -376,26 +396,46 @@ __device__ inline void op_mfma3(char *lds, int vstride, int m, const DevOp &op, 
     const int py0 = slot(pl | roff(2 * g4)) * EB, py1 = slot(pl | roff(2 * g4 + 1)) * EB;
     const int G = 1 << (m - __builtin_popcount(fixed));
     const bool valid = i16 < G;  // tiles narrower than 16 columns
-    const int waves = blockDim.x >> 6;
-    for (int c0 = (threadIdx.x >> 6) * 16; c0 < G; c0 += waves * 16) {
-        const int U = slot(spread(c0, fixed)) * EB;
+    const int step = (blockDim.x >> 6) * 16;
+    constexpr int UN = 1;  // column blocks per iteration; 4 measured no faster than 1
+    for (int c0 = (threadIdx.x >> 6) * 16; c0 < G; c0 += UN * step) {
+        int U[UN];
+        bool on[UN];
+#pragma unroll
+        for (int u = 0; u < UN; ++u) {
+            on[u] = c0 + u * step < G;  // uniform over the wave
+            U[u] = on[u] ? slot(spread(c0 + u * step, fixed)) * EB : 0;
+        }
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             char *tb = lds + v * vstride;
-            float2 x0 = make_float2(0.f, 0.f), x1 = x0;
-            if (valid) {
-                x0 = lds_ld<float2>(tb, pb0 ^ U);
-                x1 = lds_ld<float2>(tb, pb1 ^ U);
+            float2 x0[UN], x1[UN];
+#pragma unroll
+            for (int u = 0; u < UN; ++u) {
+                x0[u] = make_float2(0.f, 0.f);
+                x1[u] = x0[u];
+                if (valid && on[u]) {
+                    x0[u] = lds_ld<float2>(tb, pb0 ^ U[u]);
+                    x1[u] = lds_ld<float2>(tb, pb1 ^ U[u]);
+                }
             }
-            f32x4_t y = f32x4_t{0.f, 0.f, 0.f, 0.f};
-            y = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0], x0.x, y, 0, 0, 0);
-            y = __builtin_amdgcn_mfma_f32_16x16x4f32(a[1], x1.x, y, 0, 0, 0);
-            y = __builtin_amdgcn_mfma_f32_16x16x4f32(a[2], x0.y, y, 0, 0, 0);
-            y = __builtin_amdgcn_mfma_f32_16x16x4f32(a[3], x1.y, y, 0, 0, 0);
-            if (valid) {
-                lds_st<float2>(tb, py0 ^ U, make_float2(y[0], y[1]));
-                lds_st<float2>(tb, py1 ^ U, make_float2(y[2], y[3]));
-            }
+            f32x4_t y[UN];
+#pragma unroll
+            for (int u = 0; u < UN; ++u) y[u] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int u = 0; u < UN; ++u) y[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0], x0[u].x, y[u], 0, 0, 0);
+#pragma unroll
+            for (int u = 0; u < UN; ++u) y[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[1], x1[u].x, y[u], 0, 0, 0);
+#pragma unroll
+            for (int u = 0; u < UN; ++u) y[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[2], x0[u].y, y[u], 0, 0, 0);
+#pragma unroll
+            for (int u = 0; u < UN; ++u) y[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[3], x1[u].y, y[u], 0, 0, 0);
+#pragma unroll
+            for (int u = 0; u < UN; ++u)
+                if (valid && on[u]) {
+                    lds_st<float2>(tb, py0 ^ U[u], make_float2(y[u][0], y[u][1]));
+                    lds_st<float2>(tb, py1 ^ U[u], make_float2(y[u][2], y[u][3]));
+                }
         }
     }
 }
@@ -516,7 +556,7 @@ __device__ inline double op_grad_diag(char *lds, int vstride, int m, const DevOp
 //   1  + k = 3..5 VALU blocks;
 //   2  + k = 3..5 MFMA blocks (complex64; k = 3 VALU blocks kept as well).
 template <typename T, int NV, int VAR>
-__global__ void __launch_bounds__(kThreads) k_stage(const StageTask *__restrict__ tasks,
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(std::is_same<T, float>::value ? 4 : 1))) k_stage(const StageTask *__restrict__ tasks,
                                                     const int32_t *__restrict__ wg_task, StageTask single) {
     using C = typename VT<T>::c;
     using V = typename VT<T>::v;
