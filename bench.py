@@ -101,6 +101,8 @@ def suite_vqe20():
     H = desv.PauliSumOperator(terms, n)
     f128, _ = desv.circuit_to_energy_fn(circ, n, dtype="complex128")
     f64, _ = desv.circuit_to_energy_fn(circ, n, dtype="complex64")
+    tuned = {name: f.autotune(theta, H, cache=False)["best"] for name, f in (("c64", f64), ("c128", f128))}
+    log(f"vqe20 autotune: {tuned}")
     f128.value_and_grad(theta, H)
     f64.value_and_grad(theta, H)
     t128, (e128, g128) = _timed(lambda: f128.value_and_grad(theta, H), reps=5)
@@ -108,6 +110,7 @@ def suite_vqe20():
     tps, gps = _timed(lambda: desv.parameter_shift_grad(f64.program, theta, H), reps=1)
     return {"workload": "20q HEA 8 layers (472 gates, 320 params), 200 random Pauli terms",
             "adjoint_evals_per_s_c64": 1.0 / t64, "adjoint_evals_per_s_c128": 1.0 / t128,
+            "autotuned": tuned,
             "param_shift_evals_per_s_c64": 1.0 / tps,
             "adjoint_over_param_shift_c64": tps / t64,
             "energy_c128": e128, "abs_dE_c64_vs_c128": abs(e64 - e128),

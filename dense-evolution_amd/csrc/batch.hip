@@ -139,6 +139,9 @@ namespace {
 int choose_m(desv_batch b, int n, int nv) {
     const size_t lim = stage_lds_limit(b->device);
     int m = b->tile_bits > 0 ? b->tile_bits : (b->dtype == DESV_C64 ? 11 : 10);
+    // a lone instance fills the GPU only with enough tiles: keep >= 2^11
+    // workgroups (8 per CU) unless the tile would drop below 2^8
+    if (b->tile_bits == 0 && b->inst.size() == 1) m = std::min(m, std::max(8, n - 11));
     while (m > 6 && (size_t)nv * ((size_t)1 << m) * abytes(b->dtype) > lim) --m;
     return std::min(n, m);
 }
@@ -268,7 +271,9 @@ int build(desv_batch b) {
         o[i].fops = off; off = align256(off + in.fp.ops.size() * sizeof(DevOp));
         o[i].rops = off; off = align256(off + in.rp.ops.size() * sizeof(DevOp));
         o[i].gpart = off; off = align256(off + (size_t)std::max(1, in.rp.n_grad_slots) * in.pstride * 8);
-        in.mv_chunk = std::min<int64_t>(1ll << in.n, 4096);
+        // one amplitude per lane: many workgroups keep the gathers of lambda = H psi
+        // in flight (the kernel is latency-bound, not bandwidth-bound)
+        in.mv_chunk = std::min<int64_t>(1ll << in.n, kThreads);
         in.mv_wg = ((1ll << in.n) + in.mv_chunk - 1) / in.mv_chunk;
         o[i].epart = off; off = align256(off + in.mv_wg * 8);
         // Pauli groups (sorted by flip mask), i^#Y folded into the coefficient

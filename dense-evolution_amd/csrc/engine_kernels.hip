@@ -773,27 +773,41 @@ __global__ void __launch_bounds__(kThreads) k_matvec_tasks(const MatvecTask *__r
     const C *psi = reinterpret_cast<const C *>(tk.psi);
     C *lam = reinterpret_cast<C *>(tk.lam);
     C *psi2 = reinterpret_cast<C *>(tk.psi2);
-    const MatvecGroup *groups = reinterpret_cast<const MatvecGroup *>(tk.groups);
+    // term tables through wave-uniform global pointers (scalar loads)
+    const DESV_GLOBAL MatvecGroup *groups = gptr(reinterpret_cast<const MatvecGroup *>(tk.groups));
+    const DESV_GLOBAL uint64_t *zt = gptr(tk.z);
+    const DESV_GLOBAL double2 *ct = gptr(tk.cc);
+    const int ng = uni(tk.ngroups);
+    constexpr int UG = 4;  // flip-mask groups whose gathers are in flight together
     double e = 0;
     for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
         double2 acc = make_double2(0.0, 0.0);
-        for (int g = 0; g < tk.ngroups; ++g) {
-            const MatvecGroup gr = groups[g];
-            const uint64_t s = (uint64_t)i ^ gr.x;
-            double fr = 0, fi = 0;
-            for (int q = gr.begin; q < gr.end; ++q) {
-                const double2 c = tk.cc[q];
-                if (__popcll(s & tk.z[q]) & 1) {
-                    fr -= c.x;
-                    fi -= c.y;
-                } else {
-                    fr += c.x;
-                    fi += c.y;
+        for (int g0 = 0; g0 < ng; g0 += UG) {
+            double2 a[UG];
+            uint64_t sidx[UG];
+#pragma unroll
+            for (int u = 0; u < UG; ++u) {
+                a[u] = make_double2(0.0, 0.0);
+                sidx[u] = 0;
+                if (g0 + u < ng) {
+                    sidx[u] = (uint64_t)i ^ groups[g0 + u].x;
+                    a[u] = to_d(psi[sidx[u]]);
                 }
             }
-            const double2 a = to_d(psi[s]);
-            acc.x += fr * a.x - fi * a.y;
-            acc.y += fr * a.y + fi * a.x;
+#pragma unroll
+            for (int u = 0; u < UG; ++u) {
+                if (g0 + u >= ng) break;
+                const int qb = groups[g0 + u].begin, qe = groups[g0 + u].end;
+                double fr = 0, fi = 0;
+                for (int q = qb; q < qe; ++q) {
+                    const double2 c = gload(ct + q);
+                    const double sg = (__popcll(sidx[u] & zt[q]) & 1) ? -1.0 : 1.0;
+                    fr = __builtin_fma(sg, c.x, fr);
+                    fi = __builtin_fma(sg, c.y, fi);
+                }
+                acc.x += fr * a[u].x - fi * a[u].y;
+                acc.y += fr * a[u].y + fi * a[u].x;
+            }
         }
         const C p = psi[i];
         const double2 pd = to_d(p);

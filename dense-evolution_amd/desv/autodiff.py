@@ -106,6 +106,12 @@ class AdjointProgram:
             self._batch, self._bkey, self._bham = b, key, hamiltonian
         return self._batch
 
+    def autotune(self, theta, hamiltonian, candidates=None, reps: int = 3, cache: bool = True):
+        """Choose the launch shape of this program's adjoint evaluation for a
+        Pauli-sum Hamiltonian by timing candidates (see desv.tune)."""
+        b = self._prep_batch(theta, hamiltonian, None)
+        return b.autotune(candidates=candidates, reps=reps, cache=cache)
+
     def _prep_batch(self, theta, hamiltonian, initial_state):
         b = self._batch_for(hamiltonian)
         b.set_params(0, np.asarray(theta, dtype=np.float64).reshape(-1))
@@ -305,7 +311,13 @@ def circuit_to_energy_fn(circuit, n_qubits: int, dtype=None, device: int = 0) ->
         full[:prog.n_params] = g
         return e, full
 
+    def autotune(theta, h_matrix, **kw):
+        """Time launch shapes of the adjoint evaluation for a Pauli-sum H."""
+        th = np.asarray(theta, dtype=np.float64).reshape(-1)
+        return prog.autotune(_theta_for_prog(th), h_matrix, **kw)
+
     energy_fn.value_and_grad = value_and_grad
+    energy_fn.autotune = autotune
     energy_fn.program = prog
     return energy_fn, n_params
 
