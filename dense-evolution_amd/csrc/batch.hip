@@ -139,9 +139,9 @@ namespace {
 int choose_m(desv_batch b, int n, int nv) {
     const size_t lim = stage_lds_limit(b->device);
     int m = b->tile_bits > 0 ? b->tile_bits : (b->dtype == DESV_C64 ? 11 : 10);
-    // a lone instance fills the GPU only with enough tiles: keep >= 2^11
-    // workgroups (8 per CU) unless the tile would drop below 2^8
-    if (b->tile_bits == 0 && b->inst.size() == 1) m = std::min(m, std::max(8, n - 11));
+    // a lone instance fills the GPU only with enough tiles: keep >= 2^10
+    // workgroups (4 per CU) unless the tile would drop below 2^8
+    if (b->tile_bits == 0 && b->inst.size() == 1) m = std::min(m, std::max(8, n - 10));
     while (m > 6 && (size_t)nv * ((size_t)1 << m) * abytes(b->dtype) > lim) --m;
     return std::min(n, m);
 }
