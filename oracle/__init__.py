@@ -13,6 +13,10 @@ Contents
                 function, and c128 adjoint / parameter-shift / finite-difference
                 gradients.  Every function cites the reference file:line it
                 follows.
+``sv_oracle_c`` ctypes binding of ``c/sv_oracle.c``, a C + OpenMP
+                restatement of the gate semantics (op tables, k-qubit
+                matrices) for checks at larger sizes and the CPU baseline;
+                pinned against ``sv_oracle`` in ``tests/test_oracle_c.py``.
 
 Pinning (what keeps the oracle honest; each check lives in
 ``tests/test_oracle.py``):
