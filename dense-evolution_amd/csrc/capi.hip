@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -563,6 +564,39 @@ int desv_probabilities(desv_handle h, double *out) {
     if (e == hipSuccess) e = hipMemcpyAsync(out, d, h->dim * sizeof(double), hipMemcpyDeviceToHost, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     (void)hipFree(d);
+    if (e != hipSuccess) return fail(DESV_EDEVICE, hipGetErrorString(e));
+    return DESV_OK;
+}
+
+int desv_sample(desv_handle h, const double *uniforms, int64_t n_shots, int64_t *out) {
+    if (int rc = check_handle(h)) return rc;
+    if (n_shots < 1) return fail(DESV_EINVAL, "n_shots must be >= 1, got " + std::to_string(n_shots));
+    if (!uniforms || !out) return fail(DESV_EINVAL, "null sample buffers");
+    HIP_TRY(hipSetDevice(h->device));
+    if (int rc = ensure_aux(h, 1)) return rc;  // the CDF (2^n doubles fit in a state buffer)
+    const int64_t nb = cdf_blocks(h->dim);
+    if (int rc = ensure_part(h, (size_t)nb + 1)) return rc;
+    double *cdf = reinterpret_cast<double *>(h->aux[1]);
+    hipError_t e = h->dtype == DESV_C64 ? launch_cdf<float>(h->sv, h->dim, cdf, h->d_part, h->stream)
+                                        : launch_cdf<double>(h->sv, h->dim, cdf, h->d_part, h->stream);
+    if (e != hipSuccess) return fail(DESV_EDEVICE, hipGetErrorString(e));
+    double total = 0;
+    HIP_TRY(hipMemcpyAsync(&total, h->d_part + nb, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (!(total > 0) || !std::isfinite(total))
+        return fail(DESV_EINVAL, "statevector has zero or non-finite total probability");
+    double *du = nullptr;
+    int64_t *dout = nullptr;
+    e = hipMalloc(&du, (size_t)n_shots * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&dout, (size_t)n_shots * sizeof(int64_t));
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(du, uniforms, (size_t)n_shots * sizeof(double), hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess) e = launch_sample_search(cdf, h->dim, du, n_shots, dout, h->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(out, dout, (size_t)n_shots * sizeof(int64_t), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (du) (void)hipFree(du);
+    if (dout) (void)hipFree(dout);
     if (e != hipSuccess) return fail(DESV_EDEVICE, hipGetErrorString(e));
     return DESV_OK;
 }

@@ -1227,6 +1227,107 @@ __global__ void k_probs(const typename VT<T>::c *sv, int64_t dim, double *out) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Shot sampling: the probability CDF of the state in three deterministic
+// passes (per-workgroup scans, one scan of the workgroup totals, offset and
+// normalise), then one binary search per shot.
+// ---------------------------------------------------------------------------
+constexpr int kScanChunk = 4096;  // amplitudes per workgroup, 16 per lane
+
+__device__ inline double block_inclusive_scan(double v, double *ts) {
+    ts[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < (int)blockDim.x; off <<= 1) {
+        const double t = (int)threadIdx.x >= off ? ts[threadIdx.x - off] : 0.0;
+        __syncthreads();
+        ts[threadIdx.x] += t;
+        __syncthreads();
+    }
+    return ts[threadIdx.x];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_cdf_local(const typename VT<T>::c *sv, int64_t dim, double *cdf,
+                                                        double *blk) {
+    __shared__ double ts[kThreads];
+    constexpr int per = kScanChunk / kThreads;
+    const int64_t b = (int64_t)blockIdx.x * kScanChunk + (int64_t)threadIdx.x * per;
+    double v[per], run = 0;
+#pragma unroll
+    for (int j = 0; j < per; ++j) {
+        double p = 0;
+        if (b + j < dim) {
+            const double2 a = to_d(sv[b + j]);
+            p = a.x * a.x + a.y * a.y;
+        }
+        run += p;
+        v[j] = run;
+    }
+    const double incl = block_inclusive_scan(run, ts);
+    const double before = incl - run;
+#pragma unroll
+    for (int j = 0; j < per; ++j)
+        if (b + j < dim) cdf[b + j] = before + v[j];
+    if (threadIdx.x == blockDim.x - 1) blk[blockIdx.x] = incl;
+}
+
+// blk[0..nb) <- exclusive prefix sums; blk[nb] <- total.  One workgroup.
+__global__ void __launch_bounds__(kThreads) k_scan_blocks(double *blk, int64_t nb) {
+    __shared__ double ts[kThreads];
+    const int64_t seg = (nb + kThreads - 1) / kThreads;
+    const int64_t s0 = (int64_t)threadIdx.x * seg, s1 = s0 + seg < nb ? s0 + seg : nb;
+    double run = 0;
+    for (int64_t i = s0; i < s1; ++i) run += blk[i];
+    const double incl = block_inclusive_scan(run, ts);
+    double acc = incl - run;
+    for (int64_t i = s0; i < s1; ++i) {
+        const double x = blk[i];
+        blk[i] = acc;
+        acc += x;
+    }
+    if (threadIdx.x == blockDim.x - 1) blk[nb] = incl;
+}
+
+__global__ void k_cdf_finish(double *cdf, int64_t dim, const double *blk, int64_t nb) {
+    const double total = blk[nb];
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < dim; i += (int64_t)gridDim.x * blockDim.x)
+        cdf[i] = (cdf[i] + blk[i / kScanChunk]) / total;
+}
+
+// out[s] = number of cdf entries <= u[s] (the first index whose cumulative
+// probability exceeds u[s]), clamped to dim - 1.
+__global__ void k_sample_search(const double *cdf, int64_t dim, const double *u, int64_t shots, int64_t *out) {
+    for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < shots; s += (int64_t)gridDim.x * blockDim.x) {
+        const double x = u[s];
+        int64_t lo = 0, hi = dim;
+        while (lo < hi) {
+            const int64_t mid = lo + ((hi - lo) >> 1);
+            if (cdf[mid] <= x) lo = mid + 1;
+            else hi = mid;
+        }
+        out[s] = lo < dim ? lo : dim - 1;
+    }
+}
+
+template <typename T>
+hipError_t launch_cdf(const void *sv, int64_t dim, double *cdf, double *blk, hipStream_t s) {
+    const int64_t nb = (dim + kScanChunk - 1) / kScanChunk;
+    if (nb > 0x7fffffffll) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_cdf_local<T>), dim3((unsigned)nb), dim3(kThreads), 0, s,
+                       (const typename VT<T>::c *)sv, dim, cdf, blk);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(kThreads), 0, s, blk, nb);
+    hipLaunchKernelGGL(k_cdf_finish, dim3(ew_grid(dim)), dim3(kThreads), 0, s, cdf, dim, (const double *)blk, nb);
+    return hipGetLastError();
+}
+int64_t cdf_blocks(int64_t dim) { return (dim + kScanChunk - 1) / kScanChunk; }
+hipError_t launch_sample_search(const double *cdf, int64_t dim, const double *u, int64_t shots, int64_t *out,
+                                hipStream_t s) {
+    if (shots <= 0) return hipSuccess;
+    const int64_t grid = std::min<int64_t>((shots + kThreads - 1) / kThreads, 1 << 16);
+    hipLaunchKernelGGL(k_sample_search, dim3((unsigned)grid), dim3(kThreads), 0, s, cdf, dim, u, shots, out);
+    return hipGetLastError();
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kThreads) k_qubit_probs(const typename VT<T>::c *sv, int64_t dim, int bit,
                                                           double *partials) {
@@ -1445,6 +1546,7 @@ hipError_t launch_dense_matvec(void *out, const void *in, int n, const double2 *
     template hipError_t launch_norm2<T>(const void *, int64_t, double *, int, hipStream_t);                   \
     template hipError_t launch_inner<T>(const void *, const void *, int64_t, double *, int, hipStream_t);     \
     template hipError_t launch_probs<T>(const void *, int64_t, double *, hipStream_t);                        \
+    template hipError_t launch_cdf<T>(const void *, int64_t, double *, double *, hipStream_t);                \
     template hipError_t launch_qubit_probs<T>(const void *, int64_t, int, double *, int, hipStream_t);        \
     template hipError_t launch_collapse<T>(void *, int64_t, int, int, double, hipStream_t);                   \
     template hipError_t launch_pauli_expect<T>(const void *, int, const PauliItem *, int, const uint64_t *,   \

@@ -130,6 +130,14 @@ template <typename T>
 hipError_t launch_inner(const void *a, const void *b, int64_t dim, double *partials, int nblk, hipStream_t s);
 template <typename T>
 hipError_t launch_probs(const void *sv, int64_t dim, double *out, hipStream_t s);
+// Normalised cumulative probabilities of a state into cdf[0..dim) (blk holds
+// cdf_blocks(dim) + 1 doubles of workspace; blk[cdf_blocks(dim)] = the total
+// before normalisation), and the shot search over it.
+template <typename T>
+hipError_t launch_cdf(const void *sv, int64_t dim, double *cdf, double *blk, hipStream_t s);
+int64_t cdf_blocks(int64_t dim);
+hipError_t launch_sample_search(const double *cdf, int64_t dim, const double *u, int64_t shots, int64_t *out,
+                                hipStream_t s);
 template <typename T>
 hipError_t launch_qubit_probs(const void *sv, int64_t dim, int bit, double *partials, int nblk, hipStream_t s);
 template <typename T>

@@ -61,6 +61,7 @@ _SIGS = {
     "desv_probabilities": ([_p, _pd], _i),
     "desv_qubit_probs": ([_p, _i, _pd, _pd], _i),
     "desv_collapse": ([_p, _i, _i, _d], _i),
+    "desv_sample": ([_p, _pd, _i64, _pi64], _i),
     "desv_run_ops": ([_p, _pd, _i64], _i),
     "desv_apply_matrix": ([_p, _pd, _pi, _i], _i),
     "desv_default_opts": ([_i, C.POINTER(CompileOpts)], _i),

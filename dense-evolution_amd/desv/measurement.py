@@ -1,8 +1,10 @@
 """Readout conveniences: shot sampling and pure-state fidelity.
 
-Same behaviour as the reference's ``utils/measurement.py:17-94``; the
-probabilities and the overlap are computed on the GPU (FP64 accumulation),
-the shot draw uses the caller's NumPy generator on the host.
+Same behaviour as the reference's ``utils/measurement.py:17-94``: shots are
+drawn with the caller's NumPy generator (the same uniforms its
+``rng.choice`` draws) and located in the state's cumulative probabilities on
+the GPU (FP64 scan, one binary search per shot); the overlap of two states is
+accumulated in FP64 on the GPU.
 """
 import ctypes as C
 
@@ -33,20 +35,18 @@ def sample_counts(statevector, n_shots, rng=None):
         raise ValueError(f"statevector length {dim} is not a power of 2")
     if n_shots < 1:
         raise ValueError(f"n_shots must be >= 1, got {n_shots}")
+    if rng is None:
+        rng = np.random.default_rng()
+    # the same draws as the reference's rng.choice(dim, n_shots, p=probs):
+    # n_shots uniforms, searched in the normalised CDF (built on the GPU)
+    u = np.ascontiguousarray(rng.random(n_shots), dtype=np.float64)
+    outcomes = np.empty(n_shots, dtype=np.int64)
     h, owned = _device(statevector)
     try:
-        probs = np.empty(dim, dtype=np.float64)
-        check(lib.desv_probabilities(h.ptr, _capi.dptr(probs)))
+        check(lib.desv_sample(h.ptr, _capi.dptr(u), n_shots, _capi.i64ptr(outcomes)))
     finally:
         if owned:
             h.close()
-    total = probs.sum()
-    if not np.isfinite(total) or total <= 0:
-        raise ValueError("statevector has zero or non-finite total probability")
-    probs = probs / total
-    if rng is None:
-        rng = np.random.default_rng()
-    outcomes = rng.choice(dim, size=n_shots, p=probs)
     values, freqs = np.unique(outcomes, return_counts=True)
     return {format(int(v), f'0{n}b'): int(f) for v, f in zip(values, freqs)}
 

@@ -92,6 +92,13 @@ int desv_inner(desv_handle a, desv_handle b, double *re, double *im);
 int desv_probabilities(desv_handle h, double *out_host);
 /* Marginal probabilities of one qubit (raw sums, not normalised). */
 int desv_qubit_probs(desv_handle h, int qubit, double *p0, double *p1);
+/* Shot sampling, replacing sample_counts' rng.choice(dim, n_shots, p=|a|^2 /
+ * total) (utils/measurement.py:17-68): the caller draws the uniforms in
+ * [0, 1) (numpy's choice draws rng.random(n_shots)); out[s] is the basis
+ * index whose normalised cumulative probability first exceeds uniforms[s]
+ * (searchsorted side='right').  The CDF is built on the device in FP64.
+ * DESV_EINVAL if the total probability is zero or not finite. */
+int desv_sample(desv_handle h, const double *uniforms, int64_t n_shots, int64_t *out);
 /* Zero the amplitudes whose `qubit` differs from `outcome`, scale the rest. */
 int desv_collapse(desv_handle h, int qubit, int outcome, double scale);
 

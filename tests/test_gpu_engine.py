@@ -326,3 +326,31 @@ def test_program_autotune_keeps_parity(rng, tmp_path, monkeypatch):
     assert fid(h.download(), want) >= 1 - 1e-6
     h.close()
     assert desv.tune.autotune_program(prog, "complex64", candidates=cands, signature="brick14")["cached"]
+
+
+@pytest.mark.parametrize("n,dtype", [(12, np.complex128), (22, np.complex64), (3, np.complex128)])
+def test_device_sampling_matches_rng_choice(n, dtype, rng):
+    """Shots drawn on the GPU from the caller's generator equal the
+    reference's rng.choice(dim, n_shots, p=probs) with the same seed (the FP64
+    CDF is summed in another order, so a uniform within rounding of a bin
+    edge may land in the neighbouring bin: at most a couple of shots move)."""
+    sv = rand_state(n, rng).astype(dtype)
+    shots = 20000
+    got = desv.sample_counts(sv, shots, rng=np.random.default_rng(7))
+    p = np.abs(sv.astype(np.complex128)) ** 2
+    p /= p.sum()
+    vals, freqs = np.unique(np.random.default_rng(7).choice(1 << n, size=shots, p=p), return_counts=True)
+    want = {format(int(v), f'0{n}b'): int(f) for v, f in zip(vals, freqs)}
+    moved = sum(abs(got.get(k, 0) - want.get(k, 0)) for k in set(got) | set(want))
+    assert sum(got.values()) == shots and moved <= 4
+
+
+def test_device_sampling_on_device_state_and_errors():
+    sim = desv.DenseSVSimulator(3)
+    sim.run_circuit([('h', 0), ('cx', 0, 1), ('cx', 1, 2)])
+    counts = desv.sample_counts(sim.sv, 4000, rng=np.random.default_rng(3))
+    assert set(counts) == {'000', '111'} and abs(counts['000'] - 2000) < 200
+    with pytest.raises(ValueError, match="zero or non-finite"):
+        desv.sample_counts(np.zeros(8, dtype=np.complex128), 10)
+    with pytest.raises(ValueError, match="n_shots"):
+        desv.sample_counts(np.ones(8, dtype=np.complex128), 0)

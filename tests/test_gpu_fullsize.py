@@ -82,3 +82,33 @@ def test_circuit_then_inverse_returns_to_zero(n, dtype, tol, opts):
     assert p0 >= 1 - tol
     h.close()
     zero.close()
+
+
+@pytest.mark.parametrize("dtype", ["complex64", "complex128"])
+def test_random_op_table_24q_against_c_oracle(dtype):
+    """24 qubits (beyond one L2, several tiles per stage and remapped write
+    backs) against the C + OpenMP oracle in complex128."""
+    import os
+    import subprocess
+    from oracle import sv_oracle_c as OC
+    if not os.path.exists(OC.LIB_PATH):
+        subprocess.run(["make", "-C", os.path.dirname(OC.LIB_PATH)], check=True)
+    n = 24
+    rng = np.random.default_rng(11)
+    rows = []
+    for _ in range(400):
+        if rng.random() < 0.4:
+            a, b = rng.choice(n, 2, replace=False)
+            rows.append([int(rng.choice([20, 21, 22, 23, 24, 25])), a, b, rng.uniform(-np.pi, np.pi)])
+        else:
+            rows.append([int(rng.integers(0, 15)), int(rng.integers(0, n)), 0, rng.uniform(-np.pi, np.pi)])
+    ops = np.array(rows, dtype=np.float64)
+    want = np.zeros(1 << n, dtype=np.complex128)
+    want[0] = 1
+    OC.run_ops(want, np.concatenate([[[1, q, 0, 0.0] for q in range(n)], ops]).reshape(-1, 4))
+    sim = desv.DenseSVSimulator(n, use_float32=dtype == "complex64")
+    sim.handle.run_ops(np.concatenate([[[1, q, 0, 0.0] for q in range(n)], ops]).reshape(-1, 4))
+    got = sim.get_statevector().astype(np.complex128)
+    f = abs(np.vdot(got, want)) ** 2
+    print(f"24q random ops {dtype}: 1 - fidelity = {1 - f:.3e}")
+    assert f >= (1 - 1e-12 if dtype == "complex128" else 1 - 1e-6)
