@@ -3,7 +3,8 @@
 Drop-in surface of Dense-Evolution's statevector hot path
 (``dense_evolution/__init__.py:78-131`` names): ``DenseSVSimulator``,
 ``QASMParser``/``QASMCircuit``, ``QuantumTranspiler``, the gate tables, the
-Pauli observables and ``circuit_to_energy_fn``.  The compute runs in
+Pauli observables, ``circuit_to_energy_fn`` and the statevector noise
+trajectories (``NoiseModel``, ``NoiseSpec``).  The compute runs in
 hand-written HIP kernels (``libdesv.so``) on the GPU; importing this package
 fails if the library is not built.
 """
@@ -13,6 +14,7 @@ from .circuits import entangling_layer, pauli_rotation_ops, trotter_evolve_ops
 from .config import ensure_x64, set_precision, with_x64
 from .gates import GATE_IDS, GATES, PARAMETRIC_GATES
 from .measurement import sample_counts, statevector_fidelity
+from .noise import NoiseModel, NoiseSpec
 from .observables import (PauliSumOperator, multiply_pauli_terms, pauli_expectation,
                           pauli_hamiltonian_to_matrix, pauli_sum_expectation,
                           pauli_sum_expectation_jax, pauli_sum_matvec, pauli_sum_matvec_jax)
@@ -33,5 +35,5 @@ __all__ = [
     "pauli_expectation", "pauli_sum_expectation", "pauli_hamiltonian_to_matrix", "pauli_sum_matvec",
     "multiply_pauli_terms", "pauli_sum_matvec_jax", "pauli_sum_expectation_jax", "PauliSumOperator",
     "circuit_to_energy_fn", "AdjointProgram", "parameter_shift_grad",
-    "sample_counts", "statevector_fidelity", "device_count",
+    "sample_counts", "statevector_fidelity", "device_count", "NoiseModel", "NoiseSpec",
 ]

@@ -127,14 +127,23 @@ class AdjointProgram:
         """DeviceState view of the state after the last evaluation."""
         return DeviceState(getattr(self, "_last_state", None) or self.handle)
 
-    def value_and_grad(self, theta, hamiltonian, initial_state=None) -> Tuple[float, np.ndarray]:
-        if isinstance(hamiltonian, PauliSumOperator):
+    def value_and_grad(self, theta, hamiltonian, initial_state=None,
+                       tail: Optional[np.ndarray] = None) -> Tuple[float, np.ndarray]:
+        """Energy and gradient; `tail` (op-table rows without parameters, e.g.
+        a Pauli noise trajectory) runs after the circuit."""
+        if isinstance(hamiltonian, PauliSumOperator) and (tail is None or len(tail) == 0):
             self.ops_for(theta)  # validates the length
             b = self._prep_batch(theta, hamiltonian, initial_state)
             e, g = b.run()
             return float(e[0]), g[0]
         self._last_state = None
-        ops = np.ascontiguousarray(self.ops_for(theta))
+        ops = self.ops_for(theta)
+        param_of_op = self.param_of_op
+        if tail is not None and len(tail):
+            tail = np.asarray(tail, dtype=np.float64).reshape(-1, 4)
+            ops = np.concatenate([ops, tail])
+            param_of_op = np.concatenate([param_of_op, np.full(len(tail), -1, dtype=np.int64)])
+        ops = np.ascontiguousarray(ops)
         h = self.handle
         if initial_state is None:
             h.reset()
@@ -149,20 +158,21 @@ class AdjointProgram:
             if np.any(np.imag(cs) != 0):
                 raise ValueError("Pauli-sum Hamiltonians must have real coefficients")
             coeff = np.ascontiguousarray(np.real(cs), dtype=np.float64)
-            check(lib.desv_adjoint(h.ptr, _capi.dptr(ops), len(ops), _capi.i64ptr(self.param_of_op),
+            check(lib.desv_adjoint(h.ptr, _capi.dptr(ops), len(ops), _capi.i64ptr(param_of_op),
                                    self.n_params, _capi.u64ptr(xm), _capi.u64ptr(zm), _capi.dptr(coeff),
                                    len(xm), None, C.byref(energy), _capi.dptr(grad)))
         else:
             H = _dense_h(hamiltonian, self.n)
-            check(lib.desv_adjoint(h.ptr, _capi.dptr(ops), len(ops), _capi.i64ptr(self.param_of_op),
+            check(lib.desv_adjoint(h.ptr, _capi.dptr(ops), len(ops), _capi.i64ptr(param_of_op),
                                    self.n_params, None, None, None, 0,
                                    H.view(np.float64).ctypes.data_as(_capi._pd), C.byref(energy),
                                    _capi.dptr(grad)))
         return energy.value, grad
 
-    def energy(self, theta, hamiltonian, initial_state=None) -> float:
-        """Forward only (no gradient sweep)."""
-        if isinstance(hamiltonian, PauliSumOperator):
+    def energy(self, theta, hamiltonian, initial_state=None, noise=None) -> float:
+        """Forward only (no gradient sweep); `noise` (a NoiseSpec) applies one
+        trajectory to the final state before the energy."""
+        if isinstance(hamiltonian, PauliSumOperator) and noise is None:
             self.ops_for(theta)
             b = self._prep_batch(theta, hamiltonian, initial_state)
             return float(b.energies()[0])
@@ -174,6 +184,10 @@ class AdjointProgram:
         else:
             h.upload(np.asarray(initial_state))
         h.run_ops(ops)
+        if noise is not None:
+            from .noise import NoiseModel
+            NoiseModel.apply_to_sv(h, self.n, noise.model, noise.p, rng=noise.generator(),
+                                   qubits=list(noise.qubits) if noise.qubits is not None else None)
         return _energy_of(h, hamiltonian, self.n)
 
     def close(self):
@@ -235,15 +249,15 @@ def _make_autograd():
         """E(theta) with dE/dtheta from one adjoint sweep (computed in forward)."""
 
         @staticmethod
-        def forward(ctx, theta, program, hamiltonian, initial_state):
-            e, g = program.value_and_grad(theta.detach().cpu().numpy(), hamiltonian, initial_state)
+        def forward(ctx, theta, program, hamiltonian, initial_state, tail):
+            e, g = program.value_and_grad(theta.detach().cpu().numpy(), hamiltonian, initial_state, tail)
             ctx.save_for_backward(torch.as_tensor(g, dtype=theta.dtype, device=theta.device))
             return torch.as_tensor(e, dtype=theta.dtype, device=theta.device)
 
         @staticmethod
         def backward(ctx, grad_out):
             (g,) = ctx.saved_tensors
-            return grad_out * g, None, None, None
+            return grad_out * g, None, None, None, None
 
     return AdjointEnergy
 
@@ -251,12 +265,13 @@ def _make_autograd():
 _AUTOGRAD = None
 
 
-def adjoint_energy(theta, program: AdjointProgram, hamiltonian, initial_state=None):
-    """Torch entry point: differentiable energy of `program` at `theta`."""
+def adjoint_energy(theta, program: AdjointProgram, hamiltonian, initial_state=None, tail=None):
+    """Torch entry point: differentiable energy of `program` at `theta`
+    (`tail`: parameter-free op rows run after the circuit)."""
     global _AUTOGRAD
     if _AUTOGRAD is None:
         _AUTOGRAD = _make_autograd()
-    return _AUTOGRAD.apply(theta, program, hamiltonian, initial_state)
+    return _AUTOGRAD.apply(theta, program, hamiltonian, initial_state, tail)
 
 
 def circuit_to_energy_fn(circuit, n_qubits: int, dtype=None, device: int = 0) -> Tuple[Callable, int]:
@@ -266,8 +281,12 @@ def circuit_to_energy_fn(circuit, n_qubits: int, dtype=None, device: int = 0) ->
     ``(energy, sv)``: a float (NumPy theta) or a differentiable torch scalar
     (torch theta), and a :class:`DeviceState` view of the final state (it
     aliases the function's device buffer and changes on the next call).
-    ``energy_fn.value_and_grad(theta, h_matrix, stato_zero=None)`` returns
-    ``(energy, grad)`` as NumPy values.
+    ``energy_fn.value_and_grad(theta, h_matrix, stato_zero=None, noise=None)``
+    returns ``(energy, grad)`` as NumPy values.  ``noise`` (a
+    :class:`desv.NoiseSpec`) applies one trajectory of its channel to the
+    final state before the energy (reference ``solvers/autodiff.py:160-164``);
+    gradients are available for the Pauli channels (depolarizing, bitflip,
+    phaseflip).
     """
     template = _build_template(circuit, n_qubits)
     n_params = sum(1 for op in circuit.ops if str(op['name']).lower().strip() in _PARAMETRIC_GATES)
@@ -285,8 +304,6 @@ def circuit_to_energy_fn(circuit, n_qubits: int, dtype=None, device: int = 0) ->
         return theta[:prog.n_params] if prog.n_params != n_params else theta
 
     def energy_fn(theta, h_matrix, stato_zero: Optional[np.ndarray] = None, noise=None):
-        if noise is not None:
-            raise NotImplementedError("noise trajectories are not part of this engine")
         try:
             import torch
             is_torch = isinstance(theta, torch.Tensor)
@@ -295,18 +312,18 @@ def circuit_to_energy_fn(circuit, n_qubits: int, dtype=None, device: int = 0) ->
         if is_torch:
             th = _theta_for_prog(theta)
             if th.requires_grad or theta.requires_grad:
-                e = adjoint_energy(th, prog, h_matrix, stato_zero)
+                e = adjoint_energy(th, prog, h_matrix, stato_zero, _noise_tail(noise, n_qubits))
             else:
-                e = torch.as_tensor(prog.energy(th.detach().cpu().numpy(), h_matrix, stato_zero),
+                e = torch.as_tensor(prog.energy(th.detach().cpu().numpy(), h_matrix, stato_zero, noise),
                                     dtype=theta.dtype, device=theta.device)
             return e, prog.state()
         th = _theta_for_prog(np.asarray(theta, dtype=np.float64).reshape(-1))
-        e = prog.energy(th, h_matrix, stato_zero)
+        e = prog.energy(th, h_matrix, stato_zero, noise)
         return e, prog.state()
 
-    def value_and_grad(theta, h_matrix, stato_zero=None):
+    def value_and_grad(theta, h_matrix, stato_zero=None, noise=None):
         th = np.asarray(theta, dtype=np.float64).reshape(-1)
-        e, g = prog.value_and_grad(_theta_for_prog(th), h_matrix, stato_zero)
+        e, g = prog.value_and_grad(_theta_for_prog(th), h_matrix, stato_zero, _noise_tail(noise, n_qubits))
         full = np.zeros(len(th))
         full[:prog.n_params] = g
         return e, full
@@ -320,6 +337,20 @@ def circuit_to_energy_fn(circuit, n_qubits: int, dtype=None, device: int = 0) ->
     energy_fn.autotune = autotune
     energy_fn.program = prog
     return energy_fn, n_params
+
+
+def _noise_tail(noise, n_qubits: int):
+    """A Pauli-channel trajectory as op rows run after the circuit (the same
+    draws as ``NoiseModel.apply_to_sv``); the gradient then follows from the
+    adjoint sweep.  The trajectory's X/Y/Z are unitary, so the trajectory's
+    renormalisation only divides by (1 + 1e-15).  Amplitude damping and
+    'combined' rescale by state-dependent jump probabilities and are not
+    differentiated here."""
+    if noise is None or noise.model == 'ideal' or noise.p <= 0:
+        return None
+    from .noise import pauli_trajectory
+    return pauli_trajectory(n_qubits, noise.model, float(noise.p), noise.generator(),
+                            list(noise.qubits) if noise.qubits is not None else None)
 
 
 def parameter_shift_grad(program: AdjointProgram, theta, hamiltonian) -> np.ndarray:

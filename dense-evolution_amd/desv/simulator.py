@@ -69,6 +69,12 @@ class StateHandle:
     def scale(self, z: complex) -> None:
         check(lib.desv_scale(self.ptr, float(np.real(z)), float(np.imag(z))))
 
+    def qubit_probs(self, qubit: int) -> Tuple[float, float]:
+        """(sum |amp|^2 over qubit = 0, over qubit = 1), FP64, not normalised."""
+        p0, p1 = C.c_double(), C.c_double()
+        check(lib.desv_qubit_probs(self.ptr, int(qubit), C.byref(p0), C.byref(p1)))
+        return p0.value, p1.value
+
     def reset(self) -> None:
         check(lib.desv_reset(self.ptr))
 

@@ -615,3 +615,84 @@ def trotter_evolve_ops(terms, t, n_steps, order=1):
         for c, d in reversed(terms):
             step += pauli_rotation_ops(d, c * dt / 2)
     return step * n_steps
+
+
+# ---------------------------------------------------------------------------
+# stochastic Kraus channels (NumPy path) -- noise/kraus_channels.py, kraus/*.py
+# ---------------------------------------------------------------------------
+
+NOISE_MODELS = ['ideal', 'depolarizing', 'bitflip', 'phaseflip', 'amplitude_damping', 'combined']
+
+
+def _noise_pairs(dim, q):
+    """idx_0 (bit q clear) and idx_0 | 1 << q: the reference indexes noise
+    qubits LSB-first (``kraus_channels.py:38-58``, shift at ``:54``)."""
+    step = 1 << q
+    i = np.arange(dim)
+    i0 = i[(i & step) == 0]
+    return i0, i0 | step
+
+
+def _depolarizing(sv, i0, i1, p, rng):
+    """``kraus/depolarizing.py:18-47`` (NumPy branch): r, ch drawn in that
+    order; fire if r < p; X if ch < 1/3, Y if ch < 2/3, else Z."""
+    r, ch = rng.random(), rng.random()
+    if r < p:
+        v0, v1 = sv[i0].copy(), sv[i1].copy()
+        if ch < 1.0 / 3.0:
+            sv[i0], sv[i1] = v1, v0
+        elif ch < 2.0 / 3.0:
+            sv[i0], sv[i1] = -1j * v1, 1j * v0
+        else:
+            sv[i1] = -v1
+
+
+def _bitflip(sv, i0, i1, p, rng):
+    """``kraus/bitflip.py:10-22``."""
+    if rng.random() < p:
+        v0, v1 = sv[i0].copy(), sv[i1].copy()
+        sv[i0], sv[i1] = v1, v0
+
+
+def _phaseflip(sv, i0, i1, p, rng):
+    """``kraus/phaseflip.py:10-21``."""
+    if rng.random() < p:
+        sv[i1] = -sv[i1]
+
+
+def _amplitude_damping(sv, i0, i1, p, rng):
+    """``kraus/amplitude_damping.py:21-47``: one jump decision per qubit with
+    the whole-state probability P(K1) = gamma * sum |v1|^2."""
+    gamma = float(np.clip(p, 0.0, 1.0))
+    r = rng.random()
+    v0, v1 = sv[i0].copy(), sv[i1].copy()
+    p1 = float(np.clip(gamma * np.sum(np.abs(v1) ** 2), 0.0, 1.0))
+    if r < p1:
+        sv[i0] = v1 * np.sqrt(gamma) / np.sqrt(max(p1, 1e-15))
+        sv[i1] = 0.0
+    else:
+        nn = np.sqrt(max(1.0 - p1, 1e-15))
+        sv[i0] = v0 / nn
+        sv[i1] = v1 * np.sqrt(1.0 - gamma) / nn
+
+
+def _combined(sv, i0, i1, p, rng):
+    """``kraus/combined.py:10-16``: depolarizing(p/2) then damping(p*0.333333)."""
+    _depolarizing(sv, i0, i1, p * 0.5, rng)
+    _amplitude_damping(sv, i0, i1, p * 0.333333, rng)
+
+
+def noise_apply_to_sv(sv, n, model, p, rng, qubits=None):
+    """``NoiseModel.apply_to_sv`` NumPy path (``kraus_channels.py:104-235``):
+    'ideal' or p <= 0 returns the input; otherwise a copy gets one channel
+    application per target qubit (all qubits by default, in list order) and
+    is divided by (norm + 1e-15)."""
+    if model == 'ideal' or p <= 0.0:
+        return sv
+    channel = {'depolarizing': _depolarizing, 'bitflip': _bitflip, 'phaseflip': _phaseflip,
+               'amplitude_damping': _amplitude_damping, 'combined': _combined}[model]
+    out = np.array(sv, dtype=np.complex128, copy=True)
+    for q in (qubits if qubits is not None else range(n)):
+        i0, i1 = _noise_pairs(len(out), q)
+        channel(out, i0, i1, p, rng)
+    return out / (np.linalg.norm(out) + 1e-15)

@@ -1,0 +1,124 @@
+"""Device noise trajectories against the oracle's restatement of the
+reference's NumPy path (``noise/kraus_channels.py:104-235``): the same seeded
+generator must give the same state (complex128 to 1e-12, complex64 to 1e-5),
+and ``energy_fn(noise=...)`` must give the oracle's noisy energy and, for
+Pauli channels, its gradient."""
+import numpy as np
+import pytest
+
+import desv
+from desv import workloads
+from oracle import sv_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+MODELS = ["depolarizing", "bitflip", "phaseflip", "amplitude_damping", "combined"]
+
+
+def _random_state(n, seed):
+    rng = np.random.default_rng(seed)
+    v = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    return v / np.linalg.norm(v)
+
+
+@pytest.mark.parametrize("model", MODELS)
+@pytest.mark.parametrize("qubits", [None, [4, 0, 7], [2]])
+def test_host_array_trajectories_match_oracle(model, qubits):
+    n = 9
+    sv = _random_state(n, 3)
+    for seed in range(4):
+        got = desv.NoiseModel.apply_to_sv(sv, n, model, 0.4, rng=np.random.default_rng(seed), qubits=qubits)
+        want = O.noise_apply_to_sv(sv, n, model, 0.4, np.random.default_rng(seed), qubits)
+        assert got.dtype == np.complex128
+        assert np.max(np.abs(got - want)) < 1e-12, (model, qubits, seed)
+    # the caller's array is not modified
+    assert np.array_equal(sv, _random_state(n, 3))
+
+
+@pytest.mark.parametrize("model", MODELS)
+def test_device_state_in_place_c64(model):
+    n = 14
+    sim = desv.DenseSVSimulator(n, use_float32=True)
+    circ = workloads.random_layers_circuit(n, 3, seed=5)
+    sim.run_circuit(circ)
+    before = sim.get_statevector().astype(np.complex128)
+    view = sim.sv
+    out = desv.NoiseModel.apply_to_sv(view, n, model, 0.3, rng=np.random.default_rng(8))
+    assert out is view and out.handle is sim.handle
+    got = sim.get_statevector()
+    want = O.noise_apply_to_sv(before, n, model, 0.3, np.random.default_rng(8))
+    assert np.max(np.abs(got - want)) < 1e-5
+    sim.close()
+
+
+def test_sequence_of_trajectories_advances_the_generator():
+    n = 5
+    sv = _random_state(n, 9)
+    g_dev, g_ref = np.random.default_rng(42), np.random.default_rng(42)
+    for _ in range(5):
+        got = desv.NoiseModel.apply_to_sv(sv, n, 'combined', 0.5, rng=g_dev)
+        want = O.noise_apply_to_sv(sv, n, 'combined', 0.5, g_ref)
+        assert np.max(np.abs(got - want)) < 1e-12
+
+
+def _vqe(n=6, seed=0):
+    vc, theta, terms = workloads.vqe_instance(n, 2, 12, seed=seed)
+    return vc, theta, terms
+
+
+@pytest.mark.parametrize("model", MODELS)
+def test_energy_fn_with_noise_matches_oracle(model):
+    n = 6
+    vc, theta, terms = _vqe(n)
+    fn, _ = desv.circuit_to_energy_fn(vc, n)
+    H = desv.PauliSumOperator(terms, n)
+    spec = desv.NoiseSpec(model, 0.35, rng=np.random.default_rng(17))
+    e, sv = fn(theta, H, noise=spec)
+    template = O.build_template(vc.ops, n)
+    _, psi = O.energy(O.patch_template(template, theta), terms, n)
+    noisy = O.noise_apply_to_sv(psi, n, model, 0.35, np.random.default_rng(17))
+    want = float(np.real(np.vdot(noisy, O.pauli_sum_matvec(noisy, terms))))
+    assert abs(e - want) < 1e-10
+    assert np.max(np.abs(np.asarray(sv) - noisy)) < 1e-12
+    # dense Hamiltonian, same trajectory
+    Hd = O.pauli_matrix(terms, n)
+    e2, _ = fn(theta, Hd, noise=desv.NoiseSpec(model, 0.35, rng=np.random.default_rng(17)))
+    assert abs(e2 - want) < 1e-10
+
+
+@pytest.mark.parametrize("model", ["depolarizing", "bitflip", "phaseflip"])
+def test_gradient_through_pauli_trajectory(model):
+    """The gradient with a fixed trajectory equals the oracle's adjoint
+    gradient of the circuit followed by the drawn Paulis, and a central
+    finite difference of the noisy energy with the same draws."""
+    n = 6
+    vc, theta, terms = _vqe(n, seed=2)
+    fn, _ = desv.circuit_to_energy_fn(vc, n)
+    H = desv.PauliSumOperator(terms, n)
+    e, g = fn.value_and_grad(theta, H, noise=desv.NoiseSpec(model, 0.5, rng=np.random.default_rng(3)))
+    e_val, _ = fn(theta, H, noise=desv.NoiseSpec(model, 0.5, rng=np.random.default_rng(3)))
+    assert abs(e - e_val) < 1e-12
+    from desv.noise import pauli_trajectory
+    tail = pauli_trajectory(n, model, 0.5, np.random.default_rng(3))
+    assert len(tail) > 0
+    template = np.concatenate([O.build_template(vc.ops, n), tail])
+    e_ref, g_ref = O.adjoint_gradient(template, theta, terms, n)
+    assert abs(e - e_ref) < 1e-12
+    assert np.max(np.abs(g - g_ref)) < 1e-10
+    g_fd = O.fd_gradient(template, theta, terms, n)
+    assert np.max(np.abs(g - g_fd)) < 1e-6
+
+
+def test_torch_backward_with_noise():
+    torch = pytest.importorskip("torch")
+    n = 5
+    vc, theta, terms = _vqe(n, seed=4)
+    fn, _ = desv.circuit_to_energy_fn(vc, n)
+    H = desv.PauliSumOperator(terms, n)
+    th = torch.tensor(theta, dtype=torch.float64, requires_grad=True)
+    e, _ = fn(th, H, noise=desv.NoiseSpec('depolarizing', 0.6, rng=np.random.default_rng(1)))
+    e.backward()
+    _, g = fn.value_and_grad(theta, H, noise=desv.NoiseSpec('depolarizing', 0.6, rng=np.random.default_rng(1)))
+    assert np.max(np.abs(th.grad.numpy() - g)) < 1e-12
+    with pytest.raises(NotImplementedError):
+        fn.value_and_grad(theta, H, noise=desv.NoiseSpec('amplitude_damping', 0.2, rng=np.random.default_rng(1)))

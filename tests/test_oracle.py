@@ -259,3 +259,83 @@ def test_trotter_converges_to_expm():
     e2 = np.linalg.norm(O.run_circuit(O.trotter_evolve_ops(terms, 0.5, 40), n, sv=psi0) - exact)
     assert e2 < e1 < 0.05
     assert 1.6 < e1 / e2 < 2.4  # first order
+
+
+# ---- stochastic Kraus channels (oracle restatement of the NumPy path) -------
+
+def _kraus_average(sv, n, model, p, shots, seed):
+    rng = np.random.default_rng(seed)
+    rho = np.zeros((1 << n, 1 << n), dtype=np.complex128)
+    for _ in range(shots):
+        v = O.noise_apply_to_sv(sv, n, model, p, rng)
+        rho += np.outer(v, v.conj())
+    return rho / shots
+
+
+def _lsb_op(m, q, n):
+    """2x2 operator on bit q (LSB-first, the noise indexing) of an n-bit index."""
+    out = np.array([[1.0 + 0j]])
+    for b in reversed(range(n)):
+        out = np.kron(out, m if b == q else np.eye(2))
+    return out
+
+
+def test_noise_closed_forms():
+    rng = np.random.default_rng(2)
+    n = 4
+    sv = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    sv /= np.linalg.norm(sv)
+    # p = 1: the flip always fires on every qubit
+    assert np.allclose(O.noise_apply_to_sv(sv, n, 'bitflip', 1.0, rng), sv[::-1])
+    parity = np.array([(-1) ** bin(i).count('1') for i in range(1 << n)])
+    assert np.allclose(O.noise_apply_to_sv(sv, n, 'phaseflip', 1.0, rng), parity * sv)
+    assert O.noise_apply_to_sv(sv, n, 'ideal', 0.3, rng) is sv
+    assert O.noise_apply_to_sv(sv, n, 'depolarizing', 0.0, rng) is sv
+    # a full decay of |1...1> on every qubit lands in |0...0>
+    one = np.zeros(1 << n, dtype=np.complex128)
+    one[-1] = 1
+    assert np.allclose(O.noise_apply_to_sv(one, n, 'amplitude_damping', 1.0, rng), np.eye(1 << n)[0])
+
+
+@pytest.mark.parametrize("model,p", [("depolarizing", 0.3), ("amplitude_damping", 0.4)])
+def test_noise_trajectories_average_to_the_kraus_map(model, p):
+    """The average of |psi'><psi'| over trajectories equals the Kraus map
+    applied qubit by qubit (Monte Carlo, 6000 shots, tolerance 0.03)."""
+    n = 2
+    sv = np.array([0.6, 0.48j, -0.36, 0.52], dtype=np.complex128)
+    sv /= np.linalg.norm(sv)
+    rho = np.outer(sv, sv.conj())
+    X = np.array([[0, 1], [1, 0]], dtype=complex)
+    Y = np.array([[0, -1j], [1j, 0]])
+    Z = np.diag([1.0 + 0j, -1.0])
+    for q in range(n):
+        if model == "depolarizing":
+            ks = [np.sqrt(1 - p) * np.eye(2), np.sqrt(p / 3) * X, np.sqrt(p / 3) * Y, np.sqrt(p / 3) * Z]
+        else:
+            ks = [np.diag([1.0, np.sqrt(1 - p)]) + 0j, np.array([[0, np.sqrt(p)], [0, 0]], dtype=complex)]
+        ks = [_lsb_op(k, q, n) for k in ks]
+        rho = sum(k @ rho @ k.conj().T for k in ks)
+    est = _kraus_average(sv, n, model, p, 6000, seed=5)
+    assert np.max(np.abs(est - rho)) < 0.03
+
+
+def _measure0_freq(sv, model, p, shots, seed):
+    """Frequency of outcome 0 when each 1-qubit trajectory is measured."""
+    rng = np.random.default_rng(seed)
+    zeros = 0
+    for _ in range(shots):
+        v = O.noise_apply_to_sv(sv.copy(), 1, model, p, rng)
+        pr = np.abs(v) ** 2
+        zeros += rng.choice(2, p=pr / pr.sum()) == 0
+    return zeros / shots
+
+
+@pytest.mark.parametrize("sv,model,p,want", [
+    # the reference's own statistical known answers (tests/unit/test_registry.py:68-236)
+    (np.array([0, 1], dtype=complex), "depolarizing", 0.3, 2 * 0.3 / 3),
+    (np.array([0, 1], dtype=complex), "combined", 0.6, 2 * 0.3 / 3 + (1 - 2 * 0.3 / 3) * 0.6 * 0.333333),
+    (np.array([0, 1], dtype=complex), "amplitude_damping", 0.4, 0.4),
+    (np.array([1, 1], dtype=complex) / np.sqrt(2), "amplitude_damping", 0.5, 0.75),
+])
+def test_noise_reference_known_answers(sv, model, p, want):
+    assert abs(_measure0_freq(sv, model, p, 20000, seed=3) - want) < 0.02
