@@ -907,6 +907,69 @@ int desv_dense_matvec(desv_handle out, desv_handle in, const double *H) {
     return dense_matvec_raw(out, out->sv, in->sv, H);
 }
 
+// Handles of one shape for a Krylov-basis call; waits for every handle's
+// pending work except on `own` (the stream the call launches on).
+static int basis_check(desv_handle ref, const desv_handle *hs, int m, hipStream_t own) {
+    if (m < 1 || m > kMaxBasis)
+        return fail(DESV_EINVAL, "basis calls take 1.." + std::to_string(kMaxBasis) + " vectors, got " +
+                                     std::to_string(m));
+    if (!hs) return fail(DESV_EINVAL, "null handle array");
+    for (int j = 0; j < m; ++j) {
+        if (!hs[j]) return fail(DESV_EINVAL, "null handle in basis array");
+        if (hs[j]->n != ref->n || hs[j]->dtype != ref->dtype || hs[j]->device != ref->device)
+            return fail(DESV_EINVAL, "basis vectors must share n, dtype and device");
+        if (hs[j]->stream != own) HIP_TRY(hipStreamSynchronize(hs[j]->stream));
+    }
+    return DESV_OK;
+}
+
+int desv_inner_many(desv_handle w, const desv_handle *vs, int m, double *out) {
+    if (int rc = check_handle(w)) return rc;
+    if (!out) return fail(DESV_EINVAL, "null output buffer");
+    HIP_TRY(hipSetDevice(w->device));
+    if (int rc = basis_check(w, vs, m, w->stream)) return rc;
+    BasisPtrs ptrs{};
+    for (int j = 0; j < m; ++j) ptrs.p[j] = vs[j]->sv;
+    const int nblk = reduce_blocks(w->dim);
+    if (int rc = ensure_part(w, 2 * (size_t)m * nblk)) return rc;
+    hipError_t e = w->dtype == DESV_C64 ? launch_inner_many<float>(w->sv, ptrs, m, w->dim, w->d_part, nblk, w->stream)
+                                        : launch_inner_many<double>(w->sv, ptrs, m, w->dim, w->d_part, nblk, w->stream);
+    if (e != hipSuccess) return fail(DESV_EDEVICE, hipGetErrorString(e));
+    return sum_to_host(w, 2 * m, nblk, out);
+}
+
+int desv_combine(const desv_handle *outs, int p, const desv_handle *ins, int m, const double *coef) {
+    if (!outs || p < 1 || !outs[0]) return fail(DESV_EINVAL, "combine needs at least one output handle");
+    if (!coef) return fail(DESV_EINVAL, "null coefficient array");
+    desv_handle h = outs[0];
+    HIP_TRY(hipSetDevice(h->device));
+    if (int rc = basis_check(h, outs, p, h->stream)) return rc;
+    if (int rc = basis_check(h, ins, m, h->stream)) return rc;
+    BasisPtrs in{}, out{};
+    for (int j = 0; j < m; ++j) in.p[j] = ins[j]->sv;
+    for (int l = 0; l < p; ++l) out.p[l] = outs[l]->sv;
+    const size_t cb = (size_t)m * p * 2 * sizeof(double);
+    if (int rc = ensure_small(h, cb)) return rc;
+    HIP_TRY(hipStreamSynchronize(h->stream));  // d_small may still be read
+    HIP_TRY(hipMemcpyAsync(h->d_small, coef, cb, hipMemcpyHostToDevice, h->stream));
+    hipError_t e = h->dtype == DESV_C64
+                       ? launch_combine<float>(in, out, m, p, (const double2 *)h->d_small, h->dim, h->stream)
+                       : launch_combine<double>(in, out, m, p, (const double2 *)h->d_small, h->dim, h->stream);
+    if (e != hipSuccess) return fail(DESV_EDEVICE, hipGetErrorString(e));
+    // the outputs other than outs[0] are read next on their own streams
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return DESV_OK;
+}
+
+int desv_fill_random(desv_handle h, uint64_t seed) {
+    if (int rc = check_handle(h)) return rc;
+    HIP_TRY(hipSetDevice(h->device));
+    hipError_t e = h->dtype == DESV_C64 ? launch_fill_random<float>(h->sv, h->dim, seed, h->stream)
+                                        : launch_fill_random<double>(h->sv, h->dim, seed, h->stream);
+    if (e != hipSuccess) return fail(DESV_EDEVICE, hipGetErrorString(e));
+    return DESV_OK;
+}
+
 int desv_adjoint(desv_handle h, const double *ops, int64_t n_ops, const int64_t *param_of_op, int64_t n_params,
                  const uint64_t *xmask, const uint64_t *zmask, const double *coeff, int64_t T,
                  const double *dense_h, double *energy, double *grad) {

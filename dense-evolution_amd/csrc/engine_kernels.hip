@@ -1535,6 +1535,122 @@ hipError_t launch_dense_matvec(void *out, const void *in, int n, const double2 *
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Krylov bases (block Gram-Schmidt and Ritz rotation for Lanczos): streaming,
+// HBM-bound; FP64 accumulation whatever the state precision.
+// ---------------------------------------------------------------------------
+// <v_j|w> for j in the workgroup row's group of kInnerGroup vectors: w is read
+// once per group, each v_j once.  Partials: [(2 j + {re, im}) * nblk + block].
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_inner_many(const typename VT<T>::c *w, BasisPtrs vs, int m,
+                                                         int64_t dim, double *partials) {
+    using C = typename VT<T>::c;
+    __shared__ double red[kThreads / 64];
+    const int j0 = blockIdx.y * kInnerGroup;
+    const int cnt = min(kInnerGroup, m - j0);
+    double re[kInnerGroup], im[kInnerGroup];
+#pragma unroll
+    for (int g = 0; g < kInnerGroup; ++g) re[g] = im[g] = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < dim; i += (int64_t)gridDim.x * blockDim.x) {
+        const double2 y = to_d(w[i]);
+#pragma unroll
+        for (int g = 0; g < kInnerGroup; ++g) {
+            if (g < cnt) {
+                const double2 x = to_d(static_cast<const C *>(vs.p[j0 + g])[i]);
+                re[g] += x.x * y.x + x.y * y.y;
+                im[g] += x.x * y.y - x.y * y.x;
+            }
+        }
+    }
+    for (int g = 0; g < cnt; ++g) {
+        const double sr = block_sum(re[g], red);
+        const double si = block_sum(im[g], red);
+        if (threadIdx.x == 0) {
+            partials[(int64_t)(2 * (j0 + g)) * gridDim.x + blockIdx.x] = sr;
+            partials[(int64_t)(2 * (j0 + g) + 1) * gridDim.x + blockIdx.x] = si;
+        }
+    }
+}
+
+// out_l[i] = sum_j coef[j * p + l] * in_j[i] for l < p.  Every input element
+// of index i is loaded before any output element of index i is stored, by the
+// same lane, so outputs may alias inputs (in-place basis rotation).
+template <typename T, int M>
+__global__ void __launch_bounds__(kThreads) k_combine(BasisPtrs in, BasisPtrs out, int m, int p,
+                                                      const double2 *__restrict__ coef, int64_t dim) {
+    using C = typename VT<T>::c;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < dim; i += (int64_t)gridDim.x * blockDim.x) {
+        double2 x[M];
+#pragma unroll
+        for (int j = 0; j < M; ++j)
+            if (j < m) x[j] = to_d(static_cast<const C *>(in.p[j])[i]);
+        for (int l = 0; l < p; ++l) {
+            double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int j = 0; j < M; ++j)
+                if (j < m) acc = cfma(coef[j * p + l], x[j], acc);
+            C r;
+            r.x = (T)acc.x;
+            r.y = (T)acc.y;
+            static_cast<C *>(const_cast<void *>(out.p[l]))[i] = r;
+        }
+    }
+}
+
+// Amplitude i = (u0 - 1/2) + i (u1 - 1/2), u from SplitMix64 of (seed, i):
+// a reproducible start vector with no host transfer (not normalised).
+__device__ inline uint64_t splitmix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+template <typename T>
+__global__ void k_fill_random(typename VT<T>::c *sv, int64_t dim, uint64_t seed) {
+    using C = typename VT<T>::c;
+    const uint64_t key = splitmix64(seed);
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < dim; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t a = splitmix64(key ^ (2 * (uint64_t)i));
+        const uint64_t b = splitmix64(key ^ (2 * (uint64_t)i + 1));
+        C r;
+        r.x = (T)((double)(a >> 11) * 0x1.0p-53 - 0.5);
+        r.y = (T)((double)(b >> 11) * 0x1.0p-53 - 0.5);
+        sv[i] = r;
+    }
+}
+
+template <typename T>
+hipError_t launch_inner_many(const void *w, const BasisPtrs &vs, int m, int64_t dim, double *partials, int nblk,
+                             hipStream_t s) {
+    const int rows = (m + kInnerGroup - 1) / kInnerGroup;
+    hipLaunchKernelGGL((k_inner_many<T>), dim3(nblk, rows), dim3(kThreads), 0, s, (const typename VT<T>::c *)w, vs,
+                       m, dim, partials);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_combine(const BasisPtrs &in, const BasisPtrs &out, int m, int p, const double2 *coef, int64_t dim,
+                          hipStream_t s) {
+    const dim3 g(ew_grid(dim)), b(kThreads);
+    if (m <= 8)
+        hipLaunchKernelGGL((k_combine<T, 8>), g, b, 0, s, in, out, m, p, coef, dim);
+    else if (m <= 16)
+        hipLaunchKernelGGL((k_combine<T, 16>), g, b, 0, s, in, out, m, p, coef, dim);
+    else if (m <= 32)
+        hipLaunchKernelGGL((k_combine<T, 32>), g, b, 0, s, in, out, m, p, coef, dim);
+    else
+        hipLaunchKernelGGL((k_combine<T, kMaxBasis>), g, b, 0, s, in, out, m, p, coef, dim);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_fill_random(void *sv, int64_t dim, uint64_t seed, hipStream_t s) {
+    hipLaunchKernelGGL((k_fill_random<T>), dim3(ew_grid(dim)), dim3(kThreads), 0, s, (typename VT<T>::c *)sv, dim,
+                       seed);
+    return hipGetLastError();
+}
+
 #define DESV_INST(T)                                                                                            \
     template hipError_t launch_direct<T>(const void *, void *, int, const DirectOp &, const void *, hipStream_t);            \
     template hipError_t launch_stage<T>(int, int, const StageTask *, const int32_t *, const StageTask &, int64_t, int, \
@@ -1553,7 +1669,12 @@ hipError_t launch_dense_matvec(void *out, const void *in, int n, const double2 *
                                                double *, int, hipStream_t);                                   \
     template hipError_t launch_pauli_matvec<T>(void *, const void *, int, const MatvecGroup *, int,           \
                                                const uint64_t *, const double2 *, hipStream_t);               \
-    template hipError_t launch_dense_matvec<T>(void *, const void *, int, const double2 *, hipStream_t);
+    template hipError_t launch_dense_matvec<T>(void *, const void *, int, const double2 *, hipStream_t);        \
+    template hipError_t launch_inner_many<T>(const void *, const BasisPtrs &, int, int64_t, double *, int,     \
+                                             hipStream_t);                                                    \
+    template hipError_t launch_combine<T>(const BasisPtrs &, const BasisPtrs &, int, int, const double2 *,     \
+                                          int64_t, hipStream_t);                                              \
+    template hipError_t launch_fill_random<T>(void *, int64_t, uint64_t, hipStream_t);
 
 DESV_INST(float)
 DESV_INST(double)

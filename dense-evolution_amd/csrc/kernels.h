@@ -150,6 +150,23 @@ hipError_t launch_pauli_matvec(void *out, const void *in, int n, const MatvecGro
                                const uint64_t *zmask, const double2 *cc, hipStream_t s);
 template <typename T>
 hipError_t launch_dense_matvec(void *out, const void *in, int n, const double2 *H, hipStream_t s);
+// Krylov bases: up to kMaxBasis state pointers per launch.
+constexpr int kMaxBasis = 64;
+constexpr int kInnerGroup = 8;
+struct BasisPtrs {
+    const void *p[kMaxBasis];
+};
+// partials[(2 j + {0, 1}) * nblk + b] for <vs_j|w>, j < m (then launch_sum_partials with 2 m values).
+template <typename T>
+hipError_t launch_inner_many(const void *w, const BasisPtrs &vs, int m, int64_t dim, double *partials, int nblk,
+                             hipStream_t s);
+// out_l <- sum_j coef[j * p + l] in_j (l < p, j < m <= kMaxBasis); outputs may alias inputs.
+template <typename T>
+hipError_t launch_combine(const BasisPtrs &in, const BasisPtrs &out, int m, int p, const double2 *coef, int64_t dim,
+                          hipStream_t s);
+// Reproducible pseudo-random amplitudes in [-1/2, 1/2)^2 from a 64-bit seed.
+template <typename T>
+hipError_t launch_fill_random(void *sv, int64_t dim, uint64_t seed, hipStream_t s);
 // out[v] = sum_{b < nblk} partials[v * nblk + b], for v < nvals.
 hipError_t launch_sum_partials(const double *partials, int nvals, int nblk, double *out, hipStream_t s);
 // Max dynamic LDS bytes the stage kernel may use on this device.

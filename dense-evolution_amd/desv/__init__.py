@@ -15,6 +15,8 @@ from .config import ensure_x64, set_precision, with_x64
 from .gates import GATE_IDS, GATES, PARAMETRIC_GATES
 from .measurement import sample_counts, statevector_fidelity
 from .noise import NoiseModel, NoiseSpec
+from . import lanczos
+from .lanczos import ground_state_energy
 from .observables import (PauliSumOperator, multiply_pauli_terms, pauli_expectation,
                           pauli_hamiltonian_to_matrix, pauli_sum_expectation,
                           pauli_sum_expectation_jax, pauli_sum_matvec, pauli_sum_matvec_jax)
@@ -36,4 +38,5 @@ __all__ = [
     "multiply_pauli_terms", "pauli_sum_matvec_jax", "pauli_sum_expectation_jax", "PauliSumOperator",
     "circuit_to_energy_fn", "AdjointProgram", "parameter_shift_grad",
     "sample_counts", "statevector_fidelity", "device_count", "NoiseModel", "NoiseSpec",
+    "lanczos", "ground_state_energy",
 ]

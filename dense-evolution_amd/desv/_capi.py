@@ -58,6 +58,9 @@ _SIGS = {
     "desv_scale": ([_p, _d, _d], _i),
     "desv_norm2": ([_p, _pd], _i),
     "desv_inner": ([_p, _p, _pd, _pd], _i),
+    "desv_inner_many": ([_p, C.POINTER(_p), _i, _pd], _i),
+    "desv_combine": ([C.POINTER(_p), _i, C.POINTER(_p), _i, _pd], _i),
+    "desv_fill_random": ([_p, C.c_uint64], _i),
     "desv_probabilities": ([_p, _pd], _i),
     "desv_qubit_probs": ([_p, _i, _pd, _pd], _i),
     "desv_collapse": ([_p, _i, _i, _d], _i),

@@ -237,6 +237,17 @@ class PauliSumOperator:
     def __matmul__(self, vector):
         return pauli_sum_matvec(vector, self.terms, n_qubits=self.n_qubits)
 
+    def as_linear_operator(self):
+        """``scipy.sparse.linalg.LinearOperator`` whose matvec is the GPU
+        kernel (host vectors in and out), the operator the reference hands to
+        ``eigsh`` (``tools/dashboard/core/hamiltonians.py:569-576``).
+        ``desv.lanczos.eigsh`` solves the same problem without host round
+        trips."""
+        import scipy.sparse.linalg as sla
+        dim = 1 << self.n_qubits
+        return sla.LinearOperator(shape=(dim, dim), matvec=lambda v: self @ np.ravel(v),
+                                  dtype=np.complex128)
+
 
 _SAME_QUBIT = {
     ('X', 'X'): (1, None), ('Y', 'Y'): (1, None), ('Z', 'Z'): (1, None),

@@ -110,6 +110,16 @@ def xxz_terms(n: int, J: float = 1.0, delta: float = 0.5):
     return terms
 
 
+def tfim_terms(n: int, seed: int = 0):
+    """Open transverse-field Ising chain H = -sum J_i Z_i Z_{i+1} - sum h_i X_i
+    with J_i ~ U[0.5, 1.5), h_i ~ U[0.2, 1.2) (J drawn first)."""
+    rng = np.random.default_rng(seed)
+    J = rng.uniform(0.5, 1.5, n - 1)
+    h = rng.uniform(0.2, 1.2, n)
+    return ([(-float(J[i]), {i: 'Z', i + 1: 'Z'}) for i in range(n - 1)]
+            + [(-float(h[i]), {i: 'X'}) for i in range(n)]), J, h
+
+
 def xxz_trotter_circuit(n: int = 28, dt: float = 0.05, steps: int = 200, seed: int = 0):
     """Random product state (ry(theta_q) then rz(phi_q), theta ~ U[0, pi),
     phi ~ U[0, 2pi), drawn as the theta vector then the phi vector) followed

@@ -170,6 +170,22 @@ int desv_pauli_matvec(desv_handle out, desv_handle in, const uint64_t *xmask,
  * `h_matrix @ sv` of energy_fn (solvers/autodiff.py:166). */
 int desv_dense_matvec(desv_handle out, desv_handle in, const double *h_matrix);
 
+/* ---- Krylov bases (Lanczos ground states) --------------------------------
+ * The reference finds ground states with scipy's eigsh over a LinearOperator
+ * whose matvec is pauli_sum_matvec (tools/dashboard/core/hamiltonians.py:
+ * 512-577), one host round trip per product.  These entry points keep the
+ * whole Krylov basis on the device: handles of one n, dtype and device, at
+ * most 64 per call, FP64 accumulation.
+ * out[2j], out[2j+1] = Re, Im <vs[j]|w> for j < m. */
+int desv_inner_many(desv_handle w, const desv_handle *vs, int m, double *out);
+/* outs[l] <- sum_j (coef[2(j p + l)] + i coef[2(j p + l) + 1]) ins[j] for
+ * l < p, j < m, element by element: outputs may be among the inputs (in-place
+ * Gram-Schmidt subtraction and Ritz rotation). */
+int desv_combine(const desv_handle *outs, int p, const desv_handle *ins, int m, const double *coef);
+/* Reproducible pseudo-random amplitudes in [-1/2, 1/2) + i[-1/2, 1/2) from a
+ * 64-bit seed (a Lanczos start vector; not normalised). */
+int desv_fill_random(desv_handle h, uint64_t seed);
+
 /* ---- adjoint gradient ----------------------------------------------------
  * Energy and the full gradient of E(theta) = Re<psi|H|psi>, psi = U(ops)|h>,
  * in one forward + one reverse sweep.  Replaces jax.value_and_grad over
