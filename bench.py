@@ -13,7 +13,8 @@ the CPU baseline (the oracle's C + OpenMP restatement applying the same gates
 at 30 qubits on every host core, a bounded sample of gates); and extra workloads
 (per-gate streaming passes, full-size c64-vs-c128 parity, 32 qubits, the
 20-qubit plain circuit, VQE adjoint vs parameter shift, the 1024-instance
-batch, the 28-qubit XXZ precision study).
+batch, the 28-qubit XXZ precision study, Lanczos ground states of the XXZ
+chain).
 """
 import argparse
 import json
@@ -278,6 +279,79 @@ def suite_brickwork32(n=32, depth=24):
             "hbm_gbps": st["passes"] * 2 * (1 << n) * 8 / dt / 1e9, "norm_minus_1": nrm - 1}
 
 
+def cpu_baseline_lanczos(terms, n, ncv):
+    """The reference's ground-state pattern on the host: scipy eigsh (ARPACK)
+    over a LinearOperator whose matvec is the oracle's NumPy Pauli-sum matvec.
+    Returns (seconds, E0)."""
+    import numpy as np
+    import scipy.sparse.linalg as sla
+    from oracle import sv_oracle as O
+    dim = 1 << n
+    op = sla.LinearOperator((dim, dim), matvec=lambda v: O.pauli_sum_matvec(np.ravel(v), terms),
+                            dtype=np.complex128)
+    t0 = time.perf_counter()
+    e = sla.eigsh(op, k=1, which="SA", ncv=ncv, return_eigenvectors=False)[0]
+    return time.perf_counter() - t0, float(e)
+
+
+def suite_lanczos(n_big=26, n_cmp=16, ncv=20):
+    """Lanczos ground state of the open XXZ chain (J = 1, delta = 0.5),
+    complex128.  At `n_cmp` qubits: the device-resident solver against the
+    reference's pattern, scipy eigsh over a LinearOperator (ARPACK, one host
+    round trip per product), with the GPU matvec and with the oracle's NumPy
+    matvec on the host.  At `n_big` qubits: time to solution and the rates of
+    the three kernels of one Lanczos step (Pauli-sum matvec, 20-vector inner
+    products, 21-vector combination), as bytes streamed per second."""
+    import numpy as np
+    import scipy.sparse.linalg as sla
+    import desv
+    from desv import lanczos, workloads
+    out = {"workload": f"open XXZ chain (J=1, delta=0.5), ground state, complex128, ncv={ncv}"}
+    terms = workloads.xxz_terms(n_cmp)
+    dim = 1 << n_cmp
+    lanczos.ground_state_energy(terms, n_cmp, ncv=ncv)
+    info = {}
+    t_dev, e_dev = _timed(lambda: lanczos.ground_state_energy(terms, n_cmp, ncv=ncv, info=info))
+    op = desv.PauliSumOperator(terms, n_cmp).as_linear_operator()
+    t_op, e_op = _timed(lambda: sla.eigsh(op, k=1, which="SA", ncv=ncv, return_eigenvectors=False)[0])
+    t_cpu, e_cpu = cpu_baseline_lanczos(terms, n_cmp, ncv)
+    out[f"{n_cmp}q"] = {"device_lanczos_s": t_dev, "device_matvecs": info["matvecs"],
+                        "scipy_eigsh_gpu_matvec_s": t_op, "scipy_eigsh_numpy_matvec_s": t_cpu,
+                        "speedup_vs_scipy_numpy": t_cpu / t_dev, "speedup_vs_scipy_gpu_matvec": t_op / t_dev,
+                        "E0": e_dev, "abs_dE_vs_scipy_numpy": abs(e_dev - e_cpu),
+                        "abs_dE_vs_scipy_gpu": abs(e_dev - e_op)}
+    n = n_big
+    terms = workloads.xxz_terms(n)
+    info = {}
+    t_big, e_big = _timed(lambda: lanczos.ground_state_energy(terms, n, ncv=ncv, info=info))
+    res = {"time_to_solution_s": t_big, "E0": e_big, "E0_per_bond": e_big / (n - 1), **info}
+    # kernel rates at n_big
+    hs = [desv.StateHandle(n, True) for _ in range(ncv + 2)]
+    for i, h in enumerate(hs):
+        desv._capi.check(desv._capi.lib.desv_fill_random(h.ptr, i + 1))
+    xm, zm, cs = desv.PauliSumOperator(terms, n).encoded()
+    groups = len(set(int(x) for x in xm))
+    nb = (1 << n) * 16
+    from desv.observables import matvec_into
+    matvec_into(hs[1], hs[0], xm, zm, cs)
+    t_mv, _ = _timed(lambda: matvec_into(hs[1], hs[0], xm, zm, cs), reps=5)
+    t_in, _ = _timed(lambda: lanczos.inner_many(hs[-1], hs[:ncv]), reps=5)
+    coef = np.concatenate([[1.0], -np.full(ncv, 1e-3)])
+    t_cb, _ = _timed(lambda: lanczos.combine([hs[-1]], [hs[-1]] + hs[:ncv], coef), reps=5)
+    for h in hs:
+        h.close()
+    rows = (ncv + 7) // 8
+    res["kernels"] = {
+        "pauli_matvec": {"ms": t_mv * 1e3, "flip_mask_groups": groups,
+                         "stream_gbps": (groups + 1) * nb / t_mv / 1e9,
+                         "min_traffic_gbps": 2 * nb / t_mv / 1e9},
+        "inner_many": {"ms": t_in * 1e3, "vectors": ncv, "stream_gbps": (ncv + rows) * nb / t_in / 1e9},
+        "combine": {"ms": t_cb * 1e3, "inputs": ncv + 1, "stream_gbps": (ncv + 2) * nb / t_cb / 1e9},
+    }
+    out[f"{n}q"] = res
+    return out
+
+
 def copy_probe(n, steps):
     """Copy one 2^n complex64 state into another with the library's streaming
     copy (desv_copy): read + write bytes per second.  Under rocprofv3 --pmc,
@@ -325,7 +399,7 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="only time the library's device copy of a 2^qubits complex64 state "
                          "(calibration of HBM rate and of the rocprofv3 byte counters)")
-    ap.add_argument("--suite", default="streaming30,parity30,brickwork32,plain20,vqe20,batch,precision",
+    ap.add_argument("--suite", default="streaming30,parity30,brickwork32,plain20,vqe20,batch,precision,lanczos",
                     help="extra workloads to measure after the headline ('' for none)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -462,7 +536,7 @@ def main():
     extras = {}
     fns = {"streaming30": suite_streaming30, "parity30": suite_parity30, "brickwork32": suite_brickwork32,
            "plain20": suite_plain20, "vqe20": suite_vqe20, "batch": suite_batch,
-           "precision": suite_precision}
+           "precision": suite_precision, "lanczos": suite_lanczos}
     for name in [x for x in args.suite.split(",") if x]:
         log(f"suite {name} ...")
         t0 = time.perf_counter()

@@ -99,7 +99,7 @@ def _orthonormalise(w: StateHandle, basis: List[StateHandle]) -> (np.ndarray, fl
 def eigsh(hamiltonian, k: int = 1, which: str = "SA", ncv: Optional[int] = None, tol: float = 0.0,
           maxiter: Optional[int] = None, v0=None, return_eigenvectors: bool = False,
           n_qubits: Optional[int] = None, dtype: str = "complex128", device: int = 0, seed: int = 0,
-          return_states: bool = False):
+          return_states: bool = False, info: Optional[dict] = None):
     """k eigenvalues (and vectors) of a Pauli-sum Hamiltonian, ``scipy.sparse.
     linalg.eigsh``-style: ``which`` in SA/LA/SM/LM, ``ncv`` basis vectors
     (default ``min(dim, max(2k + 1, 20))``), eigenvalues returned in ascending
@@ -107,7 +107,9 @@ def eigsh(hamiltonian, k: int = 1, which: str = "SA", ncv: Optional[int] = None,
     ``(coeff, term)`` pairs with ``n_qubits``.  ``v0`` (host array or device
     state) is the start vector, else a seeded pseudo-random one made on the
     device.  Eigenvectors come back as a host ``(dim, k)`` array, or as
-    :class:`StateHandle` objects with ``return_states=True``.
+    :class:`StateHandle` objects with ``return_states=True``.  A dict passed
+    as ``info`` receives the counts of products and restarts and the final
+    residual norms.
 
     Device memory: ``ncv + 1`` state vectors.
     """
@@ -143,11 +145,12 @@ def eigsh(hamiltonian, k: int = 1, which: str = "SA", ncv: Optional[int] = None,
         V[0].scale(1.0 / nrm)
 
         T = np.zeros((m + 1, m), dtype=np.complex128)
-        start, restarts, m_eff, beta = 0, 0, m, 0.0
+        start, restarts, m_eff, beta, products = 0, 0, m, 0.0, 0
         while True:
             m_eff = m
             for j in range(start, m):
                 op(V[j + 1], V[j])
+                products += 1
                 c, beta = _orthonormalise(V[j + 1], V[:j + 1])
                 T[:j + 1, j] = c
                 scale = max(1.0, float(np.max(np.abs(T[:j + 2, :j + 1]))))
@@ -188,6 +191,8 @@ def eigsh(hamiltonian, k: int = 1, which: str = "SA", ncv: Optional[int] = None,
             T[:p, :p] = np.diag(theta[:p])
             T[p, :p] = b
             start = p
+        if info is not None:
+            info.update(matvecs=products, restarts=restarts, ncv=m, residuals=resid[:k].tolist())
         vals = theta[:k]
         asc = np.argsort(vals, kind="stable")
         vals = vals[asc]
