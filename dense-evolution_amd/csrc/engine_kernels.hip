@@ -267,6 +267,29 @@ __device__ inline void op_densek(char *lds, int vstride, int m, const DevOp &op,
 // ---------------------------------------------------------------------------
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 
+constexpr int kWaves = kThreads / 64;
+constexpr int kLogWaves = 2;
+static_assert((1 << kLogWaves) == kWaves, "kLogWaves");
+// MFMA ops hand column block w + kWaves g (16 groups each) to wave w.  The
+// wave-uniform byte slot of block g is looked up rather than recomputed: lane
+// l of `lo` holds the slot of g = l (including the wave's own bits), lane l of
+// `hi` the slot of g = 64 l, and slot(g) = lo[g % 64] ^ hi[g / 64] (slot and
+// spread are linear over XOR on disjoint bit fields).  Lanes past the op's
+// block count hold unused values.
+struct BlockSlots {
+    int lo, hi;
+    __device__ BlockSlots(int fixed, int eb) {
+        const int lane = threadIdx.x & 63;
+        const int wave = uni((int)threadIdx.x >> 6);
+        lo = slot(spread((wave << 4) | (lane << (4 + kLogWaves)), fixed)) * eb;
+        hi = slot(spread(lane << (10 + kLogWaves), fixed)) * eb;
+    }
+    __device__ int operator()(int g) const {
+        const int l = __builtin_amdgcn_readlane(lo, g & 63);
+        return g < 64 ? l : l ^ __builtin_amdgcn_readlane(hi, g >> 6);
+    }
+};
+
 template <int NV, int K, typename MP>
 __device__ inline void op_mfma(char *lds, int vstride, int m, const DevOp &op, MP M) {
     constexpr int D = 1 << K;
@@ -308,6 +331,62 @@ __device__ inline void op_mfma(char *lds, int vstride, int m, const DevOp &op, M
         for (int j = 0; j < 4; ++j) py[rb][j] = slot(pl | roff(rb * 16 + 4 * k4 + j)) * EB;
 
     const int G = 1 << (m - __builtin_popcount(fixed));
+    if (NV == 1 && G >= 16 * kWaves) {
+        // full column blocks for every wave: no lane predicates, block slots
+        // from the lane table, next block's loads before this block's stores
+        // (one vector: with two the prefetched fragments exceed the register
+        // budget of four waves per SIMD)
+        const int nit = G / (16 * kWaves);
+        const BlockSlots bs(fixed, EB);
+        int Uc = bs(0);
+        float br[NV][KS], bi[NV][KS];
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) {
+                const float2 x = lds_ld<float2>(lds + v * vstride, pb[ks] ^ Uc);
+                br[v][ks] = x.x;
+                bi[v][ks] = x.y;
+            }
+        for (int it = 0; it < nit; ++it) {
+            const bool more = it + 1 < nit;
+            const int Un = more ? bs(it + 1) : Uc;
+            f32x4_t yr[NV][RB], yi[NV][RB];
+#pragma unroll
+            for (int v = 0; v < NV; ++v)
+#pragma unroll
+                for (int rb = 0; rb < RB; ++rb) {
+                    yr[v][rb] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+                    yi[v][rb] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                    for (int ks = 0; ks < KS; ++ks) {
+                        yr[v][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[rb][ks], br[v][ks], yr[v][rb], 0, 0, 0);
+                        yi[v][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[rb][ks], bi[v][ks], yi[v][rb], 0, 0, 0);
+                        yr[v][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[rb][ks], -bi[v][ks], yr[v][rb], 0, 0, 0);
+                        yi[v][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[rb][ks], br[v][ks], yi[v][rb], 0, 0, 0);
+                    }
+                }
+            if (more) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v)
+#pragma unroll
+                    for (int ks = 0; ks < KS; ++ks) {
+                        const float2 x = lds_ld<float2>(lds + v * vstride, pb[ks] ^ Un);
+                        br[v][ks] = x.x;
+                        bi[v][ks] = x.y;
+                    }
+            }
+#pragma unroll
+            for (int v = 0; v < NV; ++v)
+#pragma unroll
+                for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        lds_st<float2>(lds + v * vstride, py[rb][j] ^ Uc, make_float2(yr[v][rb][j], yi[v][rb][j]));
+            Uc = Un;
+        }
+        return;
+    }
     const bool valid = i16 < G;  // tiles narrower than 16 columns
     const int step = (blockDim.x >> 6) * 16;
     // UN column blocks per iteration (independent accumulator chains with
@@ -395,6 +474,46 @@ __device__ inline void op_mfma3(char *lds, int vstride, int m, const DevOp &op, 
     const int pb0 = slot(pl | roff(g4)) * EB, pb1 = slot(pl | roff(4 + g4)) * EB;
     const int py0 = slot(pl | roff(2 * g4)) * EB, py1 = slot(pl | roff(2 * g4 + 1)) * EB;
     const int G = 1 << (m - __builtin_popcount(fixed));
+    if (G >= 16 * kWaves) {
+        // Every wave owns G / (16 kWaves) full column blocks: no lane
+        // predicates, wave-uniform (scalar) block addresses from a lane
+        // table, and the next block's loads issued before this block's stores.
+        const int nit = G / (16 * kWaves);
+        const BlockSlots bs(fixed, EB);
+        int Uc = bs(0);
+        float2 x0[NV], x1[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            x0[v] = lds_ld<float2>(lds + v * vstride, pb0 ^ Uc);
+            x1[v] = lds_ld<float2>(lds + v * vstride, pb1 ^ Uc);
+        }
+        for (int it = 0; it < nit; ++it) {
+            const bool more = it + 1 < nit;
+            const int Un = more ? bs(it + 1) : Uc;
+            f32x4_t y[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                y[v] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0], x0[v].x, f32x4_t{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+                y[v] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[1], x1[v].x, y[v], 0, 0, 0);
+                y[v] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[2], x0[v].y, y[v], 0, 0, 0);
+                y[v] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[3], x1[v].y, y[v], 0, 0, 0);
+            }
+            if (more) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    x0[v] = lds_ld<float2>(lds + v * vstride, pb0 ^ Un);
+                    x1[v] = lds_ld<float2>(lds + v * vstride, pb1 ^ Un);
+                }
+            }
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                lds_st<float2>(lds + v * vstride, py0 ^ Uc, make_float2(y[v][0], y[v][1]));
+                lds_st<float2>(lds + v * vstride, py1 ^ Uc, make_float2(y[v][2], y[v][3]));
+            }
+            Uc = Un;
+        }
+        return;
+    }
     const bool valid = i16 < G;  // tiles narrower than 16 columns
     const int step = (blockDim.x >> 6) * 16;
     constexpr int UN = 1;  // column blocks per iteration; 4 measured no faster than 1
