@@ -388,7 +388,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--qubits", type=int, default=30)
     ap.add_argument("--depth", type=int, default=24)
-    ap.add_argument("--fuse", type=int, default=4)
+    ap.add_argument("--fuse", type=int, default=None,
+                    help="fusion width; default: the program autotuner picks fusion, tile and MFMA use")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="skip the program autotuner (then fusion 4, MFMA blocks, default tile)")
     ap.add_argument("--tile-bits", type=int, default=0)
     ap.add_argument("--low-bits", type=int, default=0)
     ap.add_argument("--per-gate", action="store_true")
@@ -422,12 +425,27 @@ def main():
     prog = desv.Program(n)
     for u, a, b in gates:
         prog.add_matrix(u, [a, b])
-    kw = dict(fuse_bits=args.fuse, per_gate=args.per_gate, use_mfma=mfma)
-    if args.tile_bits:
-        kw["tile_bits"] = args.tile_bits
+    tuned = None
+    if args.fuse is None and not args.no_autotune and not args.per_gate and not args.tile_bits:
+        # launch shape chosen by measurement (desv.tune), not cached: every run
+        # times the candidates on this GPU
+        t0 = time.perf_counter()
+        tuned = desv.tune.autotune_program(prog, "complex64", cache=False)
+        if args.no_mfma and tuned["best"].get("use_mfma"):
+            tuned = desv.tune.autotune_program(
+                prog, "complex64", cache=False,
+                candidates=[c for c in desv.tune.program_candidates("complex64", n) if not c["use_mfma"]])
+        log(f"autotune ({time.perf_counter() - t0:.1f} s): {tuned['best']} "
+            f"{[(c['fuse_bits'], c['tile_bits'], c['use_mfma'], c['ms']) for c in tuned['candidates']]}")
+        kw = dict(tuned["best"])
+    else:
+        kw = dict(fuse_bits=args.fuse if args.fuse is not None else 4, per_gate=args.per_gate, use_mfma=mfma)
+        if args.tile_bits:
+            kw["tile_bits"] = args.tile_bits
     if args.low_bits:
         kw["low_bits"] = args.low_bits
     prog.compile("complex64", **kw)
+    fuse, mfma = kw["fuse_bits"], bool(kw.get("use_mfma"))
     st = prog.stats()
     kernels = prog.stage_kernels()
     log(f"compiled: {st}")
@@ -482,7 +500,8 @@ def main():
     dom = max(per, key=lambda k: per[k]["ms"])
     d = per[dom]
     avg_ms = d["ms"] / d["launches"]
-    workload_key = (f"brickwork{n}_depth{args.depth}_fuse{args.fuse}{'_mfma' if mfma and args.fuse >= 4 else ''}"
+    workload_key = (f"brickwork{n}_depth{args.depth}_fuse{fuse}{'_mfma' if mfma and fuse >= 3 else ''}"
+                    f"{'_tile' + str(kw['tile_bits']) if kw.get('tile_bits') else ''}"
                     f"{'_pergate' if args.per_gate else ''}")
     traffic = _load_traffic(dom, workload_key)
     if dom == "k_direct":
@@ -518,7 +537,8 @@ def main():
         "config": {"workload": f"brickwork Haar-SU(4), {n} qubits, depth {args.depth}, "
                                f"{len(gates)} gates, complex64 state resident in HBM",
                    "qubits": n, "depth": args.depth, "gates": len(gates),
-                   "fuse_bits": args.fuse, "per_gate": args.per_gate, "mfma": mfma,
+                   "fuse_bits": fuse, "tile_bits": kw.get("tile_bits"), "per_gate": args.per_gate,
+                   "mfma": mfma, "shape_from_autotuner": tuned is not None,
                    "hbm_passes_per_step": st["passes"], "device_ops_per_step": st["blocks"],
                    "mfma_blocks_per_step": st["mfma_blocks"]},
         # whole-job HBM rate: algorithmic bytes of the passes executed / time

@@ -310,7 +310,7 @@ __device__ inline void op_mfma(char *lds, int vstride, int m, const DevOp &op, M
         return o;
     };
     // A fragments (constant over the tile)
-    float ar[RB][KS], ai[RB][KS];
+    float ar[RB][KS], ai[RB][KS], nai[RB][KS];
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
@@ -318,6 +318,7 @@ __device__ inline void op_mfma(char *lds, int vstride, int m, const DevOp &op, M
             const float2 u = gload(M + (rb * 16 + i16) * D + ks * 4 + k4);
             ar[rb][ks] = u.x;
             ai[rb][ks] = u.y;
+            nai[rb][ks] = -u.y;  // Yr = Ur Xr + (-Ui) Xi: the sign lives on the constant operand
         }
     // per-lane byte slots: this lane's column (within a 16-column block) and
     // the rows it reads (B) and writes (accumulator)
@@ -362,7 +363,7 @@ __device__ inline void op_mfma(char *lds, int vstride, int m, const DevOp &op, M
                     for (int ks = 0; ks < KS; ++ks) {
                         yr[v][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[rb][ks], br[v][ks], yr[v][rb], 0, 0, 0);
                         yi[v][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[rb][ks], bi[v][ks], yi[v][rb], 0, 0, 0);
-                        yr[v][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[rb][ks], -bi[v][ks], yr[v][rb], 0, 0, 0);
+                        yr[v][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(nai[rb][ks], bi[v][ks], yr[v][rb], 0, 0, 0);
                         yi[v][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[rb][ks], br[v][ks], yi[v][rb], 0, 0, 0);
                     }
                 }
@@ -427,7 +428,7 @@ __device__ inline void op_mfma(char *lds, int vstride, int m, const DevOp &op, M
                     for (int u = 0; u < UN; ++u) {
                         yr[u][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[rb][ks], br[u][ks], yr[u][rb], 0, 0, 0);
                         yi[u][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[rb][ks], bi[u][ks], yi[u][rb], 0, 0, 0);
-                        yr[u][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[rb][ks], -bi[u][ks], yr[u][rb], 0, 0, 0);
+                        yr[u][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(nai[rb][ks], bi[u][ks], yr[u][rb], 0, 0, 0);
                         yi[u][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[rb][ks], br[u][ks], yi[u][rb], 0, 0, 0);
                     }
             }

@@ -82,7 +82,7 @@ def program_candidates(dtype: str, n: int) -> List[Dict]:
     tiles = [t for t in ((11, 12, 13) if c64 else (10, 11, 12)) if t <= n] or [n]
     out = [{"fuse_bits": f, "tile_bits": t, "use_mfma": False} for f in (2, 3) for t in tiles]
     if c64:
-        out += [{"fuse_bits": f, "tile_bits": t, "use_mfma": True} for f in (4, 5) for t in tiles]
+        out += [{"fuse_bits": f, "tile_bits": t, "use_mfma": True} for f in (3, 4, 5) for t in tiles]
     return out
 
 
