@@ -57,6 +57,65 @@ def cpu_baseline(n, gates, budget_s=15.0):
                       f"oracle/c (C + OpenMP, one pass over the state per gate), {dt:.1f} s"}
 
 
+def cpu_baselines_small(budget_s=8.0):
+    """Host baselines of the other workloads, each a bounded sample on the
+    oracle: the C + OpenMP restatement (all host cores) and the NumPy
+    per-gate gather formulation of the reference's kernel (one thread of
+    NumPy), complex64 for gates, complex128 for the VQE energy."""
+    import numpy as np
+    from desv import workloads
+    from oracle import sv_oracle as O
+    from oracle import sv_oracle_c as OC
+    threads = OC.default_threads()
+
+    def timed_rows(run_row, rows, budget):
+        done, t0 = 0, time.perf_counter()
+        for r in rows:
+            run_row(r)
+            done += 1
+            if time.perf_counter() - t0 > budget:
+                break
+        return done, time.perf_counter() - t0
+
+    out = {"cores": threads}
+    # plain 20-qubit circuit (1580 gates)
+    n = 20
+    ops = O.encode_ops(workloads.random_layers_circuit(n, 20, seed=0), n)
+    sv = np.zeros(1 << n, dtype=np.complex64)
+    sv[0] = 1
+    done, dt = timed_rows(lambda r: OC.run_ops(sv, r[None, :], threads), ops, budget_s)
+    out["plain20_c_gates_per_s"] = done / dt
+    sv2 = O.zero_state(n, np.complex64)
+    box = [sv2]
+    done, dt = timed_rows(lambda r: box.__setitem__(0, O.apply_op_gather(box[0], r)), ops, budget_s / 2)
+    out["plain20_numpy_gather_gates_per_s"] = done / dt
+    # 20-qubit XXZ Trotter (first steps of 200)
+    circ = workloads.xxz_trotter_circuit(n, 0.05, 200, seed=0)
+    xops = O.encode_ops(circ, n)
+    sv = np.zeros(1 << n, dtype=np.complex64)
+    sv[0] = 1
+    done, dt = timed_rows(lambda r: OC.run_ops(sv, r[None, :], threads), xops, budget_s)
+    out["xxz20_c_gates_per_s"] = done / dt
+    out["xxz20_gates"] = len(xops)
+    # 20-qubit VQE energy (forward on the C oracle in complex128, energy with
+    # the NumPy Pauli sum): one evaluation; a parameter-shift gradient costs
+    # 2 x 320 of them
+    vc, theta, terms = workloads.vqe_instance(n, 8, 200, seed=0)
+    tmpl = O.build_template(vc.ops, n)
+    t0 = time.perf_counter()
+    sv = np.zeros(1 << n, dtype=np.complex128)
+    sv[0] = 1
+    OC.run_ops(sv, O.patch_template(tmpl, theta), threads)
+    e = O.pauli_sum_expectation(sv, terms, n)
+    dt = time.perf_counter() - t0
+    out["vqe20_energy_evals_per_s"] = 1.0 / dt
+    out["vqe20_param_shift_grads_per_s"] = 1.0 / (dt * 2 * len(theta))
+    out["vqe20_energy"] = float(e)
+    out["sample"] = (f"each figure a bounded sample (about {budget_s:.0f} s): leading gates of the circuit; "
+                     "the VQE figure is one energy evaluation, the gradient figure that time x 640")
+    return out
+
+
 def _timed(fn, reps=1):
     import torch
     torch.cuda.synchronize()
@@ -551,6 +610,9 @@ def main():
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(n, gates)
+        log("cpu baselines of the other workloads ...")
+        out["cpu_baselines_other"] = cpu_baselines_small()
+        log(f"{out['cpu_baselines_other']}")
     del prog, h
     torch.cuda.synchronize()
     extras = {}
