@@ -393,7 +393,7 @@ int build(desv_batch b) {
     auto stage_launches = [&](bool rev, std::vector<PendingLaunch> &out) {
         const int maxS = rev ? maxR : maxF;
         for (int k = 0; k < maxS; ++k)
-            for (int heavy = 0; heavy < 3; ++heavy) {
+            for (int heavy = 0; heavy < 4; ++heavy) {
                 std::vector<StageTask> ts;
                 int64_t wg = 0;
                 int max_m = 0;

@@ -98,7 +98,7 @@ struct DevStage {
     int64_t n_tiles;
     int32_t op_begin, op_end;
     int32_t n_grad;    // OPK_GRAD* ops in this stage
-    int32_t heavy;     // kernel variant: 0 light, 1 k >= 3 VALU blocks, 2 MFMA blocks
+    int32_t heavy;     // kernel variant: 0 light, 1 k >= 3 VALU blocks, 2 MFMA blocks (k <= 4), 3 MFMA up to k = 5
     // Write-back bit permutation: store-order bit j (the j-th lowest global
     // bit of the tile) takes tile-local bit sperm[j].  Identity unless
     // `permuted`; lets a stage move qubits between the low (contiguous) index

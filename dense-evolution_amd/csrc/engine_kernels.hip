@@ -674,7 +674,8 @@ __device__ inline double op_grad_diag(char *lds, int vstride, int m, const DevOp
 // VAR selects the compiled micro-ops (register budget and occupancy):
 //   0  1-2 qubit dense, diagonal, scalar and gradient ops;
 //   1  + k = 3..5 VALU blocks;
-//   2  + k = 3..5 MFMA blocks (complex64; k = 3 VALU blocks kept as well).
+//   2  + k = 3, 4 MFMA blocks (complex64; k = 3 VALU blocks kept as well);
+//   3  + k = 5 MFMA blocks (their fragments double the register budget).
 template <typename T, int NV, int VAR>
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(std::is_same<T, float>::value ? 4 : 1))) k_stage(const StageTask *__restrict__ tasks,
                                                     const int32_t *__restrict__ wg_task, StageTask single) {
@@ -785,15 +786,15 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(s
                     if (op.k == 3) op_densek<T, NV, 3>(lds, vstride, m, op, M);
                     else if (op.k == 4) op_densek<T, NV, 4>(lds, vstride, m, op, M);
                     else op_densek<T, NV, 5>(lds, vstride, m, op, M);
-                } else if constexpr (VAR == 2) {
+                } else if constexpr (VAR >= 2) {
                     op_densek<T, NV, 3>(lds, vstride, m, op, M);
                 }
                 break;
             case OPK_MFMA:
-                if constexpr (VAR == 2 && std::is_same<T, float>::value) {
+                if constexpr (VAR >= 2 && std::is_same<T, float>::value) {
                     if (op.k == 3) op_mfma3<NV>(lds, vstride, m, op, M);
                     else if (op.k == 4) op_mfma<NV, 4>(lds, vstride, m, op, M);
-                    else op_mfma<NV, 5>(lds, vstride, m, op, M);
+                    else if constexpr (VAR == 3) op_mfma<NV, 5>(lds, vstride, m, op, M);
                 }
                 break;
             case OPK_DIAG: op_diag<T, NV>(lds, vstride, m, op, M, base); break;
@@ -851,17 +852,19 @@ hipError_t launch_stage(int nv, int variant, const StageTask *tasks, const int32
     const size_t lds = (size_t)nv * ((size_t)1 << max_m) * sizeof(C);
     if (grid <= 0) return hipSuccess;
     if (grid > 0x7fffffffll) return hipErrorInvalidValue;
-    if (variant < 0 || variant > 2) return hipErrorInvalidValue;
-    // complex128 never uses MFMA blocks: its variant 2 would only differ by
-    // omitting k = 4, 5 VALU blocks
-    if (!std::is_same<T, float>::value && variant == 2) variant = 1;
+    if (variant < 0 || variant > 3) return hipErrorInvalidValue;
+    // complex128 never uses MFMA blocks: its variants 2 and 3 would only
+    // differ from 1 by omitting k = 4, 5 VALU blocks
+    if (!std::is_same<T, float>::value && variant >= 2) variant = 1;
     const dim3 g((unsigned)grid);
 #define DESV_ST(NVV, VV) \
     hipLaunchKernelGGL((k_stage<T, NVV, VV>), g, dim3(kThreads), lds, s, tasks, wg_task, single)
     if (nv == 1) {
-        if (variant == 0) DESV_ST(1, 0); else if (variant == 1) DESV_ST(1, 1); else DESV_ST(1, 2);
+        if (variant == 0) DESV_ST(1, 0); else if (variant == 1) DESV_ST(1, 1);
+        else if (variant == 2) DESV_ST(1, 2); else DESV_ST(1, 3);
     } else {
-        if (variant == 0) DESV_ST(2, 0); else if (variant == 1) DESV_ST(2, 1); else DESV_ST(2, 2);
+        if (variant == 0) DESV_ST(2, 0); else if (variant == 1) DESV_ST(2, 1);
+        else if (variant == 2) DESV_ST(2, 2); else DESV_ST(2, 3);
     }
 #undef DESV_ST
     return hipGetLastError();
@@ -1220,11 +1223,13 @@ size_t stage_lds_limit(int device) {
     size_t lim = 64 * 1024;
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && v > 0)
         lim = (size_t)v;
-    const void *fns[12] = {
+    const void *fns[16] = {
         (const void *)k_stage<float, 1, 0>,  (const void *)k_stage<float, 2, 0>,  (const void *)k_stage<double, 1, 0>,
         (const void *)k_stage<double, 2, 0>, (const void *)k_stage<float, 1, 1>,  (const void *)k_stage<float, 2, 1>,
         (const void *)k_stage<double, 1, 1>, (const void *)k_stage<double, 2, 1>, (const void *)k_stage<float, 1, 2>,
-        (const void *)k_stage<float, 2, 2>,  (const void *)k_stage<double, 1, 2>, (const void *)k_stage<double, 2, 2>};
+        (const void *)k_stage<float, 2, 2>,  (const void *)k_stage<double, 1, 2>, (const void *)k_stage<double, 2, 2>,
+        (const void *)k_stage<float, 1, 3>,  (const void *)k_stage<float, 2, 3>,  (const void *)k_stage<double, 1, 3>,
+        (const void *)k_stage<double, 2, 3>};
     // leave room for the kernels' own (static) LDS arrays, as compiled
     size_t static_lds = 0;
     for (const void *f : fns) {

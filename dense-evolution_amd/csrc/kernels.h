@@ -107,7 +107,8 @@ hipError_t launch_direct(const void *in, void *out, int n, const DirectOp &op, c
 // Stage launch: `tasks` (device array) with wg_task[b] = task of workgroup b,
 // or, if tasks is null, `single`.
 template <typename T>
-// `variant`: DevStage::heavy (0 light, 1 VALU blocks k = 3..5, 2 MFMA blocks).
+// `variant`: DevStage::heavy (0 light, 1 VALU blocks k = 3..5, 2 MFMA blocks
+// k = 3, 4, 3 MFMA blocks up to k = 5).
 hipError_t launch_stage(int nv, int variant, const StageTask *tasks, const int32_t *wg_task, const StageTask &single,
                         int64_t grid, int max_m, hipStream_t s);
 template <typename T>

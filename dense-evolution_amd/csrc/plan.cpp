@@ -442,11 +442,12 @@ bool lower_stage(const std::vector<Gate> &ops, uint64_t tile, int n, const Compi
     }
     st.op_end = (int)plan.ops.size();
     st.n_grad = ngrad;
-    // kernel variant: 2 if any MFMA block (then no k >= 3 VALU blocks),
-    // 1 if any k >= 3 VALU block, else 0
+    // kernel variant: 3 if any 5-qubit MFMA block (its fragments double the
+    // register budget), 2 if any MFMA block (then no k >= 4 VALU blocks), 1
+    // if any k >= 3 VALU block, else 0
     st.heavy = 0;
     for (int i = st.op_begin; i < st.op_end; ++i) {
-        if (plan.ops[i].kind == OPK_MFMA) st.heavy = 2;
+        if (plan.ops[i].kind == OPK_MFMA) st.heavy = std::max(st.heavy, plan.ops[i].k == 5 ? 3 : 2);
         else if (plan.ops[i].kind == OPK_DENSEK && st.heavy == 0) st.heavy = 1;
     }
     plan.stages.push_back(st);
