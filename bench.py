@@ -11,7 +11,7 @@ of the dominant kernel (algorithmic FLOPs or bytes per launch / its average
 launch time, from HIP events recorded between stages on the launch stream);
 the CPU baseline (the oracle's C + OpenMP restatement applying the same gates
 at 30 qubits on every host core, a bounded sample of gates); and extra workloads
-(per-gate streaming passes, full-size c64-vs-c128 parity, 32 qubits, the
+(per-gate streaming passes, full-size c64-vs-c128 parity, 32 and 34 qubits, the
 20-qubit plain circuit, VQE adjoint vs parameter shift, the 1024-instance
 batch, the 28-qubit XXZ precision study, Lanczos ground states of the XXZ
 chain).
@@ -411,6 +411,36 @@ def suite_lanczos(n_big=26, n_cmp=16, ncv=20):
     return out
 
 
+def suite_brickwork34(n=34, depth=24):
+    """The brickwork at 34 qubits: a 128 GiB complex64 state, the largest
+    power of two that fits one MI355X's 288 GB (the state is updated in
+    place; the fused program has no out-of-place streaming stage).  Skipped
+    with a note if the device cannot hold it."""
+    import desv
+    from desv import workloads
+    gates = workloads.brickwork_su4(n, depth, seed=0)
+    prog = desv.Program(n)
+    for u, a, b in gates:
+        prog.add_matrix(u, [a, b])
+    prog.compile("complex64", fuse_bits=3, use_mfma=True, tile_bits=12)
+    try:
+        h = desv.StateHandle(n, complex128=False)
+    except MemoryError as e:
+        return {"workload": f"brickwork {n}q depth {depth}, c64 (128 GiB state)", "skipped": str(e)}
+    try:
+        h.set_double_buffer(False)
+        prog.run(h)
+        h.reset()
+        dt, _ = _timed(lambda: prog.run(h), reps=1)
+        nrm = h.norm2()
+    finally:
+        h.close()
+    st = prog.stats()
+    return {"workload": f"brickwork {n}q depth {depth}, {len(gates)} gates, c64 (128 GiB state, in place)",
+            "gates_per_s": len(gates) / dt, "ms_per_circuit": dt * 1e3, "hbm_passes": st["passes"],
+            "hbm_gbps": st["passes"] * 2 * (1 << n) * 8 / dt / 1e9, "norm_minus_1": nrm - 1}
+
+
 def copy_probe(n, steps):
     """Copy one 2^n complex64 state into another with the library's streaming
     copy (desv_copy): read + write bytes per second.  Under rocprofv3 --pmc,
@@ -461,7 +491,7 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="only time the library's device copy of a 2^qubits complex64 state "
                          "(calibration of HBM rate and of the rocprofv3 byte counters)")
-    ap.add_argument("--suite", default="streaming30,parity30,brickwork32,plain20,vqe20,batch,precision,lanczos",
+    ap.add_argument("--suite", default="streaming30,parity30,brickwork32,brickwork34,plain20,vqe20,batch,precision,lanczos",
                     help="extra workloads to measure after the headline ('' for none)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -618,7 +648,7 @@ def main():
     extras = {}
     fns = {"streaming30": suite_streaming30, "parity30": suite_parity30, "brickwork32": suite_brickwork32,
            "plain20": suite_plain20, "vqe20": suite_vqe20, "batch": suite_batch,
-           "precision": suite_precision, "lanczos": suite_lanczos}
+           "precision": suite_precision, "lanczos": suite_lanczos, "brickwork34": suite_brickwork34}
     for name in [x for x in args.suite.split(",") if x]:
         log(f"suite {name} ...")
         t0 = time.perf_counter()
