@@ -112,3 +112,21 @@ def test_random_op_table_24q_against_c_oracle(dtype):
     f = abs(np.vdot(got, want)) ** 2
     print(f"24q random ops {dtype}: 1 - fidelity = {1 - f:.3e}")
     assert f >= (1 - 1e-12 if dtype == "complex128" else 1 - 1e-6)
+
+
+def test_maximum_size_34_qubits():
+    """n = 34 is the largest width the reference accepts (statevector.py:88-89):
+    a 128 GiB complex64 state, updated in place.  Hadamards on the first and
+    last qubit and a CX between them give the marginals and the parity the
+    algebra fixes."""
+    n = 34
+    h = desv.StateHandle(n, False)
+    h.set_double_buffer(False)
+    h.run_ops(np.array([[1, 0, 0, 0.0], [1, n - 1, 0, 0.0], [20, 0, 17, 0.0]]))
+    assert abs(h.norm2() - 1) < 1e-6
+    for q, want in ((0, 0.5), (n - 1, 0.5), (17, 0.5), (9, 0.0)):
+        p0, p1 = h.qubit_probs(q)
+        assert abs(p1 - want) < 1e-6, (q, p0, p1)
+    # qubits 0 and 17 are perfectly correlated after the CX: <Z_0 Z_17> = 1
+    assert abs(desv.pauli_expectation(desv.DeviceState(h), {0: 'Z', 17: 'Z'}) - 1) < 1e-6
+    h.close()
