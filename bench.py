@@ -163,7 +163,7 @@ def suite_vqe20():
     H = desv.PauliSumOperator(terms, n)
     f128, _ = desv.circuit_to_energy_fn(circ, n, dtype="complex128")
     f64, _ = desv.circuit_to_energy_fn(circ, n, dtype="complex64")
-    tuned = {name: f.autotune(theta, H, cache=False)["best"] for name, f in (("c64", f64), ("c128", f128))}
+    tuned = {name: f.autotune(theta, H, cache=False, reps=10)["best"] for name, f in (("c64", f64), ("c128", f128))}
     log(f"vqe20 autotune: {tuned}")
     f128.value_and_grad(theta, H)
     f64.value_and_grad(theta, H)
