@@ -1037,7 +1037,7 @@ template <typename V, bool NT> __device__ inline void st(V *p, V v) {
 template <typename T, int K, int VEC, bool DIAG, int GPT, bool NT, bool OOP>
 __global__ void __launch_bounds__(kThreads) k_direct(const typename VT<T>::c *src, typename VT<T>::c *dst,
                                                     int64_t groups, DirectOp op,
-                                                    const typename VT<T>::c *__restrict__ M, int xcd) {
+                                                    const typename VT<T>::c *__restrict__ M, int xcd, int contig) {
     using C = typename VT<T>::c;
     constexpr int D = 1 << K;
     int srt[K];
@@ -1078,7 +1078,8 @@ __global__ void __launch_bounds__(kThreads) k_direct(const typename VT<T>::c *sr
     C a[GPT][D][VEC];
 #pragma unroll
     for (int u = 0; u < GPT; ++u) {
-        const int64_t g0 = (t0 + u * nthr) * VEC;
+        // contig: wave w owns the GPT consecutive 64-vector runs w GPT, ..., w GPT + GPT - 1
+        const int64_t g0 = (contig ? ((t0 >> 6) * GPT + u) * 64 + (t0 & 63) : t0 + u * nthr) * VEC;
         uint64_t x = (uint64_t)g0;
 #pragma unroll
         for (int j = 0; j < K; ++j) x = ((x >> srt[j]) << (srt[j] + 1)) | (x & ((1ull << srt[j]) - 1));
@@ -1129,39 +1130,183 @@ __global__ void __launch_bounds__(kThreads) k_direct(const typename VT<T>::c *sr
 }
 
 namespace {
-// Tuning knobs of the direct kernel (group vectors per thread, non-temporal
-// access), read once from DESV_DIRECT_GPT / DESV_DIRECT_NT.
-int direct_gpt() {
-    static int v = [] {
-        const char *e = std::getenv("DESV_DIRECT_GPT");
-        const int x = e ? std::atoi(e) : 1;
-        return (x == 1 || x == 2 || x == 4) ? x : 1;
-    }();
-    return v;
+// Launch shape of the direct kernel: group vectors per thread (GPT), whether
+// a wave's GPT vectors are consecutive 64-vector runs (contig) or strided by
+// the grid, non-temporal access (nt), and XCD-sliced workgroup order.
+// DESV_DIRECT_GPT / _CONTIG / _NT / _XCD override the per-op choice.
+int env_int(const char *name, int dflt) {
+    const char *e = std::getenv(name);
+    return e ? std::atoi(e) : dflt;
 }
-bool direct_nt() {
-    static bool v = [] {
-        const char *e = std::getenv("DESV_DIRECT_NT");
-        return e ? std::atoi(e) != 0 : false;
-    }();
-    return v;
-}
-// MI355X deals workgroups to its 8 XCDs round-robin; slicing the grid per XCD
-// measured 5.38 -> 5.52 TB/s on 30-qubit 2-qubit-gate passes.
 int direct_xcd() {
-    static int v = [] {
-        const char *e = std::getenv("DESV_DIRECT_XCD");
-        return e ? std::atoi(e) : 8;
-    }();
+    // MI355X deals workgroups to its 8 XCDs round-robin; slicing the grid per
+    // XCD measured 5.38 -> 5.52 TB/s on 30-qubit 2-qubit-gate passes.
+    static int v = env_int("DESV_DIRECT_XCD", 8);
     return v;
+}
+struct DirectShape {
+    int gpt;
+    bool nt, contig;
+};
+// Complex64 16-byte passes, by the lowest target bit lo (30-qubit brickwork,
+// ms per pass, strided GPT 1 / contig GPT 4 / contig GPT 4 + nt):
+//   lo = 1: 3.74 / 3.68 / 11.75    lo = 2: 3.43 / 3.23 / 8.44
+//   lo = 4..8: 2.93 / 2.98 / 2.96  lo >= 9: 2.67..3.60, nt + contig 2-6 % faster.
+// Non-temporal stores need whole 128-byte lines per store instruction
+// (lo >= 4) and pay off once a wave's 4 KB runs stay inside one target
+// half (lo >= 9).
+DirectShape direct_shape(bool vec16, int lo) {
+    static const int g = env_int("DESV_DIRECT_GPT", 0), c = env_int("DESV_DIRECT_CONTIG", -1),
+                     t = env_int("DESV_DIRECT_NT", -1);
+    DirectShape d{1, false, false};
+    if (vec16 && lo >= 9) d = {4, true, true};
+    else if (vec16 && lo <= 2) d = {4, false, true};
+    if (g == 1 || g == 2 || g == 4) d.gpt = g;
+    if (c >= 0) d.contig = c != 0;
+    if (t >= 0) d.nt = t != 0;
+    return d;
 }
 }  // namespace
+
+// Complex64 gates whose targets or controls include index bit 0, still with
+// 16-byte accesses: a float4 holds the amplitude pair (x, x | 1), so bit 0
+// stays inside the lane.  J0 >= 0: bit 0 is target J0 of the op (matrix bit
+// J0); J0 < 0: bit 0 is a control.  The other NH targets are "outer" bits:
+// thread u owns the 2^NH float4 pairs at x | off(c), c < 2^NH, and amplitude
+// (c, h) = amp[x | off(c) | h].  With bit 0 a control only the h = 1 half is
+// transformed; the h = 0 half is written back as read.
+template <int K, int J0, bool DIAG, bool OOP>
+__global__ void __launch_bounds__(kThreads) k_direct_b0(const float2 *src, float2 *dst, int64_t units, DirectOp op,
+                                                       const float2 *__restrict__ M, int xcd) {
+    constexpr int D = 1 << K;
+    constexpr int NH = J0 < 0 ? K : K - 1;
+    constexpr int NC = 1 << NH;
+    // matrix bit of outer bit i
+    auto mbit = [](int i) constexpr { return (J0 >= 0 && i >= J0) ? i + 1 : i; };
+    uint64_t off[NC];
+    int lo = 64, hi = 0;  // outer positions, ascending (NH <= 2)
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        uint64_t o = 0;
+#pragma unroll
+        for (int i = 0; i < NH; ++i)
+            if (c >> i & 1) o |= 1ull << op.pos[mbit(i)];
+        off[c] = o;
+    }
+#pragma unroll
+    for (int i = 0; i < NH; ++i) {
+        lo = min(lo, op.pos[mbit(i)]);
+        hi = max(hi, op.pos[mbit(i)]);
+    }
+    float2 mm[DIAG ? D : D * D];
+#pragma unroll
+    for (int i = 0; i < (DIAG ? D : D * D); ++i) mm[i] = M[i];
+    const uint64_t cm = op.cmask;
+    int64_t bid = blockIdx.x;
+    if (xcd > 1) {
+        const int64_t per = (int64_t)gridDim.x / xcd;
+        if (bid < per * xcd) bid = (bid % xcd) * per + bid / xcd;
+    }
+    const int64_t u = bid * blockDim.x + threadIdx.x;
+    if (u >= units) return;
+    // pair index y (bit b of y = index bit b + 1) with zeros at the outer bits
+    uint64_t y = (uint64_t)u;
+    if constexpr (NH >= 1) y = ((y >> (lo - 1)) << lo) | (y & ((1ull << (lo - 1)) - 1));
+    if constexpr (NH == 2) y = ((y >> (hi - 1)) << hi) | (y & ((1ull << (hi - 1)) - 1));
+    const uint64_t x = y << 1;
+    const bool act = J0 >= 0 ? (x & cm) == cm : ((x | 1) & cm) == cm;
+    if (!OOP && !act) return;
+    float2 a[NC][2];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        const float4 v = ld<float4, false>(reinterpret_cast<const float4 *>(&src[x | off[c]]));
+        a[c][0] = make_float2(v.x, v.y);
+        a[c][1] = make_float2(v.z, v.w);
+    }
+    float2 r[NC][2];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) r[c][0] = a[c][0], r[c][1] = a[c][1];
+    if (act) {
+        if constexpr (J0 >= 0) {
+            float2 in[D];  // in[matrix index]
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    int q = h << J0;
+#pragma unroll
+                    for (int i = 0; i < NH; ++i) q |= (c >> i & 1) << mbit(i);
+                    in[q] = a[c][h];
+                }
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    int q = h << J0;
+#pragma unroll
+                    for (int i = 0; i < NH; ++i) q |= (c >> i & 1) << mbit(i);
+                    if constexpr (DIAG) {
+                        r[c][h] = cmul(mm[q], in[q]);
+                    } else {
+                        float2 acc = cmul(mm[q * D], in[0]);
+#pragma unroll
+                        for (int k = 1; k < D; ++k) acc = cfma(mm[q * D + k], in[k], acc);
+                        r[c][h] = acc;
+                    }
+                }
+        } else {
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {  // matrix index of (c, 1) is c
+                if constexpr (DIAG) {
+                    r[c][1] = cmul(mm[c], a[c][1]);
+                } else {
+                    float2 acc = cmul(mm[c * D], a[0][1]);
+#pragma unroll
+                    for (int k = 1; k < D; ++k) acc = cfma(mm[c * D + k], a[k][1], acc);
+                    r[c][1] = acc;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+        st<float4, false>(reinterpret_cast<float4 *>(&dst[x | off[c]]),
+                          make_float4(r[c][0].x, r[c][0].y, r[c][1].x, r[c][1].y));
+}
+
+template <int K, int J0, bool DIAG>
+hipError_t launch_direct_b0(const float2 *src, float2 *dst, int n, const DirectOp &op, const float2 *m, hipStream_t s) {
+    constexpr int NH = J0 < 0 ? K : K - 1;
+    const int64_t units = 1ll << (n - 1 - NH);
+    const int64_t grid = std::max<int64_t>(1, (units + kThreads - 1) / kThreads);
+    if (grid > 0x7fffffffll) return hipErrorInvalidValue;
+    if (src != dst)
+        hipLaunchKernelGGL((k_direct_b0<K, J0, DIAG, true>), dim3((unsigned)grid), dim3(kThreads), 0, s, src, dst,
+                           units, op, m, direct_xcd());
+    else
+        hipLaunchKernelGGL((k_direct_b0<K, J0, DIAG, false>), dim3((unsigned)grid), dim3(kThreads), 0, s, src, dst,
+                           units, op, m, direct_xcd());
+    return hipGetLastError();
+}
+
+template <int K, bool DIAG>
+hipError_t dispatch_direct_b0(const float2 *src, float2 *dst, int n, const DirectOp &op, const float2 *m,
+                              hipStream_t s) {
+    if (op.pos[0] == 0) return launch_direct_b0<K, 0, DIAG>(src, dst, n, op, m, s);
+    if constexpr (K == 2)
+        if (op.pos[1] == 0) return launch_direct_b0<K, 1, DIAG>(src, dst, n, op, m, s);
+    return launch_direct_b0<K, -1, DIAG>(src, dst, n, op, m, s);
+}
 
 template <typename T, int K, int VEC, bool DIAG>
 hipError_t launch_direct_k(const typename VT<T>::c *src, typename VT<T>::c *dst, int64_t groups, const DirectOp &op,
                            const typename VT<T>::c *m, hipStream_t s) {
-    const int gpt = direct_gpt();
-    const bool nt = direct_nt();
+    int lo = 64;
+    for (int j = 0; j < K; ++j) lo = std::min(lo, (int)op.pos[j]);
+    const DirectShape shape = direct_shape(VEC == 2, lo);
+    const int gpt = shape.gpt;
+    const bool nt = shape.nt;
+    const int contig = shape.contig;
     const int xcd = direct_xcd();
     const int64_t vecs = (groups + VEC - 1) / VEC;
     const int64_t threads = (vecs + gpt - 1) / gpt;
@@ -1169,7 +1314,7 @@ hipError_t launch_direct_k(const typename VT<T>::c *src, typename VT<T>::c *dst,
     if (grid > 0x7fffffffll) return hipErrorInvalidValue;
 #define DESV_DK(G, N, O)                                                                               \
     hipLaunchKernelGGL((k_direct<T, K, VEC, DIAG, G, N, O>), dim3((unsigned)grid), dim3(kThreads), 0, s, src, dst, \
-                       groups, op, m, xcd)
+                       groups, op, m, xcd, contig)
     if (src != dst) {
         if (nt) {
             if (gpt == 1) DESV_DK(1, true, true); else if (gpt == 2) DESV_DK(2, true, true); else DESV_DK(4, true, true);
@@ -1197,6 +1342,12 @@ hipError_t launch_direct(const void *in, void *out, int n, const DirectOp &op, c
     C *q = reinterpret_cast<C *>(out);
     const C *m = reinterpret_cast<const C *>(M);
     if constexpr (std::is_same<T, float>::value) {
+        if (bit0) {
+            if (K == 1) return op.diag ? dispatch_direct_b0<1, true>(p, q, n, op, m, s)
+                                       : dispatch_direct_b0<1, false>(p, q, n, op, m, s);
+            return op.diag ? dispatch_direct_b0<2, true>(p, q, n, op, m, s)
+                           : dispatch_direct_b0<2, false>(p, q, n, op, m, s);
+        }
         if (K == 1) {
             if (op.diag) return vec ? launch_direct_k<T, 1, 2, true>(p, q, groups, op, m, s)
                                     : launch_direct_k<T, 1, 1, true>(p, q, groups, op, m, s);

@@ -132,6 +132,47 @@ def test_streaming_passes_double_buffer(dtype, rng):
         assert abs(nrm - 1) < 1e-5
 
 
+@pytest.mark.parametrize("dtype", ["complex128", "complex64"])
+def test_streaming_index_bit0(dtype, rng):
+    """Every gate id with index bit 0 (the last qubit) as target, as control
+    and as one of two targets, on the streaming kernel (complex64 runs these
+    with the amplitude pair (x, x | 1) in one 16-byte lane), in place and out
+    of place, and dense 4x4 unitaries on pairs that include it; amplitudes
+    within 1e-10 (complex128) or 2e-6 (complex64) of the oracle."""
+    n = 9
+    last = n - 1
+    rows = [[g, last, 0, rng.uniform(-np.pi, np.pi)] for g in range(15)]
+    for g in (20, 21, 22, 23, 24, 25):
+        for a, b in ((last, 3), (2, last), (last, 0), (0, last), (last, 7), (7, last)):
+            rows.append([g, a, b, rng.uniform(-np.pi, np.pi)])
+    ops = np.array(rows, dtype=np.float64)
+    sv0 = rand_state(n, rng)
+    prog = desv.Program(n).add_ops(ops).compile(dtype, fuse_bits=1, per_gate=True)
+    want = O.run_ops_fast(sv0, ops)
+    tol = 1e-10 if dtype == "complex128" else 2e-6
+    for mode in (True, False):
+        h = desv.StateHandle(n, dtype == "complex128")
+        h.set_double_buffer(mode)
+        h.upload(sv0)
+        prog.run(h)
+        got = h.download()
+        h.close()
+        assert np.max(np.abs(got - want)) < tol, mode
+    sim = desv.DenseSVSimulator(n, use_float32=dtype == "complex64")
+    sim.sv = sv0.astype(sim.dtype)
+    want = sv0.copy()
+    for a, b in ((last, 3), (1, last), (last, 7), (0, last)):
+        z = rng.normal(size=(4, 4)) + 1j * rng.normal(size=(4, 4))
+        u, _ = np.linalg.qr(z)
+        sim.apply_gate_2q(u, a, b)
+        want = O.apply_matrix(want, u, [a, b])
+        v, _ = np.linalg.qr(rng.normal(size=(2, 2)) + 1j * rng.normal(size=(2, 2)))
+        sim.apply_gate_1q(v, last)
+        want = O.apply_matrix(want, v, [last])
+    assert np.max(np.abs(sim.get_statevector() - want)) < tol
+    sim.close()
+
+
 @pytest.mark.parametrize("k", [1, 2, 3, 4, 5])
 @pytest.mark.parametrize("use_float32", [False, True])
 def test_apply_arbitrary_unitaries(k, use_float32, rng):
