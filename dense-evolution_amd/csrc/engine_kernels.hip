@@ -1166,6 +1166,13 @@ DirectShape direct_shape(bool vec16, int lo) {
     if (t >= 0) d.nt = t != 0;
     return d;
 }
+// The lane-exchange kernel for targets in bits 1..6: used where the strided
+// 16-byte kernel is slow (lowest target bit 1 or 2); DESV_DIRECT_LANE = 0 / 1
+// turns it off / on for every eligible op.
+bool direct_lane(int lo) {
+    static const int v = env_int("DESV_DIRECT_LANE", -1);
+    return v < 0 ? lo <= 2 : v != 0;
+}
 }  // namespace
 
 // Complex64 gates whose targets or controls include index bit 0, still with
@@ -1298,6 +1305,84 @@ hipError_t dispatch_direct_b0(const float2 *src, float2 *dst, int n, const Direc
     return launch_direct_b0<K, -1, DIAG>(src, dst, n, op, m, s);
 }
 
+// Complex64 gates whose targets all lie in index bits 1..6 (no target at
+// bit 0): every access instruction is lane-contiguous (wave w, step s, lane l
+// owns float4 f = 256 w + 64 s + l, the amplitude pair (2f, 2f + 1)), so the
+// members of a group sit in lanes that differ at lane bits pos - 1 and are
+// exchanged by xor shuffles.  A lane's member index m depends on the lane
+// only, so its matrix row M[m][m ^ d] is loaded once.  Controls never touch
+// target bits, so a group shares its control outcome (per half: bit 0 may be
+// a control).  nvec4 must be a multiple of 256 (whole waves in range).
+template <int K, bool DIAG, bool OOP>
+__global__ void __launch_bounds__(kThreads) k_direct_lane(const float2 *src, float2 *dst, int64_t nvec4, DirectOp op,
+                                                         const float2 *__restrict__ M, int xcd) {
+    constexpr int D = 1 << K, S = 4;
+    const int lane = threadIdx.x & 63;
+    int m = 0, dep[D];
+#pragma unroll
+    for (int j = 0; j < K; ++j) m |= ((lane >> (op.pos[j] - 1)) & 1) << j;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        int v = 0;
+#pragma unroll
+        for (int j = 0; j < K; ++j) v |= ((d >> j) & 1) << (op.pos[j] - 1);
+        dep[d] = v;
+    }
+    float2 c[DIAG ? 1 : D];
+    if constexpr (DIAG) {
+        c[0] = M[m];
+    } else {
+#pragma unroll
+        for (int d = 0; d < D; ++d) c[d] = M[m * D + (m ^ d)];
+    }
+    int64_t bid = blockIdx.x;
+    if (xcd > 1) {
+        const int64_t per = (int64_t)gridDim.x / xcd;
+        if (bid < per * xcd) bid = (bid % xcd) * per + bid / xcd;
+    }
+    const int64_t f0 = ((bid * blockDim.x + threadIdx.x) >> 6) * (S * 64);
+    if (f0 >= nvec4) return;  // the whole wave
+    const uint64_t cm = op.cmask;
+    float4 a[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) a[s] = ld<float4, false>(reinterpret_cast<const float4 *>(src) + f0 + s * 64 + lane);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const uint64_t e0 = (uint64_t)(f0 + s * 64 + lane) << 1;
+        const bool c0 = (e0 & cm) == cm, c1 = ((e0 | 1) & cm) == cm;
+        float2 lo = make_float2(a[s].x, a[s].y), hi = make_float2(a[s].z, a[s].w);
+        float2 r0 = cmul(c[0], lo), r1 = cmul(c[0], hi);
+        if constexpr (!DIAG) {
+#pragma unroll
+            for (int d = 1; d < D; ++d) {
+                const float2 plo = make_float2(__shfl_xor(a[s].x, dep[d]), __shfl_xor(a[s].y, dep[d]));
+                const float2 phi = make_float2(__shfl_xor(a[s].z, dep[d]), __shfl_xor(a[s].w, dep[d]));
+                r0 = cfma(c[d], plo, r0);
+                r1 = cfma(c[d], phi, r1);
+            }
+        }
+        if (!c0) r0 = lo;
+        if (!c1) r1 = hi;
+        if (OOP || c0 || c1)
+            st<float4, false>(reinterpret_cast<float4 *>(dst) + f0 + s * 64 + lane, make_float4(r0.x, r0.y, r1.x, r1.y));
+    }
+}
+
+template <int K, bool DIAG>
+hipError_t launch_direct_lane(const float2 *src, float2 *dst, int n, const DirectOp &op, const float2 *m,
+                              hipStream_t s) {
+    const int64_t nvec4 = 1ll << (n - 1);
+    const int64_t grid = std::max<int64_t>(1, nvec4 / 4 / kThreads + (nvec4 / 4 % kThreads != 0));
+    if (grid > 0x7fffffffll) return hipErrorInvalidValue;
+    if (src != dst)
+        hipLaunchKernelGGL((k_direct_lane<K, DIAG, true>), dim3((unsigned)grid), dim3(kThreads), 0, s, src, dst,
+                           nvec4, op, m, direct_xcd());
+    else
+        hipLaunchKernelGGL((k_direct_lane<K, DIAG, false>), dim3((unsigned)grid), dim3(kThreads), 0, s, src, dst,
+                           nvec4, op, m, direct_xcd());
+    return hipGetLastError();
+}
+
 template <typename T, int K, int VEC, bool DIAG>
 hipError_t launch_direct_k(const typename VT<T>::c *src, typename VT<T>::c *dst, int64_t groups, const DirectOp &op,
                            const typename VT<T>::c *m, hipStream_t s) {
@@ -1347,6 +1432,14 @@ hipError_t launch_direct(const void *in, void *out, int n, const DirectOp &op, c
                                        : dispatch_direct_b0<1, false>(p, q, n, op, m, s);
             return op.diag ? dispatch_direct_b0<2, true>(p, q, n, op, m, s)
                            : dispatch_direct_b0<2, false>(p, q, n, op, m, s);
+        }
+        int lo = 64, hi = 0;
+        for (int j = 0; j < K; ++j) lo = std::min(lo, (int)op.pos[j]), hi = std::max(hi, (int)op.pos[j]);
+        if (lo >= 1 && hi <= 6 && n >= 9 && direct_lane(lo)) {
+            if (K == 1) return op.diag ? launch_direct_lane<1, true>(p, q, n, op, m, s)
+                                       : launch_direct_lane<1, false>(p, q, n, op, m, s);
+            return op.diag ? launch_direct_lane<2, true>(p, q, n, op, m, s)
+                           : launch_direct_lane<2, false>(p, q, n, op, m, s);
         }
         if (K == 1) {
             if (op.diag) return vec ? launch_direct_k<T, 1, 2, true>(p, q, groups, op, m, s)

@@ -132,18 +132,21 @@ def test_streaming_passes_double_buffer(dtype, rng):
         assert abs(nrm - 1) < 1e-5
 
 
+@pytest.mark.parametrize("n", [9, 12])
 @pytest.mark.parametrize("dtype", ["complex128", "complex64"])
-def test_streaming_index_bit0(dtype, rng):
-    """Every gate id with index bit 0 (the last qubit) as target, as control
-    and as one of two targets, on the streaming kernel (complex64 runs these
-    with the amplitude pair (x, x | 1) in one 16-byte lane), in place and out
-    of place, and dense 4x4 unitaries on pairs that include it; amplitudes
-    within 1e-10 (complex128) or 2e-6 (complex64) of the oracle."""
-    n = 9
+def test_streaming_low_index_bits(n, dtype, rng):
+    """Every gate id on the lowest index bits on the streaming kernels: bit 0
+    (the last qubit) as target, as control and as one of two targets
+    (complex64 keeps the amplitude pair (x, x | 1) in one 16-byte lane), and
+    targets in bits 1 and 2 (complex64 exchanges group members between lanes),
+    in place and out of place, plus dense 4x4 unitaries on such pairs;
+    amplitudes within 1e-10 (complex128) or 2e-6 (complex64) of the oracle."""
     last = n - 1
-    rows = [[g, last, 0, rng.uniform(-np.pi, np.pi)] for g in range(15)]
+    rows = [[g, q, 0, rng.uniform(-np.pi, np.pi)] for q in (last, n - 2, n - 3) for g in range(15)]
+    pairs = ((last, 3), (2, last), (last, 0), (0, last), (last, n - 2), (n - 2, last), (n - 2, n - 3),
+             (n - 3, n - 2), (n - 2, n - 5), (n - 7, n - 3), (n - 3, 0), (n - 2, 1))
     for g in (20, 21, 22, 23, 24, 25):
-        for a, b in ((last, 3), (2, last), (last, 0), (0, last), (last, 7), (7, last)):
+        for a, b in pairs:
             rows.append([g, a, b, rng.uniform(-np.pi, np.pi)])
     ops = np.array(rows, dtype=np.float64)
     sv0 = rand_state(n, rng)
@@ -161,14 +164,14 @@ def test_streaming_index_bit0(dtype, rng):
     sim = desv.DenseSVSimulator(n, use_float32=dtype == "complex64")
     sim.sv = sv0.astype(sim.dtype)
     want = sv0.copy()
-    for a, b in ((last, 3), (1, last), (last, 7), (0, last)):
+    for a, b in ((last, 3), (1, last), (last, n - 2), (0, last), (n - 2, n - 3), (n - 5, n - 2)):
         z = rng.normal(size=(4, 4)) + 1j * rng.normal(size=(4, 4))
         u, _ = np.linalg.qr(z)
         sim.apply_gate_2q(u, a, b)
         want = O.apply_matrix(want, u, [a, b])
         v, _ = np.linalg.qr(rng.normal(size=(2, 2)) + 1j * rng.normal(size=(2, 2)))
-        sim.apply_gate_1q(v, last)
-        want = O.apply_matrix(want, v, [last])
+        sim.apply_gate_1q(v, b)
+        want = O.apply_matrix(want, v, [b])
     assert np.max(np.abs(sim.get_statevector() - want)) < tol
     sim.close()
 
