@@ -342,11 +342,34 @@ std::vector<Gate> fuse(const std::vector<Gate> &gs, int K, uint64_t allowed, std
             keep.push_back(std::move(nb));
             open.swap(keep);
         } else {
-            emit_touching(am);
+            // Too wide for one block: emit the widest touching blocks until
+            // the rest fit with the gate, and merge the rest into it.  The
+            // touching blocks are disjoint and all precede the gate, so any
+            // of them may go first (a lone 1-qubit gate waiting on the new
+            // gate's other bit is absorbed instead of emitted on its own).
+            std::vector<int> touch;
+            for (int b = 0; b < (int)open.size(); ++b)
+                if (open[b].bits & am) touch.push_back(b);
+            std::stable_sort(touch.begin(), touch.end(),
+                             [&](int x, int y) { return popc(open[x].bits) < popc(open[y].bits); });
+            uint64_t keep_bits = am;
+            std::vector<char> merge(open.size(), 0);
+            for (int b : touch)
+                if (popc(keep_bits | open[b].bits) <= K) {
+                    keep_bits |= open[b].bits;
+                    merge[b] = 1;
+                }
             Block nb;
-            nb.bits = am;
+            nb.bits = keep_bits;
+            std::vector<Block> keep;
+            for (int b = 0; b < (int)open.size(); ++b) {
+                if (merge[b]) nb.gates.insert(nb.gates.end(), open[b].gates.begin(), open[b].gates.end());
+                else if (open[b].bits & am) emit(open[b]);
+                else keep.push_back(std::move(open[b]));
+            }
             nb.gates.push_back(i);
-            open.push_back(std::move(nb));
+            keep.push_back(std::move(nb));
+            open.swap(keep);
         }
     }
     for (auto &b : open) emit(b);
@@ -488,6 +511,47 @@ int scan_take(const std::vector<Gate> &gates, const std::vector<int> &rem, uint6
     return cnt;
 }
 
+// A stage's trailing 1-qubit gates (dense or diagonal) on a qubit that has a
+// later gate among the deferred ones move to the deferred list: they commute
+// with everything off their qubit, and in the later stage they fuse into the
+// block of that gate instead of costing a tile sweep of their own here (the
+// last gates of a qubit before a stage boundary are otherwise emitted as lone
+// 1-qubit ops).  Per-qubit order is kept: only a qubit's final run in `take`
+// moves.  Both lists stay in gate order; `take` is never emptied.
+void defer_trailing_singles(const std::vector<Gate> &gates, std::vector<int> &take, std::vector<int> &defer) {
+    uint64_t later = 0;
+    for (int gi : defer) later |= gates[gi].all_mask();
+    uint64_t closed = 0;
+    std::vector<char> mv(take.size(), 0);
+    size_t nmv = 0;
+    for (int t = (int)take.size() - 1; t >= 0; --t) {
+        const Gate &g = gates[take[t]];
+        const uint64_t am = g.all_mask();
+        if (popc(am) == 1 && (g.kind == G_DENSE || g.kind == G_DIAG) && (later & am) && !(closed & am)) {
+            mv[t] = 1;
+            ++nmv;
+        } else {
+            closed |= am;
+        }
+    }
+    if (nmv == 0 || nmv == take.size()) return;
+    std::vector<int> t2, d2;
+    t2.reserve(take.size() - nmv);
+    d2.reserve(defer.size() + nmv);
+    size_t j = 0;
+    for (size_t t = 0; t < take.size(); ++t) {
+        if (!mv[t]) {
+            t2.push_back(take[t]);
+            continue;
+        }
+        while (j < defer.size() && defer[j] < take[t]) d2.push_back(defer[j++]);
+        d2.push_back(take[t]);
+    }
+    while (j < defer.size()) d2.push_back(defer[j++]);
+    take.swap(t2);
+    defer.swap(d2);
+}
+
 Gate to_physical(const Gate &g, const std::vector<int> &pi) {
     Gate p = g;
     for (int j = 0; j < g.k; ++j) p.bits[j] = pi[g.bits[j]];
@@ -594,6 +658,7 @@ bool build_plan(const std::vector<Gate> &gates, int n, const CompileOpts &o, Pla
             err = "internal: scheduler made no progress";
             return false;
         }
+        if (m < n && K > 1) defer_trailing_singles(gates, take, defer);
         // fill the tile: bits the next deferred gates need, then the lowest physical bits
         for (size_t i = 0; i < defer.size() && popc(S) < m && i < 256; ++i) {
             const uint64_t need = gates[defer[i]].tile_mask() & ~S;
