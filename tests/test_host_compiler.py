@@ -119,6 +119,29 @@ def test_remapped_schedules_restore_the_layout(rng):
         assert np.allclose(interpret(prog.plan(), sv0), want, atol=1e-10)
 
 
+@pytest.mark.parametrize("fuse,tile,low", [(2, 7, 3), (3, 8, 4), (4, 7, 3)])
+def test_trotter_staircase_fuses_single_qubit_gates(fuse, tile, low):
+    """A Trotter staircase (XX, YY, ZZ on every bond, as h/sdg/cx/rz
+    sequences) compiles to blocks only: every 1-qubit gate joins a block of a
+    neighbouring 2-qubit gate, including the last gates of a qubit before a
+    stage boundary, which wait for the stage of that qubit's next gate.  The
+    plan still replays the circuit exactly."""
+    n = 12
+    circ = desv.workloads.xxz_trotter_circuit(n, 0.05, 6, seed=0)
+    ops = desv.simulator.encode_op_table(desv.QuantumTranspiler.transpile(circ), n)
+    prog = desv.Program(n).add_ops(ops).compile("complex128", fuse_bits=fuse, tile_bits=tile, low_bits=low)
+    tm, so, dops, mats = prog.plan()
+    assert prog.stats()["passes"] > 1
+    lone = [o for o in dops if int(o[1]) == 1 and int(o[0]) in (OPK_DENSE1, OPK_DIAG)]
+    assert lone == []
+    # at most one block per bond and step (each bond's 21 h/sdg/s/cx/rz gates fuse into one)
+    assert len(dops) <= (n - 1) * 6
+    rng = np.random.default_rng(5)
+    sv0 = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    sv0 /= np.linalg.norm(sv0)
+    assert np.allclose(interpret((tm, so, dops, mats), sv0), O.run_ops(sv0.copy(), ops), atol=1e-10)
+
+
 def test_cache_blocking_reduces_passes(rng):
     n = 16
     prog = desv.Program(n).add_circuit(desv.workloads.random_layers_circuit(n, 4, seed=1))
