@@ -137,7 +137,8 @@ def suite_plain20():
     circ = workloads.random_layers_circuit(n, 20, seed=0)
     s64 = desv.DenseSVSimulator(n, use_float32=True)
     ops = s64.compile_ops(circ)
-    prog = desv.Program(n).add_ops(ops).compile("complex64")
+    prog = desv.Program(n).add_ops(ops)
+    tuned = desv.tune.autotune_program(prog, "complex64", reps=5, cache=False)["best"]
     prog.run(s64.handle)
     dt, _ = _timed(lambda: (s64.handle.reset(), prog.run(s64.handle)), reps=20)
     s128 = desv.DenseSVSimulator(n)
@@ -148,7 +149,7 @@ def suite_plain20():
              for q in range(n))
     return {"workload": "20q random rx/ry/rz + cx ladder, 20 layers, 1580 gates, c64",
             "gates_per_s": len(ops) / dt, "ms_per_circuit": dt * 1e3, "hbm_passes": prog.stats()["passes"],
-            "fidelity_vs_c128": fid, "max_abs_dZ_vs_c128": dz}
+            "autotuned": tuned, "fidelity_vs_c128": fid, "max_abs_dZ_vs_c128": dz}
 
 
 def suite_vqe20():
@@ -230,11 +231,18 @@ def suite_precision(n=28, steps=200):
     import desv
     from desv import workloads
     circ = workloads.xxz_trotter_circuit(n, 0.05, steps, seed=0)
-    res = {}
+    # the launch shape is tuned on the first 20 steps (same preparation, same
+    # per-step structure), then the whole evolution runs with it
+    head = workloads.xxz_trotter_circuit(n, 0.05, min(steps, 20), seed=0)
+    res, tuned = {}, {}
     for dt_name, f32 in (("c64", True), ("c128", False)):
+        dtype = "complex64" if f32 else "complex128"
         sim = desv.DenseSVSimulator(n, use_float32=f32)
         ops = sim.compile_ops(circ)
-        prog = desv.Program(n).add_ops(ops).compile("complex64" if f32 else "complex128")
+        probe = desv.Program(n).add_ops(sim.compile_ops(head))
+        tuned[dt_name] = desv.tune.autotune_program(probe, dtype, reps=1, cache=False)["best"]
+        probe.close()
+        prog = desv.Program(n).add_ops(ops).compile(dtype, **tuned[dt_name])
         t, _ = _timed(lambda: prog.run(sim.handle), reps=1)
         zz = desv.pauli_sum_expectation(sim.sv, [(1.0, {i: 'Z', i + 1: 'Z'}) for i in range(n - 1)])
         zzs = [desv.pauli_expectation(sim.sv, {i: 'Z', i + 1: 'Z'}) for i in range(n - 1)]
@@ -243,7 +251,7 @@ def suite_precision(n=28, steps=200):
     (t64, n64, z64, st64, g), (t128, n128, z128, st128, _) = res["c64"], res["c128"]
     return {"workload": f"{n}q XXZ Trotter, {steps} steps, {g} gates",
             "gates_per_s_c64": g / t64, "gates_per_s_c128": g / t128,
-            "hbm_passes_c64": st64["passes"],
+            "hbm_passes_c64": st64["passes"], "autotuned": tuned,
             "norm_c64_minus_1": n64 - 1.0, "norm_c128_minus_1": n128 - 1.0,
             "max_abs_dZZ_c64_vs_c128": float(np.max(np.abs(z64 - z128)))}
 
