@@ -605,6 +605,69 @@ __device__ inline void op_diag(char *lds, int vstride, int m, const DevOp &op, M
     }
 }
 
+// Diagonal op with K = 1, 2 matrix bits: factors and slots of the entries
+// present in this tile (those whose out-of-tile bits match `base`) are held in
+// registers, and two groups are loaded before either is stored.
+template <typename T, int NV, int K, typename MP>
+__device__ inline void op_diag_lo(char *lds, int vstride, int m, const DevOp &op, MP Dg, uint64_t base) {
+    using C = typename VT<T>::c;
+    constexpr int D = 1 << K;
+    constexpr int EB = sizeof(C);
+    const int cl = (int)op.cloc;
+    int gsub = 0, gmask = 0, lmask = 0;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        const int pj = op.pos[j];
+        if (pj < 0) {
+            gmask |= 1 << j;
+            if (base >> (-1 - pj) & 1) gsub |= 1 << j;
+        } else {
+            lmask |= 1 << pj;
+        }
+    }
+    const Groups g(m, lmask | cl, EB);
+    if (g.idle()) return;
+    bool on[D];
+    int so[D];
+    C f[D];
+#pragma unroll
+    for (int s = 0; s < D; ++s) {
+        on[s] = (s & gmask) == gsub;
+        int o = cl;
+#pragma unroll
+        for (int j = 0; j < K; ++j)
+            if (op.pos[j] >= 0 && (s >> j & 1)) o |= 1 << op.pos[j];
+        so[s] = slot(o) * EB;
+        f[s] = gload(Dg + s);
+    }
+    for (int v = 0; v < NV; ++v) {  // not unrolled: one vector's registers at a time
+        char *tb = lds + v * vstride;
+        int j = 0;
+        for (; j + 2 <= g.J; j += 2) {
+            const int A0 = g.P ^ g.U(j, EB), A1 = g.P ^ g.U(j + 1, EB);
+            C a0[D], a1[D];
+#pragma unroll
+            for (int s = 0; s < D; ++s)
+                if (on[s]) {
+                    a0[s] = lds_ld<C>(tb, A0 ^ so[s]);
+                    a1[s] = lds_ld<C>(tb, A1 ^ so[s]);
+                }
+#pragma unroll
+            for (int s = 0; s < D; ++s)
+                if (on[s]) {
+                    lds_st<C>(tb, A0 ^ so[s], cmul(f[s], a0[s]));
+                    lds_st<C>(tb, A1 ^ so[s], cmul(f[s], a1[s]));
+                }
+        }
+        if (j < g.J) {
+            const int A0 = g.P ^ g.U(j, EB);
+#pragma unroll
+            for (int s = 0; s < D; ++s)
+                if (on[s]) lds_st<C>(tb, A0 ^ so[s], cmul(f[s], lds_ld<C>(tb, A0 ^ so[s])));
+        }
+    }
+}
+
 // Global phase on the whole tile (order of elements irrelevant).
 template <typename T, int NV, typename MP>
 __device__ inline void op_scalar(char *lds, int vstride, int m, MP S) {
@@ -797,7 +860,10 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(s
                     else if constexpr (VAR == 3) op_mfma<NV, 5>(lds, vstride, m, op, M);
                 }
                 break;
-            case OPK_DIAG: op_diag<T, NV>(lds, vstride, m, op, M, base); break;
+            case OPK_DIAG:
+                if (op.k == 1) op_diag_lo<T, NV, 1>(lds, vstride, m, op, M, base);
+                else op_diag<T, NV>(lds, vstride, m, op, M, base);
+                break;
             case OPK_SCALAR: op_scalar<T, NV>(lds, vstride, m, M); break;
             case OPK_GRAD:
             case OPK_GRADDIAG:
