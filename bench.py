@@ -144,7 +144,8 @@ def suite_plain20():
     s128 = desv.DenseSVSimulator(n)
     s128.run_circuit(circ)
     a, b = s64.get_statevector(), s128.get_statevector()
-    fid = float(abs(np.vdot(a.astype(np.complex128), b)) ** 2)
+    a = a.astype(np.complex128)
+    fid = float(abs(np.vdot(a, b)) ** 2 / (np.vdot(a, a).real * np.vdot(b, b).real))
     dz = max(abs(desv.pauli_expectation(s64.sv, {q: 'Z'}) - desv.pauli_expectation(s128.sv, {q: 'Z'}))
              for q in range(n))
     return {"workload": "20q random rx/ry/rz + cx ladder, 20 layers, 1580 gates, c64",
