@@ -659,6 +659,30 @@ bool build_plan(const std::vector<Gate> &gates, int n, const CompileOpts &o, Pla
             return false;
         }
         if (m < n && K > 1) defer_trailing_singles(gates, take, defer);
+        // Last compute stage: grow the tile from the bits its gates need and
+        // the low bits' occupants by the occupant of each member's home bit
+        // (so that member can go home), then by bits already home; the
+        // write-back below then sends every bit it can home (the low bits
+        // are not held for later gates), leaving the restore passes less.
+        const bool endgame = defer.empty() && m < n && o.remap;
+        if (endgame) {
+            uint64_t T = 0;
+            for (int gi : take) T |= gates[gi].tile_mask();
+            for (int p = 0; p < L; ++p) T |= 1ull << inv[p];
+            if (popc(T) <= m) {
+                for (bool grew = true; grew && popc(T) < m;) {
+                    grew = false;
+                    for (int u = 0; u < n && popc(T) < m; ++u)
+                        if ((T >> u & 1) && pi[u] != u && !(T >> inv[u] & 1)) {
+                            T |= 1ull << inv[u];
+                            grew = true;
+                        }
+                }
+                for (int u = 0; u < n && popc(T) < m; ++u)
+                    if (pi[u] == u) T |= 1ull << u;
+                S = T;  // holds every bit the stage's gates need
+            }
+        }
         // fill the tile: bits the next deferred gates need, then the lowest physical bits
         for (size_t i = 0; i < defer.size() && popc(S) < m && i < 256; ++i) {
             const uint64_t need = gates[defer[i]].tile_mask() & ~S;
@@ -681,7 +705,7 @@ bool build_plan(const std::vector<Gate> &gates, int n, const CompileOpts &o, Pla
                         (int)want.size() < L)
                         want.push_back(u);
             }
-            for (int p = 0; p < L && (int)want.size() < L; ++p)
+            for (int p = 0; p < L && (int)want.size() < L && !endgame; ++p)
                 if (std::find(want.begin(), want.end(), inv[p]) == want.end()) want.push_back(inv[p]);
             uint64_t used = 0, placed = 0;
             for (int u : want)
@@ -703,7 +727,7 @@ bool build_plan(const std::vector<Gate> &gates, int n, const CompileOpts &o, Pla
                 for (int u = 0; u < n; ++u) {
                     if (!(S >> u & 1) || (placed >> u & 1)) continue;
                     int slot = -1;
-                    if (pass == 0 && (P >> u & 1) && !(used >> u & 1) && u >= L) slot = u;
+                    if (pass == 0 && (P >> u & 1) && !(used >> u & 1) && (u >= L || endgame)) slot = u;
                     if (pass == 1 && !(used >> pi[u] & 1)) slot = pi[u];
                     if (pass == 2)
                         for (int b = 0; b < n; ++b)
