@@ -176,6 +176,30 @@ def test_streaming_low_index_bits(n, dtype, rng):
     sim.close()
 
 
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 8, 9, 10])
+def test_streaming_small_widths_c64(n, rng):
+    """One streaming pass per gate on complex64 states of 1..10 qubits (grids
+    of a single partly used workgroup), in place and out of place: every
+    gate id on every qubit and qubit pair against the oracle."""
+    rows = [[g, q, 0, rng.uniform(-np.pi, np.pi)] for q in range(n) for g in range(1, 15)]
+    for a in range(n):
+        for b in range(n):
+            if a != b:
+                rows.append([int(rng.choice([20, 21, 22, 23, 24, 25])), a, b, rng.uniform(-np.pi, np.pi)])
+    ops = np.array(rows, dtype=np.float64)
+    sv0 = rand_state(n, rng)
+    prog = desv.Program(n).add_ops(ops).compile("complex64", fuse_bits=1, per_gate=True)
+    want = O.run_ops_fast(sv0, ops)
+    for mode in (True, False):
+        h = desv.StateHandle(n, False)
+        h.set_double_buffer(mode)
+        h.upload(sv0)
+        prog.run(h)
+        got = h.download()
+        h.close()
+        assert np.max(np.abs(got - want)) < 5e-6, mode
+
+
 @pytest.mark.parametrize("k", [1, 2, 3, 4, 5])
 @pytest.mark.parametrize("use_float32", [False, True])
 def test_apply_arbitrary_unitaries(k, use_float32, rng):
