@@ -1214,18 +1214,21 @@ struct DirectShape {
     int gpt;
     bool nt, contig;
 };
-// Complex64 16-byte passes, by the lowest target bit lo (30-qubit brickwork,
-// ms per pass, strided GPT 1 / contig GPT 4 / contig GPT 4 + nt):
-//   lo = 1: 3.74 / 3.68 / 11.75    lo = 2: 3.43 / 3.23 / 8.44
-//   lo = 4..8: 2.93 / 2.98 / 2.96  lo >= 9: 2.67..3.60, nt + contig 2-6 % faster.
-// Non-temporal stores need whole 128-byte lines per store instruction
-// (lo >= 4) and pay off once a wave's 4 KB runs stay inside one target
-// half (lo >= 9).
-DirectShape direct_shape(bool vec16, int lo) {
+// By the lowest target bit lo, ms per pass of a 2-qubit brickwork gate
+// (strided GPT 1 / contig GPT 4 / contig GPT 4 + nt):
+//   complex64, 2 amplitudes per 16-byte access, 30 qubits:
+//     lo = 1: 3.74 / 3.68 / 11.75    lo = 2: 3.43 / 3.23 / 8.44
+//     lo = 4..8: 2.93 / 2.98 / 2.96  lo >= 9: 2.67..3.60, nt + contig 2-6 % faster
+//   complex128, 1 amplitude per access, 29 qubits:
+//     lo = 0: 4.01 / 3.76 / 10.66    lo = 1: 3.71 / 3.37 / 7.14
+//     lo = 3..9: 2.95..2.97 / 2.90..3.00 / 2.76..2.81; lo >= 10: nt + contig mostly best.
+// Non-temporal stores need whole 128-byte lines per store instruction.
+DirectShape direct_shape(bool dbl, bool vec16, int lo) {
     static const int g = env_int("DESV_DIRECT_GPT", 0), c = env_int("DESV_DIRECT_CONTIG", -1),
                      t = env_int("DESV_DIRECT_NT", -1);
     DirectShape d{1, false, false};
-    if (vec16 && lo >= 9) d = {4, true, true};
+    if (dbl) d = lo >= 3 ? DirectShape{4, true, true} : DirectShape{4, false, true};
+    else if (vec16 && lo >= 9) d = {4, true, true};
     else if (vec16 && lo <= 2) d = {4, false, true};
     if (g == 1 || g == 2 || g == 4) d.gpt = g;
     if (c >= 0) d.contig = c != 0;
@@ -1454,7 +1457,7 @@ hipError_t launch_direct_k(const typename VT<T>::c *src, typename VT<T>::c *dst,
                            const typename VT<T>::c *m, hipStream_t s) {
     int lo = 64;
     for (int j = 0; j < K; ++j) lo = std::min(lo, (int)op.pos[j]);
-    const DirectShape shape = direct_shape(VEC == 2, lo);
+    const DirectShape shape = direct_shape(!std::is_same<T, float>::value, VEC == 2, lo);
     const int gpt = shape.gpt;
     const bool nt = shape.nt;
     const int contig = shape.contig;
