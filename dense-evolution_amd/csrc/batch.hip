@@ -271,9 +271,8 @@ int build(desv_batch b) {
         o[i].fops = off; off = align256(off + in.fp.ops.size() * sizeof(DevOp));
         o[i].rops = off; off = align256(off + in.rp.ops.size() * sizeof(DevOp));
         o[i].gpart = off; off = align256(off + (size_t)std::max(1, in.rp.n_grad_slots) * in.pstride * 8);
-        // one amplitude per lane: many workgroups keep the gathers of lambda = H psi
-        // in flight (the kernel is latency-bound, not bandwidth-bound)
-        in.mv_chunk = std::min<int64_t>(1ll << in.n, kThreads);
+        // lambda = H psi: one aligned LDS tile of amplitudes per workgroup
+        in.mv_chunk = 1ll << std::min(in.n, ab == 8 ? kMatvecTileBitsC64 : kMatvecTileBitsC128);
         in.mv_wg = ((1ll << in.n) + in.mv_chunk - 1) / in.mv_chunk;
         o[i].epart = off; off = align256(off + in.mv_wg * 8);
         // Pauli groups (sorted by flip mask), i^#Y folded into the coefficient

@@ -950,64 +950,92 @@ __device__ inline int find_task(const Task *tasks, int ntasks) {
     return lo;
 }
 
+// lambda = H psi for one aligned tile of 2^tb amplitudes per workgroup
+// (tb = log2 chunk: 11, fewer for small states).  Flip-mask groups arrive sorted by mask, so the groups sharing the
+// mask's bits above the tile are consecutive: their gathers psi[i ^ x] all
+// fall in one aligned input tile, loaded into LDS once per such run instead
+// of once per group.  The sign (-1)^popc((i ^ x) & z) splits into the part
+// above the tile (uniform, folded into the coefficient) and the part inside.
+// FP64 accumulation; the energy partial <psi|lambda> and the psi copy are
+// formed at the end.
 template <typename T>
 __global__ void __launch_bounds__(kThreads) k_matvec_tasks(const MatvecTask *__restrict__ tasks, int ntasks) {
     using C = typename VT<T>::c;
+    constexpr int TB = std::is_same<T, float>::value ? kMatvecTileBitsC64 : kMatvecTileBitsC128;
+    constexpr int E = (1 << TB) / kThreads;  // amplitudes per lane
+    __shared__ double2 tile[1 << TB];  // converted once per load
     __shared__ double red[kThreads / 64];
     const MatvecTask tk = tasks[find_task(tasks, ntasks)];
     const int64_t w = (int64_t)blockIdx.x - tk.wg_begin;
-    const int64_t dim = 1ll << tk.n;
-    const int64_t i0 = w * tk.chunk;
-    const int64_t i1 = i0 + tk.chunk < dim ? i0 + tk.chunk : dim;
+    const int64_t tsize = tk.chunk;  // power of two <= 2^TB
+    const uint64_t lomask = (uint64_t)tsize - 1;
+    const uint64_t i0 = (uint64_t)w * (uint64_t)tsize;
     const C *psi = reinterpret_cast<const C *>(tk.psi);
     C *lam = reinterpret_cast<C *>(tk.lam);
     C *psi2 = reinterpret_cast<C *>(tk.psi2);
-    // term tables through wave-uniform global pointers (scalar loads)
     const DESV_GLOBAL MatvecGroup *groups = gptr(reinterpret_cast<const MatvecGroup *>(tk.groups));
     const DESV_GLOBAL uint64_t *zt = gptr(tk.z);
     const DESV_GLOBAL double2 *ct = gptr(tk.cc);
     const int ng = uni(tk.ngroups);
-    constexpr int UG = 4;  // flip-mask groups whose gathers are in flight together
-    double e = 0;
-    for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-        double2 acc = make_double2(0.0, 0.0);
-        for (int g0 = 0; g0 < ng; g0 += UG) {
-            double2 a[UG];
-            uint64_t sidx[UG];
+    const int t = threadIdx.x;
+    double2 acc[E];
 #pragma unroll
-            for (int u = 0; u < UG; ++u) {
-                a[u] = make_double2(0.0, 0.0);
-                sidx[u] = 0;
-                if (g0 + u < ng) {
-                    sidx[u] = (uint64_t)i ^ groups[g0 + u].x;
-                    a[u] = to_d(psi[sidx[u]]);
-                }
+    for (int e = 0; e < E; ++e) acc[e] = make_double2(0.0, 0.0);
+    uint64_t cur = ~0ull;
+    for (int g = 0; g < ng; ++g) {
+        const uint64_t x = uni64(groups[g].x);
+        const uint64_t xh = x & ~lomask;
+        const uint32_t xl = (uint32_t)(x & lomask);
+        if (xh != cur) {
+            __syncthreads();  // previous tile fully read
+            const C *src = psi + (i0 ^ xh);
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const int j = t + e * kThreads;
+                if (j < tsize) tile[j] = to_d(src[j]);
             }
+            __syncthreads();
+            cur = xh;
+        }
+        double2 a[E];
 #pragma unroll
-            for (int u = 0; u < UG; ++u) {
-                if (g0 + u >= ng) break;
-                const int qb = groups[g0 + u].begin, qe = groups[g0 + u].end;
-                double fr = 0, fi = 0;
-                for (int q = qb; q < qe; ++q) {
-                    const double2 c = gload(ct + q);
-                    const double sg = (__popcll(sidx[u] & zt[q]) & 1) ? -1.0 : 1.0;
-                    fr = __builtin_fma(sg, c.x, fr);
-                    fi = __builtin_fma(sg, c.y, fi);
-                }
-                acc.x += fr * a[u].x - fi * a[u].y;
-                acc.y += fr * a[u].y + fi * a[u].x;
+        for (int e = 0; e < E; ++e) {
+            const int j = t + e * kThreads;
+            a[e] = j < tsize ? tile[j ^ xl] : make_double2(0.0, 0.0);
+        }
+        const uint64_t hi = i0 ^ xh;  // bits of i ^ x above the tile
+        const int qb = uni(groups[g].begin), qe = uni(groups[g].end);
+        for (int q = qb; q < qe; ++q) {
+            const uint64_t z = uni64(zt[q]);
+            double2 c = gload(ct + q);
+            if (__popcll(hi & z) & 1) c = make_double2(-c.x, -c.y);
+            const uint32_t zl = (uint32_t)(z & lomask);
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const uint32_t jl = (uint32_t)(t + e * kThreads) ^ xl;
+                const double sg = (__popc(jl & zl) & 1) ? -1.0 : 1.0;
+                const double2 ad = a[e];
+                const double pr = c.x * ad.x - c.y * ad.y, pi = c.x * ad.y + c.y * ad.x;
+                acc[e].x = __builtin_fma(sg, pr, acc[e].x);
+                acc[e].y = __builtin_fma(sg, pi, acc[e].y);
             }
         }
-        const C p = psi[i];
-        const double2 pd = to_d(p);
-        e += pd.x * acc.x + pd.y * acc.y;
-        C r;
-        r.x = (T)acc.x;
-        r.y = (T)acc.y;
-        lam[i] = r;
-        if (psi2) psi2[i] = p;
     }
-    const double sum = block_sum(e, red);
+    double en = 0;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int j = t + e * kThreads;
+        if (j >= tsize) continue;
+        const C p = psi[i0 + j];
+        const double2 pd = to_d(p);
+        en += pd.x * acc[e].x + pd.y * acc[e].y;
+        C r;
+        r.x = (T)acc[e].x;
+        r.y = (T)acc[e].y;
+        lam[i0 + j] = r;
+        if (psi2) psi2[i0 + j] = p;
+    }
+    const double sum = block_sum(en, red);
     if (threadIdx.x == 0) tk.epart[w] = sum;
 }
 

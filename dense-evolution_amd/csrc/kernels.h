@@ -46,6 +46,10 @@ struct StageTask {
 
 // lambda = H psi, psi2 = psi and FP64 partials of Re<psi|lambda>, for a chunk of
 // one instance's amplitudes per workgroup.
+// Amplitudes per workgroup of the batched Pauli matvec (one aligned tile in
+// LDS, 8 per lane): 2^11, or the whole state if smaller.
+constexpr int kMatvecTileBitsC64 = 11, kMatvecTileBitsC128 = 11;
+
 struct MatvecTask {
     int32_t n;
     int32_t ngroups;
@@ -57,7 +61,7 @@ struct MatvecTask {
     const double2 *cc;
     double *epart;       // one double per workgroup of this task
     int64_t wg_begin;
-    int64_t chunk;       // amplitudes per workgroup
+    int64_t chunk;       // amplitudes per workgroup: 2^min(n, kMatvecTileBits*)
 };
 
 // dst[0..dim) = src, or |0...0> when src is null.
