@@ -272,7 +272,7 @@ int build(desv_batch b) {
         o[i].rops = off; off = align256(off + in.rp.ops.size() * sizeof(DevOp));
         o[i].gpart = off; off = align256(off + (size_t)std::max(1, in.rp.n_grad_slots) * in.pstride * 8);
         // lambda = H psi: one aligned LDS tile of amplitudes per workgroup
-        in.mv_chunk = 1ll << std::min(in.n, ab == 8 ? kMatvecTileBitsC64 : kMatvecTileBitsC128);
+        in.mv_chunk = 1ll << matvec_tile_bits(in.n);
         in.mv_wg = ((1ll << in.n) + in.mv_chunk - 1) / in.mv_chunk;
         o[i].epart = off; off = align256(off + in.mv_wg * 8);
         // Pauli groups (sorted by flip mask), i^#Y folded into the coefficient

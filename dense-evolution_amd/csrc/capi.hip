@@ -326,24 +326,33 @@ int pauli_matvec_raw(desv_handle h, void *out, const void *in, const uint64_t *x
         else groups.back().end = (int32_t)r + 1;
     }
     const size_t gb = groups.size() * sizeof(MatvecGroup), zb = T * sizeof(uint64_t), cb = T * sizeof(double2);
-    const size_t ga = (gb + 255) & ~size_t(255), za = (zb + 255) & ~size_t(255);
-    int rc = ensure_small(h, ga + za + cb + 256);
+    const size_t ga = (gb + 255) & ~size_t(255), za = (zb + 255) & ~size_t(255), ca = (cb + 255) & ~size_t(255);
+    int rc = ensure_small(h, ga + za + ca + sizeof(MatvecTask) + 256);
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(h->stream));
-    std::vector<unsigned char> host(ga + za + cb);
+    unsigned char *base = reinterpret_cast<unsigned char *>(h->d_small);
+    // one task of the tiled matvec kernel (no energy partials, no psi copy)
+    MatvecTask tk;
+    std::memset(&tk, 0, sizeof(tk));
+    tk.n = h->n;
+    tk.ngroups = (int32_t)groups.size();
+    tk.psi = in;
+    tk.lam = out;
+    tk.groups = base;
+    tk.z = reinterpret_cast<const uint64_t *>(base + ga);
+    tk.cc = reinterpret_cast<const double2 *>(base + ga + za);
+    tk.chunk = 1ll << matvec_tile_bits(h->n);
+    const int64_t grid = (1ll << h->n) / tk.chunk;
+    std::vector<unsigned char> host(ga + za + ca + sizeof(MatvecTask));
     std::memcpy(host.data(), groups.data(), gb);
     std::memcpy(host.data() + ga, z.data(), zb);
     std::memcpy(host.data() + ga + za, cc.data(), cb);
+    std::memcpy(host.data() + ga + za + ca, &tk, sizeof(tk));
     HIP_TRY(hipMemcpyAsync(h->d_small, host.data(), host.size(), hipMemcpyHostToDevice, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
-    unsigned char *base = reinterpret_cast<unsigned char *>(h->d_small);
-    hipError_t e = h->dtype == DESV_C64
-                       ? launch_pauli_matvec<float>(out, in, h->n, (const MatvecGroup *)base, (int)groups.size(),
-                                                    (const uint64_t *)(base + ga), (const double2 *)(base + ga + za),
-                                                    h->stream)
-                       : launch_pauli_matvec<double>(out, in, h->n, (const MatvecGroup *)base, (int)groups.size(),
-                                                     (const uint64_t *)(base + ga), (const double2 *)(base + ga + za),
-                                                     h->stream);
+    const MatvecTask *dtk = reinterpret_cast<const MatvecTask *>(base + ga + za + ca);
+    hipError_t e = h->dtype == DESV_C64 ? launch_matvec_tasks<float>(dtk, 1, grid, h->stream)
+                                        : launch_matvec_tasks<double>(dtk, 1, grid, h->stream);
     if (e != hipSuccess) return fail(DESV_EDEVICE, hipGetErrorString(e));
     return DESV_OK;
 }

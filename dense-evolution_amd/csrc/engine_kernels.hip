@@ -1021,22 +1021,28 @@ __global__ void __launch_bounds__(kThreads) k_matvec_tasks(const MatvecTask *__r
             }
         }
     }
+    // the energy partial and the psi copy only when asked for (uniform)
+    const bool full = tk.epart != nullptr || psi2 != nullptr;
     double en = 0;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const int j = t + e * kThreads;
         if (j >= tsize) continue;
-        const C p = psi[i0 + j];
-        const double2 pd = to_d(p);
-        en += pd.x * acc[e].x + pd.y * acc[e].y;
         C r;
         r.x = (T)acc[e].x;
         r.y = (T)acc[e].y;
         lam[i0 + j] = r;
-        if (psi2) psi2[i0 + j] = p;
+        if (full) {
+            const C p = psi[i0 + j];
+            const double2 pd = to_d(p);
+            en += pd.x * acc[e].x + pd.y * acc[e].y;
+            if (psi2) psi2[i0 + j] = p;
+        }
     }
-    const double sum = block_sum(en, red);
-    if (threadIdx.x == 0) tk.epart[w] = sum;
+    if (tk.epart) {
+        const double sum = block_sum(en, red);
+        if (threadIdx.x == 0) tk.epart[w] = sum;
+    }
 }
 
 template <typename T>
@@ -1960,39 +1966,6 @@ __global__ void __launch_bounds__(kThreads) k_pauli_expect(const typename VT<T>:
 
 // out_i = sum_groups (sum_{t in g} cc_t (-1)^popcount((i^x)&z_t)) in_{i^x}, FP64 accumulation.
 template <typename T>
-__global__ void __launch_bounds__(kThreads) k_pauli_matvec(typename VT<T>::c *out, const typename VT<T>::c *in, int n,
-                                                           const MatvecGroup *groups, int n_groups,
-                                                           const uint64_t *zmask, const double2 *cc) {
-    using C = typename VT<T>::c;
-    const int64_t dim = 1ll << n;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < dim; i += (int64_t)gridDim.x * blockDim.x) {
-        double2 acc = make_double2(0.0, 0.0);
-        for (int g = 0; g < n_groups; ++g) {
-            const MatvecGroup gr = groups[g];
-            const uint64_t s = (uint64_t)i ^ gr.x;
-            double fr = 0, fi = 0;
-            for (int t = gr.begin; t < gr.end; ++t) {
-                const double2 c = cc[t];
-                if (__popcll(s & zmask[t]) & 1) {
-                    fr -= c.x;
-                    fi -= c.y;
-                } else {
-                    fr += c.x;
-                    fi += c.y;
-                }
-            }
-            const double2 a = to_d(in[s]);
-            acc.x += fr * a.x - fi * a.y;
-            acc.y += fr * a.y + fi * a.x;
-        }
-        C r;
-        r.x = (T)acc.x;
-        r.y = (T)acc.y;
-        out[i] = r;
-    }
-}
-
-template <typename T>
 __global__ void k_dense_matvec(typename VT<T>::c *out, const typename VT<T>::c *in, int n, const double2 *H) {
     using C = typename VT<T>::c;
     const int64_t dim = 1ll << n;
@@ -2012,13 +1985,6 @@ hipError_t launch_pauli_expect(const void *sv, int n, const PauliItem *items, in
                                double *partials, int nblk, hipStream_t s) {
     hipLaunchKernelGGL((k_pauli_expect<T>), dim3(nblk, n_items), dim3(kThreads), 0, s,
                        (const typename VT<T>::c *)sv, n, items, zmask, partials, nblk);
-    return hipGetLastError();
-}
-template <typename T>
-hipError_t launch_pauli_matvec(void *out, const void *in, int n, const MatvecGroup *groups, int n_groups,
-                               const uint64_t *zmask, const double2 *cc, hipStream_t s) {
-    hipLaunchKernelGGL((k_pauli_matvec<T>), dim3(ew_grid(1ll << n)), dim3(kThreads), 0, s,
-                       (typename VT<T>::c *)out, (const typename VT<T>::c *)in, n, groups, n_groups, zmask, cc);
     return hipGetLastError();
 }
 template <typename T>
@@ -2160,8 +2126,6 @@ hipError_t launch_fill_random(void *sv, int64_t dim, uint64_t seed, hipStream_t 
     template hipError_t launch_collapse<T>(void *, int64_t, int, int, double, hipStream_t);                   \
     template hipError_t launch_pauli_expect<T>(const void *, int, const PauliItem *, int, const uint64_t *,   \
                                                double *, int, hipStream_t);                                   \
-    template hipError_t launch_pauli_matvec<T>(void *, const void *, int, const MatvecGroup *, int,           \
-                                               const uint64_t *, const double2 *, hipStream_t);               \
     template hipError_t launch_dense_matvec<T>(void *, const void *, int, const double2 *, hipStream_t);        \
     template hipError_t launch_inner_many<T>(const void *, const BasisPtrs &, int, int64_t, double *, int,     \
                                              hipStream_t);                                                    \

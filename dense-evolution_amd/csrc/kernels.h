@@ -6,6 +6,7 @@
 // reproducible run to run.
 #pragma once
 
+#include <cstdlib>
 #include <hip/hip_runtime.h>
 
 #include "desv_internal.h"
@@ -46,9 +47,19 @@ struct StageTask {
 
 // lambda = H psi, psi2 = psi and FP64 partials of Re<psi|lambda>, for a chunk of
 // one instance's amplitudes per workgroup.
-// Amplitudes per workgroup of the batched Pauli matvec (one aligned tile in
-// LDS, 8 per lane): 2^11, or the whole state if smaller.
+// Amplitudes per workgroup of the tiled Pauli matvec (one aligned tile in
+// LDS, at most 8 per lane): 2^11 at most.
 constexpr int kMatvecTileBitsC64 = 11, kMatvecTileBitsC128 = 11;
+// log2 amplitudes per workgroup for an n-qubit state (DESV_MATVEC_TILE_BITS
+// caps it below 11).
+inline int matvec_tile_bits(int n) {
+    static const int cap = [] {
+        const char *e = std::getenv("DESV_MATVEC_TILE_BITS");
+        const int v = e ? std::atoi(e) : kMatvecTileBitsC64;
+        return v >= 1 && v <= kMatvecTileBitsC64 ? v : kMatvecTileBitsC64;
+    }();
+    return n < cap ? n : cap;
+}
 
 struct MatvecTask {
     int32_t n;
@@ -150,9 +161,6 @@ hipError_t launch_collapse(void *sv, int64_t dim, int bit, int outcome, double s
 template <typename T>
 hipError_t launch_pauli_expect(const void *sv, int n, const PauliItem *items, int n_items,
                                const uint64_t *zmask, double *partials, int nblk, hipStream_t s);
-template <typename T>
-hipError_t launch_pauli_matvec(void *out, const void *in, int n, const MatvecGroup *groups, int n_groups,
-                               const uint64_t *zmask, const double2 *cc, hipStream_t s);
 template <typename T>
 hipError_t launch_dense_matvec(void *out, const void *in, int n, const double2 *H, hipStream_t s);
 // Krylov bases: up to kMaxBasis state pointers per launch.
