@@ -683,9 +683,62 @@ __device__ inline void op_scalar(char *lds, int vstride, int m, MP S) {
 }
 
 // Re <lam| M |psi> over the tile (tile 0 = psi, tile 1 = lam); M is a dense
-// 2x2 generator on tile bit pos[0] with local controls.
+// 2x2 generator on tile bit pos[0] with local controls.  Complex64: each
+// pair's product and inner product in FP32 (gate precision), the sum over the
+// tile in FP64, two groups in flight.
+template <typename MP>
+__device__ inline double op_grad_dense_f32(char *lds, int vstride, int m, const DevOp &op, MP M) {
+    constexpr int EB = sizeof(float2);
+    const float2 m00 = gload(M), m01 = gload(M + 1), m10 = gload(M + 2), m11 = gload(M + 3);
+    const int cl = (int)op.cloc, tb = 1 << op.pos[0];
+    const int o0 = slot(cl) * EB, o1 = slot(cl | tb) * EB;
+    const Groups g(m, cl | tb, EB);
+    if (g.idle()) return 0.0;
+    const char *ls = lds + vstride;
+    auto pair = [&](int A) {
+        const float2 s0 = lds_ld<float2>(lds, A ^ o0), s1 = lds_ld<float2>(lds, A ^ o1);
+        const float2 l0 = lds_ld<float2>(ls, A ^ o0), l1 = lds_ld<float2>(ls, A ^ o1);
+        const float2 g0 = cfma(m01, s1, cmul(m00, s0));
+        const float2 g1 = cfma(m11, s1, cmul(m10, s0));
+        return __builtin_fmaf(l1.y, g1.y, __builtin_fmaf(l1.x, g1.x, __builtin_fmaf(l0.y, g0.y, l0.x * g0.x)));
+    };
+    double acc = 0;
+    int j = 0;
+    for (; j + 2 <= g.J; j += 2) {
+        const float a0 = pair(g.P ^ g.U(j, EB)), a1 = pair(g.P ^ g.U(j + 1, EB));
+        acc += (double)a0 + (double)a1;
+    }
+    if (j < g.J) acc += (double)pair(g.P ^ g.U(j, EB));
+    return acc;
+}
+
+template <typename MP>
+__device__ inline double op_grad_diag_f32(char *lds, int vstride, int m, const DevOp &op, MP Dg) {
+    constexpr int EB = sizeof(float2);
+    const float2 d0 = gload(Dg), d1 = gload(Dg + 1);
+    const int cl = (int)op.cloc, tb = 1 << op.pos[0];
+    const int o0 = slot(cl) * EB, o1 = slot(cl | tb) * EB;
+    const Groups g(m, cl | tb, EB);
+    if (g.idle()) return 0.0;
+    const char *ls = lds + vstride;
+    auto pair = [&](int A) {
+        const float2 g0 = cmul(d0, lds_ld<float2>(lds, A ^ o0)), g1 = cmul(d1, lds_ld<float2>(lds, A ^ o1));
+        const float2 l0 = lds_ld<float2>(ls, A ^ o0), l1 = lds_ld<float2>(ls, A ^ o1);
+        return __builtin_fmaf(l1.y, g1.y, __builtin_fmaf(l1.x, g1.x, __builtin_fmaf(l0.y, g0.y, l0.x * g0.x)));
+    };
+    double acc = 0;
+    int j = 0;
+    for (; j + 2 <= g.J; j += 2) {
+        const float a0 = pair(g.P ^ g.U(j, EB)), a1 = pair(g.P ^ g.U(j + 1, EB));
+        acc += (double)a0 + (double)a1;
+    }
+    if (j < g.J) acc += (double)pair(g.P ^ g.U(j, EB));
+    return acc;
+}
+
 template <typename T, typename MP>
 __device__ inline double op_grad_dense(char *lds, int vstride, int m, const DevOp &op, MP M) {
+    if constexpr (std::is_same<T, float>::value) return op_grad_dense_f32(lds, vstride, m, op, M);
     using C = typename VT<T>::c;
     constexpr int EB = sizeof(C);
     const double2 m00 = to_d(gload(M)), m01 = to_d(gload(M + 1)), m10 = to_d(gload(M + 2)),
@@ -710,6 +763,7 @@ __device__ inline double op_grad_dense(char *lds, int vstride, int m, const DevO
 // Same for a diagonal generator diag(d0, d1) on tile bit pos[0].
 template <typename T, typename MP>
 __device__ inline double op_grad_diag(char *lds, int vstride, int m, const DevOp &op, MP Dg) {
+    if constexpr (std::is_same<T, float>::value) return op_grad_diag_f32(lds, vstride, m, op, Dg);
     using C = typename VT<T>::c;
     constexpr int EB = sizeof(C);
     const double2 d0 = to_d(gload(Dg)), d1 = to_d(gload(Dg + 1));
