@@ -783,6 +783,93 @@ __device__ inline double op_grad_diag(char *lds, int vstride, int m, const DevOp
     return acc;
 }
 
+// A gradient probe on tile bit p (generator G, dense 2x2 or diagonal)
+// followed by a block of KB = 1, 2 bits (dense or diagonal, BD) that contains
+// p, in one sweep: each group of the block is loaded once from both tiles,
+// the probe's pairs along p are read off the loaded values, then the block is
+// applied to both tiles.  Probe products in the state precision, the tile sum
+// in FP64.
+template <typename T, int KB, bool BD, typename MP>
+__device__ inline double op_grad_block_k(char *lds, int vstride, int m, const DevOp &gop, MP G, const DevOp &bop,
+                                         MP B) {
+    using C = typename VT<T>::c;
+    using R = T;
+    constexpr int D = 1 << KB;
+    constexpr int EB = sizeof(C);
+    const bool gdiag = gop.kind == OPK_GRADDIAG;
+    const C g00 = gload(G), g11 = gdiag ? gload(G + 1) : gload(G + 3);
+    const C g01 = gdiag ? make_zero<C>() : gload(G + 1), g10 = gdiag ? make_zero<C>() : gload(G + 2);
+    const bool jp1 = KB == 2 && bop.pos[1] == gop.pos[0];  // probe on matrix bit 1
+    int fixed = 0, so[D];
+#pragma unroll
+    for (int j = 0; j < KB; ++j) fixed |= 1 << bop.pos[j];
+#pragma unroll
+    for (int s = 0; s < D; ++s) {
+        int o = 0;
+#pragma unroll
+        for (int j = 0; j < KB; ++j)
+            if (s >> j & 1) o |= 1 << bop.pos[j];
+        so[s] = slot(o) * EB;
+    }
+    const Groups g(m, fixed, EB);
+    if (g.idle()) return 0.0;
+    C mm[BD ? D : D * D];
+#pragma unroll
+    for (int i = 0; i < (BD ? D : D * D); ++i) mm[i] = gload(B + i);
+    // Re(conj(l0) (G s)_0 + conj(l1) (G s)_1) for the pair (0, 1) along p
+    auto probe = [&](C s0, C s1, C l0, C l1) -> R {
+        const C a0 = cfma(g01, s1, cmul(g00, s0)), a1 = cfma(g11, s1, cmul(g10, s0));
+        return l0.x * a0.x + l0.y * a0.y + l1.x * a1.x + l1.y * a1.y;
+    };
+    char *lt = lds + vstride;
+    double acc = 0;
+    for (int j = 0; j < g.J; ++j) {
+        const int A = g.P ^ g.U(j, EB);
+        C x[D], y[D];
+#pragma unroll
+        for (int s = 0; s < D; ++s) {
+            x[s] = lds_ld<C>(lds, A ^ so[s]);
+            y[s] = lds_ld<C>(lt, A ^ so[s]);
+        }
+        if constexpr (KB == 1) {
+            acc += (double)probe(x[0], x[1], y[0], y[1]);
+        } else if (jp1) {
+            acc += (double)probe(x[0], x[2], y[0], y[2]) + (double)probe(x[1], x[3], y[1], y[3]);
+        } else {
+            acc += (double)probe(x[0], x[1], y[0], y[1]) + (double)probe(x[2], x[3], y[2], y[3]);
+        }
+#pragma unroll
+        for (int r = 0; r < D; ++r) {
+            C ox, oy;
+            if constexpr (BD) {
+                ox = cmul(mm[r], x[r]);
+                oy = cmul(mm[r], y[r]);
+            } else {
+                ox = cmul(mm[r * D], x[0]);
+                oy = cmul(mm[r * D], y[0]);
+#pragma unroll
+                for (int c = 1; c < D; ++c) {
+                    ox = cfma(mm[r * D + c], x[c], ox);
+                    oy = cfma(mm[r * D + c], y[c], oy);
+                }
+            }
+            lds_st<C>(lds, A ^ so[r], ox);
+            lds_st<C>(lt, A ^ so[r], oy);
+        }
+    }
+    return acc;
+}
+
+template <typename T, typename MP>
+__device__ inline double op_grad_block(char *lds, int vstride, int m, const DevOp &gop, MP G, const DevOp &bop,
+                                       MP B) {
+    if (bop.kind == OPK_DIAG)
+        return bop.k == 1 ? op_grad_block_k<T, 1, true>(lds, vstride, m, gop, G, bop, B)
+                          : op_grad_block_k<T, 2, true>(lds, vstride, m, gop, G, bop, B);
+    return bop.k == 1 ? op_grad_block_k<T, 1, false>(lds, vstride, m, gop, G, bop, B)
+                      : op_grad_block_k<T, 2, false>(lds, vstride, m, gop, G, bop, B);
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -922,8 +1009,15 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(s
             case OPK_GRAD:
             case OPK_GRADDIAG:
                 if constexpr (NV == 2) {
-                    const double part = op.kind == OPK_GRAD ? op_grad_dense<T>(lds, vstride, m, op, M)
-                                                            : op_grad_diag<T>(lds, vstride, m, op, M);
+                    double part;
+                    if (op.pos[4] == 1) {  // fused with the next op (a block containing the probe's bit)
+                        const DevOp nx = gload_op(ops + o + 1);
+                        part = op_grad_block<T>(lds, vstride, m, op, M, nx, mats + nx.mat);
+                        ++o;
+                    } else {
+                        part = op.kind == OPK_GRAD ? op_grad_dense<T>(lds, vstride, m, op, M)
+                                                   : op_grad_diag<T>(lds, vstride, m, op, M);
+                    }
                     // one wave-level reduction; each wave owns its own slot
                     const double ws = wave_sum(part);
                     if ((threadIdx.x & 63) == 0) gacc[gi * (kThreads / 64) + (threadIdx.x >> 6)] = ws;

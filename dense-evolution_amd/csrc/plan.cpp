@@ -465,6 +465,29 @@ bool lower_stage(const std::vector<Gate> &ops, uint64_t tile, int n, const Compi
     }
     st.op_end = (int)plan.ops.size();
     st.n_grad = ngrad;
+    // Reverse sweeps: a gradient probe followed by a 1-/2-qubit dense or
+    // diagonal block that contains the probe's bit (the inverse of the
+    // probed gate, possibly fused with its neighbours) run as one sweep of
+    // the tile; pos[4] = 1 on the probe marks the pair.
+    if (o.nv == 2)
+        for (int i = st.op_begin; i + 1 < st.op_end; ++i) {
+            DevOp &gp = plan.ops[i];
+            const DevOp &nx = plan.ops[i + 1];
+            if ((gp.kind != OPK_GRAD && gp.kind != OPK_GRADDIAG) || gp.k != 1 || gp.cloc || gp.cglob ||
+                gp.pos[0] < 0)
+                continue;
+            if ((nx.kind != OPK_DENSE1 && nx.kind != OPK_DENSE2 && nx.kind != OPK_DIAG) || nx.k < 1 || nx.k > 2 ||
+                nx.cloc || nx.cglob)
+                continue;
+            bool local = true, has = false;
+            for (int j = 0; j < nx.k; ++j) {
+                local &= nx.pos[j] >= 0;
+                has |= nx.pos[j] == gp.pos[0];
+            }
+            if (!local || !has) continue;
+            gp.pos[4] = 1;
+            ++i;
+        }
     // kernel variant: 3 if any 5-qubit MFMA block (its fragments double the
     // register budget), 2 if any MFMA block (then no k >= 4 VALU blocks), 1
     // if any k >= 3 VALU block, else 0
@@ -550,6 +573,33 @@ void defer_trailing_singles(const std::vector<Gate> &gates, std::vector<int> &ta
     while (j < defer.size()) d2.push_back(defer[j++]);
     take.swap(t2);
     defer.swap(d2);
+}
+
+// Reverse sweeps apply every op to both psi and lambda, so a gradient probe
+// Re<lam|G_q|psi> commutes with ops on other bits: each uncontrolled 1-qubit
+// probe moves forward to just before the first later op that touches its
+// bit, when that op is an uncontrolled 1-/2-qubit dense or diagonal block (the
+// kernel then runs the two in one sweep).  A probe separated from that op by
+// probes only stays (two probes before one 2-qubit block would otherwise
+// displace each other forever); every move removes non-probe ops from between
+// a probe and its block, so the loop ends.  Recipes move with their ops.
+void move_probes_to_blocks(std::vector<Gate> &g, std::vector<Recipe> &rec) {
+    for (size_t i = 0; i < g.size(); ++i) {
+        if (g[i].kind != G_GRAD || g[i].cmask || g[i].k != 1) continue;
+        const uint64_t pb = 1ull << g[i].bits[0];
+        size_t j = i + 1;
+        bool only_probes = true;
+        while (j < g.size() && !(g[j].all_mask() & pb)) {
+            only_probes &= g[j].kind == G_GRAD;
+            ++j;
+        }
+        if (j == g.size() || only_probes) continue;
+        const Gate &b = g[j];
+        if (!((b.kind == G_DENSE || b.kind == G_DIAG) && b.k >= 1 && b.k <= 2 && b.cmask == 0)) continue;
+        std::rotate(g.begin() + i, g.begin() + i + 1, g.begin() + j);
+        std::rotate(rec.begin() + i, rec.begin() + i + 1, rec.begin() + j);
+        --i;  // position i now holds the next op
+    }
 }
 
 Gate to_physical(const Gate &g, const std::vector<int> &pi) {
@@ -752,6 +802,7 @@ bool build_plan(const std::vector<Gate> &gates, int n, const CompileOpts &o, Pla
             fused = lg;
             for (size_t i = 0; i < lg.size(); ++i) rec.push_back(Recipe{{(int)i}, 0});
         }
+        if (o.nv == 2) move_probes_to_blocks(fused, rec);
         for (Recipe &r : rec)
             for (int &gi : r.gates) gi = take[gi];
         plan.recipes.insert(plan.recipes.end(), rec.begin(), rec.end());
