@@ -301,8 +301,11 @@ int apply_gates(desv_handle h, const std::vector<Gate> &gates) {
 }
 
 // Pauli-sum matvec into `out` (device buffer) from `in`.
-int pauli_matvec_raw(desv_handle h, void *out, const void *in, const uint64_t *xmask, const uint64_t *zmask,
-                     const double *cre, const double *cim, int64_t T) {
+// The tiled Pauli-sum kernel on `in`: lambda = H in into `out` (may be null)
+// and, if `epart` is set, one partial Re<in|H in> per workgroup (`*grid_out`
+// workgroups).
+int pauli_tiles_raw(desv_handle h, void *out, const void *in, const uint64_t *xmask, const uint64_t *zmask,
+                    const double *cre, const double *cim, int64_t T, double *epart, int64_t *grid_out) {
     // group terms by flip mask; fold i^{#Y} into the coefficient
     std::vector<int64_t> order(T);
     for (int64_t i = 0; i < T; ++i) order[i] = i;
@@ -331,18 +334,20 @@ int pauli_matvec_raw(desv_handle h, void *out, const void *in, const uint64_t *x
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(h->stream));
     unsigned char *base = reinterpret_cast<unsigned char *>(h->d_small);
-    // one task of the tiled matvec kernel (no energy partials, no psi copy)
+    // one task of the tiled matvec kernel (no psi copy)
     MatvecTask tk;
     std::memset(&tk, 0, sizeof(tk));
     tk.n = h->n;
     tk.ngroups = (int32_t)groups.size();
     tk.psi = in;
     tk.lam = out;
+    tk.epart = epart;
     tk.groups = base;
     tk.z = reinterpret_cast<const uint64_t *>(base + ga);
     tk.cc = reinterpret_cast<const double2 *>(base + ga + za);
     tk.chunk = 1ll << matvec_tile_bits(h->n);
     const int64_t grid = (1ll << h->n) / tk.chunk;
+    if (grid_out) *grid_out = grid;
     std::vector<unsigned char> host(ga + za + ca + sizeof(MatvecTask));
     std::memcpy(host.data(), groups.data(), gb);
     std::memcpy(host.data() + ga, z.data(), zb);
@@ -355,6 +360,11 @@ int pauli_matvec_raw(desv_handle h, void *out, const void *in, const uint64_t *x
                                         : launch_matvec_tasks<double>(dtk, 1, grid, h->stream);
     if (e != hipSuccess) return fail(DESV_EDEVICE, hipGetErrorString(e));
     return DESV_OK;
+}
+
+int pauli_matvec_raw(desv_handle h, void *out, const void *in, const uint64_t *xmask, const uint64_t *zmask,
+                     const double *cre, const double *cim, int64_t T) {
+    return pauli_tiles_raw(h, out, in, xmask, zmask, cre, cim, T, nullptr, nullptr);
 }
 
 int check_masks(desv_handle h, const uint64_t *xmask, const uint64_t *zmask, int64_t T) {
@@ -903,6 +913,24 @@ int desv_pauli_matvec(desv_handle out, desv_handle in, const uint64_t *xmask, co
         return DESV_OK;
     }
     return pauli_matvec_raw(out, out->sv, in->sv, xmask, zmask, cre, cim, T);
+}
+
+int desv_pauli_sum_expect(desv_handle h, const uint64_t *xmask, const uint64_t *zmask, const double *coeff,
+                          int64_t T, double *out) {
+    if (int rc = check_handle(h)) return rc;
+    if (!out) return fail(DESV_EINVAL, "null output");
+    if (T < 0 || (T > 0 && (!xmask || !zmask || !coeff))) return fail(DESV_EINVAL, "bad Pauli term arrays");
+    if (int rc = check_masks(h, xmask, zmask, T)) return rc;
+    HIP_TRY(hipSetDevice(h->device));
+    if (T == 0) {
+        *out = 0.0;
+        return DESV_OK;
+    }
+    const int64_t grid = (1ll << h->n) >> matvec_tile_bits(h->n);
+    if (int rc = ensure_part(h, (size_t)grid)) return rc;
+    int64_t g = 0;
+    if (int rc = pauli_tiles_raw(h, nullptr, h->sv, xmask, zmask, coeff, nullptr, T, h->d_part, &g)) return rc;
+    return sum_to_host(h, 1, (int)g, out);
 }
 
 int desv_dense_matvec(desv_handle out, desv_handle in, const double *H) {

@@ -1104,8 +1104,8 @@ __device__ inline int find_task(const Task *tasks, int ntasks) {
 // fall in one aligned input tile, loaded into LDS once per such run instead
 // of once per group.  The sign (-1)^popc((i ^ x) & z) splits into the part
 // above the tile (uniform, folded into the coefficient) and the part inside.
-// FP64 accumulation; the energy partial <psi|lambda> and the psi copy are
-// formed at the end.
+// FP64 accumulation; lambda (if requested), the energy partial <psi|lambda>
+// and the psi copy (if requested) are formed at the end.
 template <typename T>
 __global__ void __launch_bounds__(kThreads) k_matvec_tasks(const MatvecTask *__restrict__ tasks, int ntasks) {
     using C = typename VT<T>::c;
@@ -1176,10 +1176,12 @@ __global__ void __launch_bounds__(kThreads) k_matvec_tasks(const MatvecTask *__r
     for (int e = 0; e < E; ++e) {
         const int j = t + e * kThreads;
         if (j >= tsize) continue;
-        C r;
-        r.x = (T)acc[e].x;
-        r.y = (T)acc[e].y;
-        lam[i0 + j] = r;
+        if (lam) {
+            C r;
+            r.x = (T)acc[e].x;
+            r.y = (T)acc[e].y;
+            lam[i0 + j] = r;
+        }
         if (full) {
             const C p = psi[i0 + j];
             const double2 pd = to_d(p);

@@ -147,6 +147,14 @@ def pauli_sum_expectation(statevector, terms, n_qubits=None):
     xm, zm, cs = encode_pauli_terms(terms, n, n_qubits)
     h, owned = _as_device(statevector)
     try:
+        if len(xm) and not np.any(cs.imag):
+            # real coefficients: Re<psi|H|psi> in one tiled pass
+            c = np.ascontiguousarray(cs.real, dtype=np.float64)
+            out = np.zeros(1, dtype=np.float64)
+            check(lib.desv_pauli_sum_expect(h.ptr, _capi.u64ptr(np.ascontiguousarray(xm)),
+                                            _capi.u64ptr(np.ascontiguousarray(zm)), _capi.dptr(c), len(xm),
+                                            _capi.dptr(out)))
+            return float(out[0])
         vals = expectations(h, xm, zm)
     finally:
         if owned:

@@ -166,6 +166,12 @@ int desv_pauli_expect(desv_handle h, const uint64_t *xmask, const uint64_t *zmas
 int desv_pauli_matvec(desv_handle out, desv_handle in, const uint64_t *xmask,
                       const uint64_t *zmask, const double *coeff_re,
                       const double *coeff_im, int64_t n_terms);
+/* *out = sum_t coeff[t] <psi|P_t|psi> for real coefficients, i.e.
+ * Re<psi|H|psi>, in one pass of the tiled Pauli-sum kernel with FP64 partial
+ * sums (no per-term values, no output state); the fast path of
+ * pauli_sum_expectation (physics/observables.py:152-186). */
+int desv_pauli_sum_expect(desv_handle h, const uint64_t *xmask, const uint64_t *zmask,
+                          const double *coeff, int64_t n_terms, double *out);
 /* out <- H in for a dense row-major complex128 H (2^n x 2^n, n <= 14); the
  * `h_matrix @ sv` of energy_fn (solvers/autodiff.py:166). */
 int desv_dense_matvec(desv_handle out, desv_handle in, const double *h_matrix);

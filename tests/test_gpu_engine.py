@@ -307,6 +307,10 @@ def test_pauli_matvec_across_tiles(n, use_float32, rng):
     want = O.pauli_sum_matvec(ref_sv, terms)
     got = desv.pauli_sum_matvec(sim.sv, terms)
     assert np.max(np.abs(got - want)) < (1e-5 if use_float32 else 1e-10)
+    # the real-coefficient sum takes the one-pass energy path
+    real_terms = [(float(np.real(c)), t) for c, t in terms]
+    e = desv.pauli_sum_expectation(sim.sv, real_terms)
+    assert abs(e - O.pauli_sum_expectation(ref_sv, real_terms)) < (1e-5 if use_float32 else 1e-10)
     sim.close()
 
 
