@@ -420,6 +420,44 @@ def suite_lanczos(n_big=26, n_cmp=16, ncv=20):
     return out
 
 
+def suite_readout(n=30, shots=1 << 20):
+    """Device readout and noise on a 30-qubit complex64 state: shot sampling
+    (FP64 CDF of |a|^2 on the GPU, then one binary search per shot), a
+    projective single-qubit measurement (probabilities, collapse,
+    renormalisation) and one noise trajectory per model (every qubit draws;
+    Pauli channels cost one streaming pass per fired qubit, amplitude damping
+    a probability pass and an update pass per qubit)."""
+    import numpy as np
+    import desv
+    from desv import _capi, workloads
+    sim = desv.DenseSVSimulator(n, use_float32=True)
+    sim.run_circuit(workloads.random_layers_circuit(n, 2, seed=3))
+    h = sim.handle
+    rng = np.random.default_rng(0)
+    u = np.ascontiguousarray(rng.random(shots), dtype=np.float64)
+    out = np.empty(shots, dtype=np.int64)
+    sample = lambda: _capi.check(_capi.lib.desv_sample(h.ptr, _capi.dptr(u), shots, _capi.i64ptr(out)))
+    sample()
+    t_sample, _ = _timed(sample, reps=3)
+    # the CDF alone: the same call with one shot
+    one = np.ascontiguousarray(u[:1])
+    o1 = np.empty(1, dtype=np.int64)
+    t_cdf, _ = _timed(lambda: _capi.check(_capi.lib.desv_sample(h.ptr, _capi.dptr(one), 1, _capi.i64ptr(o1))),
+                      reps=3)
+    t_measure, _ = _timed(lambda: sim.measure(n // 2, rng=np.random.default_rng(1)), reps=3)
+    noise = {}
+    for model, p in (("depolarizing", 0.05), ("amplitude_damping", 0.05)):
+        t, _ = _timed(lambda: desv.NoiseModel.apply_to_sv(sim.sv, n, model, p, rng=np.random.default_rng(7)),
+                      reps=2)
+        noise[model] = {"p": p, "ms_per_trajectory": t * 1e3}
+    sim.close()
+    cdf_bytes = (1 << n) * (8 + 8 + 8)  # chunk totals (read), then read again and write the normalised CDF
+    return {"workload": f"{n}q complex64 state: sampling, measurement, noise trajectories",
+            "shots": shots, "sample_ms": t_sample * 1e3, "shots_per_s": shots / t_sample,
+            "cdf_ms": t_cdf * 1e3, "cdf_gbps": cdf_bytes / t_cdf / 1e9,
+            "measure_ms": t_measure * 1e3, "noise": noise}
+
+
 def suite_brickwork34(n=34, depth=24):
     """The brickwork at 34 qubits: a 128 GiB complex64 state, the largest
     power of two that fits one MI355X's 288 GB (the state is updated in
@@ -500,7 +538,8 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="only time the library's device copy of a 2^qubits complex64 state "
                          "(calibration of HBM rate and of the rocprofv3 byte counters)")
-    ap.add_argument("--suite", default="streaming30,parity30,brickwork32,brickwork34,plain20,vqe20,batch,precision,lanczos",
+    ap.add_argument("--suite", default="streaming30,parity30,brickwork32,brickwork34,plain20,vqe20,batch,precision,lanczos,"
+                    "readout",
                     help="extra workloads to measure after the headline ('' for none)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -657,7 +696,8 @@ def main():
     extras = {}
     fns = {"streaming30": suite_streaming30, "parity30": suite_parity30, "brickwork32": suite_brickwork32,
            "plain20": suite_plain20, "vqe20": suite_vqe20, "batch": suite_batch,
-           "precision": suite_precision, "lanczos": suite_lanczos, "brickwork34": suite_brickwork34}
+           "precision": suite_precision, "lanczos": suite_lanczos, "brickwork34": suite_brickwork34,
+           "readout": suite_readout}
     for name in [x for x in args.suite.split(",") if x]:
         log(f"suite {name} ...")
         t0 = time.perf_counter()

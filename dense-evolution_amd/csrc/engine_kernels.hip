@@ -1895,29 +1895,56 @@ __device__ inline double block_inclusive_scan(double v, double *ts) {
     return ts[threadIdx.x];
 }
 
-template <typename T>
-__global__ void __launch_bounds__(kThreads) k_cdf_local(const typename VT<T>::c *sv, int64_t dim, double *cdf,
-                                                        double *blk) {
-    __shared__ double ts[kThreads];
+// One 4096-amplitude chunk per workgroup: lane t sums |a|^2 over its 16
+// consecutive amplitudes, then a workgroup inclusive scan.  WRITE = false
+// stores the chunk total in blk[chunk]; WRITE = true (after k_scan_blocks)
+// recomputes the same scan and stores the normalised CDF
+// ((before + v) + offset) / total, the same operations in the same order as
+// a separate local-scan, offset and normalisation pass.  Global loads and
+// stores are coalesced through LDS (element e at e + e / 16: lane t's 16
+// elements then fall in distinct banks).
+template <typename T, bool WRITE>
+__global__ void __launch_bounds__(kThreads) k_cdf_pass(const typename VT<T>::c *sv, int64_t dim, double *cdf,
+                                                       double *blk, int64_t nb) {
     constexpr int per = kScanChunk / kThreads;
-    const int64_t b = (int64_t)blockIdx.x * kScanChunk + (int64_t)threadIdx.x * per;
+    __shared__ double ts[kThreads];
+    __shared__ double buf[kScanChunk + kScanChunk / per];
+    auto at = [](int e) { return e + e / per; };
+    const int64_t c0 = (int64_t)blockIdx.x * kScanChunk;
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < per; ++k) {
+        const int e = k * kThreads + t;
+        double p = 0;
+        if (c0 + e < dim) {
+            const double2 a = to_d(sv[c0 + e]);
+            p = a.x * a.x + a.y * a.y;
+        }
+        buf[at(e)] = p;
+    }
+    __syncthreads();
     double v[per], run = 0;
 #pragma unroll
     for (int j = 0; j < per; ++j) {
-        double p = 0;
-        if (b + j < dim) {
-            const double2 a = to_d(sv[b + j]);
-            p = a.x * a.x + a.y * a.y;
-        }
-        run += p;
+        run += buf[at(t * per + j)];
         v[j] = run;
     }
     const double incl = block_inclusive_scan(run, ts);
-    const double before = incl - run;
+    if constexpr (WRITE) {
+        const double before = incl - run;
+        const double off = blk[blockIdx.x], total = blk[nb];
+        __syncthreads();  // every lane has read its elements
 #pragma unroll
-    for (int j = 0; j < per; ++j)
-        if (b + j < dim) cdf[b + j] = before + v[j];
-    if (threadIdx.x == blockDim.x - 1) blk[blockIdx.x] = incl;
+        for (int j = 0; j < per; ++j) buf[at(t * per + j)] = ((before + v[j]) + off) / total;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < per; ++k) {
+            const int e = k * kThreads + t;
+            if (c0 + e < dim) cdf[c0 + e] = buf[at(e)];
+        }
+    } else {
+        if (threadIdx.x == blockDim.x - 1) blk[blockIdx.x] = incl;
+    }
 }
 
 // blk[0..nb) <- exclusive prefix sums; blk[nb] <- total.  One workgroup.
@@ -1935,12 +1962,6 @@ __global__ void __launch_bounds__(kThreads) k_scan_blocks(double *blk, int64_t n
         acc += x;
     }
     if (threadIdx.x == blockDim.x - 1) blk[nb] = incl;
-}
-
-__global__ void k_cdf_finish(double *cdf, int64_t dim, const double *blk, int64_t nb) {
-    const double total = blk[nb];
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < dim; i += (int64_t)gridDim.x * blockDim.x)
-        cdf[i] = (cdf[i] + blk[i / kScanChunk]) / total;
 }
 
 // out[s] = number of cdf entries <= u[s] (the first index whose cumulative
@@ -1962,10 +1983,11 @@ template <typename T>
 hipError_t launch_cdf(const void *sv, int64_t dim, double *cdf, double *blk, hipStream_t s) {
     const int64_t nb = (dim + kScanChunk - 1) / kScanChunk;
     if (nb > 0x7fffffffll) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_cdf_local<T>), dim3((unsigned)nb), dim3(kThreads), 0, s,
-                       (const typename VT<T>::c *)sv, dim, cdf, blk);
+    hipLaunchKernelGGL((k_cdf_pass<T, false>), dim3((unsigned)nb), dim3(kThreads), 0, s,
+                       (const typename VT<T>::c *)sv, dim, cdf, blk, nb);
     hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(kThreads), 0, s, blk, nb);
-    hipLaunchKernelGGL(k_cdf_finish, dim3(ew_grid(dim)), dim3(kThreads), 0, s, cdf, dim, (const double *)blk, nb);
+    hipLaunchKernelGGL((k_cdf_pass<T, true>), dim3((unsigned)nb), dim3(kThreads), 0, s,
+                       (const typename VT<T>::c *)sv, dim, cdf, blk, nb);
     return hipGetLastError();
 }
 int64_t cdf_blocks(int64_t dim) { return (dim + kScanChunk - 1) / kScanChunk; }
