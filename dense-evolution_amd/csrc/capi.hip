@@ -637,6 +637,27 @@ int desv_qubit_probs(desv_handle h, int qubit, double *p0, double *p1) {
     return DESV_OK;
 }
 
+int desv_apply1q_probs(desv_handle h, int qubit, const double *m, int next_qubit, double *p0, double *p1) {
+    if (int rc = check_handle(h)) return rc;
+    if (qubit < 0 || qubit >= h->n) return fail(DESV_EINVAL, "Qubit " + std::to_string(qubit) + " out of range");
+    if (next_qubit >= h->n) return fail(DESV_EINVAL, "Qubit " + std::to_string(next_qubit) + " out of range");
+    if (!m) return fail(DESV_EINVAL, "null matrix");
+    if (next_qubit >= 0 && (!p0 || !p1)) return fail(DESV_EINVAL, "null probability outputs");
+    HIP_TRY(hipSetDevice(h->device));
+    // one streaming pass: enough workgroups to keep the pairs' loads in flight
+    const int nblk = (int)std::max<int64_t>(1, std::min<int64_t>(h->dim / (2 * kThreads * 8), 8192));
+    if (int rc = ensure_part(h, 2 * (size_t)nblk)) return rc;
+    const int bit = h->n - 1 - qubit, pbit = next_qubit >= 0 ? h->n - 1 - next_qubit : -1;
+    HIP_TRY(h->dtype == DESV_C64 ? launch_apply1q_probs<float>(h->sv, h->dim, bit, m, pbit, h->d_part, nblk, h->stream)
+                                 : launch_apply1q_probs<double>(h->sv, h->dim, bit, m, pbit, h->d_part, nblk, h->stream));
+    if (next_qubit < 0) return DESV_OK;
+    double out[2];
+    if (int rc = sum_to_host(h, 2, nblk, out)) return rc;
+    *p0 = out[0];
+    *p1 = out[1];
+    return DESV_OK;
+}
+
 int desv_collapse(desv_handle h, int qubit, int outcome, double scale) {
     if (int rc = check_handle(h)) return rc;
     if (qubit < 0 || qubit >= h->n) return fail(DESV_EINVAL, "Qubit " + std::to_string(qubit) + " out of range");

@@ -2043,6 +2043,52 @@ __global__ void __launch_bounds__(kThreads) k_sum_partials(const double *partial
     if (threadIdx.x == 0) out[blockIdx.x] = s;
 }
 
+// In place: the 2x2 matrix m on index bit `bit` (state precision, the same
+// products as the streaming kernel), and, if pbit >= 0, the FP64 populations
+// of index bit pbit in the updated state: partials[blk] (bit = 0) and
+// partials[nblk + blk] (bit = 1).
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_apply1q_probs(typename VT<T>::c *sv, int64_t dim, int bit, double2 m00,
+                                                            double2 m01, double2 m10, double2 m11, int pbit,
+                                                            double *partials, int nblk) {
+    using C = typename VT<T>::c;
+    __shared__ double red[kThreads / 64];
+    C a00, a01, a10, a11;
+    a00.x = (T)m00.x, a00.y = (T)m00.y, a01.x = (T)m01.x, a01.y = (T)m01.y;
+    a10.x = (T)m10.x, a10.y = (T)m10.y, a11.x = (T)m11.x, a11.y = (T)m11.y;
+    const int64_t half = dim >> 1, tb = 1ll << bit;
+    double s0 = 0, s1 = 0;
+    for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < half; u += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t x = ((u >> bit) << (bit + 1)) | (u & (tb - 1)), y = x | tb;
+        const C v0 = sv[x], v1 = sv[y];
+        const C w0 = cfma(a01, v1, cmul(a00, v0)), w1 = cfma(a11, v1, cmul(a10, v0));
+        sv[x] = w0;
+        sv[y] = w1;
+        if (pbit >= 0) {
+            const double2 d0 = to_d(w0), d1 = to_d(w1);
+            const double q0 = d0.x * d0.x + d0.y * d0.y, q1 = d1.x * d1.x + d1.y * d1.y;
+            if ((x >> pbit) & 1) s1 += q0; else s0 += q0;
+            if ((y >> pbit) & 1) s1 += q1; else s0 += q1;
+        }
+    }
+    if (pbit >= 0) {  // uniform
+        const double t0 = block_sum(s0, red), t1 = block_sum(s1, red);
+        if (threadIdx.x == 0) {
+            partials[blockIdx.x] = t0;
+            partials[nblk + blockIdx.x] = t1;
+        }
+    }
+}
+
+template <typename T>
+hipError_t launch_apply1q_probs(void *sv, int64_t dim, int bit, const double *m, int pbit, double *partials, int nblk,
+                                hipStream_t s) {
+    hipLaunchKernelGGL((k_apply1q_probs<T>), dim3(nblk), dim3(kThreads), 0, s, (typename VT<T>::c *)sv, dim, bit,
+                       make_double2(m[0], m[1]), make_double2(m[2], m[3]), make_double2(m[4], m[5]),
+                       make_double2(m[6], m[7]), pbit, partials, nblk);
+    return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch_norm2(const void *sv, int64_t dim, double *partials, int nblk, hipStream_t s) {
     hipLaunchKernelGGL((k_norm2<T>), dim3(nblk), dim3(kThreads), 0, s, (const typename VT<T>::c *)sv, dim, partials);
@@ -2303,7 +2349,9 @@ hipError_t launch_fill_random(void *sv, int64_t dim, uint64_t seed, hipStream_t 
                                              hipStream_t);                                                    \
     template hipError_t launch_combine<T>(const BasisPtrs &, const BasisPtrs &, int, int, const double2 *,     \
                                           int64_t, hipStream_t);                                              \
-    template hipError_t launch_fill_random<T>(void *, int64_t, uint64_t, hipStream_t);
+    template hipError_t launch_fill_random<T>(void *, int64_t, uint64_t, hipStream_t);                          \
+    template hipError_t launch_apply1q_probs<T>(void *, int64_t, int, const double *, int, double *, int,        \
+                                                hipStream_t);
 
 DESV_INST(float)
 DESV_INST(double)

@@ -152,6 +152,11 @@ hipError_t launch_probs(const void *sv, int64_t dim, double *out, hipStream_t s)
 template <typename T>
 hipError_t launch_cdf(const void *sv, int64_t dim, double *cdf, double *blk, hipStream_t s);
 int64_t cdf_blocks(int64_t dim);
+// In place 2x2 matrix (row-major re/im, 8 doubles) on index bit `bit`; if
+// pbit >= 0 also the populations of index bit pbit afterwards (2 x nblk partials).
+template <typename T>
+hipError_t launch_apply1q_probs(void *sv, int64_t dim, int bit, const double *m, int pbit, double *partials, int nblk,
+                                hipStream_t s);
 hipError_t launch_sample_search(const double *cdf, int64_t dim, const double *u, int64_t shots, int64_t *out,
                                 hipStream_t s);
 template <typename T>

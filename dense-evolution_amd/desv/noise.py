@@ -109,10 +109,14 @@ class NoiseModel:
         try:
             if handle.n != n:
                 raise ValueError(f"statevector has {handle.n} qubits, n = {n}")
-            for q in (qubits if qubits is not None else range(n)):
+            qs = list(qubits if qubits is not None else range(n))
+            for q in qs:
                 if not 0 <= q < n:
                     raise ValueError(f"noise qubit {q} out of range for {n} qubits")
-                _CHANNELS[model](handle, n - 1 - q, float(p), rng)
+            st = _Trajectory(handle)
+            for i, q in enumerate(qs):
+                nxt = n - 1 - qs[i + 1] if i + 1 < len(qs) else None
+                _CHANNELS[model](st, n - 1 - q, float(p), rng, nxt)
             norm = np.sqrt(handle.norm2())
             handle.scale(1.0 / (norm + 1e-15))
             if owned:
@@ -143,47 +147,73 @@ class NoiseModel:
 
 # ---- channels: one decision per qubit, one device pass when it fires --------
 
-def _depolarizing(h: StateHandle, q: int, p: float, rng) -> Optional[np.ndarray]:
+class _Trajectory:
+    """The state of one trajectory: 2x2 operators are applied in one
+    streaming pass each, and a damping step's pass also returns the
+    populations of the qubit the next damping step draws against, so no
+    separate probability pass is needed after the first.  A Pauli on qubit q
+    leaves every other qubit's populations as they are and swaps (X, Y) or
+    keeps (Z) q's own, so a cached pair survives it."""
+
+    def __init__(self, handle: StateHandle):
+        self.h = handle
+        self.q = None    # engine qubit whose populations are cached
+        self.p = None
+
+    def probs(self, q):
+        return self.p if self.q == q else self.h.qubit_probs(q)
+
+    def pauli(self, m, q, swaps):
+        self.h.apply_matrix(m, [q])
+        if swaps and self.q == q:
+            self.p = (self.p[1], self.p[0])
+
+    def kraus(self, m, q, nxt):
+        if nxt is None:
+            self.h.apply_matrix(m, [q])
+            self.q = self.p = None
+        else:
+            self.p = self.h.apply_1q_probs(m, q, nxt)
+            self.q = nxt
+
+
+def _depolarizing(st: _Trajectory, q: int, p: float, rng, nxt=None):
     r, ch = rng.random(), rng.random()
     if r < p:
-        m = _X if ch < _THIRD else (_Y if ch < 2.0 * _THIRD else _Z)
-        h.apply_matrix(m, [q])
-        return m
-    return None
+        if ch < _THIRD:
+            st.pauli(_X, q, True)
+        elif ch < 2.0 * _THIRD:
+            st.pauli(_Y, q, True)
+        else:
+            st.pauli(_Z, q, False)
 
 
-def _bitflip(h, q, p, rng):
+def _bitflip(st, q, p, rng, nxt=None):
     if rng.random() < p:
-        h.apply_matrix(_X, [q])
-        return _X
-    return None
+        st.pauli(_X, q, True)
 
 
-def _phaseflip(h, q, p, rng):
+def _phaseflip(st, q, p, rng, nxt=None):
     if rng.random() < p:
-        h.apply_matrix(_Z, [q])
-        return _Z
-    return None
+        st.pauli(_Z, q, False)
 
 
-def _amplitude_damping(h, q, p, rng):
+def _amplitude_damping(st, q, p, rng, nxt=None):
     gamma = float(np.clip(p, 0.0, 1.0))
     r = rng.random()
-    p0, p1 = h.qubit_probs(q)
+    p0, p1 = st.probs(q)
     p1 = float(np.clip(gamma * p1, 0.0, 1.0))
     if r < p1:
         m = np.array([[0, np.sqrt(gamma) / np.sqrt(max(p1, 1e-15))], [0, 0]], dtype=np.complex128)
     else:
         nn = np.sqrt(max(1.0 - p1, 1e-15))
         m = np.array([[1.0 / nn, 0], [0, np.sqrt(1.0 - gamma) / nn]], dtype=np.complex128)
-    h.apply_matrix(m, [q])
-    return None
+    st.kraus(m, q, nxt)
 
 
-def _combined(h, q, p, rng):
-    _depolarizing(h, q, p * 0.5, rng)
-    _amplitude_damping(h, q, p * 0.333333, rng)
-    return None
+def _combined(st, q, p, rng, nxt=None):
+    _depolarizing(st, q, p * 0.5, rng)
+    _amplitude_damping(st, q, p * 0.333333, rng, nxt)
 
 
 _CHANNELS = {'depolarizing': _depolarizing, 'bitflip': _bitflip, 'phaseflip': _phaseflip,

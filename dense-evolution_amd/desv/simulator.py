@@ -75,6 +75,17 @@ class StateHandle:
         check(lib.desv_qubit_probs(self.ptr, int(qubit), C.byref(p0), C.byref(p1)))
         return p0.value, p1.value
 
+    def apply_1q_probs(self, mat: np.ndarray, qubit: int, next_qubit: Optional[int] = None):
+        """2x2 `mat` on `qubit` in place (one streaming pass); with
+        `next_qubit`, the same pass's (p0, p1) populations of that qubit in
+        the updated state (FP64, not normalised)."""
+        m = np.ascontiguousarray(np.asarray(mat, dtype=np.complex128).reshape(2, 2))
+        p0, p1 = C.c_double(), C.c_double()
+        nq = -1 if next_qubit is None else int(next_qubit)
+        check(lib.desv_apply1q_probs(self.ptr, int(qubit), m.view(np.float64).ctypes.data_as(_capi._pd), nq,
+                                     C.byref(p0), C.byref(p1)))
+        return None if next_qubit is None else (p0.value, p1.value)
+
     def reset(self) -> None:
         check(lib.desv_reset(self.ptr))
 

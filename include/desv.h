@@ -101,6 +101,13 @@ int desv_qubit_probs(desv_handle h, int qubit, double *p0, double *p1);
 int desv_sample(desv_handle h, const double *uniforms, int64_t n_shots, int64_t *out);
 /* Zero the amplitudes whose `qubit` differs from `outcome`, scale the rest. */
 int desv_collapse(desv_handle h, int qubit, int outcome, double scale);
+/* In place 2x2 matrix m (row-major, interleaved re/im, 8 doubles) on `qubit`
+ * in one streaming pass; if next_qubit >= 0 the same pass returns the FP64
+ * populations (p0, p1) of next_qubit in the updated state (not normalised).
+ * Serves the per-qubit Kraus steps of the noise trajectories
+ * (noise/kraus_channels.py:104-235: amplitude damping draws against the next
+ * qubit's |1> population, computed here without a separate pass). */
+int desv_apply1q_probs(desv_handle h, int qubit, const double *m, int next_qubit, double *p0, double *p1);
 
 /* ---- gate application ----------------------------------------------------
  * desv_run_ops replaces _compile_and_run_circuit_jit(sv, ops) and its

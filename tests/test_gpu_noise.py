@@ -51,6 +51,30 @@ def test_device_state_in_place_c64(model):
     sim.close()
 
 
+@pytest.mark.parametrize("complex128", [True, False])
+def test_apply_1q_with_next_populations(complex128):
+    """One pass applies a 2x2 operator in place and returns the populations
+    of another qubit (or the same one) in the updated state."""
+    n = 12
+    rng = np.random.default_rng(11)
+    h = desv.StateHandle(n, complex128)
+    sv = _random_state(n, 4)
+    for q, nq in ((3, 7), (0, 11), (11, 0), (5, 5), (8, None)):
+        m = rng.normal(size=(2, 2)) + 1j * rng.normal(size=(2, 2))
+        h.upload(sv)
+        got = h.apply_1q_probs(m, q, nq)
+        want = O.apply_matrix(sv, m, [q])
+        tol = 1e-12 if complex128 else 1e-5
+        assert np.max(np.abs(h.download() - want)) < tol
+        if nq is None:
+            assert got is None
+        else:
+            bits = (np.arange(1 << n) >> (n - 1 - nq)) & 1
+            pw = np.abs(want) ** 2
+            assert abs(got[0] - pw[bits == 0].sum()) < tol and abs(got[1] - pw[bits == 1].sum()) < tol
+    h.close()
+
+
 def test_sequence_of_trajectories_advances_the_generator():
     n = 5
     sv = _random_state(n, 9)
