@@ -342,8 +342,10 @@ class DenseSVSimulator:
         else:
             result = int(np.random.choice([0, 1], p=[pr0, pr1]))
         keep = pr0 if result == 0 else pr1
+        # zero the other half and divide by the kept half's norm, sqrt(keep *
+        # total) = sqrt(sum |a|^2 over the outcome): the reference's mask then
+        # sv / norm(sv) (statevector.py:343-358), in one pass
         check(lib.desv_collapse(self._h.ptr, qubit_idx, result, 1.0 / np.sqrt(keep * total)))
-        self.normalize()
         return result
 
     # -- circuits --------------------------------------------------------------
