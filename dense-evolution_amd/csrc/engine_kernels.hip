@@ -2080,9 +2080,107 @@ __global__ void __launch_bounds__(kThreads) k_apply1q_probs(typename VT<T>::c *s
     }
 }
 
+// Complex64 form of k_apply1q_probs with 16-byte accesses.  B0 (bit = 0): a
+// unit is one float4, the pair (x, x | 1).  Otherwise a unit is the float4s
+// at x and x | 2^bit (two neighbouring pairs).  Each wave walks GPT
+// consecutive 64-unit runs per iteration, so every access instruction of a
+// wave is one contiguous run; the products per amplitude are those of the
+// generic kernel and the populations are accumulated in FP64 the same way.
+template <bool B0, bool NT>
+__global__ void __launch_bounds__(kThreads) k_apply1q_probs_c64(float4 *sv, int64_t units, int bit, float2 a00,
+                                                                float2 a01, float2 a10, float2 a11, int pbit,
+                                                                double *partials, int nblk) {
+    constexpr int GPT = 4;
+    __shared__ double red[kThreads / 64];
+    const int64_t tb = 1ll << bit;
+    const int lane = threadIdx.x & 63;
+    const int64_t wave0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    double s0 = 0, s1 = 0;
+    auto pop = [&](int64_t e, float2 w) {
+        const double2 d = to_d(w);
+        const double q = d.x * d.x + d.y * d.y;
+        if ((e >> pbit) & 1) s1 += q; else s0 += q;
+    };
+    for (int64_t w = wave0; w * (GPT * 64) < units; w += nwaves) {
+        int64_t ix[GPT], iy[GPT];
+        float4 v[GPT], u[GPT];
+        bool on[GPT];
+#pragma unroll
+        for (int j = 0; j < GPT; ++j) {
+            const int64_t un = (w * GPT + j) * 64 + lane;
+            on[j] = un < units;
+            if constexpr (B0) {
+                ix[j] = un;
+                iy[j] = un;
+                if (on[j]) v[j] = ld<float4, NT>(sv + un);
+            } else {
+                const int64_t a = 2 * un, x = ((a >> bit) << (bit + 1)) | (a & (tb - 1));
+                ix[j] = x >> 1;
+                iy[j] = (x | tb) >> 1;
+                if (on[j]) {
+                    v[j] = ld<float4, NT>(sv + ix[j]);
+                    u[j] = ld<float4, NT>(sv + iy[j]);
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < GPT; ++j) {
+            if (!on[j]) continue;
+            if constexpr (B0) {
+                const float2 v0 = make_float2(v[j].x, v[j].y), v1 = make_float2(v[j].z, v[j].w);
+                const float2 w0 = cfma(a01, v1, cmul(a00, v0)), w1 = cfma(a11, v1, cmul(a10, v0));
+                st<float4, NT>(sv + ix[j], make_float4(w0.x, w0.y, w1.x, w1.y));
+                if (pbit >= 0) {
+                    pop(2 * ix[j], w0);
+                    pop(2 * ix[j] + 1, w1);
+                }
+            } else {
+                const float2 p0 = make_float2(v[j].x, v[j].y), p1 = make_float2(v[j].z, v[j].w);
+                const float2 q0 = make_float2(u[j].x, u[j].y), q1 = make_float2(u[j].z, u[j].w);
+                const float2 w0 = cfma(a01, q0, cmul(a00, p0)), y0 = cfma(a11, q0, cmul(a10, p0));
+                const float2 w1 = cfma(a01, q1, cmul(a00, p1)), y1 = cfma(a11, q1, cmul(a10, p1));
+                st<float4, NT>(sv + ix[j], make_float4(w0.x, w0.y, w1.x, w1.y));
+                st<float4, NT>(sv + iy[j], make_float4(y0.x, y0.y, y1.x, y1.y));
+                if (pbit >= 0) {
+                    pop(2 * ix[j], w0);
+                    pop(2 * ix[j] + 1, w1);
+                    pop(2 * iy[j], y0);
+                    pop(2 * iy[j] + 1, y1);
+                }
+            }
+        }
+    }
+    if (pbit >= 0) {  // uniform
+        const double t0 = block_sum(s0, red), t1 = block_sum(s1, red);
+        if (threadIdx.x == 0) {
+            partials[blockIdx.x] = t0;
+            partials[nblk + blockIdx.x] = t1;
+        }
+    }
+}
+
 template <typename T>
 hipError_t launch_apply1q_probs(void *sv, int64_t dim, int bit, const double *m, int pbit, double *partials, int nblk,
                                 hipStream_t s) {
+    if constexpr (sizeof(T) == 4) {
+        // 16-byte units from 2^10 amplitudes up (every unit count a multiple of 256)
+        if (dim >= 1024) {
+            const float2 a00 = make_float2((float)m[0], (float)m[1]), a01 = make_float2((float)m[2], (float)m[3]);
+            const float2 a10 = make_float2((float)m[4], (float)m[5]), a11 = make_float2((float)m[6], (float)m[7]);
+            float4 *p = (float4 *)sv;
+            if (bit == 0)
+                hipLaunchKernelGGL((k_apply1q_probs_c64<true, false>), dim3(nblk), dim3(kThreads), 0, s, p, dim / 2,
+                                   bit, a00, a01, a10, a11, pbit, partials, nblk);
+            else if (direct_shape(false, true, bit).nt)
+                hipLaunchKernelGGL((k_apply1q_probs_c64<false, true>), dim3(nblk), dim3(kThreads), 0, s, p, dim / 4,
+                                   bit, a00, a01, a10, a11, pbit, partials, nblk);
+            else
+                hipLaunchKernelGGL((k_apply1q_probs_c64<false, false>), dim3(nblk), dim3(kThreads), 0, s, p, dim / 4,
+                                   bit, a00, a01, a10, a11, pbit, partials, nblk);
+            return hipGetLastError();
+        }
+    }
     hipLaunchKernelGGL((k_apply1q_probs<T>), dim3(nblk), dim3(kThreads), 0, s, (typename VT<T>::c *)sv, dim, bit,
                        make_double2(m[0], m[1]), make_double2(m[2], m[3]), make_double2(m[4], m[5]),
                        make_double2(m[6], m[7]), pbit, partials, nblk);

@@ -59,7 +59,9 @@ def test_apply_1q_with_next_populations(complex128):
     rng = np.random.default_rng(11)
     h = desv.StateHandle(n, complex128)
     sv = _random_state(n, 4)
-    for q, nq in ((3, 7), (0, 11), (11, 0), (5, 5), (8, None)):
+    # index bits 8, 11, 0, 6, 3, 1, 2: the complex64 kernel's pair, strided
+    # and non-temporal forms
+    for q, nq in ((3, 7), (0, 11), (11, 0), (5, 5), (8, None), (10, 1), (9, 10)):
         m = rng.normal(size=(2, 2)) + 1j * rng.normal(size=(2, 2))
         h.upload(sv)
         got = h.apply_1q_probs(m, q, nq)
