@@ -182,9 +182,12 @@ int ensure_aux(desv_handle h, int i) {
     return DESV_OK;
 }
 
+// Workgroups of a grid-stride FP64 reduction: one per 4 x 256 amplitudes, at
+// most 8192 (32 waves per CU: enough loads in flight to stream a large state
+// from HBM).
 int reduce_blocks(int64_t dim) {
     int64_t b = (dim + kThreads * 4 - 1) / (kThreads * 4);
-    return (int)std::max<int64_t>(1, std::min<int64_t>(b, 1024));
+    return (int)std::max<int64_t>(1, std::min<int64_t>(b, 8192));
 }
 
 // Read `nvals` sums of partials back to the host (synchronous).

@@ -2187,9 +2187,60 @@ hipError_t launch_apply1q_probs(void *sv, int64_t dim, int bit, const double *m,
     return hipGetLastError();
 }
 
+// Complex64 forms of k_norm2 (PROBS = false) and k_qubit_probs (PROBS =
+// true) with 16-byte loads: a float4 holds amplitudes 2f and 2f + 1, and each
+// wave reads 4 contiguous 64-float4 runs per iteration, so every load
+// instruction of a wave is one contiguous run and four are in flight per
+// lane.  The FP64 terms per amplitude are those of the generic kernels.
+template <bool PROBS>
+__global__ void __launch_bounds__(kThreads) k_pops_c64(const float4 *sv, int64_t nf4, int bit, double *partials) {
+    constexpr int GPT = 4;
+    __shared__ double red[kThreads / 64];
+    const int lane = threadIdx.x & 63;
+    const int64_t wave0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    double p0 = 0, p1 = 0;
+    for (int64_t w = wave0; w * (GPT * 64) < nf4; w += nwaves) {
+        float4 v[GPT];
+#pragma unroll
+        for (int j = 0; j < GPT; ++j) {
+            const int64_t f = (w * GPT + j) * 64 + lane;
+            v[j] = f < nf4 ? sv[f] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (int j = 0; j < GPT; ++j) {
+            const int64_t e = 2 * ((w * GPT + j) * 64 + lane);
+            const double2 a = to_d(make_float2(v[j].x, v[j].y)), b = to_d(make_float2(v[j].z, v[j].w));
+            const double pa = a.x * a.x + a.y * a.y, pb = b.x * b.x + b.y * b.y;
+            if constexpr (PROBS) {
+                if ((e >> bit) & 1) p1 += pa; else p0 += pa;
+                if (((e + 1) >> bit) & 1) p1 += pb; else p0 += pb;
+            } else {
+                p0 += pa;
+                p0 += pb;
+            }
+        }
+    }
+    const double s0 = block_sum(p0, red);
+    if constexpr (PROBS) {
+        const double s1 = block_sum(p1, red);
+        if (threadIdx.x == 0) {
+            partials[blockIdx.x] = s0;
+            partials[gridDim.x + blockIdx.x] = s1;
+        }
+    } else if (threadIdx.x == 0) {
+        partials[blockIdx.x] = s0;
+    }
+}
+
 template <typename T>
 hipError_t launch_norm2(const void *sv, int64_t dim, double *partials, int nblk, hipStream_t s) {
-    hipLaunchKernelGGL((k_norm2<T>), dim3(nblk), dim3(kThreads), 0, s, (const typename VT<T>::c *)sv, dim, partials);
+    if (sizeof(T) == 4 && dim >= 2)
+        hipLaunchKernelGGL((k_pops_c64<false>), dim3(nblk), dim3(kThreads), 0, s, (const float4 *)sv, dim / 2, 0,
+                           partials);
+    else
+        hipLaunchKernelGGL((k_norm2<T>), dim3(nblk), dim3(kThreads), 0, s, (const typename VT<T>::c *)sv, dim,
+                           partials);
     return hipGetLastError();
 }
 template <typename T>
@@ -2205,8 +2256,12 @@ hipError_t launch_probs(const void *sv, int64_t dim, double *out, hipStream_t s)
 }
 template <typename T>
 hipError_t launch_qubit_probs(const void *sv, int64_t dim, int bit, double *partials, int nblk, hipStream_t s) {
-    hipLaunchKernelGGL((k_qubit_probs<T>), dim3(nblk), dim3(kThreads), 0, s, (const typename VT<T>::c *)sv, dim, bit,
-                       partials);
+    if (sizeof(T) == 4 && dim >= 2)
+        hipLaunchKernelGGL((k_pops_c64<true>), dim3(nblk), dim3(kThreads), 0, s, (const float4 *)sv, dim / 2, bit,
+                           partials);
+    else
+        hipLaunchKernelGGL((k_qubit_probs<T>), dim3(nblk), dim3(kThreads), 0, s, (const typename VT<T>::c *)sv, dim,
+                           bit, partials);
     return hipGetLastError();
 }
 template <typename T>
