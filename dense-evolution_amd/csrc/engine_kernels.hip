@@ -1878,8 +1878,8 @@ __global__ void k_probs(const typename VT<T>::c *sv, int64_t dim, double *out) {
 
 // ---------------------------------------------------------------------------
 // Shot sampling: the probability CDF of the state in three deterministic
-// passes (per-workgroup scans, one scan of the workgroup totals, offset and
-// normalise), then one binary search per shot.
+// passes (per-chunk totals, one scan of the chunk totals, per-chunk scans
+// with offset and normalisation), then one binary search per shot.
 // ---------------------------------------------------------------------------
 constexpr int kScanChunk = 4096;  // amplitudes per workgroup, 16 per lane
 
@@ -1895,17 +1895,53 @@ __device__ inline double block_inclusive_scan(double v, double *ts) {
     return ts[threadIdx.x];
 }
 
-// One 4096-amplitude chunk per workgroup: lane t sums |a|^2 over its 16
-// consecutive amplitudes, then a workgroup inclusive scan.  WRITE = false
-// stores the chunk total in blk[chunk]; WRITE = true (after k_scan_blocks)
-// recomputes the same scan and stores the normalised CDF
-// ((before + v) + offset) / total, the same operations in the same order as
-// a separate local-scan, offset and normalisation pass.  Global loads and
+// First CDF pass: blk[chunk] = sum of |a|^2 over each kScanChunk-amplitude
+// chunk, one workgroup per chunk, 16-byte coalesced loads (a float4 is two
+// complex64 amplitudes or one complex128 amplitude) and no LDS staging.
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_cdf_totals(const typename VT<T>::c *sv, int64_t dim, double *blk) {
+    using C = typename VT<T>::c;
+    constexpr int APU = 16 / (int)sizeof(C);  // amplitudes per 16-byte unit
+    constexpr int per = kScanChunk / APU / kThreads;
+    __shared__ double red[kThreads / 64];
+    const int64_t c0 = (int64_t)blockIdx.x * kScanChunk;
+    double acc = 0;
+    if (c0 + kScanChunk <= dim) {
+        const float4 *p = reinterpret_cast<const float4 *>(sv + c0);
+        float4 v[per];
+#pragma unroll
+        for (int k = 0; k < per; ++k) v[k] = p[k * kThreads + threadIdx.x];
+#pragma unroll
+        for (int k = 0; k < per; ++k) {
+            if constexpr (APU == 2) {
+                const double2 a = to_d(make_float2(v[k].x, v[k].y)), b = to_d(make_float2(v[k].z, v[k].w));
+                acc += a.x * a.x + a.y * a.y;
+                acc += b.x * b.x + b.y * b.y;
+            } else {
+                double2 a;
+                __builtin_memcpy(&a, &v[k], sizeof(a));
+                acc += a.x * a.x + a.y * a.y;
+            }
+        }
+    } else {
+        for (int64_t e = c0 + threadIdx.x; e < dim; e += kThreads) {
+            const double2 a = to_d(sv[e]);
+            acc += a.x * a.x + a.y * a.y;
+        }
+    }
+    const double t = block_sum(acc, red);
+    if (threadIdx.x == 0) blk[blockIdx.x] = t;
+}
+
+// Second CDF pass (after k_scan_blocks), one 4096-amplitude chunk per
+// workgroup: lane t sums |a|^2 over its 16 consecutive amplitudes, a
+// workgroup inclusive scan gives each lane's prefix `before`, and the
+// normalised CDF is ((before + v) + offset) / total.  Global loads and
 // stores are coalesced through LDS (element e at e + e / 16: lane t's 16
 // elements then fall in distinct banks).
-template <typename T, bool WRITE>
+template <typename T>
 __global__ void __launch_bounds__(kThreads) k_cdf_pass(const typename VT<T>::c *sv, int64_t dim, double *cdf,
-                                                       double *blk, int64_t nb) {
+                                                       const double *blk, int64_t nb) {
     constexpr int per = kScanChunk / kThreads;
     __shared__ double ts[kThreads];
     __shared__ double buf[kScanChunk + kScanChunk / per];
@@ -1930,20 +1966,16 @@ __global__ void __launch_bounds__(kThreads) k_cdf_pass(const typename VT<T>::c *
         v[j] = run;
     }
     const double incl = block_inclusive_scan(run, ts);
-    if constexpr (WRITE) {
-        const double before = incl - run;
-        const double off = blk[blockIdx.x], total = blk[nb];
-        __syncthreads();  // every lane has read its elements
+    const double before = incl - run;
+    const double off = blk[blockIdx.x], total = blk[nb];
+    __syncthreads();  // every lane has read its elements
 #pragma unroll
-        for (int j = 0; j < per; ++j) buf[at(t * per + j)] = ((before + v[j]) + off) / total;
-        __syncthreads();
+    for (int j = 0; j < per; ++j) buf[at(t * per + j)] = ((before + v[j]) + off) / total;
+    __syncthreads();
 #pragma unroll
-        for (int k = 0; k < per; ++k) {
-            const int e = k * kThreads + t;
-            if (c0 + e < dim) cdf[c0 + e] = buf[at(e)];
-        }
-    } else {
-        if (threadIdx.x == blockDim.x - 1) blk[blockIdx.x] = incl;
+    for (int k = 0; k < per; ++k) {
+        const int e = k * kThreads + t;
+        if (c0 + e < dim) cdf[c0 + e] = buf[at(e)];
     }
 }
 
@@ -1983,10 +2015,10 @@ template <typename T>
 hipError_t launch_cdf(const void *sv, int64_t dim, double *cdf, double *blk, hipStream_t s) {
     const int64_t nb = (dim + kScanChunk - 1) / kScanChunk;
     if (nb > 0x7fffffffll) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_cdf_pass<T, false>), dim3((unsigned)nb), dim3(kThreads), 0, s,
-                       (const typename VT<T>::c *)sv, dim, cdf, blk, nb);
+    hipLaunchKernelGGL((k_cdf_totals<T>), dim3((unsigned)nb), dim3(kThreads), 0, s, (const typename VT<T>::c *)sv,
+                       dim, blk);
     hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(kThreads), 0, s, blk, nb);
-    hipLaunchKernelGGL((k_cdf_pass<T, true>), dim3((unsigned)nb), dim3(kThreads), 0, s,
+    hipLaunchKernelGGL((k_cdf_pass<T>), dim3((unsigned)nb), dim3(kThreads), 0, s,
                        (const typename VT<T>::c *)sv, dim, cdf, blk, nb);
     return hipGetLastError();
 }
