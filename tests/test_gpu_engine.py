@@ -344,6 +344,25 @@ def test_readout_measure_and_normalisation(rng):
     assert sim.memory_mb() == (1 << n) * 16 / 1e6
 
 
+@pytest.mark.parametrize("n", [1, 2, 7, 13, 20])
+@pytest.mark.parametrize("complex128", [False, True])
+def test_norm_and_qubit_populations(n, complex128, rng):
+    """FP64 norm and per-qubit populations of a state (complex64: 16-byte
+    loads, a partial last wave below 2^9 amplitudes) against NumPy sums of the
+    same amplitudes."""
+    sv = (rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)).astype(np.complex128 if complex128 else np.complex64)
+    h = desv.StateHandle(n, complex128)
+    h.upload(sv)
+    p = np.abs(sv.astype(np.complex128)) ** 2
+    assert abs(h.norm2() - p.sum()) < 1e-12 * p.sum()
+    idx = np.arange(1 << n)
+    for q in sorted({0, n // 2, n - 1}):
+        bits = (idx >> (n - 1 - q)) & 1
+        p0, p1 = h.qubit_probs(q)
+        assert abs(p0 - p[bits == 0].sum()) < 1e-12 * p.sum() and abs(p1 - p[bits == 1].sum()) < 1e-12 * p.sum()
+    h.close()
+
+
 def test_errors_match_reference_messages():
     sim = desv.DenseSVSimulator(3)
     with pytest.raises(ValueError, match="unknown gate"):
@@ -421,7 +440,8 @@ def test_program_autotune_keeps_parity(rng, tmp_path, monkeypatch):
     assert desv.tune.autotune_program(prog, "complex64", candidates=cands, signature="brick14")["cached"]
 
 
-@pytest.mark.parametrize("n,dtype", [(12, np.complex128), (22, np.complex64), (3, np.complex128)])
+@pytest.mark.parametrize("n,dtype", [(12, np.complex128), (22, np.complex64), (3, np.complex128), (9, np.complex64),
+                                     (16, np.complex128)])
 def test_device_sampling_matches_rng_choice(n, dtype, rng):
     """Shots drawn on the GPU from the caller's generator equal the
     reference's rng.choice(dim, n_shots, p=probs) with the same seed (the FP64
