@@ -31,7 +31,7 @@ enum {
     DESV_EINVAL = 1,      /* bad argument: maps to ValueError             */
     DESV_ENOMEM = 2,      /* device or host allocation failed: MemoryError */
     DESV_EDEVICE = 3,     /* HIP runtime error: RuntimeError               */
-    DESV_EUNSUPPORTED = 4 /* valid request this build does not implement   */
+    DESV_EUNSUPPORTED = 4 /* valid request the library does not implement  */
 };
 
 enum { DESV_C64 = 0, DESV_C128 = 1 };
