@@ -2275,10 +2275,52 @@ hipError_t launch_norm2(const void *sv, int64_t dim, double *partials, int nblk,
                            partials);
     return hipGetLastError();
 }
+// Complex64 form of k_inner: 16-byte loads of both vectors (a float4 is
+// amplitudes 2f and 2f + 1), 4 contiguous 64-float4 runs per wave and
+// iteration; the FP64 terms per amplitude are those of the generic kernel.
+__global__ void __launch_bounds__(kThreads) k_inner_c64(const float4 *a, const float4 *b, int64_t nf4,
+                                                        double *partials) {
+    constexpr int GPT = 4;
+    __shared__ double red[kThreads / 64];
+    const int lane = threadIdx.x & 63;
+    const int64_t wave0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    double re = 0, im = 0;
+    for (int64_t w = wave0; w * (GPT * 64) < nf4; w += nwaves) {
+        float4 x[GPT], y[GPT];
+#pragma unroll
+        for (int j = 0; j < GPT; ++j) {
+            const int64_t f = (w * GPT + j) * 64 + lane;
+            const bool on = f < nf4;
+            x[j] = on ? a[f] : make_float4(0.f, 0.f, 0.f, 0.f);
+            y[j] = on ? b[f] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (int j = 0; j < GPT; ++j) {
+            const double2 x0 = to_d(make_float2(x[j].x, x[j].y)), x1 = to_d(make_float2(x[j].z, x[j].w));
+            const double2 y0 = to_d(make_float2(y[j].x, y[j].y)), y1 = to_d(make_float2(y[j].z, y[j].w));
+            re += x0.x * y0.x + x0.y * y0.y;
+            im += x0.x * y0.y - x0.y * y0.x;
+            re += x1.x * y1.x + x1.y * y1.y;
+            im += x1.x * y1.y - x1.y * y1.x;
+        }
+    }
+    const double sr = block_sum(re, red);
+    const double si = block_sum(im, red);
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = sr;
+        partials[gridDim.x + blockIdx.x] = si;
+    }
+}
+
 template <typename T>
 hipError_t launch_inner(const void *a, const void *b, int64_t dim, double *partials, int nblk, hipStream_t s) {
-    hipLaunchKernelGGL((k_inner<T>), dim3(nblk), dim3(kThreads), 0, s, (const typename VT<T>::c *)a,
-                       (const typename VT<T>::c *)b, dim, partials);
+    if (sizeof(T) == 4 && dim >= 2)
+        hipLaunchKernelGGL(k_inner_c64, dim3(nblk), dim3(kThreads), 0, s, (const float4 *)a, (const float4 *)b,
+                           dim / 2, partials);
+    else
+        hipLaunchKernelGGL((k_inner<T>), dim3(nblk), dim3(kThreads), 0, s, (const typename VT<T>::c *)a,
+                           (const typename VT<T>::c *)b, dim, partials);
     return hipGetLastError();
 }
 template <typename T>

@@ -360,6 +360,13 @@ def test_norm_and_qubit_populations(n, complex128, rng):
         bits = (idx >> (n - 1 - q)) & 1
         p0, p1 = h.qubit_probs(q)
         assert abs(p0 - p[bits == 0].sum()) < 1e-12 * p.sum() and abs(p1 - p[bits == 1].sum()) < 1e-12 * p.sum()
+    # <sv|other> in FP64 from the stored amplitudes
+    other = (rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)).astype(sv.dtype)
+    g = desv.StateHandle(n, complex128)
+    g.upload(other)
+    want = np.vdot(sv.astype(np.complex128), other.astype(np.complex128))
+    assert abs(h.inner(g) - want) < 1e-12 * np.sqrt(p.sum() * (np.abs(other.astype(np.complex128)) ** 2).sum())
+    g.close()
     h.close()
 
 
