@@ -200,9 +200,9 @@ int run_launches(desv_batch b, const std::vector<Launch> &ls) {
                       : launch_fill_tasks<double>((const FillTask *)l.tasks, l.ntasks, l.grid, b->stream);
                 break;
             case 1:
-                e = f ? launch_stage<float>(l.nv, l.heavy, (const StageTask *)l.tasks, l.wg_task, dummy, l.grid,
+                e = f ? launch_stage<float>(l.nv, l.heavy, false, (const StageTask *)l.tasks, l.wg_task, dummy, l.grid,
                                             l.max_m, b->stream)
-                      : launch_stage<double>(l.nv, l.heavy, (const StageTask *)l.tasks, l.wg_task, dummy, l.grid,
+                      : launch_stage<double>(l.nv, l.heavy, false, (const StageTask *)l.tasks, l.wg_task, dummy, l.grid,
                                              l.max_m, b->stream);
                 break;
             case 2:

@@ -47,6 +47,10 @@ struct DevPlan {
     size_t cap = 0;
     DevOp *ops = nullptr;
     void *mats = nullptr;
+    // compensated (complex64): mats holds the FP32 hi pool followed by the lo
+    // pool, lo_off complex elements further on
+    int comp = 0;
+    int32_t lo_off = 0;
 
     void release() {
         if (buf) (void)hipFree(buf);
@@ -55,10 +59,12 @@ struct DevPlan {
     }
 };
 
-int upload_plan(const Plan &p, int dtype, int device, hipStream_t s, DevPlan &d) {
+int upload_plan(const Plan &p, int dtype, int device, hipStream_t s, DevPlan &d, bool comp = false) {
+    comp = comp && dtype == DESV_C64;
+    if (comp && p.mats.size() > (size_t)0x3fffffff) return fail(DESV_EINVAL, "matrix pool too large");
     const size_t ob = std::max<size_t>(p.ops.size(), 1) * sizeof(DevOp);
     const size_t oba = (ob + 255) & ~size_t(255);
-    const size_t mb = std::max<size_t>(p.mats.size(), 1) * amp_bytes(dtype);
+    const size_t mb = std::max<size_t>(p.mats.size(), 1) * amp_bytes(dtype) * (comp ? 2 : 1);
     const size_t need = oba + mb;
     HIP_TRY(hipStreamSynchronize(s));  // the previous upload may still be read
     if (d.cap < need || d.device != device) {
@@ -69,7 +75,9 @@ int upload_plan(const Plan &p, int dtype, int device, hipStream_t s, DevPlan &d)
     std::vector<unsigned char> host(need);
     std::memcpy(host.data(), p.ops.data(), p.ops.size() * sizeof(DevOp));
     if (dtype == DESV_C64) {
-        round_mats_f32(p, reinterpret_cast<float *>(host.data() + oba));
+        float *f = reinterpret_cast<float *>(host.data() + oba);
+        round_mats_f32(p, f);
+        if (comp) round_mats_f32_lo(p, f, f + 2 * p.mats.size());
     } else {
         double *f = reinterpret_cast<double *>(host.data() + oba);
         for (size_t i = 0; i < p.mats.size(); ++i) {
@@ -83,6 +91,8 @@ int upload_plan(const Plan &p, int dtype, int device, hipStream_t s, DevPlan &d)
     d.mats = reinterpret_cast<unsigned char *>(d.buf) + oba;
     d.device = device;
     d.dtype = dtype;
+    d.comp = comp;
+    d.lo_off = comp ? (int32_t)p.mats.size() : 0;
     return DESV_OK;
 }
 
@@ -118,6 +128,9 @@ struct desv_state {
     // alt, swap): allocated on first use unless double buffering is off.
     void *alt = nullptr;
     int double_buffer = 1;
+    // complex64: gate tables of desv_run_ops / desv_apply_matrix are applied
+    // with compensated (hi + lo) matrices
+    int compensated = 0;
     hipStream_t own = nullptr;
     hipStream_t stream = nullptr;
     double *d_part = nullptr;
@@ -247,7 +260,8 @@ int run_plan_on(desv_handle h, const Plan &plan, const DevPlan &dp, int nv, void
         if (evs) HIP_TRY(hipEventRecord(evs[si], h->stream));
         ++si;
         DirectOp dop;
-        if (nv == 1 && direct_op(plan, st, dop)) {
+        // compensated programs run every stage on the stage kernel
+        if (nv == 1 && !dp.comp && direct_op(plan, st, dop)) {
             const size_t eb = h->dtype == DESV_C64 ? 8 : 16;
             const void *M = reinterpret_cast<const char *>(dp.mats) + (size_t)plan.ops[st.op_begin].mat * eb;
             void *out = own_state ? ensure_alt(h) : nullptr;
@@ -270,12 +284,13 @@ int run_plan_on(desv_handle h, const Plan &plan, const DevPlan &dp, int nv, void
         t.sv1 = sv1;
         t.ops = dp.ops;
         t.mats = dp.mats;
+        t.lo_off = dp.lo_off;
         t.partials = h->d_part;
         t.pstride = pstride;
         t.wg_begin = 0;
         hipError_t e = h->dtype == DESV_C64
-                           ? launch_stage<float>(nv, st.heavy, nullptr, nullptr, t, st.n_tiles, st.m, h->stream)
-                           : launch_stage<double>(nv, st.heavy, nullptr, nullptr, t, st.n_tiles, st.m, h->stream);
+                           ? launch_stage<float>(nv, st.heavy, dp.comp, nullptr, nullptr, t, st.n_tiles, st.m, h->stream)
+                           : launch_stage<double>(nv, st.heavy, false, nullptr, nullptr, t, st.n_tiles, st.m, h->stream);
         if (e != hipSuccess) return fail(DESV_EDEVICE, std::string("stage launch: ") + hipGetErrorString(e));
     }
     if (evs) HIP_TRY(hipEventRecord(evs[si], h->stream));
@@ -298,7 +313,7 @@ int apply_gates(desv_handle h, const std::vector<Gate> &gates) {
     Plan plan;
     std::string err;
     if (!build_plan(gates, h->n, o, plan, err)) return fail(DESV_EINVAL, err);
-    int rc = upload_plan(plan, h->dtype, h->device, h->stream, h->scratch_plan);
+    int rc = upload_plan(plan, h->dtype, h->device, h->stream, h->scratch_plan, h->compensated != 0);
     if (rc) return rc;
     return run_plan_on(h, plan, h->scratch_plan, 1, h->sv, nullptr, 0);
 }
@@ -413,7 +428,7 @@ extern "C" {
 
 const char *desv_last_error(void) { return g_err.c_str(); }
 
-int desv_version(void) { return 1; }
+int desv_version(void) { return 2; }
 
 int desv_device_count(int *out) {
     int c = 0;
@@ -496,6 +511,12 @@ int desv_set_double_buffer(desv_handle h, int enable) {
         HIP_TRY(hipFree(h->alt));
         h->alt = nullptr;
     }
+    return DESV_OK;
+}
+
+int desv_set_compensated(desv_handle h, int enable) {
+    if (int rc = check_handle(h)) return rc;
+    h->compensated = enable && h->dtype == DESV_C64 ? 1 : 0;
     return DESV_OK;
 }
 
@@ -703,6 +724,7 @@ int desv_default_opts(int dtype, desv_compile_opts *opts) {
     opts->per_gate = o.per_gate;
     opts->use_mfma = o.use_mfma;
     opts->dtype = dtype;
+    opts->compensated = o.compensated;
     return DESV_OK;
 }
 
@@ -759,6 +781,7 @@ int desv_program_compile(desv_program p, const desv_compile_opts *opts) {
         // FP64 MFMA has the same peak as FP64 VALU on MI355X, and its 16x16x4
         // path did not match the oracle: complex128 blocks stay on VALU.
         o.use_mfma = dtype == DESV_C64 ? opts->use_mfma : 0;
+        o.compensated = dtype == DESV_C64 ? opts->compensated : 0;
     }
     if (o.fuse_bits < 1 || o.fuse_bits > 5) return fail(DESV_EINVAL, "fuse_bits must be in [1, 5]");
     const size_t lim = stage_lds_limit(dev);
@@ -836,7 +859,7 @@ int prepare_program(desv_handle h, desv_program p) {
         if (rc) return rc;
     }
     if (p->dev.dtype != h->dtype || p->dev.device != h->device) {
-        int rc = upload_plan(p->plan, h->dtype, h->device, h->stream, p->dev);
+        int rc = upload_plan(p->plan, h->dtype, h->device, h->stream, p->dev, p->opts.compensated != 0);
         if (rc) return rc;
     }
     return DESV_OK;

@@ -133,6 +133,7 @@ struct CompileOpts {
     int low_bits = 5;
     int per_gate = 0;
     int use_mfma = 0;
+    int compensated = 0;   // complex64: matrices as FP32 hi + lo pairs, lo applied first
     int nv = 1;            // vectors per tile (2 for the adjoint reverse sweep)
     int remap = 1;         // let stages permute which qubits sit on the low bits
     int window = 4096;     // gates scanned ahead when choosing a tile
@@ -151,5 +152,8 @@ void refresh_plan_mats(Plan &plan, const std::vector<Gate> &gates);
 // plan.mats rounded to complex64 (interleaved floats, 2 per entry), round to
 // nearest (products were formed in double).
 void round_mats_f32(const Plan &p, float *out);
+// lo[i] = the FP32 rounding of plan.mats[i] - hi[i] (hi from round_mats_f32):
+// hi + lo carries each entry to about 2^-48 relative.
+void round_mats_f32_lo(const Plan &p, const float *hi, float *lo);
 
 }  // namespace desv

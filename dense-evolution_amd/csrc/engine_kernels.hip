@@ -152,9 +152,19 @@ struct Groups {
 // Tile micro-ops.  `lds` = tile 0; tile v is at lds + v * vstride.
 // ---------------------------------------------------------------------------
 
+// Compensated ops (CP = true, complex64 programs): the exact matrix is the
+// FP32 pair M + ML (ML = the FP32 rounding of the FP64 entries minus M).  Each
+// output is accumulated lo part first, so the ML products (about an ulp of the
+// result) enter the accumulator before the M products are added; every later
+// FMA rounds the exact sum of an unrounded product and the accumulator, so the
+// result is M + ML applied with unbiased rounding.  Adding the ML products
+// after the M products (or their sum to the rounded M result) would round them
+// away, and the rounding error of M would then repeat every time the same
+// matrix is applied (a norm drift linear in the number of applications).
+
 // Dense 2^K x 2^K matrix, K = 1, 2, on tile bits pos[] with local controls.
-template <typename T, int NV, int K, typename MP>
-__device__ inline void op_dense_lo(char *lds, int vstride, int m, const DevOp &op, MP M) {
+template <typename T, int NV, int K, bool CP, typename MP>
+__device__ inline void op_dense_lo(char *lds, int vstride, int m, const DevOp &op, MP M, MP ML) {
     using C = typename VT<T>::c;
     constexpr int D = 1 << K;
     constexpr int EB = sizeof(C);
@@ -172,13 +182,26 @@ __device__ inline void op_dense_lo(char *lds, int vstride, int m, const DevOp &o
     }
     const Groups g(m, fixed, EB);
     if (g.idle()) return;
-    C mm[D * D];
+    C mm[D * D], ml[CP ? D * D : 1];
 #pragma unroll
     for (int i = 0; i < D * D; ++i) mm[i] = gload(M + i);
-    auto row = [&](const C *a, int r) {
-        C acc = cmul(mm[r * D], a[0]);
+    if constexpr (CP) {
 #pragma unroll
-        for (int c = 1; c < D; ++c) acc = cfma(mm[r * D + c], a[c], acc);
+        for (int i = 0; i < D * D; ++i) ml[i] = gload(ML + i);
+    }
+    auto row = [&](const C *a, int r) {
+        C acc;
+        if constexpr (CP) {
+            acc = cmul(ml[r * D], a[0]);
+#pragma unroll
+            for (int c = 1; c < D; ++c) acc = cfma(ml[r * D + c], a[c], acc);
+#pragma unroll
+            for (int c = 0; c < D; ++c) acc = cfma(mm[r * D + c], a[c], acc);
+        } else {
+            acc = cmul(mm[r * D], a[0]);
+#pragma unroll
+            for (int c = 1; c < D; ++c) acc = cfma(mm[r * D + c], a[c], acc);
+        }
         return acc;
     };
 #pragma unroll
@@ -212,8 +235,8 @@ __device__ inline void op_dense_lo(char *lds, int vstride, int m, const DevOp &o
 
 // Dense 2^K x 2^K block on VALU (K = 3..5): one group's amplitudes in
 // registers, rows formed one at a time from the (scalar) matrix row.
-template <typename T, int NV, int K, typename MP>
-__device__ inline void op_densek(char *lds, int vstride, int m, const DevOp &op, MP M) {
+template <typename T, int NV, int K, bool CP, typename MP>
+__device__ inline void op_densek(char *lds, int vstride, int m, const DevOp &op, MP M, MP ML) {
     using C = typename VT<T>::c;
     constexpr int D = 1 << K;
     constexpr int EB = sizeof(C);
@@ -242,9 +265,19 @@ __device__ inline void op_densek(char *lds, int vstride, int m, const DevOp &op,
 #pragma unroll 1
             for (int r = 0; r < D; ++r) {
                 const MP Mr = M + r * D;
-                C acc = cmul(gload(Mr), a[0]);
+                C acc;
+                if constexpr (CP) {
+                    const MP Lr = ML + r * D;
+                    acc = cmul(gload(Lr), a[0]);
 #pragma unroll
-                for (int c = 1; c < D; ++c) acc = cfma(gload(Mr + c), a[c], acc);
+                    for (int c = 1; c < D; ++c) acc = cfma(gload(Lr + c), a[c], acc);
+#pragma unroll
+                    for (int c = 0; c < D; ++c) acc = cfma(gload(Mr + c), a[c], acc);
+                } else {
+                    acc = cmul(gload(Mr), a[0]);
+#pragma unroll
+                    for (int c = 1; c < D; ++c) acc = cfma(gload(Mr + c), a[c], acc);
+                }
                 lds_st<C>(tb, A ^ roff(r), acc);
             }
         }
@@ -290,8 +323,8 @@ struct BlockSlots {
     }
 };
 
-template <int NV, int K, typename MP>
-__device__ inline void op_mfma(char *lds, int vstride, int m, const DevOp &op, MP M) {
+template <int NV, int K, bool CP, typename MP>
+__device__ inline void op_mfma(char *lds, int vstride, int m, const DevOp &op, MP M, MP ML) {
     constexpr int D = 1 << K;
     constexpr int RB = D / 16;  // 16-row blocks of U
     constexpr int KS = D / 4;   // 4-wide k slices
@@ -320,6 +353,19 @@ __device__ inline void op_mfma(char *lds, int vstride, int m, const DevOp &op, M
             ai[rb][ks] = u.y;
             nai[rb][ks] = -u.y;  // Yr = Ur Xr + (-Ui) Xi: the sign lives on the constant operand
         }
+    // lo parts of the compensated form (accumulated first)
+    float lr[CP ? RB : 1][CP ? KS : 1], li[CP ? RB : 1][CP ? KS : 1], nli[CP ? RB : 1][CP ? KS : 1];
+    if constexpr (CP) {
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) {
+                const float2 u = gload(ML + (rb * 16 + i16) * D + ks * 4 + k4);
+                lr[rb][ks] = u.x;
+                li[rb][ks] = u.y;
+                nli[rb][ks] = -u.y;
+            }
+    }
     // per-lane byte slots: this lane's column (within a 16-column block) and
     // the rows it reads (B) and writes (accumulator)
     const int pl = spread(i16, fixed) | cl;
@@ -359,6 +405,15 @@ __device__ inline void op_mfma(char *lds, int vstride, int m, const DevOp &op, M
                 for (int rb = 0; rb < RB; ++rb) {
                     yr[v][rb] = f32x4_t{0.f, 0.f, 0.f, 0.f};
                     yi[v][rb] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+                    if constexpr (CP) {
+#pragma unroll
+                        for (int ks = 0; ks < KS; ++ks) {
+                            yr[v][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(lr[rb][ks], br[v][ks], yr[v][rb], 0, 0, 0);
+                            yi[v][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(lr[rb][ks], bi[v][ks], yi[v][rb], 0, 0, 0);
+                            yr[v][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(nli[rb][ks], bi[v][ks], yr[v][rb], 0, 0, 0);
+                            yi[v][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(li[rb][ks], br[v][ks], yi[v][rb], 0, 0, 0);
+                        }
+                    }
 #pragma unroll
                     for (int ks = 0; ks < KS; ++ks) {
                         yr[v][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[rb][ks], br[v][ks], yr[v][rb], 0, 0, 0);
@@ -422,6 +477,17 @@ __device__ inline void op_mfma(char *lds, int vstride, int m, const DevOp &op, M
                     yr[u][rb] = f32x4_t{0.f, 0.f, 0.f, 0.f};
                     yi[u][rb] = f32x4_t{0.f, 0.f, 0.f, 0.f};
                 }
+                if constexpr (CP) {
+#pragma unroll
+                    for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+                        for (int u = 0; u < UN; ++u) {
+                            yr[u][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(lr[rb][ks], br[u][ks], yr[u][rb], 0, 0, 0);
+                            yi[u][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(lr[rb][ks], bi[u][ks], yi[u][rb], 0, 0, 0);
+                            yr[u][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(nli[rb][ks], bi[u][ks], yr[u][rb], 0, 0, 0);
+                            yi[u][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(li[rb][ks], br[u][ks], yi[u][rb], 0, 0, 0);
+                        }
+                }
 #pragma unroll
                 for (int ks = 0; ks < KS; ++ks)
 #pragma unroll
@@ -455,8 +521,8 @@ __device__ inline void op_mfma(char *lds, int vstride, int m, const DevOp &op, M
 // parts for ks < 2 and imaginary parts for ks >= 2; accumulator lane l, reg j
 // -> R row 4 (l/16) + j, so lane group g holds outputs 2g and 2g+1 as
 // (re, im) register pairs.
-template <int NV, typename MP>
-__device__ inline void op_mfma3(char *lds, int vstride, int m, const DevOp &op, MP M) {
+template <int NV, bool CP, typename MP>
+__device__ inline void op_mfma3(char *lds, int vstride, int m, const DevOp &op, MP M, MP ML) {
     constexpr int EB = sizeof(float2);
     const int lane = threadIdx.x & 63;
     const int i16 = lane & 15, g4 = lane >> 4;
@@ -471,6 +537,31 @@ __device__ inline void op_mfma3(char *lds, int vstride, int m, const DevOp &op, 
         const float2 u = gload(M + (i16 >> 1) * 8 + (kk & 7));
         a[ks] = kk < 8 ? ((i16 & 1) ? u.y : u.x) : ((i16 & 1) ? u.x : -u.y);
     }
+    float la[4];  // lo parts of the compensated form (accumulated first)
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+        la[ks] = 0.f;
+        if constexpr (CP) {
+            const int kk = 4 * ks + g4;
+            const float2 u = gload(ML + (i16 >> 1) * 8 + (kk & 7));
+            la[ks] = kk < 8 ? ((i16 & 1) ? u.y : u.x) : ((i16 & 1) ? u.x : -u.y);
+        }
+    }
+    // y = R [Xr; Xi] (+ the lo part first when compensated) for one column block
+    auto mv = [&](float2 x0, float2 x1) {
+        f32x4_t y = f32x4_t{0.f, 0.f, 0.f, 0.f};
+        if constexpr (CP) {
+            y = __builtin_amdgcn_mfma_f32_16x16x4f32(la[0], x0.x, y, 0, 0, 0);
+            y = __builtin_amdgcn_mfma_f32_16x16x4f32(la[1], x1.x, y, 0, 0, 0);
+            y = __builtin_amdgcn_mfma_f32_16x16x4f32(la[2], x0.y, y, 0, 0, 0);
+            y = __builtin_amdgcn_mfma_f32_16x16x4f32(la[3], x1.y, y, 0, 0, 0);
+        }
+        y = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0], x0.x, y, 0, 0, 0);
+        y = __builtin_amdgcn_mfma_f32_16x16x4f32(a[1], x1.x, y, 0, 0, 0);
+        y = __builtin_amdgcn_mfma_f32_16x16x4f32(a[2], x0.y, y, 0, 0, 0);
+        y = __builtin_amdgcn_mfma_f32_16x16x4f32(a[3], x1.y, y, 0, 0, 0);
+        return y;
+    };
     const int pl = spread(i16, fixed) | cl;
     const int pb0 = slot(pl | roff(g4)) * EB, pb1 = slot(pl | roff(4 + g4)) * EB;
     const int py0 = slot(pl | roff(2 * g4)) * EB, py1 = slot(pl | roff(2 * g4 + 1)) * EB;
@@ -493,12 +584,7 @@ __device__ inline void op_mfma3(char *lds, int vstride, int m, const DevOp &op, 
             const int Un = more ? bs(it + 1) : Uc;
             f32x4_t y[NV];
 #pragma unroll
-            for (int v = 0; v < NV; ++v) {
-                y[v] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0], x0[v].x, f32x4_t{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
-                y[v] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[1], x1[v].x, y[v], 0, 0, 0);
-                y[v] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[2], x0[v].y, y[v], 0, 0, 0);
-                y[v] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[3], x1[v].y, y[v], 0, 0, 0);
-            }
+            for (int v = 0; v < NV; ++v) y[v] = mv(x0[v], x1[v]);
             if (more) {
 #pragma unroll
                 for (int v = 0; v < NV; ++v) {
@@ -541,15 +627,7 @@ __device__ inline void op_mfma3(char *lds, int vstride, int m, const DevOp &op, 
             }
             f32x4_t y[UN];
 #pragma unroll
-            for (int u = 0; u < UN; ++u) y[u] = f32x4_t{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int u = 0; u < UN; ++u) y[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0], x0[u].x, y[u], 0, 0, 0);
-#pragma unroll
-            for (int u = 0; u < UN; ++u) y[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[1], x1[u].x, y[u], 0, 0, 0);
-#pragma unroll
-            for (int u = 0; u < UN; ++u) y[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[2], x0[u].y, y[u], 0, 0, 0);
-#pragma unroll
-            for (int u = 0; u < UN; ++u) y[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[3], x1[u].y, y[u], 0, 0, 0);
+            for (int u = 0; u < UN; ++u) y[u] = mv(x0[u], x1[u]);
 #pragma unroll
             for (int u = 0; u < UN; ++u)
                 if (valid && on[u]) {
@@ -564,8 +642,8 @@ __device__ inline void op_mfma3(char *lds, int vstride, int m, const DevOp &op, 
 // -1-pos[j] outside the tile (constant over the tile, taken from `base`).
 // Groups enumerate the tile bits that are neither local targets nor
 // controls; each group holds 2^nl amplitudes with wave-uniform factors.
-template <typename T, int NV, typename MP>
-__device__ inline void op_diag(char *lds, int vstride, int m, const DevOp &op, MP Dg, uint64_t base) {
+template <typename T, int NV, bool CP, typename MP>
+__device__ inline void op_diag(char *lds, int vstride, int m, const DevOp &op, MP Dg, MP DL, uint64_t base) {
     using C = typename VT<T>::c;
     constexpr int EB = sizeof(C);
     int gsub = 0, gmask = 0, lmask = 0;
@@ -595,11 +673,12 @@ __device__ inline void op_diag(char *lds, int vstride, int m, const DevOp &op, M
             for (int q = 0; q < 5; ++q)
                 if (q < op.k && op.pos[q] >= 0 && (d >> q & 1)) o |= 1 << op.pos[q];
             o = slot(o) * EB;
-            const C f = gload(Dg + d);
+            const C f = gload(Dg + d), fl = CP ? gload(DL + d) : f;
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
                 char *tb = lds + v * vstride;
-                lds_st<C>(tb, A ^ o, cmul(f, lds_ld<C>(tb, A ^ o)));
+                const C a = lds_ld<C>(tb, A ^ o);
+                lds_st<C>(tb, A ^ o, CP ? cfma(f, a, cmul(fl, a)) : cmul(f, a));
             }
         }
     }
@@ -608,8 +687,8 @@ __device__ inline void op_diag(char *lds, int vstride, int m, const DevOp &op, M
 // Diagonal op with K = 1, 2 matrix bits: factors and slots of the entries
 // present in this tile (those whose out-of-tile bits match `base`) are held in
 // registers, and two groups are loaded before either is stored.
-template <typename T, int NV, int K, typename MP>
-__device__ inline void op_diag_lo(char *lds, int vstride, int m, const DevOp &op, MP Dg, uint64_t base) {
+template <typename T, int NV, int K, bool CP, typename MP>
+__device__ inline void op_diag_lo(char *lds, int vstride, int m, const DevOp &op, MP Dg, MP DL, uint64_t base) {
     using C = typename VT<T>::c;
     constexpr int D = 1 << K;
     constexpr int EB = sizeof(C);
@@ -629,7 +708,7 @@ __device__ inline void op_diag_lo(char *lds, int vstride, int m, const DevOp &op
     if (g.idle()) return;
     bool on[D];
     int so[D];
-    C f[D];
+    C f[D], fl[CP ? D : 1];
 #pragma unroll
     for (int s = 0; s < D; ++s) {
         on[s] = (s & gmask) == gsub;
@@ -639,7 +718,12 @@ __device__ inline void op_diag_lo(char *lds, int vstride, int m, const DevOp &op
             if (op.pos[j] >= 0 && (s >> j & 1)) o |= 1 << op.pos[j];
         so[s] = slot(o) * EB;
         f[s] = gload(Dg + s);
+        if constexpr (CP) fl[s] = gload(DL + s);
     }
+    auto mul = [&](int s, C a) {
+        if constexpr (CP) return cfma(f[s], a, cmul(fl[s], a));
+        else return cmul(f[s], a);
+    };
     for (int v = 0; v < NV; ++v) {  // not unrolled: one vector's registers at a time
         char *tb = lds + v * vstride;
         int j = 0;
@@ -655,30 +739,30 @@ __device__ inline void op_diag_lo(char *lds, int vstride, int m, const DevOp &op
 #pragma unroll
             for (int s = 0; s < D; ++s)
                 if (on[s]) {
-                    lds_st<C>(tb, A0 ^ so[s], cmul(f[s], a0[s]));
-                    lds_st<C>(tb, A1 ^ so[s], cmul(f[s], a1[s]));
+                    lds_st<C>(tb, A0 ^ so[s], mul(s, a0[s]));
+                    lds_st<C>(tb, A1 ^ so[s], mul(s, a1[s]));
                 }
         }
         if (j < g.J) {
             const int A0 = g.P ^ g.U(j, EB);
 #pragma unroll
             for (int s = 0; s < D; ++s)
-                if (on[s]) lds_st<C>(tb, A0 ^ so[s], cmul(f[s], lds_ld<C>(tb, A0 ^ so[s])));
+                if (on[s]) lds_st<C>(tb, A0 ^ so[s], mul(s, lds_ld<C>(tb, A0 ^ so[s])));
         }
     }
 }
 
 // Global phase on the whole tile (order of elements irrelevant).
-template <typename T, int NV, typename MP>
-__device__ inline void op_scalar(char *lds, int vstride, int m, MP S) {
+template <typename T, int NV, bool CP, typename MP>
+__device__ inline void op_scalar(char *lds, int vstride, int m, MP S, MP SL) {
     using C = typename VT<T>::c;
-    const C f = gload(S);
+    const C f = gload(S), fl = CP ? gload(SL) : f;
     const int size = 1 << m;
     for (int i = threadIdx.x; i < size; i += blockDim.x)
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             C *tb = reinterpret_cast<C *>(lds + v * vstride);
-            tb[i] = cmul(f, tb[i]);
+            tb[i] = CP ? cfma(f, tb[i], cmul(fl, tb[i])) : cmul(f, tb[i]);
         }
 }
 
@@ -880,8 +964,12 @@ __device__ inline double op_grad_block(char *lds, int vstride, int m, const DevO
 //   1  + k = 3..5 VALU blocks;
 //   2  + k = 3, 4 MFMA blocks (complex64; k = 3 VALU blocks kept as well);
 //   3  + k = 5 MFMA blocks (their fragments double the register budget).
-template <typename T, int NV, int VAR>
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(std::is_same<T, float>::value ? 4 : 1))) k_stage(const StageTask *__restrict__ tasks,
+//   CP compensated matrices (lo parts in registers beside the hi parts; with
+//   k = 5 MFMA blocks they need more than the 128 VGPRs of 4 waves per SIMD).
+template <typename T, int NV, int VAR, bool CP>
+__global__ void __launch_bounds__(kThreads)
+__attribute__((amdgpu_waves_per_eu(std::is_same<T, float>::value ? (CP && VAR == 3 ? 2 : 4) : 1)))
+k_stage(const StageTask *__restrict__ tasks,
                                                     const int32_t *__restrict__ wg_task, StageTask single) {
     using C = typename VT<T>::c;
     using V = typename VT<T>::v;
@@ -914,6 +1002,8 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(s
     const int n = uni(tk.n);
     const uint64_t all = (n >= 64) ? ~0ull : ((1ull << n) - 1);
     const DESV_GLOBAL C *mats = gptr(reinterpret_cast<const C *>(tk.mats));
+    // lo halves of a compensated program's matrices: the pool's second half
+    const DESV_GLOBAL C *matl = CP ? mats + uni(tk.lo_off) : mats;
     const DESV_GLOBAL DevOp *ops = gptr(tk.ops);
     const bool permuted = uni(tk.st.permuted) != 0;
     const int n_grad = uni(tk.st.n_grad);
@@ -981,31 +1071,31 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(s
             gi += is_grad;
             continue;
         }
-        const DESV_GLOBAL C *M = mats + op.mat;
+        const DESV_GLOBAL C *M = mats + op.mat, *ML = matl + op.mat;
         switch (op.kind) {
-            case OPK_DENSE1: op_dense_lo<T, NV, 1>(lds, vstride, m, op, M); break;
-            case OPK_DENSE2: op_dense_lo<T, NV, 2>(lds, vstride, m, op, M); break;
+            case OPK_DENSE1: op_dense_lo<T, NV, 1, CP>(lds, vstride, m, op, M, ML); break;
+            case OPK_DENSE2: op_dense_lo<T, NV, 2, CP>(lds, vstride, m, op, M, ML); break;
             case OPK_DENSEK:
                 if constexpr (VAR == 1) {
-                    if (op.k == 3) op_densek<T, NV, 3>(lds, vstride, m, op, M);
-                    else if (op.k == 4) op_densek<T, NV, 4>(lds, vstride, m, op, M);
-                    else op_densek<T, NV, 5>(lds, vstride, m, op, M);
+                    if (op.k == 3) op_densek<T, NV, 3, CP>(lds, vstride, m, op, M, ML);
+                    else if (op.k == 4) op_densek<T, NV, 4, CP>(lds, vstride, m, op, M, ML);
+                    else op_densek<T, NV, 5, CP>(lds, vstride, m, op, M, ML);
                 } else if constexpr (VAR >= 2) {
-                    op_densek<T, NV, 3>(lds, vstride, m, op, M);
+                    op_densek<T, NV, 3, CP>(lds, vstride, m, op, M, ML);
                 }
                 break;
             case OPK_MFMA:
                 if constexpr (VAR >= 2 && std::is_same<T, float>::value) {
-                    if (op.k == 3) op_mfma3<NV>(lds, vstride, m, op, M);
-                    else if (op.k == 4) op_mfma<NV, 4>(lds, vstride, m, op, M);
-                    else if constexpr (VAR == 3) op_mfma<NV, 5>(lds, vstride, m, op, M);
+                    if (op.k == 3) op_mfma3<NV, CP>(lds, vstride, m, op, M, ML);
+                    else if (op.k == 4) op_mfma<NV, 4, CP>(lds, vstride, m, op, M, ML);
+                    else if constexpr (VAR == 3) op_mfma<NV, 5, CP>(lds, vstride, m, op, M, ML);
                 }
                 break;
             case OPK_DIAG:
-                if (op.k == 1) op_diag_lo<T, NV, 1>(lds, vstride, m, op, M, base);
-                else op_diag<T, NV>(lds, vstride, m, op, M, base);
+                if (op.k == 1) op_diag_lo<T, NV, 1, CP>(lds, vstride, m, op, M, ML, base);
+                else op_diag<T, NV, CP>(lds, vstride, m, op, M, ML, base);
                 break;
-            case OPK_SCALAR: op_scalar<T, NV>(lds, vstride, m, M); break;
+            case OPK_SCALAR: op_scalar<T, NV, CP>(lds, vstride, m, M, ML); break;
             case OPK_GRAD:
             case OPK_GRADDIAG:
                 if constexpr (NV == 2) {
@@ -1060,8 +1150,8 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(s
 }
 
 template <typename T>
-hipError_t launch_stage(int nv, int variant, const StageTask *tasks, const int32_t *wg_task, const StageTask &single,
-                        int64_t grid, int max_m, hipStream_t s) {
+hipError_t launch_stage(int nv, int variant, bool comp, const StageTask *tasks, const int32_t *wg_task,
+                        const StageTask &single, int64_t grid, int max_m, hipStream_t s) {
     using C = typename VT<T>::c;
     const size_t lds = (size_t)nv * ((size_t)1 << max_m) * sizeof(C);
     if (grid <= 0) return hipSuccess;
@@ -1070,9 +1160,20 @@ hipError_t launch_stage(int nv, int variant, const StageTask *tasks, const int32
     // complex128 never uses MFMA blocks: its variants 2 and 3 would only
     // differ from 1 by omitting k = 4, 5 VALU blocks
     if (!std::is_same<T, float>::value && variant >= 2) variant = 1;
+    // compensated matrices: complex64 forward programs only
+    if (comp && (!std::is_same<T, float>::value || nv != 1)) return hipErrorInvalidValue;
     const dim3 g((unsigned)grid);
 #define DESV_ST(NVV, VV) \
-    hipLaunchKernelGGL((k_stage<T, NVV, VV>), g, dim3(kThreads), lds, s, tasks, wg_task, single)
+    hipLaunchKernelGGL((k_stage<T, NVV, VV, false>), g, dim3(kThreads), lds, s, tasks, wg_task, single)
+#define DESV_STC(VV) \
+    hipLaunchKernelGGL((k_stage<float, 1, VV, true>), g, dim3(kThreads), lds, s, tasks, wg_task, single)
+    if constexpr (std::is_same<T, float>::value) {
+        if (comp) {
+            if (variant == 0) DESV_STC(0); else if (variant == 1) DESV_STC(1);
+            else if (variant == 2) DESV_STC(2); else DESV_STC(3);
+            return hipGetLastError();
+        }
+    }
     if (nv == 1) {
         if (variant == 0) DESV_ST(1, 0); else if (variant == 1) DESV_ST(1, 1);
         else if (variant == 2) DESV_ST(1, 2); else DESV_ST(1, 3);
@@ -1081,6 +1182,7 @@ hipError_t launch_stage(int nv, int variant, const StageTask *tasks, const int32
         else if (variant == 2) DESV_ST(2, 2); else DESV_ST(2, 3);
     }
 #undef DESV_ST
+#undef DESV_STC
     return hipGetLastError();
 }
 
@@ -1720,13 +1822,17 @@ size_t stage_lds_limit(int device) {
     size_t lim = 64 * 1024;
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && v > 0)
         lim = (size_t)v;
-    const void *fns[16] = {
-        (const void *)k_stage<float, 1, 0>,  (const void *)k_stage<float, 2, 0>,  (const void *)k_stage<double, 1, 0>,
-        (const void *)k_stage<double, 2, 0>, (const void *)k_stage<float, 1, 1>,  (const void *)k_stage<float, 2, 1>,
-        (const void *)k_stage<double, 1, 1>, (const void *)k_stage<double, 2, 1>, (const void *)k_stage<float, 1, 2>,
-        (const void *)k_stage<float, 2, 2>,  (const void *)k_stage<double, 1, 2>, (const void *)k_stage<double, 2, 2>,
-        (const void *)k_stage<float, 1, 3>,  (const void *)k_stage<float, 2, 3>,  (const void *)k_stage<double, 1, 3>,
-        (const void *)k_stage<double, 2, 3>};
+    const void *fns[20] = {
+        (const void *)k_stage<float, 1, 0, false>,  (const void *)k_stage<float, 2, 0, false>,
+        (const void *)k_stage<double, 1, 0, false>, (const void *)k_stage<double, 2, 0, false>,
+        (const void *)k_stage<float, 1, 1, false>,  (const void *)k_stage<float, 2, 1, false>,
+        (const void *)k_stage<double, 1, 1, false>, (const void *)k_stage<double, 2, 1, false>,
+        (const void *)k_stage<float, 1, 2, false>,  (const void *)k_stage<float, 2, 2, false>,
+        (const void *)k_stage<double, 1, 2, false>, (const void *)k_stage<double, 2, 2, false>,
+        (const void *)k_stage<float, 1, 3, false>,  (const void *)k_stage<float, 2, 3, false>,
+        (const void *)k_stage<double, 1, 3, false>, (const void *)k_stage<double, 2, 3, false>,
+        (const void *)k_stage<float, 1, 0, true>,   (const void *)k_stage<float, 1, 1, true>,
+        (const void *)k_stage<float, 1, 2, true>,   (const void *)k_stage<float, 1, 3, true>};
     // leave room for the kernels' own (static) LDS arrays, as compiled
     size_t static_lds = 0;
     for (const void *f : fns) {
@@ -2557,8 +2663,8 @@ hipError_t launch_fill_random(void *sv, int64_t dim, uint64_t seed, hipStream_t 
 
 #define DESV_INST(T)                                                                                            \
     template hipError_t launch_direct<T>(const void *, void *, int, const DirectOp &, const void *, hipStream_t);            \
-    template hipError_t launch_stage<T>(int, int, const StageTask *, const int32_t *, const StageTask &, int64_t, int, \
-                                        hipStream_t);                                                          \
+    template hipError_t launch_stage<T>(int, int, bool, const StageTask *, const int32_t *, const StageTask &,  \
+                                        int64_t, int, hipStream_t);                                            \
     template hipError_t launch_matvec_tasks<T>(const MatvecTask *, int, int64_t, hipStream_t);                 \
     template hipError_t launch_fill_tasks<T>(const FillTask *, int, int64_t, hipStream_t);                     \
     template hipError_t launch_fill_basis<T>(void *, int64_t, int64_t, hipStream_t);                          \

@@ -35,7 +35,7 @@ constexpr int kMaxGradLDS = 64;
 struct StageTask {
     DevStage st;
     int32_t n;
-    int32_t pad0;
+    int32_t lo_off;      // compensated launch: offset (complex elements) of the lo matrix pool
     void *sv0;           // psi
     void *sv1;           // lambda (adjoint reverse sweep), else unused
     const DevOp *ops;
@@ -123,9 +123,10 @@ hipError_t launch_direct(const void *in, void *out, int n, const DirectOp &op, c
 // or, if tasks is null, `single`.
 template <typename T>
 // `variant`: DevStage::heavy (0 light, 1 VALU blocks k = 3..5, 2 MFMA blocks
-// k = 3, 4, 3 MFMA blocks up to k = 5).
-hipError_t launch_stage(int nv, int variant, const StageTask *tasks, const int32_t *wg_task, const StageTask &single,
-                        int64_t grid, int max_m, hipStream_t s);
+// k = 3, 4, 3 MFMA blocks up to k = 5).  `comp`: compensated matrices (the lo
+// pool at mats + lo_off; complex64, nv = 1 only).
+hipError_t launch_stage(int nv, int variant, bool comp, const StageTask *tasks, const int32_t *wg_task,
+                        const StageTask &single, int64_t grid, int max_m, hipStream_t s);
 template <typename T>
 hipError_t launch_matvec_tasks(const MatvecTask *tasks, int ntasks, int64_t grid, hipStream_t s);
 template <typename T>

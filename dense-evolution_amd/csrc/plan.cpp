@@ -906,4 +906,11 @@ void round_mats_f32(const Plan &p, float *out) {
     }
 }
 
+void round_mats_f32_lo(const Plan &p, const float *hi, float *lo) {
+    for (size_t i = 0; i < p.mats.size(); ++i) {
+        lo[2 * i] = (float)(p.mats[i].real() - (double)hi[2 * i]);
+        lo[2 * i + 1] = (float)(p.mats[i].imag() - (double)hi[2 * i + 1]);
+    }
+}
+
 }  // namespace desv

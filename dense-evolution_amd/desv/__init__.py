@@ -11,7 +11,7 @@ fails if the library is not built.
 from ._capi import device_count
 from .autodiff import AdjointProgram, circuit_to_energy_fn, parameter_shift_grad
 from .circuits import entangling_layer, pauli_rotation_ops, trotter_evolve_ops
-from .config import ensure_x64, set_precision, with_x64
+from .config import ensure_x64, set_compensation, set_precision, with_x64
 from .gates import GATE_IDS, GATES, PARAMETRIC_GATES
 from .measurement import sample_counts, statevector_fidelity
 from .noise import NoiseModel, NoiseSpec
@@ -29,7 +29,7 @@ from .transpiler import QuantumTranspiler
 __version__ = "0.1.0"
 
 __all__ = [
-    "__version__", "set_precision", "ensure_x64", "with_x64",
+    "__version__", "set_precision", "set_compensation", "ensure_x64", "with_x64",
     "DenseSVSimulator", "DeviceState", "StateHandle", "Program",
     "QASMParser", "QASMCircuit", "QuantumTranspiler",
     "GATES", "PARAMETRIC_GATES", "GATE_IDS",

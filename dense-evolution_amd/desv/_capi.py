@@ -39,7 +39,7 @@ _pi = C.POINTER(C.c_int)
 class CompileOpts(C.Structure):
     """``desv_compile_opts``."""
     _fields_ = [("fuse_bits", _i), ("tile_bits", _i), ("low_bits", _i),
-                ("per_gate", _i), ("use_mfma", _i), ("dtype", _i)]
+                ("per_gate", _i), ("use_mfma", _i), ("dtype", _i), ("compensated", _i)]
 
 
 _SIGS = {
@@ -51,6 +51,7 @@ _SIGS = {
     "desv_set_stream": ([_p, _p], _i),
     "desv_synchronize": ([_p], _i),
     "desv_set_double_buffer": ([_p, _i], _i),
+    "desv_set_compensated": ([_p, _i], _i),
     "desv_reset": ([_p], _i),
     "desv_set_state": ([_p, _p], _i),
     "desv_get_state": ([_p, _p], _i),

@@ -11,6 +11,7 @@ import functools
 
 _explicit = False
 _float64 = True
+_compensated = True
 
 
 def set_precision(float64: bool = True) -> None:
@@ -29,6 +30,19 @@ def ensure_x64() -> None:
 
 def default_float64() -> bool:
     return _float64
+
+
+def set_compensation(enable: bool = True) -> None:
+    """Default gate arithmetic of complex64 simulators: compensated matrices
+    (each FP32 matrix applied as hi + lo, lo part first; on by default) or
+    plain FP32-rounded matrices (the reference's complex64 arithmetic,
+    ``circuits/compiler.py:172-179``).  Has no effect on complex128."""
+    global _compensated
+    _compensated = bool(enable)
+
+
+def default_compensation() -> bool:
+    return _compensated
 
 
 def with_x64(fn):

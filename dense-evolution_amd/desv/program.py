@@ -53,7 +53,12 @@ class Program:
         return self
 
     def compile(self, dtype="complex64", fuse_bits: Optional[int] = None, tile_bits: Optional[int] = None,
-                low_bits: Optional[int] = None, per_gate: bool = False, use_mfma: bool = False) -> "Program":
+                low_bits: Optional[int] = None, per_gate: bool = False, use_mfma: bool = False,
+                compensated: bool = False) -> "Program":
+        """Compile for a state of `dtype`.  ``compensated`` (complex64):
+        matrices uploaded as FP32 hi + lo pairs and applied lo part first, for
+        long programs whose repeated matrices would otherwise accumulate their
+        FP32 rounding coherently; about twice the gate arithmetic."""
         code = _capi.C64 if np.dtype(dtype) == np.complex64 else _capi.C128
         o = _capi.CompileOpts()
         check(lib.desv_default_opts(code, C.byref(o)))
@@ -65,6 +70,7 @@ class Program:
             o.low_bits = int(low_bits)
         o.per_gate = int(bool(per_gate))
         o.use_mfma = int(bool(use_mfma))
+        o.compensated = int(bool(compensated))
         o.dtype = code
         check(lib.desv_program_compile(self.ptr, C.byref(o)))
         return self

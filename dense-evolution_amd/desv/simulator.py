@@ -18,6 +18,7 @@ import numpy as np
 
 from . import _capi
 from ._capi import check, lib
+from .config import default_compensation
 from .gates import GATE_IDS, GATES, PARAMETRIC_GATES, _TWO_QUBIT_PARAMETRIC_GATES
 from .transpiler import QuantumTranspiler
 
@@ -88,6 +89,11 @@ class StateHandle:
 
     def reset(self) -> None:
         check(lib.desv_reset(self.ptr))
+
+    def set_compensated(self, enable: bool) -> None:
+        """complex64: apply gate tables and matrices with compensated (FP32
+        hi + lo) matrices, lo part accumulated first (no-op for complex128)."""
+        check(lib.desv_set_compensated(self.ptr, int(bool(enable))))
 
     def set_double_buffer(self, enable: bool) -> None:
         """Out-of-place streaming passes into a second state-sized buffer
@@ -215,9 +221,15 @@ class DenseSVSimulator:
     n_qubits : int in [1, 34]
     use_float32 : complex64 state if True, complex128 otherwise
     device : HIP device index
+    compensated : complex64 only: apply gates with compensated matrices (each
+        FP32 matrix carried as hi + lo, lo products accumulated first), so the
+        rounding of a repeated matrix does not accumulate coherently over long
+        circuits (Trotter evolutions); None takes the process default
+        (:func:`desv.config.set_compensation`, on unless changed).
     """
 
-    def __init__(self, n_qubits: int, use_float32: bool = False, device: int = 0):
+    def __init__(self, n_qubits: int, use_float32: bool = False, device: int = 0,
+                 compensated: Optional[bool] = None):
         if n_qubits < 1 or n_qubits > 34:
             raise ValueError(f"n_qubits must be in [1, 34], got {n_qubits}")
         self.n = n_qubits
@@ -226,6 +238,10 @@ class DenseSVSimulator:
         self.dtype = np.complex64 if use_float32 else np.complex128
         self.xp = np
         self._h = StateHandle(n_qubits, not use_float32, device)
+        if compensated is None:
+            compensated = default_compensation()
+        self.compensated = bool(compensated) and bool(use_float32)
+        self._h.set_compensated(self.compensated)
 
     # -- state -------------------------------------------------------------
     @property
@@ -416,6 +432,7 @@ class DenseSVSimulator:
                 f"positional column.")
         out = np.empty((parameter_batch.shape[0], self.dim), dtype=self.dtype)
         work = StateHandle(self.n, not self.use_float32, self._h.device)
+        work.set_compensated(self.compensated)
         try:
             for b, row in enumerate(np.asarray(parameter_batch, dtype=np.float64)):
                 ops = template.copy()

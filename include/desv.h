@@ -70,6 +70,12 @@ int desv_synchronize(desv_handle h);
  * hands back a new buffer.  enable = 0 frees the second buffer and updates in
  * place; an allocation failure also falls back to in place. */
 int desv_set_double_buffer(desv_handle h, int enable);
+/* complex64 handles: enable = 1 applies the gate tables of desv_run_ops and
+ * the matrices of desv_apply_matrix with compensated matrices (see
+ * desv_compile_opts.compensated); a no-op for complex128.  No counterpart in
+ * the reference (its complex64 path, use_float32, rounds each gate matrix
+ * once: circuits/compiler.py:172-179). */
+int desv_set_compensated(desv_handle h, int enable);
 /* |0...0> */
 int desv_reset(desv_handle h);
 /* Copy 2^n host amplitudes in the handle's dtype (no normalisation). */
@@ -131,6 +137,14 @@ typedef struct {
     int per_gate;   /* 1: one HBM pass per fused block (no cache blocking)   */
     int use_mfma;   /* 1: 3-5-qubit blocks run as MFMA GEMMs (complex64)     */
     int dtype;      /* DESV_C64 / DESV_C128: matrix precision of the upload */
+    int compensated;/* 1 (complex64): every matrix is uploaded as an FP32 pair
+                       hi + lo (lo = the rounding error of hi) and applied lo
+                       part first, so the FP32 rounding of a matrix does not
+                       act coherently when the same gates repeat (long Trotter
+                       programs); about twice the gate arithmetic.  No
+                       counterpart in the reference, whose complex64 path
+                       rounds each gate matrix once (circuits/compiler.py:
+                       172-179).                                              */
 } desv_compile_opts;
 
 /* Fill `opts` with the defaults for a dtype. */

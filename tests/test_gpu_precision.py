@@ -7,16 +7,15 @@ bond's 21 Trotter gates (built from the oracle's own gate tables, checked
 below against gate-by-gate application), which keeps the 20-qubit reference
 to about 3,800 two-qubit applications on the CPU.
 
-Bars (the errors measured on MI355X are printed): complex128 state fidelity
->= 1 - 1e-12 and <Z_i Z_i+1> within 1e-10.  complex64 (FP32 gate arithmetic,
-FP64 norm and expectation accumulation): the fidelity |<ref|psi>|^2 (the
-reference's statevector_fidelity, unnormalised) is dominated by a slow norm
-loss (about 3e-5 after these 79,800 gates: each step applies the same
-FP32-rounded fused matrices, whose rounding error acts coherently from step to
-step; rounding choices that fix column norms or the Frobenius distance to
-unitarity were tried on the CPU and did not reduce it), so the bars are: norm
-within 1e-4 of 1, fidelity >= 1 - 1e-4, the fidelity of the normalised state
->= 1 - 1e-6, and <Z_i Z_i+1> within 2e-5.
+Bars: complex128 state fidelity >= 1 - 1e-12 and <Z_i Z_i+1> within 1e-10;
+complex64 (FP32 gate arithmetic with compensated matrices, the simulator's
+default; FP64 norm and expectation accumulation) fidelity |<ref|psi>|^2 (the
+reference's statevector_fidelity, unnormalised) >= 1 - 1e-6, |norm - 1| <=
+1e-6 and <Z_i Z_i+1> within 1e-6.  Plain complex64 (each fused matrix rounded
+once to FP32, as the reference's complex64 path rounds its gate matrices)
+loses about 3e-5 of norm over these 79,800 gates, because the same rounded
+matrices act every step and their rounding error adds up coherently; it is
+kept as a strict expected failure of the same bars.
 """
 import numpy as np
 import pytest
@@ -66,24 +65,55 @@ def zz(sv_like, n):
     return np.array([desv.pauli_expectation(sv_like, {i: 'Z', i + 1: 'Z'}) for i in range(n - 1)])
 
 
-def test_xxz_20q_against_complex128_oracle():
-    n = 20
-    ref = oracle_state(n)
-    zz_ref = np.array([O.pauli_expectation(ref, {i: 'Z', i + 1: 'Z'}) for i in range(n - 1)])
-    circ = workloads.xxz_trotter_circuit(n, DT, STEPS, seed=0)
-    res = {}
-    for f32 in (False, True):
-        sim = desv.DenseSVSimulator(n, use_float32=f32)
+_STATES = {}
+
+
+def run_xxz(n, f32, compensated):
+    key = (n, f32, compensated)
+    if key not in _STATES:
+        circ = workloads.xxz_trotter_circuit(n, DT, STEPS, seed=0)
+        sim = desv.DenseSVSimulator(n, use_float32=f32, compensated=compensated)
         sim.run_circuit(circ)
-        sv = sim.get_statevector()
-        nrm = sim.handle.norm2()
-        fd = O.fidelity(sv.astype(np.complex128), ref)
-        res[f32] = (fd, fd / nrm, np.max(np.abs(zz(sim.sv, n) - zz_ref)), nrm - 1.0)
+        _STATES[key] = (sim.get_statevector(), sim.handle.norm2(), zz(sim.sv, n))
         sim.close()
-    for f32, (fd, fdn, dzz, dn) in res.items():
-        print(f"20q XXZ {'c64' if f32 else 'c128'} vs c128 oracle: 1 - fidelity = {1 - fd:.3e}, "
-              f"normalised {1 - fdn:.3e}, max |d<ZZ>| = {dzz:.3e}, norm - 1 = {dn:.3e}")
-    fd, fdn, dzz, dn = res[False]
+    return _STATES[key]
+
+
+_REF = {}
+
+
+def reference(n):
+    if n not in _REF:
+        ref = oracle_state(n)
+        _REF[n] = (ref, np.array([O.pauli_expectation(ref, {i: 'Z', i + 1: 'Z'}) for i in range(n - 1)]))
+    return _REF[n]
+
+
+def errors(n, f32, compensated):
+    ref, zz_ref = reference(n)
+    sv, nrm, zzs = run_xxz(n, f32, compensated)
+    fd = O.fidelity(sv.astype(np.complex128), ref)
+    out = (fd, fd / nrm, float(np.max(np.abs(zzs - zz_ref))), nrm - 1.0)
+    name = 'c128' if not f32 else ('c64 compensated' if compensated else 'c64 plain')
+    print(f"20q XXZ {name} vs c128 oracle: 1 - fidelity = {1 - out[0]:.3e}, normalised {1 - out[1]:.3e}, "
+          f"max |d<ZZ>| = {out[2]:.3e}, norm - 1 = {out[3]:.3e}")
+    return out
+
+
+def test_xxz_20q_complex128_against_oracle():
+    fd, fdn, dzz, dn = errors(20, False, False)
     assert fd >= 1 - 1e-12 and dzz < 1e-10 and abs(dn) < 1e-10
-    fd, fdn, dzz, dn = res[True]
-    assert abs(dn) < 1e-4 and fd >= 1 - 1e-4 and fdn >= 1 - 1e-6 and dzz < 2e-5
+
+
+def test_xxz_20q_complex64_compensated_against_oracle():
+    fd, fdn, dzz, dn = errors(20, True, True)
+    assert fd >= 1 - 1e-6 and abs(dn) <= 1e-6 and dzz <= 1e-6
+
+
+@pytest.mark.xfail(strict=True, reason="plain complex64 (matrices rounded once to FP32) loses about 3e-5 of norm "
+                                       "over 200 Trotter steps: the rounding of the repeated matrices adds up "
+                                       "coherently; the compensated default meets the bars")
+def test_xxz_20q_complex64_plain_against_oracle():
+    fd, fdn, dzz, dn = errors(20, True, False)
+    assert fdn >= 1 - 1e-6  # the direction stays accurate; the norm does not
+    assert fd >= 1 - 1e-6 and abs(dn) <= 1e-6 and dzz <= 1e-6
