@@ -2,9 +2,10 @@
 
 Workload (one step): depth-24 brickwork of Haar-random SU(4) gates on 30
 qubits (348 gates, seed 0), applied to a resident 8 GiB complex64 state.
-The circuit is compiled once (fusion into blocks of up to 4 qubits, 4-qubit
-blocks on MFMA, cache-blocked LDS stages); a step is one replay of the
-compiled program.
+The circuit is compiled once in the launch shape the program autotuner
+measures fastest on this GPU (fusion width 2-5, 3-5-qubit blocks on MFMA or
+VALU, tile of 2^11-2^13 amplitudes; cache-blocked LDS stages); a step is one
+replay of the compiled program.
 
 Printed JSON line: gates/s as `value`; the whole-job HBM rate; the roofline
 of the dominant kernel (algorithmic FLOPs or bytes per launch / its average
@@ -227,7 +228,9 @@ def suite_batch(count=1024, solo_sample=128):
 
 def suite_precision(n=28, steps=200):
     """XXZ chain (J=1, Delta=0.5) first-order Trotter, dt=0.05, 200 steps, from a
-    random product state: complex64 (FP64 norm and <Z_i Z_i+1>) vs complex128."""
+    random product state: complex64 with compensated matrices (the simulator's
+    default) and with plain FP32-rounded matrices, each against complex128 on
+    the GPU (FP64 norm, <Z_i Z_i+1> and fidelity |<c128|c64>|^2)."""
     import numpy as np
     import desv
     from desv import workloads
@@ -235,26 +238,39 @@ def suite_precision(n=28, steps=200):
     # the launch shape is tuned on the first 20 steps (same preparation, same
     # per-step structure), then the whole evolution runs with it
     head = workloads.xxz_trotter_circuit(n, 0.05, min(steps, 20), seed=0)
+    from desv.simulator import encode_op_table
+    from desv.transpiler import QuantumTranspiler
+    ops = encode_op_table(QuantumTranspiler.transpile(circ), n)
+    hops = encode_op_table(QuantumTranspiler.transpile(head), n)
+    bonds = [(1.0, {i: 'Z', i + 1: 'Z'}) for i in range(n - 1)]
     res, tuned = {}, {}
-    for dt_name, f32 in (("c64", True), ("c128", False)):
-        dtype = "complex64" if f32 else "complex128"
-        sim = desv.DenseSVSimulator(n, use_float32=f32)
-        ops = sim.compile_ops(circ)
-        probe = desv.Program(n).add_ops(sim.compile_ops(head))
-        tuned[dt_name] = desv.tune.autotune_program(probe, dtype, reps=1, cache=False)["best"]
+    ref = None
+    for name, dtype, comp in (("c128", "complex128", False), ("c64_compensated", "complex64", True),
+                              ("c64_plain", "complex64", False)):
+        probe = desv.Program(n).add_ops(hops)
+        tuned[name] = desv.tune.autotune_program(probe, dtype, reps=1, cache=False, compensated=comp)["best"]
         probe.close()
-        prog = desv.Program(n).add_ops(ops).compile(dtype, **tuned[dt_name])
-        t, _ = _timed(lambda: prog.run(sim.handle), reps=1)
-        zz = desv.pauli_sum_expectation(sim.sv, [(1.0, {i: 'Z', i + 1: 'Z'}) for i in range(n - 1)])
-        zzs = [desv.pauli_expectation(sim.sv, {i: 'Z', i + 1: 'Z'}) for i in range(n - 1)]
-        res[dt_name] = (t, sim.handle.norm2(), np.array(zzs), prog.stats(), len(ops))
-        sim.close()
-    (t64, n64, z64, st64, g), (t128, n128, z128, st128, _) = res["c64"], res["c128"]
-    return {"workload": f"{n}q XXZ Trotter, {steps} steps, {g} gates",
-            "gates_per_s_c64": g / t64, "gates_per_s_c128": g / t128,
-            "hbm_passes_c64": st64["passes"], "autotuned": tuned,
-            "norm_c64_minus_1": n64 - 1.0, "norm_c128_minus_1": n128 - 1.0,
-            "max_abs_dZZ_c64_vs_c128": float(np.max(np.abs(z64 - z128)))}
+        prog = desv.Program(n).add_ops(ops).compile(dtype, compensated=comp, **tuned[name])
+        h = desv.StateHandle(n, complex128=dtype == "complex128")
+        t, _ = _timed(lambda: prog.run(h), reps=1)
+        zzs = np.array([desv.pauli_expectation(desv.DeviceState(h), t_[1]) for t_ in bonds])
+        r = {"gates_per_s": len(ops) / t, "hbm_passes": prog.stats()["passes"], "norm_minus_1": h.norm2() - 1.0}
+        prog.close()
+        if ref is None:
+            ref, zref = h, zzs
+        else:
+            up = desv.StateHandle(n, complex128=True)
+            up.copy_from(h)
+            h.close()
+            r["fidelity_vs_c128"] = abs(up.inner(ref)) ** 2
+            r["infidelity_vs_c128"] = 1.0 - r["fidelity_vs_c128"]
+            r["max_abs_dZZ_vs_c128"] = float(np.max(np.abs(zzs - zref)))
+            up.close()
+        res[name] = r
+    ref.close()
+    return {"workload": f"{n}q XXZ Trotter, {steps} steps, {len(ops)} gates", "autotuned": tuned,
+            **{f"{k}": v for k, v in res.items()},
+            "compensation_cost_time_ratio": res["c64_plain"]["gates_per_s"] / res["c64_compensated"]["gates_per_s"]}
 
 
 MFMA_FP32_PEAK_TFLOPS = 157.3   # MI355X dense FP32 matrix peak
@@ -298,16 +314,19 @@ def suite_streaming30(n=30, depth=24, steps=2):
             "hbm_frac": gbps / HBM_PEAK_GBPS}
 
 
-def suite_parity30(n=30, depth=24):
-    """Full-size parity: the benchmark circuit in complex64 (as benchmarked)
-    against complex128, fidelity and norms computed on the GPU."""
+def suite_parity30(n=30, depth=24, shape=None):
+    """Full-size parity: the benchmark circuit in complex64, compiled in the
+    launch shape the headline was timed with (`shape`), against complex128;
+    fidelity and norms computed on the GPU."""
     import desv
     from desv import workloads
     gates = workloads.brickwork_su4(n, depth, seed=0)
     prog = desv.Program(n)
     for u, a, b in gates:
         prog.add_matrix(u, [a, b])
-    prog.compile("complex64", fuse_bits=4, use_mfma=True)
+    shape = dict(shape or dict(fuse_bits=4, use_mfma=True))
+    prog.compile("complex64", **shape)
+    plan_stats = prog.stats()
     h64 = desv.StateHandle(n, complex128=False)
     prog.run(h64)
     n64 = h64.norm2()
@@ -318,12 +337,14 @@ def suite_parity30(n=30, depth=24):
     up = desv.StateHandle(n, complex128=True)
     up.copy_from(h64)
     h64.close()
-    fid = abs(up.inner(h128)) ** 2 / (n64 * n128)
+    raw = abs(up.inner(h128)) ** 2
+    fid = raw / (n64 * n128)
     up.close()
     h128.close()
-    return {"workload": f"brickwork {n}q depth {depth}: c64 (fused MFMA blocks) vs c128",
-            "fidelity_c64_vs_c128": fid, "infidelity": 1 - fid, "norm_c64_minus_1": n64 - 1,
-            "norm_c128_minus_1": n128 - 1}
+    return {"workload": f"brickwork {n}q depth {depth}: c64 in the timed launch shape vs c128",
+            "shape": shape, "plan": plan_stats,
+            "fidelity_c64_vs_c128": raw, "infidelity": 1 - raw, "fidelity_normalised": fid,
+            "norm_c64_minus_1": n64 - 1, "norm_c128_minus_1": n128 - 1}
 
 
 def suite_brickwork32(n=32, depth=24):
@@ -701,7 +722,7 @@ def main():
     for name in [x for x in args.suite.split(",") if x]:
         log(f"suite {name} ...")
         t0 = time.perf_counter()
-        extras[name] = fns[name]()
+        extras[name] = fns[name](shape=kw) if name == "parity30" else fns[name]()
         log(f"suite {name}: {time.perf_counter() - t0:.1f} s {extras[name]}")
     if extras:
         out["extras"] = extras

@@ -10,7 +10,7 @@ fails if the library is not built.
 """
 from ._capi import device_count
 from .autodiff import AdjointProgram, circuit_to_energy_fn, parameter_shift_grad
-from .circuits import entangling_layer, pauli_rotation_ops, trotter_evolve_ops
+from .circuits import entangling_layer, pauli_rotation_ops, random_circuit, trotter_evolve_ops
 from .config import ensure_x64, set_compensation, set_precision, with_x64
 from .gates import GATE_IDS, GATES, PARAMETRIC_GATES
 from .measurement import sample_counts, statevector_fidelity
@@ -33,7 +33,7 @@ __all__ = [
     "DenseSVSimulator", "DeviceState", "StateHandle", "Program",
     "QASMParser", "QASMCircuit", "QuantumTranspiler",
     "GATES", "PARAMETRIC_GATES", "GATE_IDS",
-    "entangling_layer", "pauli_rotation_ops", "trotter_evolve_ops",
+    "entangling_layer", "pauli_rotation_ops", "trotter_evolve_ops", "random_circuit",
     "pauli_expectation", "pauli_sum_expectation", "pauli_hamiltonian_to_matrix", "pauli_sum_matvec",
     "multiply_pauli_terms", "pauli_sum_matvec_jax", "pauli_sum_expectation_jax", "PauliSumOperator",
     "circuit_to_energy_fn", "AdjointProgram", "parameter_shift_grad",

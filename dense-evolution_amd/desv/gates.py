@@ -86,9 +86,11 @@ def _crz(theta):
     return np.diag([1, 1, np.exp(-0.5j * theta), np.exp(0.5j * theta)]).astype(np.complex128)
 
 
+#: the reference's key set exactly (no 'phase' / 'cphase' entries: those
+#: names exist only in GATE_IDS, so its eager loop rejects them).
 PARAMETRIC_GATES = {
     'rx': _rx, 'ry': _ry, 'rz': _rz,
-    'cp': _cp, 'cphase': _cp, 'crz': _crz,
-    'u3': _u3, 'u2': _u2, 'u1': _phase, 'p': _phase, 'phase': _phase,
+    'cp': _cp, 'crz': _crz,
+    'u3': _u3, 'u2': _u2, 'u1': _phase, 'p': _phase,
     'gphase': lambda alpha: np.exp(1j * alpha) * np.eye(2, dtype=np.complex128),
 }

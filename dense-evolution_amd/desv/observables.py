@@ -33,14 +33,6 @@ __all__ = [
     'pauli_sum_expectation_jax', 'PauliSumOperator', 'encode_pauli_terms',
 ]
 
-_PAULI = {
-    'I': np.eye(2, dtype=np.complex128),
-    'X': np.array([[0, 1], [1, 0]], dtype=np.complex128),
-    'Y': np.array([[0, -1j], [1j, 0]], dtype=np.complex128),
-    'Z': np.array([[1, 0], [0, -1]], dtype=np.complex128),
-}
-
-
 def _normalize_terms(pauli_terms, n_qubits=None):
     """str | dict | pairs -> {qubit: 'X'|'Y'|'Z'} with identities dropped."""
     if isinstance(pauli_terms, str):
@@ -166,19 +158,30 @@ def pauli_sum_expectation(statevector, terms, n_qubits=None):
 
 
 def pauli_hamiltonian_to_matrix(terms, n_qubits):
-    """Dense H = sum_i c_i kron(P_i) (host matrix builder, qubit 0 first)."""
+    """Dense H = sum_i c_i P_i as a (2^n, 2^n) complex128 host matrix.
+
+    Same matrix as the reference's Kronecker-product builder
+    (``physics/observables.py:189-247``, qubit 0 = most significant index
+    bit), filled column by column from the flip-mask rule instead:
+    P|i> = i^(#Y) (-1)^popcount(i & zmask) |i ^ xmask>, one scatter per term.
+    """
     if n_qubits < 1:
         raise ValueError(f"n_qubits must be >= 1, got {n_qubits}")
     dim = 1 << n_qubits
     H = np.zeros((dim, dim), dtype=np.complex128)
+    cols = np.arange(dim, dtype=np.int64)
     for coeff, p in terms:
         d = _normalize_terms(p, n_qubits)
         if d and max(d) >= n_qubits:
             raise ValueError(f"term references qubit {max(d)}, but n_qubits={n_qubits}")
-        m = np.array([[1.0]], dtype=np.complex128)
-        for q in range(n_qubits):
-            m = np.kron(m, _PAULI[d.get(q, 'I')])
-        H += coeff * m
+        x, z = _masks(d, n_qubits)
+        ny = bin(x & z).count("1")
+        par = np.zeros(dim, dtype=np.int64)
+        zz = cols & z
+        while np.any(zz):
+            par ^= zz & 1
+            zz >>= 1
+        H[cols ^ x, cols] += (coeff * (1j ** ny)) * (1 - 2 * par)
     return H
 
 
@@ -257,28 +260,37 @@ class PauliSumOperator:
                                   dtype=np.complex128)
 
 
-_SAME_QUBIT = {
-    ('X', 'X'): (1, None), ('Y', 'Y'): (1, None), ('Z', 'Z'): (1, None),
-    ('X', 'Y'): (1j, 'Z'), ('Y', 'X'): (-1j, 'Z'),
-    ('Y', 'Z'): (1j, 'X'), ('Z', 'Y'): (-1j, 'X'),
-    ('Z', 'X'): (1j, 'Y'), ('X', 'Z'): (-1j, 'Y'),
-}
+def _letter_bits(letter):
+    return {'X': (1, 0), 'Y': (1, 1), 'Z': (0, 1)}[letter]
 
 
 def multiply_pauli_terms(factors):
-    """Operator product of (coeff, pauli) factors, left to right: (coeff, dict)."""
+    """Operator product of (coeff, pauli) factors, left to right, as
+    (coeff, {qubit: letter}) with identities dropped (the reference's
+    ``physics/observables.py:461-523``).
+
+    Letters are multiplied in the symplectic form sigma(x, z) =
+    i^(x z) X^x Z^z: sigma(x1, z1) sigma(x2, z2) = i^(x1 z1 + x2 z2 - x3 z3 + 2 z1 x2)
+    sigma(x3, z3) with (x3, z3) = (x1 ^ x2, z1 ^ z2), so X Y = i Z, Y X = -i Z
+    and P P = I.
+    """
     merged = {}
+    power = 0          # accumulated phase i^power
     total = 1.0 + 0j
     for coeff, p in factors:
         total *= coeff
         for q, letter in _normalize_terms(p).items():
+            x2, z2 = _letter_bits(letter)
             if q not in merged:
-                merged[q] = letter
+                merged[q] = (x2, z2)
                 continue
-            phase, new = _SAME_QUBIT[(merged[q], letter)]
-            total *= phase
-            if new is None:
-                del merged[q]
+            x1, z1 = merged[q]
+            x3, z3 = x1 ^ x2, z1 ^ z2
+            power += x1 * z1 + x2 * z2 - x3 * z3 + 2 * z1 * x2
+            if x3 or z3:
+                merged[q] = (x3, z3)
             else:
-                merged[q] = new
-    return total, merged
+                del merged[q]
+    total *= 1j ** (power % 4)
+    letters = {(1, 0): 'X', (1, 1): 'Y', (0, 1): 'Z'}
+    return total, {q: letters[b] for q, b in merged.items()}

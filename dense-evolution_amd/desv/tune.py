@@ -136,9 +136,11 @@ def autotune_batch(batch, candidates: Optional[List[Dict]] = None, reps: int = 2
 
 
 def autotune_program(prog, dtype: str = "complex64", candidates: Optional[List[Dict]] = None, reps: int = 2,
-                     cache: bool = True, signature: Optional[str] = None, device: int = 0) -> Dict:
+                     cache: bool = True, signature: Optional[str] = None, device: int = 0,
+                     compensated: bool = False) -> Dict:
     """Compile `prog` with the fastest shape among `candidates` (cached only
-    when `signature` names the workload).
+    when `signature` names the workload).  ``compensated`` compiles every
+    candidate with compensated matrices (complex64).
 
     Timing runs the program on a scratch |0...0> state of the program's width
     (its own allocation; the caller's states are untouched), one untimed run
@@ -146,22 +148,22 @@ def autotune_program(prog, dtype: str = "complex64", candidates: Optional[List[D
     """
     from .simulator import StateHandle
     cache = cache and signature is not None   # a result is cached only under a caller's signature
-    key = f"program/{_device_key(device)}/{np.dtype(dtype).name}/n{prog.n}/{signature}"
+    key = f"program/{_device_key(device)}/{np.dtype(dtype).name}{'-comp' if compensated else ''}/n{prog.n}/{signature}"
     table = _load() if cache else {}
     if key in table:
-        prog.compile(dtype, **table[key]["best"])
+        prog.compile(dtype, compensated=compensated, **table[key]["best"])
         return {**table[key], "cached": True}
     h = StateHandle(prog.n, np.dtype(dtype) == np.complex128, device)
     try:
         def trial(cfg):
-            prog.compile(dtype, **cfg)
+            prog.compile(dtype, compensated=compensated, **cfg)
             prog.run(h)
             return float(sum(prog.run_timed(h).sum() for _ in range(reps)) / reps)
 
         res = _timed_best(candidates or program_candidates(dtype, prog.n), trial)
     finally:
         h.close()
-    prog.compile(dtype, **res["best"])
+    prog.compile(dtype, compensated=compensated, **res["best"])
     if cache:
         table[key] = res
         _store(table)

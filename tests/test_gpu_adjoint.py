@@ -1,10 +1,9 @@
 """GPU adjoint gradients against the oracle (c128 adjoint, parameter shift
 and central finite differences) and the torch autograd binding.
 
-Bars: complex128 energy and gradient within 1e-8 of the oracle; complex64
-(FP32 state, FP64 reductions) within the stated 2e-4 absolute on the small
-cases below (the error of a float32 state, measured and recorded in the bench
-output for the full-size cases).
+Bars: complex128 energy and gradient within 1e-9 of the oracle; complex64
+(FP32 state, FP64 reductions) within 1e-6 relative to the complex128 oracle:
+|dE| <= 1e-6 max(1, |E|) and max |dgrad| <= 1e-6 max(1, max |grad|).
 """
 import numpy as np
 import pytest
@@ -14,6 +13,12 @@ from desv import workloads
 from oracle import sv_oracle as O
 
 pytestmark = pytest.mark.gpu
+
+
+def close_rel(got, ref, rtol=1e-6):
+    """max |got - ref| <= rtol max(1, max |ref|)."""
+    got, ref = np.asarray(got, dtype=np.float64), np.asarray(ref, dtype=np.float64)
+    return float(np.max(np.abs(got - ref), initial=0.0)) <= rtol * max(1.0, float(np.max(np.abs(ref), initial=0.0)))
 
 VQE_QASM = ('OPENQASM 2.0; include "qelib1.inc"; qreg q[2]; creg c[2]; '
             'ry(0.5) q[0]; rx(0.5) q[1]; cx q[0],q[1]; rz(0.2) q[1]; cx q[0],q[1]; '
@@ -67,8 +72,7 @@ def test_hea_gradients_c64_error_bound():
     e_ref, g_ref = O.adjoint_gradient(tmpl, theta, terms, n)
     energy_fn, _ = desv.circuit_to_energy_fn(circ, n, dtype="complex64")
     e, g = energy_fn.value_and_grad(theta, desv.PauliSumOperator(terms, n))
-    assert abs(e - e_ref) < 2e-4
-    assert np.max(np.abs(g - g_ref)) < 2e-4
+    assert close_rel(e, e_ref) and close_rel(g, g_ref)
 
 
 def test_phase_type_generators():
@@ -127,7 +131,7 @@ def test_batch_matches_single_instances():
         assert np.allclose(g, g_ref, atol=1e-9)
 
 
-@pytest.mark.parametrize("dtype,tol", [("complex128", 1e-9), ("complex64", 2e-4)])
+@pytest.mark.parametrize("dtype,tol", [("complex128", 1e-9), ("complex64", 1e-6)])
 def test_batch_refresh_graph_and_states(dtype, tol):
     """A compiled batch re-run with new angles (matrices refreshed, graph
     replayed) must match the oracle at the new angles; graph and direct
@@ -148,10 +152,9 @@ def test_batch_refresh_graph_and_states(dtype, tol):
             es, gs = b.run()
             for (n, circ, _, terms), th, e, g in zip(insts, thetas, es, gs):
                 e_ref, g_ref = O.adjoint_gradient(O.build_template(circ.ops, n), th, terms, n)
-                assert abs(e - e_ref) < tol
-                assert np.max(np.abs(g - g_ref)) < tol
+                assert close_rel(e, e_ref, tol) and close_rel(g, g_ref, tol)
         e_only = b.energies()
-        assert np.allclose(e_only, es, atol=tol)
+        assert close_rel(e_only, es, tol)
         # initial state for instance 0
         n0, c0, _, t0 = insts[0]
         psi0 = rng.normal(size=1 << n0) + 1j * rng.normal(size=1 << n0)
@@ -161,15 +164,15 @@ def test_batch_refresh_graph_and_states(dtype, tol):
         tmpl = O.build_template(c0.ops, n0)
         ops = O.patch_template(tmpl, thetas[0])
         sv = O.run_ops_fast(psi0.astype(np.complex128), ops)
-        assert abs(es2[0] - np.real(np.vdot(sv, O.pauli_sum_matvec(sv, t0)))) < tol
-        assert O.fidelity(b.get_state(0, n0), sv) > 1 - 10 * tol
+        assert close_rel(es2[0], np.real(np.vdot(sv, O.pauli_sum_matvec(sv, t0))), tol)
+        assert O.fidelity(b.get_state(0, n0), sv) > 1 - max(tol, 1e-9)
         b.close()
 
 
 @pytest.mark.parametrize("dtype,tol,shape", [
     ("complex128", 1e-9, dict(tile_bits=8, fuse_bits=3)),
-    ("complex64", 2e-4, dict(tile_bits=8, fuse_bits=4, use_mfma=True)),
-    ("complex64", 2e-4, dict(tile_bits=10, fuse_bits=5, use_mfma=True)),
+    ("complex64", 1e-6, dict(tile_bits=8, fuse_bits=4, use_mfma=True)),
+    ("complex64", 1e-6, dict(tile_bits=10, fuse_bits=5, use_mfma=True)),
 ])
 def test_batch_launch_shapes(dtype, tol, shape):
     """Wider fusion (VALU k = 3 blocks, forward MFMA blocks) in the batched
@@ -183,8 +186,7 @@ def test_batch_launch_shapes(dtype, tol, shape):
     es, gs = b.run()
     for (n, circ, theta, terms), e, g in zip(insts, es, gs):
         e_ref, g_ref = O.adjoint_gradient(O.build_template(circ.ops, n), theta, terms, n)
-        assert abs(e - e_ref) < tol
-        assert np.max(np.abs(g - g_ref)) < tol
+        assert close_rel(e, e_ref, tol) and close_rel(g, g_ref, tol)
     b.close()
 
 
@@ -201,7 +203,6 @@ def test_batch_autotune_keeps_parity(tmp_path, monkeypatch):
     es, gs = b.run()
     for (n, circ, theta, terms), e, g in zip(insts, es, gs):
         e_ref, g_ref = O.adjoint_gradient(O.build_template(circ.ops, n), theta, terms, n)
-        assert abs(e - e_ref) < 2e-4
-        assert np.max(np.abs(g - g_ref)) < 2e-4
+        assert close_rel(e, e_ref) and close_rel(g, g_ref)
     assert b.autotune()["cached"]
     b.close()

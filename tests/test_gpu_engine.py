@@ -474,3 +474,78 @@ def test_device_sampling_on_device_state_and_errors():
         desv.sample_counts(np.zeros(8, dtype=np.complex128), 10)
     with pytest.raises(ValueError, match="n_shots"):
         desv.sample_counts(np.ones(8, dtype=np.complex128), 0)
+
+
+@pytest.mark.parametrize("use_float32", [False, True])
+def test_named_gate_helpers(use_float32, rng):
+    """apply_cx / apply_cz / apply_rx / apply_ry / apply_rz against the oracle
+    (reference statevector.py:207-268) and the cases of
+    tests/unit/test_simulator.py:536-588."""
+    tol = 1e-6 if use_float32 else 1e-12
+    sim2 = desv.DenseSVSimulator(2, use_float32=use_float32)
+    sim2.apply_cx(0, 1)
+    assert sim2.get_probabilities()[0] > 0.99
+    sim2.apply_gate_1q(desv.GATES['x'], 0)
+    sim2.apply_cx(0, 1)
+    assert sim2.get_probabilities()[3] > 0.99          # |10> -> |11>
+    sim2.apply_cx(0, 1)
+    assert sim2.get_probabilities()[2] > 0.99          # CX twice = identity
+    sim2.set_initial_state(None)
+    sim2.apply_cz(0, 1)
+    assert sim2.get_probabilities()[0] > 0.99
+    sim2.apply_gate_1q(desv.GATES['h'], 0)
+    sim2.apply_cx(0, 1)
+    assert abs(np.sum(sim2.get_probabilities()) - 1) < tol
+    for bad in (lambda: sim2.apply_cx(0, 0), lambda: sim2.apply_cx(0, 5), lambda: sim2.apply_cz(1, 1),
+                lambda: sim2.apply_cz(0, 5), lambda: sim2.apply_gate_2q(np.eye(4), 0, 0),
+                lambda: sim2.apply_gate_2q(np.eye(4), 0, 5), lambda: sim2.apply_rx(2, 0.3)):
+        with pytest.raises(ValueError):
+            bad()
+    # random sequences of the helpers on 7 qubits against the oracle's table path
+    n = 7
+    sv0 = rand_state(n, rng)
+    sim = desv.DenseSVSimulator(n, use_float32=use_float32)
+    sim.sv = sv0
+    want = sv0.copy()
+    for _ in range(60):
+        k = int(rng.integers(5))
+        if k < 2:
+            a, b = (int(x) for x in rng.choice(n, 2, replace=False))
+            (sim.apply_cx if k == 0 else sim.apply_cz)(a, b)
+            row = [20 if k == 0 else 21, a, b, 0.0]
+        else:
+            q, th = int(rng.integers(n)), float(rng.uniform(-np.pi, np.pi))
+            (sim.apply_rx, sim.apply_ry, sim.apply_rz)[k - 2](q, th)
+            row = [(9, 10, 11)[k - 2], q, 0, th]
+        want = O.run_ops(want, np.array([row], dtype=np.float64))
+    assert fid(sim.get_statevector(), want) >= 1 - tol
+
+
+@pytest.mark.parametrize("use_float32", [False, True])
+def test_normalize(use_float32, rng):
+    n = 9
+    v = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    sim = desv.DenseSVSimulator(n, use_float32=use_float32)
+    sim.sv = 3.7 * v / np.linalg.norm(v)           # assignment does not normalise
+    assert abs(sim.handle.norm2() - 3.7 ** 2) < 1e-4
+    sim.normalize()
+    got = sim.get_statevector()
+    assert abs(np.vdot(got, got).real - 1) < (1e-6 if use_float32 else 1e-12)
+    assert np.allclose(got, (v / np.linalg.norm(v)).astype(got.dtype), atol=1e-6 if use_float32 else 1e-12)
+    z = desv.DenseSVSimulator(3, use_float32=use_float32)
+    z.sv = np.zeros(8)
+    z.normalize()                                   # a zero state stays zero (norm below 1e-12)
+    assert np.all(z.get_statevector() == 0)
+
+
+@pytest.mark.parametrize("pattern", ['linear', 'circular', 'full', 'star', 'brick'])
+def test_entangling_layers_and_random_circuits_on_device(pattern):
+    """Every entangling pattern (tests/unit/test_topology.py:56-64) and a
+    generated random circuit (tests/unit/test_random_circuit.py:27-32) run
+    through run_circuit and match the oracle."""
+    circ = [('h', 0)] + desv.entangling_layer(5, pattern=pattern) + desv.random_circuit(5, 50, seed=7)
+    sim = desv.DenseSVSimulator(5)
+    sim.run_circuit(circ)
+    want = O.run_circuit(O.transpile(circ), 5)
+    assert np.allclose(sim.get_statevector(), want, atol=1e-12)
+    assert abs(sim.get_probabilities().sum() - 1.0) < 1e-9
