@@ -185,24 +185,44 @@ def suite_vqe20():
 
 def suite_batch(count=1024, solo_sample=128):
     """1024 heterogeneous VQE instances (n ~ U{10..22}, 4-8 layers, 50 terms):
-    instances/s of energy + adjoint gradient in one graph-captured batch, and
-    the per-instance latency: in the batch every instance completes with the
-    batch; run alone (its own compiled program, energy + gradient), the
-    latency distribution over a sample of the instances (p50 / p99)."""
+    instances/s of energy + adjoint gradient in one graph-captured batch
+    (complex64 with compensated matrices, the default, and plain), the
+    per-instance latency inside the batch (device wall-clock completion stamp
+    of each instance's last result, p50 / p99), the error against
+    single-instance complex128 runs on a sample, and, run alone (its own
+    compiled program, energy + gradient), the latency distribution over a
+    sample of the instances."""
     import numpy as np
     import desv
     from desv import workloads
     from desv.batch import Batch
     insts = workloads.batch_instances(count=count)
-    b = Batch("complex64")
-    for n, circ, theta, terms in insts:
-        b.add(circ, n, theta, terms)
-    tuned = b.autotune(reps=1, cache=False)   # launch shape measured on this device
-    log(f"batch autotune: best {tuned['best']} "
-        f"{[(c['tile_bits'], c['fuse_bits'], c['use_mfma'], c['ms'] and round(c['ms'], 1)) for c in tuned['candidates']]}")
-    b.run()
-    dt, (es, gs) = _timed(lambda: b.run(), reps=2)
-    b.close()
+    res = {}
+    for label, comp in (("compensated", True), ("plain", False)):
+        b = Batch("complex64", compensated=comp)
+        for n, circ, theta, terms in insts:
+            b.add(circ, n, theta, terms)
+        tuned = b.autotune(reps=1, cache=False)   # launch shape measured on this device
+        log(f"batch ({label}) autotune: best {tuned['best']} "
+            f"{[(c['tile_bits'], c['fuse_bits'], c['use_mfma'], c['ms'] and round(c['ms'], 1)) for c in tuned['candidates']]}")
+        b.run()
+        dt, (es, gs) = _timed(lambda: b.run(), reps=2)
+        done_ms = b.completion_us() / 1e3
+        b.close()
+        res[label] = (dt, tuned, es, gs, done_ms)
+    dt, tuned, es, gs, done_ms = res["compensated"]
+    # error of the batch against single-instance complex128 runs (a sample)
+    errs = []
+    gs_plain = res["plain"][3]
+    for i in range(0, count, max(1, count // 64)):
+        n, circ, theta, terms = insts[i]
+        fn, _ = desv.circuit_to_energy_fn(circ, n, dtype="complex128")
+        e1, g1 = fn.value_and_grad(theta, desv.PauliSumOperator(terms, n))
+        fn.program.close()
+        sc = max(1.0, float(np.max(np.abs(g1))))
+        errs.append((abs(es[i] - e1) / max(1.0, abs(e1)), float(np.max(np.abs(gs[i] - g1))) / sc,
+                     float(np.max(np.abs(gs_plain[i] - g1))) / sc))
+    errs = np.array(errs)
     lat = []
     for i in range(0, count, max(1, count // solo_sample)):
         n, circ, theta, terms = insts[i]
@@ -217,7 +237,17 @@ def suite_batch(count=1024, solo_sample=128):
             "autotuned": tuned["best"],
             "autotune_candidates_ms": {f"t{c['tile_bits']}f{c['fuse_bits']}{'m' if c['use_mfma'] else ''}": c["ms"]
                                        for c in tuned["candidates"]},
-            "batch_latency_ms_p50": dt * 1e3, "batch_latency_ms_p99": dt * 1e3,
+            "instances_per_s_plain": count / res["plain"][0], "autotuned_plain": res["plain"][1]["best"],
+            "max_rel_dE_vs_c128_sample": float(errs[:, 0].max()),
+            "max_rel_dgrad_vs_c128_sample": float(errs[:, 1].max()),
+            "max_rel_dgrad_vs_c128_sample_plain": float(errs[:, 2].max()),
+            "error_sample": f"{len(errs)} instances (every {max(1, count // 64)}th) vs their single-instance "
+                            "complex128 runs; relative to max(1, max |ref|)",
+            "completion_ms_p50": float(np.percentile(done_ms, 50)),
+            "completion_ms_p99": float(np.percentile(done_ms, 99)),
+            "completion_ms_min": float(done_ms.min()), "completion_ms_max": float(done_ms.max()),
+            "completion_note": "device wall-clock time from the batch's first kernel to each instance's last "
+                               "reduced result (compensated run)",
             "amortized_ms_per_instance": dt * 1e3 / count,
             "solo_latency_ms_p50": float(np.percentile(lat, 50)),
             "solo_latency_ms_p99": float(np.percentile(lat, 99)),

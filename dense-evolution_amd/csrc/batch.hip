@@ -60,7 +60,7 @@ struct BInst {
 };
 
 struct Launch {
-    int kind;        // 0 fill, 1 stage, 2 matvec, 3 sum
+    int kind;        // 0 fill, 1 stage, 2 matvec, 3 sum, 4 start mark
     int nv = 1;
     int heavy = 0;  // stage kernel variant
     const void *tasks = nullptr;
@@ -104,6 +104,10 @@ struct desv_batch_s {
     int fuse_bits = 2;
     int use_mfma = 0;
     int use_graph = 1;
+    // complex64: matrices as FP32 hi + lo pairs (pool doubled: lo part
+    // mat_entries elements after the hi part), applied lo part first
+    int compensated = 0;
+    size_t mat_entries = 0;
     double last_us = 0;
     std::vector<BInst> inst;
     bool built = false;
@@ -115,6 +119,10 @@ struct desv_batch_s {
     size_t mats_bytes = 0;
     double *out_dev = nullptr, *out_host = nullptr;  // pinned host
     int n_out = 0;
+    // device wall-clock stamps of the last run: [0] start, [1 + i] the time
+    // instance i's last result was summed
+    unsigned long long *stamps_dev = nullptr;
+    int64_t n_stamps = 0;
     std::vector<Launch> full, energy_only;
     hipGraphExec_t graph_full = nullptr, graph_energy = nullptr;
     std::string err;
@@ -176,9 +184,13 @@ int decode_instance(BInst &in, bool structure) {
     return DESV_OK;
 }
 
-void write_mats(const Plan &p, int dtype, void *host, size_t off) {
+// lo_entries > 0: compensated pool, the lo parts lo_entries elements after
+// the hi parts
+void write_mats(const Plan &p, int dtype, void *host, size_t off, size_t lo_entries) {
     if (dtype == DESV_C64) {
-        round_mats_f32(p, reinterpret_cast<float *>(host) + 2 * off);
+        float *hi = reinterpret_cast<float *>(host) + 2 * off;
+        round_mats_f32(p, hi);
+        if (lo_entries) round_mats_f32_lo(p, hi, hi + 2 * lo_entries);
     } else {
         double *f = reinterpret_cast<double *>(host) + 2 * off;
         for (size_t i = 0; i < p.mats.size(); ++i) {
@@ -200,8 +212,8 @@ int run_launches(desv_batch b, const std::vector<Launch> &ls) {
                       : launch_fill_tasks<double>((const FillTask *)l.tasks, l.ntasks, l.grid, b->stream);
                 break;
             case 1:
-                e = f ? launch_stage<float>(l.nv, l.heavy, false, (const StageTask *)l.tasks, l.wg_task, dummy, l.grid,
-                                            l.max_m, b->stream)
+                e = f ? launch_stage<float>(l.nv, l.heavy, b->mat_entries != 0, (const StageTask *)l.tasks, l.wg_task,
+                                            dummy, l.grid, l.max_m, b->stream)
                       : launch_stage<double>(l.nv, l.heavy, false, (const StageTask *)l.tasks, l.wg_task, dummy, l.grid,
                                              l.max_m, b->stream);
                 break;
@@ -210,6 +222,7 @@ int run_launches(desv_batch b, const std::vector<Launch> &ls) {
                       : launch_matvec_tasks<double>((const MatvecTask *)l.tasks, l.ntasks, l.grid, b->stream);
                 break;
             case 3: e = launch_sum_tasks((const SumTask *)l.tasks, l.ntasks, b->stream); break;
+            case 4: e = launch_mark(b->stamps_dev, b->n_stamps, b->stream); break;
         }
         if (e != hipSuccess) return bfail(DESV_EDEVICE, std::string("batch launch: ") + hipGetErrorString(e));
     }
@@ -307,8 +320,13 @@ int build(desv_batch b) {
     }
     const size_t out_off = off;
     off = align256(off + (size_t)std::max(1, n_out) * 8);
+    const size_t stamps_off = off;
+    off = align256(off + (b->inst.size() + 1) * sizeof(unsigned long long));
     const size_t mats_off = off;
-    b->mats_bytes = std::max<size_t>(moff, 1) * ab;
+    const bool comp = b->compensated && b->dtype == DESV_C64;
+    if (comp && moff > (size_t)0x3fffffff) return bfail(DESV_EINVAL, "matrix pool too large");
+    b->mat_entries = comp ? moff : 0;
+    b->mats_bytes = std::max<size_t>(moff, 1) * ab * (comp ? 2 : 1);
     off = align256(off + b->mats_bytes);
     // task arrays are appended below; compute them after pointers are known
     const size_t tasks_off = off;
@@ -343,7 +361,7 @@ int build(desv_batch b) {
             for (auto &st : in.fp.stages) tiles += st.n_tiles;
             for (auto &st : in.rp.stages) tiles += st.n_tiles;
         }
-        est_tasks = nt * sizeof(StageTask) + tiles * sizeof(int32_t) + (size_t)(maxF + maxR + 8) * 4 * 256 + 4096;
+        est_tasks = nt * sizeof(StageTask) + tiles * sizeof(int32_t) + (size_t)(maxF + maxR + 8) * 16 * 256 + 4096;
     }
     b->arena_bytes = tasks_off + align256(est_tasks);
     BHIP(hipMalloc(&b->arena, b->arena_bytes));
@@ -351,6 +369,8 @@ int build(desv_batch b) {
     BHIP(hipHostMalloc(&b->mats_host, b->mats_bytes, hipHostMallocDefault));
     BHIP(hipHostMalloc((void **)&b->out_host, (size_t)std::max(1, n_out) * 8, hipHostMallocDefault));
     b->out_dev = reinterpret_cast<double *>(A + out_off);
+    b->stamps_dev = reinterpret_cast<unsigned long long *>(A + stamps_off);
+    b->n_stamps = (int64_t)b->inst.size();
     b->mats_dev = A + mats_off;
     b->n_out = n_out;
     for (size_t i = 0; i < b->inst.size(); ++i) {
@@ -366,6 +386,13 @@ int build(desv_batch b) {
         in.groups = A + o[i].groups;
         in.zz = reinterpret_cast<uint64_t *>(A + o[i].zz);
         in.cc = reinterpret_cast<double2 *>(A + o[i].cc);
+    }
+    // start mark (clears the completion stamps)
+    {
+        Launch l;
+        l.kind = 4;
+        pend_full.push_back({l, 0});
+        pend_energy.push_back({l, 0});
     }
     // fill
     {
@@ -389,50 +416,50 @@ int build(desv_batch b) {
         pend_full.push_back({l, at});
         pend_energy.push_back({l, at});
     }
-    auto stage_launches = [&](bool rev, std::vector<PendingLaunch> &out) {
-        const int maxS = rev ? maxR : maxF;
-        for (int k = 0; k < maxS; ++k)
-            for (int heavy = 0; heavy < 4; ++heavy) {
-                std::vector<StageTask> ts;
-                int64_t wg = 0;
-                int max_m = 0;
-                for (auto &in : b->inst) {
-                    const Plan &p = rev ? in.rp : in.fp;
-                    if ((int)p.stages.size() <= k || p.stages[k].heavy != heavy) continue;
-                    StageTask t;
-                    std::memset(&t, 0, sizeof(t));
-                    t.st = p.stages[k];
-                    t.n = in.n;
-                    t.sv0 = rev ? in.psi2 : in.psi;
-                    t.sv1 = rev ? in.lam : nullptr;
-                    t.ops = rev ? in.rops : in.fops;
-                    t.mats = reinterpret_cast<char *>(b->mats_dev) + (rev ? in.rmat : in.fmat) * ab;
-                    t.partials = in.gpart;
-                    t.pstride = in.pstride;
-                    t.wg_begin = wg;
-                    wg += t.st.n_tiles;
-                    max_m = std::max(max_m, (int)t.st.m);
-                    ts.push_back(t);
-                }
-                if (ts.empty()) continue;
-                Launch l;
-                l.kind = 1;
-                l.nv = rev ? 2 : 1;
-                l.heavy = heavy;
-                l.ntasks = (int)ts.size();
-                l.grid = wg;
-                l.max_m = max_m;
-                std::vector<int32_t> wt((size_t)wg);
-                for (size_t ti = 0; ti < ts.size(); ++ti)
-                    for (int64_t w = 0; w < ts[ti].st.n_tiles; ++w) wt[ts[ti].wg_begin + w] = (int32_t)ti;
-                PendingLaunch pl{l, push_blob(ts.data(), ts.size() * sizeof(StageTask))};
-                pl.wg_off = push_blob(wt.data(), wt.size() * sizeof(int32_t));
-                out.push_back(pl);
+    // the launches of stage k of every instance that has one (one per kernel variant)
+    auto stage_launches = [&](bool rev, int k, std::vector<PendingLaunch> &out) {
+        for (int heavy = 0; heavy < 4; ++heavy) {
+            std::vector<StageTask> ts;
+            int64_t wg = 0;
+            int max_m = 0;
+            for (auto &in : b->inst) {
+                const Plan &p = rev ? in.rp : in.fp;
+                if ((int)p.stages.size() <= k || p.stages[k].heavy != heavy) continue;
+                StageTask t;
+                std::memset(&t, 0, sizeof(t));
+                t.st = p.stages[k];
+                t.n = in.n;
+                t.sv0 = rev ? in.psi2 : in.psi;
+                t.sv1 = rev ? in.lam : nullptr;
+                t.ops = rev ? in.rops : in.fops;
+                t.mats = reinterpret_cast<char *>(b->mats_dev) + (rev ? in.rmat : in.fmat) * ab;
+                t.lo_off = (int32_t)b->mat_entries;
+                t.partials = in.gpart;
+                t.pstride = in.pstride;
+                t.wg_begin = wg;
+                wg += t.st.n_tiles;
+                max_m = std::max(max_m, (int)t.st.m);
+                ts.push_back(t);
             }
+            if (ts.empty()) continue;
+            Launch l;
+            l.kind = 1;
+            l.nv = rev ? 2 : 1;
+            l.heavy = heavy;
+            l.ntasks = (int)ts.size();
+            l.grid = wg;
+            l.max_m = max_m;
+            std::vector<int32_t> wt((size_t)wg);
+            for (size_t ti = 0; ti < ts.size(); ++ti)
+                for (int64_t w = 0; w < ts[ti].st.n_tiles; ++w) wt[ts[ti].wg_begin + w] = (int32_t)ti;
+            PendingLaunch pl{l, push_blob(ts.data(), ts.size() * sizeof(StageTask))};
+            pl.wg_off = push_blob(wt.data(), wt.size() * sizeof(int32_t));
+            out.push_back(pl);
+        }
     };
     {
         const size_t before = pend_full.size();
-        stage_launches(false, pend_full);
+        for (int k = 0; k < maxF; ++k) stage_launches(false, k, pend_full);
         for (size_t i = before; i < pend_full.size(); ++i) pend_energy.push_back(pend_full[i]);
     }
     {
@@ -462,30 +489,44 @@ int build(desv_batch b) {
         pend_full.push_back({l, at});
         pend_energy.push_back({l, at});
     }
-    stage_launches(true, pend_full);
-    {
-        std::vector<SumTask> sf, se;
-        for (auto &in : b->inst) {
-            SumTask t;
-            t.src = in.epart;
-            t.count = in.mv_wg;
-            t.dst = b->out_dev + in.e_out;
-            sf.push_back(t);
-            se.push_back(t);
-            for (int s = 0; s < in.rp.n_grad_slots; ++s) {
-                SumTask g;
-                g.src = in.gpart + (int64_t)s * in.pstride;
-                g.count = in.pstride;
-                g.dst = b->out_dev + in.s_out + s;
-                sf.push_back(g);
-            }
-        }
+    // Sums of an instance's energy and gradient partials run right after its
+    // last reverse stage (or after the matvec, without one), so each instance's
+    // results are final, and time-stamped, as soon as its own work is done.
+    auto sum_launch = [&](const std::vector<SumTask> &ts, std::vector<PendingLaunch> &out) {
+        if (ts.empty()) return;
         Launch l;
         l.kind = 3;
-        l.ntasks = (int)sf.size();
-        pend_full.push_back({l, push_blob(sf.data(), sf.size() * sizeof(SumTask))});
-        l.ntasks = (int)se.size();
-        pend_energy.push_back({l, push_blob(se.data(), se.size() * sizeof(SumTask))});
+        l.ntasks = (int)ts.size();
+        out.push_back({l, push_blob(ts.data(), ts.size() * sizeof(SumTask))});
+    };
+    auto sums_of = [&](size_t i, bool grads, std::vector<SumTask> &ts) {
+        const BInst &in = b->inst[i];
+        SumTask t;
+        t.src = in.epart;
+        t.count = in.mv_wg;
+        t.dst = b->out_dev + in.e_out;
+        t.stamp = b->stamps_dev + 1 + i;
+        ts.push_back(t);
+        for (int s = 0; grads && s < in.rp.n_grad_slots; ++s) {
+            SumTask g;
+            g.src = in.gpart + (int64_t)s * in.pstride;
+            g.count = in.pstride;
+            g.dst = b->out_dev + in.s_out + s;
+            g.stamp = b->stamps_dev + 1 + i;
+            ts.push_back(g);
+        }
+    };
+    for (int k = -1; k < maxR; ++k) {
+        if (k >= 0) stage_launches(true, k, pend_full);
+        std::vector<SumTask> done;
+        for (size_t i = 0; i < b->inst.size(); ++i)
+            if ((int)b->inst[i].rp.stages.size() == k + 1) sums_of(i, true, done);
+        sum_launch(done, pend_full);
+    }
+    {
+        std::vector<SumTask> se;
+        for (size_t i = 0; i < b->inst.size(); ++i) sums_of(i, false, se);
+        sum_launch(se, pend_energy);
     }
     if (task_blob.size() > b->arena_bytes - tasks_off) return bfail(DESV_EDEVICE, "internal: task area too small");
     // upload the static per-instance tables
@@ -514,25 +555,35 @@ int build(desv_batch b) {
     }
     // matrices
     for (auto &in : b->inst) {
-        write_mats(in.fp, b->dtype, b->mats_host, in.fmat);
-        write_mats(in.rp, b->dtype, b->mats_host, in.rmat);
+        write_mats(in.fp, b->dtype, b->mats_host, in.fmat, b->mat_entries);
+        write_mats(in.rp, b->dtype, b->mats_host, in.rmat, b->mat_entries);
     }
     if (!b->stream) BHIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
-    b->built = true;
     if (b->use_graph) {
+        // the batch counts as built only once both graphs exist; a failed
+        // capture or instantiation releases what was made and leaves it unbuilt
         for (int which = 0; which < 2; ++which) {
             hipGraph_t g = nullptr;
-            BHIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-            int r = run_launches(b, which == 0 ? b->full : b->energy_only);
-            hipError_t e = hipStreamEndCapture(b->stream, &g);
-            if (r) return r;
-            BHIP(e);
+            hipError_t e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
+            if (e != hipSuccess) {
+                b->release();
+                return bfail(DESV_EDEVICE, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
+            }
+            const int r = run_launches(b, which == 0 ? b->full : b->energy_only);
+            e = hipStreamEndCapture(b->stream, &g);
             hipGraphExec_t ex = nullptr;
-            BHIP(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
-            (void)hipGraphDestroy(g);
+            if (!r && e == hipSuccess) e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+            if (g) (void)hipGraphDestroy(g);
+            if (r || e != hipSuccess) {
+                const std::string msg = r ? std::string(desv_last_error())
+                                          : std::string("graph capture: ") + hipGetErrorString(e);
+                b->release();
+                return bfail(r ? r : DESV_EDEVICE, msg);
+            }
             (which == 0 ? b->graph_full : b->graph_energy) = ex;
         }
     }
+    b->built = true;
     return DESV_OK;
 }
 
@@ -547,8 +598,8 @@ int refresh(desv_batch b) {
         }
         refresh_plan_mats(in.fp, in.fg);
         refresh_plan_mats(in.rp, in.rg);
-        write_mats(in.fp, b->dtype, b->mats_host, in.fmat);
-        write_mats(in.rp, b->dtype, b->mats_host, in.rmat);
+        write_mats(in.fp, b->dtype, b->mats_host, in.fmat, b->mat_entries);
+        write_mats(in.rp, b->dtype, b->mats_host, in.rmat, b->mat_entries);
         in.dirty = false;
     });
     for (int r : rc)
@@ -682,6 +733,14 @@ int desv_batch_configure(desv_batch b, int tile_bits, int fuse_bits, int use_mfm
     return DESV_OK;
 }
 
+int desv_batch_set_compensated(desv_batch b, int enable) {
+    if (!b) return bfail(DESV_EINVAL, "null batch");
+    const int c = enable && b->dtype == DESV_C64 ? 1 : 0;
+    if (c != b->compensated) b->built = false;
+    b->compensated = c;
+    return DESV_OK;
+}
+
 int desv_batch_run(desv_batch b, double *energies, double *grads) {
     if (!b) return bfail(DESV_EINVAL, "null batch");
     auto t0 = std::chrono::steady_clock::now();
@@ -715,6 +774,22 @@ int desv_batch_get_state(desv_batch b, int64_t index, void *host_amps) {
 int desv_batch_last_time_us(desv_batch b, double *us) {
     if (!b || !us) return bfail(DESV_EINVAL, "null argument");
     *us = b->last_us;
+    return DESV_OK;
+}
+
+int desv_batch_completion_us(desv_batch b, double *us) {
+    if (!b || !us) return bfail(DESV_EINVAL, "null argument");
+    if (!b->built || !b->stamps_dev) return bfail(DESV_EINVAL, "the batch has not run yet");
+    BHIP(hipSetDevice(b->device));
+    std::vector<unsigned long long> st((size_t)b->n_stamps + 1);
+    BHIP(hipMemcpyAsync(st.data(), b->stamps_dev, st.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                        b->stream));
+    BHIP(hipStreamSynchronize(b->stream));
+    int khz = 0;
+    BHIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, b->device));
+    if (khz <= 0) return bfail(DESV_EDEVICE, "device reports no wall clock rate");
+    for (int64_t i = 0; i < b->n_stamps; ++i)
+        us[i] = st[1 + i] >= st[0] ? (double)(st[1 + i] - st[0]) * 1e3 / khz : -1.0;
     return DESV_OK;
 }
 

@@ -289,7 +289,8 @@ int run_plan_on(desv_handle h, const Plan &plan, const DevPlan &dp, int nv, void
         t.pstride = pstride;
         t.wg_begin = 0;
         hipError_t e = h->dtype == DESV_C64
-                           ? launch_stage<float>(nv, st.heavy, dp.comp, nullptr, nullptr, t, st.n_tiles, st.m, h->stream)
+                           ? launch_stage<float>(nv, st.heavy, dp.comp != 0, nullptr, nullptr, t, st.n_tiles, st.m,
+                                                 h->stream)
                            : launch_stage<double>(nv, st.heavy, false, nullptr, nullptr, t, st.n_tiles, st.m, h->stream);
         if (e != hipSuccess) return fail(DESV_EDEVICE, std::string("stage launch: ") + hipGetErrorString(e));
     }
@@ -1120,7 +1121,7 @@ int desv_adjoint(desv_handle h, const double *ops, int64_t n_ops, const int64_t 
     const int64_t pstride = 1ll << (h->n - std::min(h->n, o.tile_bits));
     rc = ensure_part(h, (size_t)plan.n_grad_slots * pstride);
     if (rc) return rc;
-    rc = upload_plan(plan, h->dtype, h->device, h->stream, h->scratch_plan);
+    rc = upload_plan(plan, h->dtype, h->device, h->stream, h->scratch_plan, h->compensated != 0);
     if (rc) return rc;
     rc = run_plan_on(h, plan, h->scratch_plan, 2, h->aux[0], h->aux[1], pstride);
     if (rc) return rc;

@@ -873,9 +873,9 @@ __device__ inline double op_grad_diag(char *lds, int vstride, int m, const DevOp
 // the probe's pairs along p are read off the loaded values, then the block is
 // applied to both tiles.  Probe products in the state precision, the tile sum
 // in FP64.
-template <typename T, int KB, bool BD, typename MP>
+template <typename T, int KB, bool BD, bool CP, typename MP>
 __device__ inline double op_grad_block_k(char *lds, int vstride, int m, const DevOp &gop, MP G, const DevOp &bop,
-                                         MP B) {
+                                         MP B, MP BL) {
     using C = typename VT<T>::c;
     using R = T;
     constexpr int D = 1 << KB;
@@ -897,9 +897,13 @@ __device__ inline double op_grad_block_k(char *lds, int vstride, int m, const De
     }
     const Groups g(m, fixed, EB);
     if (g.idle()) return 0.0;
-    C mm[BD ? D : D * D];
+    C mm[BD ? D : D * D], ml[CP ? (BD ? D : D * D) : 1];
 #pragma unroll
     for (int i = 0; i < (BD ? D : D * D); ++i) mm[i] = gload(B + i);
+    if constexpr (CP) {
+#pragma unroll
+        for (int i = 0; i < (BD ? D : D * D); ++i) ml[i] = gload(BL + i);
+    }
     // Re(conj(l0) (G s)_0 + conj(l1) (G s)_1) for the pair (0, 1) along p
     auto probe = [&](C s0, C s1, C l0, C l1) -> R {
         const C a0 = cfma(g01, s1, cmul(g00, s0)), a1 = cfma(g11, s1, cmul(g10, s0));
@@ -925,9 +929,25 @@ __device__ inline double op_grad_block_k(char *lds, int vstride, int m, const De
 #pragma unroll
         for (int r = 0; r < D; ++r) {
             C ox, oy;
-            if constexpr (BD) {
+            if constexpr (BD && CP) {
+                ox = cfma(mm[r], x[r], cmul(ml[r], x[r]));
+                oy = cfma(mm[r], y[r], cmul(ml[r], y[r]));
+            } else if constexpr (BD) {
                 ox = cmul(mm[r], x[r]);
                 oy = cmul(mm[r], y[r]);
+            } else if constexpr (CP) {
+                ox = cmul(ml[r * D], x[0]);
+                oy = cmul(ml[r * D], y[0]);
+#pragma unroll
+                for (int c = 1; c < D; ++c) {
+                    ox = cfma(ml[r * D + c], x[c], ox);
+                    oy = cfma(ml[r * D + c], y[c], oy);
+                }
+#pragma unroll
+                for (int c = 0; c < D; ++c) {
+                    ox = cfma(mm[r * D + c], x[c], ox);
+                    oy = cfma(mm[r * D + c], y[c], oy);
+                }
             } else {
                 ox = cmul(mm[r * D], x[0]);
                 oy = cmul(mm[r * D], y[0]);
@@ -944,14 +964,14 @@ __device__ inline double op_grad_block_k(char *lds, int vstride, int m, const De
     return acc;
 }
 
-template <typename T, typename MP>
+template <typename T, bool CP, typename MP>
 __device__ inline double op_grad_block(char *lds, int vstride, int m, const DevOp &gop, MP G, const DevOp &bop,
-                                       MP B) {
+                                       MP B, MP BL) {
     if (bop.kind == OPK_DIAG)
-        return bop.k == 1 ? op_grad_block_k<T, 1, true>(lds, vstride, m, gop, G, bop, B)
-                          : op_grad_block_k<T, 2, true>(lds, vstride, m, gop, G, bop, B);
-    return bop.k == 1 ? op_grad_block_k<T, 1, false>(lds, vstride, m, gop, G, bop, B)
-                      : op_grad_block_k<T, 2, false>(lds, vstride, m, gop, G, bop, B);
+        return bop.k == 1 ? op_grad_block_k<T, 1, true, CP>(lds, vstride, m, gop, G, bop, B, BL)
+                          : op_grad_block_k<T, 2, true, CP>(lds, vstride, m, gop, G, bop, B, BL);
+    return bop.k == 1 ? op_grad_block_k<T, 1, false, CP>(lds, vstride, m, gop, G, bop, B, BL)
+                      : op_grad_block_k<T, 2, false, CP>(lds, vstride, m, gop, G, bop, B, BL);
 }
 
 }  // namespace
@@ -1102,7 +1122,7 @@ k_stage(const StageTask *__restrict__ tasks,
                     double part;
                     if (op.pos[4] == 1) {  // fused with the next op (a block containing the probe's bit)
                         const DevOp nx = gload_op(ops + o + 1);
-                        part = op_grad_block<T>(lds, vstride, m, op, M, nx, mats + nx.mat);
+                        part = op_grad_block<T, CP>(lds, vstride, m, op, M, nx, mats + nx.mat, matl + nx.mat);
                         ++o;
                     } else {
                         part = op.kind == OPK_GRAD ? op_grad_dense<T>(lds, vstride, m, op, M)
@@ -1160,17 +1180,22 @@ hipError_t launch_stage(int nv, int variant, bool comp, const StageTask *tasks, 
     // complex128 never uses MFMA blocks: its variants 2 and 3 would only
     // differ from 1 by omitting k = 4, 5 VALU blocks
     if (!std::is_same<T, float>::value && variant >= 2) variant = 1;
-    // compensated matrices: complex64 forward programs only
-    if (comp && (!std::is_same<T, float>::value || nv != 1)) return hipErrorInvalidValue;
+    // compensated matrices: complex64 only
+    if (comp && !std::is_same<T, float>::value) return hipErrorInvalidValue;
     const dim3 g((unsigned)grid);
 #define DESV_ST(NVV, VV) \
     hipLaunchKernelGGL((k_stage<T, NVV, VV, false>), g, dim3(kThreads), lds, s, tasks, wg_task, single)
-#define DESV_STC(VV) \
-    hipLaunchKernelGGL((k_stage<float, 1, VV, true>), g, dim3(kThreads), lds, s, tasks, wg_task, single)
+#define DESV_STC(NVV, VV) \
+    hipLaunchKernelGGL((k_stage<float, NVV, VV, true>), g, dim3(kThreads), lds, s, tasks, wg_task, single)
     if constexpr (std::is_same<T, float>::value) {
         if (comp) {
-            if (variant == 0) DESV_STC(0); else if (variant == 1) DESV_STC(1);
-            else if (variant == 2) DESV_STC(2); else DESV_STC(3);
+            if (nv == 1) {
+                if (variant == 0) DESV_STC(1, 0); else if (variant == 1) DESV_STC(1, 1);
+                else if (variant == 2) DESV_STC(1, 2); else DESV_STC(1, 3);
+            } else {
+                if (variant == 0) DESV_STC(2, 0); else if (variant == 1) DESV_STC(2, 1);
+                else if (variant == 2) DESV_STC(2, 2); else DESV_STC(2, 3);
+            }
             return hipGetLastError();
         }
     }
@@ -1324,7 +1349,20 @@ __global__ void __launch_bounds__(kThreads) k_sum_tasks(const SumTask *__restric
     double acc = 0;
     for (int64_t i = threadIdx.x; i < tk.count; i += blockDim.x) acc += tk.src[i];
     const double s = block_sum(acc, red);
-    if (threadIdx.x == 0) *tk.dst = s;
+    if (threadIdx.x == 0) {
+        *tk.dst = s;
+        if (tk.stamp) atomicMax(tk.stamp, (unsigned long long)wall_clock64());
+    }
+}
+
+__global__ void __launch_bounds__(kThreads) k_mark(unsigned long long *stamps, int64_t count) {
+    for (int64_t i = threadIdx.x; i < count; i += blockDim.x) stamps[1 + i] = 0ull;
+    if (threadIdx.x == 0) stamps[0] = (unsigned long long)wall_clock64();
+}
+
+hipError_t launch_mark(unsigned long long *stamps, int64_t count, hipStream_t s) {
+    hipLaunchKernelGGL(k_mark, dim3(1), dim3(kThreads), 0, s, stamps, count);
+    return hipGetLastError();
 }
 
 template <typename T>
@@ -1822,7 +1860,7 @@ size_t stage_lds_limit(int device) {
     size_t lim = 64 * 1024;
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && v > 0)
         lim = (size_t)v;
-    const void *fns[20] = {
+    const void *fns[24] = {
         (const void *)k_stage<float, 1, 0, false>,  (const void *)k_stage<float, 2, 0, false>,
         (const void *)k_stage<double, 1, 0, false>, (const void *)k_stage<double, 2, 0, false>,
         (const void *)k_stage<float, 1, 1, false>,  (const void *)k_stage<float, 2, 1, false>,
@@ -1832,7 +1870,9 @@ size_t stage_lds_limit(int device) {
         (const void *)k_stage<float, 1, 3, false>,  (const void *)k_stage<float, 2, 3, false>,
         (const void *)k_stage<double, 1, 3, false>, (const void *)k_stage<double, 2, 3, false>,
         (const void *)k_stage<float, 1, 0, true>,   (const void *)k_stage<float, 1, 1, true>,
-        (const void *)k_stage<float, 1, 2, true>,   (const void *)k_stage<float, 1, 3, true>};
+        (const void *)k_stage<float, 1, 2, true>,   (const void *)k_stage<float, 1, 3, true>,
+        (const void *)k_stage<float, 2, 0, true>,   (const void *)k_stage<float, 2, 1, true>,
+        (const void *)k_stage<float, 2, 2, true>,   (const void *)k_stage<float, 2, 3, true>};
     // leave room for the kernels' own (static) LDS arrays, as compiled
     size_t static_lds = 0;
     for (const void *f : fns) {

@@ -84,11 +84,13 @@ struct FillTask {
     int64_t chunk;
 };
 
-// *dst = sum of src[0..count)
+// *dst = sum of src[0..count); then, if stamp is set, *stamp = max(*stamp,
+// device wall clock) (completion time of the result).
 struct SumTask {
     const double *src;
     int64_t count;
     double *dst;
+    unsigned long long *stamp;
 };
 
 // Pauli-expectation work item: one flip mask, up to kTermsPerItem terms.
@@ -124,7 +126,7 @@ hipError_t launch_direct(const void *in, void *out, int n, const DirectOp &op, c
 template <typename T>
 // `variant`: DevStage::heavy (0 light, 1 VALU blocks k = 3..5, 2 MFMA blocks
 // k = 3, 4, 3 MFMA blocks up to k = 5).  `comp`: compensated matrices (the lo
-// pool at mats + lo_off; complex64, nv = 1 only).
+// pool at mats + lo_off; complex64 only).
 hipError_t launch_stage(int nv, int variant, bool comp, const StageTask *tasks, const int32_t *wg_task,
                         const StageTask &single, int64_t grid, int max_m, hipStream_t s);
 template <typename T>
@@ -132,6 +134,8 @@ hipError_t launch_matvec_tasks(const MatvecTask *tasks, int ntasks, int64_t grid
 template <typename T>
 hipError_t launch_fill_tasks(const FillTask *tasks, int ntasks, int64_t grid, hipStream_t s);
 hipError_t launch_sum_tasks(const SumTask *tasks, int ntasks, hipStream_t s);
+// stamps[1 .. count] = 0, stamps[0] = device wall clock (start of a timed run).
+hipError_t launch_mark(unsigned long long *stamps, int64_t count, hipStream_t s);
 // dst <- src on the device (bytes a multiple of 16).
 hipError_t launch_copy16(void *dst, const void *src, size_t bytes, hipStream_t s);
 // dst <- src converted between complex64 (float) and complex128 (double).

@@ -52,6 +52,8 @@ _SIGS = {
     "desv_synchronize": ([_p], _i),
     "desv_set_double_buffer": ([_p, _i], _i),
     "desv_set_compensated": ([_p, _i], _i),
+    "desv_batch_set_compensated": ([_p, _i], _i),
+    "desv_batch_completion_us": ([_p, _pd], _i),
     "desv_reset": ([_p], _i),
     "desv_set_state": ([_p, _p], _i),
     "desv_get_state": ([_p, _p], _i),

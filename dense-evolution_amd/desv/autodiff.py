@@ -80,6 +80,9 @@ class AdjointProgram:
         self.param_of_op[self.slots] = np.arange(len(self.slots))
         self.n_params = len(self.slots)
         self.handle = StateHandle(self.n, complex128, device)
+        from .config import default_compensation
+        self.compensated = default_compensation() and not complex128
+        self.handle.set_compensated(self.compensated)
 
     def ops_for(self, theta) -> np.ndarray:
         theta = np.asarray(theta, dtype=np.float64).reshape(-1)
@@ -100,7 +103,8 @@ class AdjointProgram:
             xm, zm, cs = hamiltonian.encoded()
             if np.any(np.imag(cs) != 0):
                 raise ValueError("Pauli-sum Hamiltonians must have real coefficients")
-            b = Batch("complex128" if self.handle.code == _capi.C128 else "complex64", self.handle.device)
+            b = Batch("complex128" if self.handle.code == _capi.C128 else "complex64", self.handle.device,
+                      compensated=self.compensated)
             b.add_ops(self.n, self.ops_for(np.zeros(self.n_params)), self.param_of_op, self.n_params,
                       xm, zm, np.real(cs))
             self._batch, self._bkey, self._bham = b, key, hamiltonian

@@ -20,11 +20,19 @@ from .observables import encode_pauli_terms
 
 
 class Batch:
-    def __init__(self, dtype="complex64", device: int = 0):
+    """``compensated`` (complex64): forward and reverse sweeps with
+    compensated matrices (FP32 hi + lo pairs, lo part first); None takes the
+    process default (:func:`desv.config.set_compensation`)."""
+
+    def __init__(self, dtype="complex64", device: int = 0, compensated=None):
         self.code = _capi.C64 if np.dtype(dtype) == np.complex64 else _capi.C128
         p = C.c_void_p()
         check(lib.desv_batch_create(self.code, device, C.byref(p)))
         self.ptr = p
+        if compensated is None:
+            from .config import default_compensation
+            compensated = default_compensation()
+        self.set_compensated(compensated)
         self.n_params: List[int] = []
         self.slots: List[np.ndarray] = []
         self._shapes: List[Tuple[int, int, int]] = []
@@ -78,6 +86,11 @@ class Batch:
         check(lib.desv_batch_get_state(self.ptr, index, _capi.vptr(out)))
         return out
 
+    def set_compensated(self, enable: bool) -> None:
+        """complex64: compensated matrices in both sweeps (rebuilds on the next run)."""
+        self.compensated = bool(enable) and self.code == _capi.C64
+        check(lib.desv_batch_set_compensated(self.ptr, int(self.compensated)))
+
     def configure(self, tile_bits: int = 0, use_graph: bool = True, fuse_bits: int = 0,
                   use_mfma: bool = False) -> None:
         """Launch shape for the next run (the instances are recompiled)."""
@@ -88,7 +101,8 @@ class Batch:
 
     def signature(self) -> str:
         """Workload class of the batch for the autotuner's cache."""
-        return f"c{'64' if self.code == _capi.C64 else '128'}-inst{len(self.n_params)}-" + _shape_sig(self._shapes)
+        return (f"c{'64' if self.code == _capi.C64 else '128'}{'-comp' if self.compensated else ''}"
+                f"-inst{len(self.n_params)}-" + _shape_sig(self._shapes))
 
     def autotune(self, candidates=None, reps: int = 2, cache=True):
         """Time candidate launch shapes and keep the fastest (see desv.tune)."""
@@ -124,6 +138,14 @@ class Batch:
             out.append(g[off:off + k].copy())
             off += k
         return e, out
+
+    def completion_us(self) -> np.ndarray:
+        """Per-instance completion times (microseconds from the start) of the
+        last run, from device wall-clock stamps written as each instance's
+        results are reduced."""
+        out = np.zeros(len(self.n_params), dtype=np.float64)
+        check(lib.desv_batch_completion_us(self.ptr, _capi.dptr(out)))
+        return out
 
     def last_time_us(self) -> float:
         v = C.c_double()

@@ -253,13 +253,25 @@ int desv_batch_get_state(desv_batch b, int64_t index, void *host_amps);
  * 3-5-qubit blocks on MFMA, complex64), use_graph (capture + replay).  The
  * Python autotuner (desv.tune) times candidate shapes and sets the fastest. */
 int desv_batch_configure(desv_batch b, int tile_bits, int fuse_bits, int use_mfma, int use_graph);
+/* complex64 batches: enable = 1 applies every gate of the forward and
+ * reverse sweeps with compensated matrices (see desv_compile_opts.
+ * compensated), which keeps energies and gradients within 1e-6 relative of
+ * the FP64 result; the next run rebuilds.  No counterpart in the reference. */
+int desv_batch_set_compensated(desv_batch b, int enable);
 /* Evaluate everything; energies[i], grads[offset_i ...] in instance order
  * (grads packed by instance, n_params_i each).  grads == NULL runs the
  * forward pass and energies only. */
 int desv_batch_run(desv_batch b, double *energies, double *grads);
-/* Per-instance device time of the last run is not separable; this returns
- * the wall time of the last desv_batch_run in microseconds. */
+/* Wall time of the last desv_batch_run in microseconds (host clock). */
 int desv_batch_last_time_us(desv_batch b, double *us);
+/* Per-instance completion times of the last run, us[i] in microseconds from
+ * the run's first kernel to the moment instance i's last result (energy or
+ * gradient entry) was reduced, read from the device wall clock.  The sums of
+ * each instance run right after its own last reverse stage, so these times
+ * spread with the instances' circuit depths.  No counterpart in the
+ * reference (its vmapped run_batch_jit, statevector.py:496-584, returns all
+ * rows at once). */
+int desv_batch_completion_us(desv_batch b, double *us);
 
 #ifdef __cplusplus
 }

@@ -206,3 +206,31 @@ def test_batch_autotune_keeps_parity(tmp_path, monkeypatch):
         assert close_rel(e, e_ref) and close_rel(g, g_ref)
     assert b.autotune()["cached"]
     b.close()
+
+
+@pytest.mark.parametrize("compensated", [False, True])
+def test_batch_compensation_modes_and_completion_times(compensated):
+    """Both complex64 modes of a batch match the oracle; the device completion
+    stamps are positive, within the run, and ordered by the instances' number
+    of reverse stages (an instance's sums run right after its last one)."""
+    from desv.batch import Batch
+    insts = workloads.batch_instances(count=12, n_lo=5, n_hi=12, l_lo=1, l_hi=5, n_terms=10)
+    for n, layers, seed in ((5, 1, 1), (12, 5, 2)):
+        insts.append((n, *workloads.vqe_instance(n, layers, 10, seed=seed)))
+    b = Batch("complex64", compensated=compensated)
+    b.configure(tile_bits=8)
+    for n, circ, theta, terms in insts:
+        b.add(circ, n, theta, terms)
+    es, gs = b.run()
+    for (n, circ, theta, terms), e, g in zip(insts, es, gs):
+        e_ref, g_ref = O.adjoint_gradient(O.build_template(circ.ops, n), theta, terms, n)
+        assert close_rel(e, e_ref) and close_rel(g, g_ref)
+    done = b.completion_us()
+    assert np.all(done > 0) and done.max() <= b.last_time_us() + 1.0
+    # a 5-qubit, 1-layer instance (one reverse stage at 2^8-amplitude tiles)
+    # finishes before a 12-qubit, 5-layer one (several reverse stages): its
+    # sums are launched earlier on the same stream
+    assert done[-2] < done[-1], done
+    e_only = b.energies()
+    assert close_rel(e_only, es) and np.all(b.completion_us() > 0)
+    b.close()
