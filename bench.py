@@ -154,6 +154,43 @@ def suite_plain20():
             "autotuned": tuned, "fidelity_vs_c128": fid, "max_abs_dZ_vs_c128": dz}
 
 
+def suite_batchjit20(n=20, rows=256):
+    """run_batch_jit on the 20-qubit plain circuit (20 layers of rx, ry, rz on
+    every qubit + a CX ladder, 1200 angle columns), 256 parameter rows in one
+    device batch, complex64 (compensated, the simulator default): rows/s
+    including the host encode, the batch build and the (256, 2^20) download,
+    and the device-only forward time of a second run."""
+    import numpy as np
+    import desv
+    from desv import workloads
+    circ = workloads.random_layers_circuit(n, 20, seed=0)
+    sim = desv.DenseSVSimulator(n, use_float32=True)
+    cols = sum(1 for c in circ if c[0] in ('rx', 'ry', 'rz'))
+    P = np.random.default_rng(1).uniform(0, 2 * np.pi, (rows, cols))
+    sim.run_batch_jit(circ, P[:2])                       # warm up (library load, first launch)
+    t, out = _timed(lambda: sim.run_batch_jit(circ, P), reps=1)
+    # device-only: the same batch object run again
+    from desv.batch import Batch
+    from desv.simulator import encode_op_table
+    from desv.transpiler import QuantumTranspiler
+    tmpl = encode_op_table(QuantumTranspiler.transpile(circ), n, param_slots=True)
+    slots = np.nonzero(tmpl[:, 3] == -1.0)[0]
+    b = Batch("complex64")
+    b.set_forward_only(True)
+    b.configure(use_graph=False)
+    pm = np.full(len(tmpl), -1, dtype=np.int64)
+    for r in P:
+        ops = tmpl.copy()
+        ops[slots, 3] = r
+        b.add_ops(n, ops, pm, 0, np.zeros(0, np.uint64), np.zeros(0, np.uint64), np.zeros(0))
+    b.run_forward()
+    td, _ = _timed(lambda: b.run_forward(), reps=3)
+    b.close()
+    return {"workload": f"run_batch_jit, {n}q plain circuit ({len(circ)} gates, {cols} columns), {rows} rows, c64",
+            "rows_per_s": rows / t, "s_per_call": t, "device_rows_per_s": rows / td,
+            "device_ms_per_run": td * 1e3, "gates_per_s_device": rows * len(circ) / td}
+
+
 def suite_vqe20():
     """20-qubit HEA VQE (8 layers, 320 params, 200 Pauli terms): adjoint energy+gradient
     evals/s (c64 with FP64 reductions, and c128) vs parameter shift on the plain forward
@@ -589,8 +626,8 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="only time the library's device copy of a 2^qubits complex64 state "
                          "(calibration of HBM rate and of the rocprofv3 byte counters)")
-    ap.add_argument("--suite", default="streaming30,parity30,brickwork32,brickwork34,plain20,vqe20,batch,precision,lanczos,"
-                    "readout",
+    ap.add_argument("--suite", default="streaming30,parity30,brickwork32,brickwork34,plain20,batchjit20,vqe20,batch,"
+                    "precision,lanczos,readout",
                     help="extra workloads to measure after the headline ('' for none)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -746,6 +783,7 @@ def main():
     torch.cuda.synchronize()
     extras = {}
     fns = {"streaming30": suite_streaming30, "parity30": suite_parity30, "brickwork32": suite_brickwork32,
+           "batchjit20": suite_batchjit20,
            "plain20": suite_plain20, "vqe20": suite_vqe20, "batch": suite_batch,
            "precision": suite_precision, "lanczos": suite_lanczos, "brickwork34": suite_brickwork34,
            "readout": suite_readout}

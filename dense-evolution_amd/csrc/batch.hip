@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <new>
 #include <string>
@@ -108,6 +109,12 @@ struct desv_batch_s {
     // mat_entries elements after the hi part), applied lo part first
     int compensated = 0;
     size_t mat_entries = 0;
+    // forward only: no Hamiltonian, no reverse sweep; a run applies each
+    // instance's circuit to |0...0> (or its initial state) and the final
+    // states sit back to back in the arena (desv_batch_get_states)
+    int forward_only = 0;
+    void *states_dev = nullptr;
+    size_t states_bytes = 0;
     double last_us = 0;
     std::vector<BInst> inst;
     bool built = false;
@@ -229,20 +236,56 @@ int run_launches(desv_batch b, const std::vector<Launch> &ls) {
     return DESV_OK;
 }
 
+// Byte signature of an instance's gate structure (qubit count and, per gate of
+// the forward and reverse lists, kind, width, bits, controls and probe
+// slot): instances with equal signatures differ only in their matrices, so
+// they share one compiled plan whose matrices are recomputed from their own
+// gates.
+std::string gate_signature(const BInst &in) {
+    std::string s;
+    auto put = [&](const void *p, size_t nb) { s.append(reinterpret_cast<const char *>(p), nb); };
+    put(&in.n, sizeof(in.n));
+    for (const auto *list : {&in.fg, &in.rg}) {
+        const size_t cnt = list->size();
+        put(&cnt, sizeof(cnt));
+        for (const Gate &g : *list) {
+            const int hdr[5] = {g.kind, g.k, g.grad_diag ? 1 : 0, g.grad_slot, (int)g.m.size()};
+            put(hdr, sizeof(hdr));
+            put(g.bits, sizeof(g.bits));
+            put(&g.cmask, sizeof(g.cmask));
+        }
+    }
+    return s;
+}
+
 int build(desv_batch b) {
     b->release();
     BHIP(hipSetDevice(b->device));
     const size_t ab = abytes(b->dtype);
-    // host compile
+    // host compile: decode every instance, compile one plan per distinct gate
+    // structure, then give the other instances a copy with their own matrices
     std::vector<int> rc(b->inst.size(), 0);
     std::vector<std::string> errs(b->inst.size());
     parallel_for(b->inst.size(), [&](size_t i) {
-        BInst &in = b->inst[i];
-        if (decode_instance(in, true)) {
+        if (decode_instance(b->inst[i], true)) {
             rc[i] = DESV_EINVAL;
             errs[i] = desv_last_error();
-            return;
         }
+    });
+    for (size_t i = 0; i < rc.size(); ++i)
+        if (rc[i]) return bfail(rc[i], "instance " + std::to_string(i) + ": " + errs[i]);
+    std::vector<int64_t> rep(b->inst.size());
+    {
+        std::map<std::string, int64_t> first;
+        for (size_t i = 0; i < b->inst.size(); ++i)
+            rep[i] = first.emplace(gate_signature(b->inst[i]), (int64_t)i).first->second;
+    }
+    std::vector<size_t> reps;
+    for (size_t i = 0; i < b->inst.size(); ++i)
+        if (rep[i] == (int64_t)i) reps.push_back(i);
+    parallel_for(reps.size(), [&](size_t r) {
+        const size_t i = reps[r];
+        BInst &in = b->inst[i];
         CompileOpts of, orv;
         of.tile_bits = in.mf = choose_m(b, in.n, 1);
         orv.tile_bits = in.mr = choose_m(b, in.n, 2);
@@ -253,7 +296,9 @@ int build(desv_batch b) {
         of.use_mfma = b->dtype == DESV_C64 ? b->use_mfma : 0;
         orv.nv = 2;
         std::string err;
-        if (!build_plan(in.fg, in.n, of, in.fp, err) || !build_plan(in.rg, in.n, orv, in.rp, err)) {
+        in.rp = Plan();
+        if (!build_plan(in.fg, in.n, of, in.fp, err) ||
+            (!b->forward_only && !build_plan(in.rg, in.n, orv, in.rp, err))) {
             rc[i] = DESV_EINVAL;
             errs[i] = err;
             return;
@@ -263,6 +308,19 @@ int build(desv_batch b) {
     });
     for (size_t i = 0; i < rc.size(); ++i)
         if (rc[i]) return bfail(rc[i], "instance " + std::to_string(i) + ": " + errs[i]);
+    parallel_for(b->inst.size(), [&](size_t i) {
+        if (rep[i] == (int64_t)i) return;
+        BInst &in = b->inst[i];
+        const BInst &r = b->inst[rep[i]];
+        in.fp = r.fp;
+        in.rp = r.rp;
+        in.mf = r.mf;
+        in.mr = r.mr;
+        in.pstride = r.pstride;
+        refresh_plan_mats(in.fp, in.fg);
+        if (!b->forward_only) refresh_plan_mats(in.rp, in.rg);
+        in.dirty = false;
+    });
 
     // arena layout
     size_t off = 0, moff = 0;
@@ -274,12 +332,19 @@ int build(desv_batch b) {
     std::vector<std::vector<MatvecGroup>> groups(b->inst.size());
     std::vector<std::vector<uint64_t>> zs(b->inst.size());
     std::vector<std::vector<double2>> ccs(b->inst.size());
+    // the final states first, back to back in instance order
+    for (size_t i = 0; i < b->inst.size(); ++i) {
+        o[i].psi = off;
+        off += ((size_t)1 << b->inst[i].n) * ab;
+    }
+    const size_t states_bytes = off;
+    off = align256(off);
+    const bool rev = !b->forward_only;
     for (size_t i = 0; i < b->inst.size(); ++i) {
         BInst &in = b->inst[i];
         const size_t sb = ((size_t)1 << in.n) * ab;
-        o[i].psi = off; off = align256(off + sb);
-        o[i].psi2 = off; off = align256(off + sb);
-        o[i].lam = off; off = align256(off + sb);
+        o[i].psi2 = off; off = align256(off + (rev ? sb : 0));
+        o[i].lam = off; off = align256(off + (rev ? sb : 0));
         o[i].init = off; off = align256(off + (in.init.empty() ? 0 : sb));
         o[i].fops = off; off = align256(off + in.fp.ops.size() * sizeof(DevOp));
         o[i].rops = off; off = align256(off + in.rp.ops.size() * sizeof(DevOp));
@@ -314,6 +379,7 @@ int build(desv_batch b) {
         o[i].cc = off; off = align256(off + ccs[i].size() * sizeof(double2) + 16);
         in.fmat = moff; moff += in.fp.mats.size();
         in.rmat = moff; moff += in.rp.mats.size();
+        if (!rev) continue;
         in.e_out = n_out++;
         in.s_out = n_out;
         n_out += in.rp.n_grad_slots;
@@ -370,6 +436,8 @@ int build(desv_batch b) {
     BHIP(hipHostMalloc((void **)&b->out_host, (size_t)std::max(1, n_out) * 8, hipHostMallocDefault));
     b->out_dev = reinterpret_cast<double *>(A + out_off);
     b->stamps_dev = reinterpret_cast<unsigned long long *>(A + stamps_off);
+    b->states_dev = A;
+    b->states_bytes = states_bytes;
     b->n_stamps = (int64_t)b->inst.size();
     b->mats_dev = A + mats_off;
     b->n_out = n_out;
@@ -462,7 +530,7 @@ int build(desv_batch b) {
         for (int k = 0; k < maxF; ++k) stage_launches(false, k, pend_full);
         for (size_t i = before; i < pend_full.size(); ++i) pend_energy.push_back(pend_full[i]);
     }
-    {
+    if (rev) {
         std::vector<MatvecTask> mt;
         int64_t wg = 0;
         for (auto &in : b->inst) {
@@ -516,14 +584,14 @@ int build(desv_batch b) {
             ts.push_back(g);
         }
     };
-    for (int k = -1; k < maxR; ++k) {
+    for (int k = -1; rev && k < maxR; ++k) {
         if (k >= 0) stage_launches(true, k, pend_full);
         std::vector<SumTask> done;
         for (size_t i = 0; i < b->inst.size(); ++i)
             if ((int)b->inst[i].rp.stages.size() == k + 1) sums_of(i, true, done);
         sum_launch(done, pend_full);
     }
-    {
+    if (rev) {
         std::vector<SumTask> se;
         for (size_t i = 0; i < b->inst.size(); ++i) sums_of(i, false, se);
         sum_launch(se, pend_energy);
@@ -733,6 +801,23 @@ int desv_batch_configure(desv_batch b, int tile_bits, int fuse_bits, int use_mfm
     return DESV_OK;
 }
 
+int desv_batch_set_forward_only(desv_batch b, int enable) {
+    if (!b) return bfail(DESV_EINVAL, "null batch");
+    const int f = enable ? 1 : 0;
+    if (f != b->forward_only) b->built = false;
+    b->forward_only = f;
+    return DESV_OK;
+}
+
+int desv_batch_get_states(desv_batch b, void *host) {
+    if (!b || !host) return bfail(DESV_EINVAL, "null argument");
+    if (!b->built || !b->states_dev) return bfail(DESV_EINVAL, "the batch has not run yet");
+    BHIP(hipSetDevice(b->device));
+    BHIP(hipMemcpyAsync(host, b->states_dev, b->states_bytes, hipMemcpyDeviceToHost, b->stream));
+    BHIP(hipStreamSynchronize(b->stream));
+    return DESV_OK;
+}
+
 int desv_batch_set_compensated(desv_batch b, int enable) {
     if (!b) return bfail(DESV_EINVAL, "null batch");
     const int c = enable && b->dtype == DESV_C64 ? 1 : 0;
@@ -743,6 +828,8 @@ int desv_batch_set_compensated(desv_batch b, int enable) {
 
 int desv_batch_run(desv_batch b, double *energies, double *grads) {
     if (!b) return bfail(DESV_EINVAL, "null batch");
+    if (b->forward_only && (energies || grads))
+        return bfail(DESV_EINVAL, "a forward-only batch has no energies or gradients: pass NULL for both");
     auto t0 = std::chrono::steady_clock::now();
     if (int r = execute(b, grads != nullptr)) return r;
     int64_t goff = 0;

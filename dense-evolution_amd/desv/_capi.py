@@ -54,6 +54,8 @@ _SIGS = {
     "desv_set_compensated": ([_p, _i], _i),
     "desv_batch_set_compensated": ([_p, _i], _i),
     "desv_batch_completion_us": ([_p, _pd], _i),
+    "desv_batch_get_states": ([_p, _p], _i),
+    "desv_batch_set_forward_only": ([_p, _i], _i),
     "desv_reset": ([_p], _i),
     "desv_set_state": ([_p, _p], _i),
     "desv_get_state": ([_p, _p], _i),

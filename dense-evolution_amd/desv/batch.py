@@ -81,6 +81,24 @@ class Batch:
         a = np.ascontiguousarray(state, dtype=np.complex64 if self.code == _capi.C64 else np.complex128)
         check(lib.desv_batch_set_initial_state(self.ptr, index, _capi.vptr(a)))
 
+    def set_forward_only(self, enable: bool = True) -> None:
+        """Only apply the circuits (no Hamiltonian, no gradient sweep)."""
+        check(lib.desv_batch_set_forward_only(self.ptr, int(bool(enable))))
+
+    def run_forward(self) -> None:
+        """One forward run of every instance (forward-only batches)."""
+        check(lib.desv_batch_run(self.ptr, None, None))
+
+    def get_states(self) -> np.ndarray:
+        """Every instance's final state, concatenated in instance order, in
+        one device-to-host copy; shaped (B, 2^n) when all widths agree."""
+        ns = [s[0] for s in self._shapes]
+        out = np.empty(sum(1 << n for n in ns), dtype=np.complex64 if self.code == _capi.C64 else np.complex128)
+        check(lib.desv_batch_get_states(self.ptr, _capi.vptr(out)))
+        if ns and all(n == ns[0] for n in ns):
+            return out.reshape(len(ns), 1 << ns[0])
+        return out
+
     def get_state(self, index: int, n_qubits: int) -> np.ndarray:
         out = np.empty(1 << n_qubits, dtype=np.complex64 if self.code == _capi.C64 else np.complex128)
         check(lib.desv_batch_get_state(self.ptr, index, _capi.vptr(out)))

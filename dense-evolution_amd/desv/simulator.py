@@ -430,19 +430,30 @@ class DenseSVSimulator:
                 f"phase/cp/crz/cphase) -- one column per parametric gate, in gate-appearance order. "
                 f"A literal float passed for one of these gates is NOT exempt: it still consumes a "
                 f"positional column.")
-        out = np.empty((parameter_batch.shape[0], self.dim), dtype=self.dtype)
-        work = StateHandle(self.n, not self.use_float32, self._h.device)
-        work.set_compensated(self.compensated)
+        rows = np.asarray(parameter_batch, dtype=np.float64)
+        if rows.shape[0] == 0:
+            return np.empty((0, self.dim), dtype=self.dtype)
+        # every row an instance of one forward-only device batch: the rows
+        # share one compiled plan (same gate structure), their matrices are
+        # refreshed from their own angles, each stage runs for all rows in one
+        # launch, and the (B, 2^n) result comes back in one copy
+        from .batch import Batch
+        b = Batch("complex64" if self.use_float32 else "complex128", self._h.device,
+                  compensated=self.compensated)
         try:
-            for b, row in enumerate(np.asarray(parameter_batch, dtype=np.float64)):
+            b.set_forward_only(True)
+            b.configure(use_graph=False)
+            pmap = np.full(len(template), -1, dtype=np.int64)
+            none_u = np.zeros(0, dtype=np.uint64)
+            none_d = np.zeros(0, dtype=np.float64)
+            for row in rows:
                 ops = template.copy()
                 ops[slots, 3] = row
-                work.reset()
-                work.run_ops(ops)
-                out[b] = work.download()
+                b.add_ops(self.n, ops, pmap, 0, none_u, none_u, none_d)
+            b.run_forward()
+            return b.get_states().reshape(len(rows), self.dim)
         finally:
-            work.close()
-        return out
+            b.close()
 
     def run_parametric_batch_jit(self, base_circuit: List, parameter_batch: np.ndarray) -> np.ndarray:
         warnings.warn(

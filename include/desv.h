@@ -248,6 +248,16 @@ int desv_batch_set_params(desv_batch b, int64_t index, const double *theta);
 int desv_batch_set_initial_state(desv_batch b, int64_t index, const void *host_amps);
 /* Final state psi of instance i after the last run. */
 int desv_batch_get_state(desv_batch b, int64_t index, void *host_amps);
+/* Every instance's final state, concatenated in instance order (sum over i of
+ * 2^n_i amplitudes in the batch dtype), in one device-to-host copy. */
+int desv_batch_get_states(desv_batch b, void *host_amps);
+/* enable = 1: the batch only applies each instance's circuit (instances may be
+ * added with n_terms = 0; desv_batch_run takes NULL energies and gradients;
+ * no completion stamps are written).  Serves run_batch_jit
+ * (statevector.py:496-584): one row per instance, instances of equal gate
+ * structure share one compiled plan, all rows advance together one stage
+ * launch at a time. */
+int desv_batch_set_forward_only(desv_batch b, int enable);
 /* Launch shape of the next build: tile_bits (0 = default: 11 for complex64,
  * 10 for complex128, capped by LDS), fuse_bits (0 = 2), use_mfma (forward
  * 3-5-qubit blocks on MFMA, complex64), use_graph (capture + replay).  The

@@ -406,6 +406,46 @@ def test_run_batch_jit(rng):
         assert np.allclose(out[b], O.run_circuit(c, n), atol=1e-10)
 
 
+@pytest.mark.parametrize("use_float32", [False, True])
+def test_run_batch_jit_device_batch(use_float32, rng):
+    """Many rows in one device batch (reference statevector.py:496-584): every
+    parametric gate consumes a column even with a literal angle, gphase from a
+    transpiled u3 gets angle 0 (the reference's template), other literals are
+    kept; each row against the oracle; column-count errors keep the reference's
+    message."""
+    n, B = 10, 24
+    circ = ([('h', q) for q in range(n)] + [('rx', q, 1.5) for q in range(n)] + desv.entangling_layer(n) +
+            [('u3', 3, 0.3, 0.2, 0.1), ('crz', 2, 7, 0.9), ('t', 4), ('p', 5, 2.0), ('swap', 1, 8)] +
+            [('ry', q, 0.0) for q in range(n)])
+    target = desv.QuantumTranspiler.transpile(circ)
+    cols = sum(1 for c in target if c[0] in ('rx', 'ry', 'rz', 'p', 'u1', 'phase', 'cp', 'crz', 'cphase'))
+    P = rng.uniform(-np.pi, np.pi, (B, cols))
+    sim = desv.DenseSVSimulator(n, use_float32=use_float32)
+    out = sim.run_batch_jit(circ, P)
+    assert out.shape == (B, 1 << n) and out.dtype == (np.complex64 if use_float32 else np.complex128)
+    for b in range(B):
+        k, rows = 0, []
+        for c in target:
+            if c[0] in ('rx', 'ry', 'rz', 'p', 'u1', 'phase'):
+                rows.append((c[0], c[1], P[b, k]))
+                k += 1
+            elif c[0] in ('cp', 'crz', 'cphase'):
+                rows.append((c[0], c[1], c[2], P[b, k]))
+                k += 1
+            elif c[0] == 'gphase':
+                rows.append(('gphase', c[1], 0.0))
+            else:
+                rows.append(c)
+        want = O.run_ops_fast(O.zero_state(n), O.encode_ops(rows, n))
+        if use_float32:
+            assert O.fidelity(out[b], want) >= 1 - 1e-6
+        else:
+            assert np.allclose(out[b], want, atol=1e-10)
+    with pytest.raises(ValueError, match="parameter_batch"):
+        sim.run_batch_jit(circ, P[:, :-1])
+    assert sim.run_batch_jit(circ, P[:0]).shape == (0, 1 << n)
+
+
 def test_qasm_to_state_integration():
     qasm = ('OPENQASM 2.0; include "qelib1.inc"; qreg q[6]; h q[0]; cx q[0],q[1]; rx(pi/3) q[2]; '
             'ry(0.4) q[3]; cz q[2],q[3]; crz(0.7) q[1],q[4]; ccx q[0],q[4],q[5]; u3(0.1,0.2,0.3) q[5]; '
