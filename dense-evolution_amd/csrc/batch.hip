@@ -130,14 +130,22 @@ struct desv_batch_s {
     // instance i's last result was summed
     unsigned long long *stamps_dev = nullptr;
     int64_t n_stamps = 0;
-    std::vector<Launch> full, energy_only;
-    hipGraphExec_t graph_full = nullptr, graph_energy = nullptr;
+    // launch lists: energies + gradients, energies only (forward, lambda,
+    // energy sums), and the reverse sweep alone (gradients of the state an
+    // energies-only run left: psi copy and lambda), each with its graph
+    std::vector<Launch> full, energy_only, reverse_only;
+    hipGraphExec_t graph_full = nullptr, graph_energy = nullptr, graph_reverse = nullptr;
+    // the device holds the psi copy and lambda of an energies-only run made
+    // with the current matrices and initial states
+    bool reverse_ready = false;
     std::string err;
 
     void release() {
         if (graph_full) (void)hipGraphExecDestroy(graph_full);
         if (graph_energy) (void)hipGraphExecDestroy(graph_energy);
-        graph_full = graph_energy = nullptr;
+        if (graph_reverse) (void)hipGraphExecDestroy(graph_reverse);
+        graph_full = graph_energy = graph_reverse = nullptr;
+        reverse_ready = false;
         if (arena) (void)hipFree(arena);
         if (mats_host) (void)hipHostFree(mats_host);
         if (out_host) (void)hipHostFree(out_host);
@@ -145,6 +153,7 @@ struct desv_batch_s {
         out_host = nullptr;
         full.clear();
         energy_only.clear();
+        reverse_only.clear();
         built = false;
     }
 };
@@ -409,7 +418,7 @@ int build(desv_batch b) {
         size_t blob_off;
         size_t wg_off = 0;
     };
-    std::vector<PendingLaunch> pend_full, pend_energy;
+    std::vector<PendingLaunch> pend_full, pend_energy, pend_reverse;
     auto push_blob = [&](const void *p, size_t bytes) {
         const size_t at = align256(task_blob.size());
         task_blob.resize(at + bytes);
@@ -584,12 +593,21 @@ int build(desv_batch b) {
             ts.push_back(g);
         }
     };
+    {
+        // the reverse-only list starts with its own mark (completion times of
+        // a reverse run count from its first kernel)
+        Launch l;
+        l.kind = 4;
+        pend_reverse.push_back({l, 0});
+    }
     for (int k = -1; rev && k < maxR; ++k) {
+        const size_t before = pend_full.size();
         if (k >= 0) stage_launches(true, k, pend_full);
         std::vector<SumTask> done;
         for (size_t i = 0; i < b->inst.size(); ++i)
             if ((int)b->inst[i].rp.stages.size() == k + 1) sums_of(i, true, done);
         sum_launch(done, pend_full);
+        for (size_t i = before; i < pend_full.size(); ++i) pend_reverse.push_back(pend_full[i]);
     }
     if (rev) {
         std::vector<SumTask> se;
@@ -621,6 +639,11 @@ int build(desv_batch b) {
         if (p.l.kind == 1) p.l.wg_task = reinterpret_cast<const int32_t *>(A + tasks_off + p.wg_off);
         b->energy_only.push_back(p.l);
     }
+    for (auto &p : pend_reverse) {
+        p.l.tasks = A + tasks_off + p.blob_off;
+        if (p.l.kind == 1) p.l.wg_task = reinterpret_cast<const int32_t *>(A + tasks_off + p.wg_off);
+        b->reverse_only.push_back(p.l);
+    }
     // matrices
     for (auto &in : b->inst) {
         write_mats(in.fp, b->dtype, b->mats_host, in.fmat, b->mat_entries);
@@ -630,14 +653,14 @@ int build(desv_batch b) {
     if (b->use_graph) {
         // the batch counts as built only once both graphs exist; a failed
         // capture or instantiation releases what was made and leaves it unbuilt
-        for (int which = 0; which < 2; ++which) {
+        for (int which = 0; which < 3; ++which) {
             hipGraph_t g = nullptr;
             hipError_t e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
             if (e != hipSuccess) {
                 b->release();
                 return bfail(DESV_EDEVICE, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
             }
-            const int r = run_launches(b, which == 0 ? b->full : b->energy_only);
+            const int r = run_launches(b, which == 0 ? b->full : which == 1 ? b->energy_only : b->reverse_only);
             e = hipStreamEndCapture(b->stream, &g);
             hipGraphExec_t ex = nullptr;
             if (!r && e == hipSuccess) e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
@@ -648,7 +671,7 @@ int build(desv_batch b) {
                 b->release();
                 return bfail(r ? r : DESV_EDEVICE, msg);
             }
-            (which == 0 ? b->graph_full : b->graph_energy) = ex;
+            (which == 0 ? b->graph_full : which == 1 ? b->graph_energy : b->graph_reverse) = ex;
         }
     }
     b->built = true;
@@ -675,8 +698,15 @@ int refresh(desv_batch b) {
     return DESV_OK;
 }
 
-int execute(desv_batch b, bool grads) {
-    if (!b->built) {
+// mode 0: energies only, 1: energies and gradients, 2: the reverse sweep of
+// the last energies-only run (gradients; energies re-summed)
+int execute(desv_batch b, int mode) {
+    if (mode == 2) {
+        bool dirty = false;
+        for (const auto &in : b->inst) dirty |= in.dirty;
+        if (!b->built || !b->reverse_ready || dirty)
+            return bfail(DESV_EINVAL, "the reverse sweep needs an energies-only run with the current parameters");
+    } else if (!b->built) {
         int r = build(b);
         if (r) return r;
     } else {
@@ -684,13 +714,16 @@ int execute(desv_batch b, bool grads) {
         if (r) return r;
     }
     BHIP(hipSetDevice(b->device));
-    BHIP(hipMemcpyAsync(b->mats_dev, b->mats_host, b->mats_bytes, hipMemcpyHostToDevice, b->stream));
+    if (mode != 2)
+        BHIP(hipMemcpyAsync(b->mats_dev, b->mats_host, b->mats_bytes, hipMemcpyHostToDevice, b->stream));
+    b->reverse_ready = false;
     if (b->use_graph) {
-        BHIP(hipGraphLaunch(grads ? b->graph_full : b->graph_energy, b->stream));
+        BHIP(hipGraphLaunch(mode == 1 ? b->graph_full : mode == 0 ? b->graph_energy : b->graph_reverse, b->stream));
     } else {
-        int r = run_launches(b, grads ? b->full : b->energy_only);
+        int r = run_launches(b, mode == 1 ? b->full : mode == 0 ? b->energy_only : b->reverse_only);
         if (r) return r;
     }
+    b->reverse_ready = mode == 0 && !b->forward_only;
     BHIP(hipMemcpyAsync(b->out_host, b->out_dev, (size_t)std::max(1, b->n_out) * 8, hipMemcpyDeviceToHost,
                         b->stream));
     BHIP(hipStreamSynchronize(b->stream));
@@ -771,6 +804,7 @@ int desv_batch_set_initial_state(desv_batch b, int64_t index, const void *host_a
     BInst &in = b->inst[index];
     const size_t sb = ((size_t)1 << in.n) * abytes(b->dtype);
     const bool had = !in.init.empty();
+    b->reverse_ready = false;
     if (!host_amps) {
         in.init.clear();
         if (had) b->built = false;
@@ -831,7 +865,7 @@ int desv_batch_run(desv_batch b, double *energies, double *grads) {
     if (b->forward_only && (energies || grads))
         return bfail(DESV_EINVAL, "a forward-only batch has no energies or gradients: pass NULL for both");
     auto t0 = std::chrono::steady_clock::now();
-    if (int r = execute(b, grads != nullptr)) return r;
+    if (int r = execute(b, grads != nullptr ? 1 : 0)) return r;
     int64_t goff = 0;
     for (size_t i = 0; i < b->inst.size(); ++i) {
         const BInst &in = b->inst[i];
@@ -844,6 +878,29 @@ int desv_batch_run(desv_batch b, double *energies, double *grads) {
         goff += in.n_params;
     }
     b->last_us = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
+    return DESV_OK;
+}
+
+int desv_batch_run_reverse(desv_batch b, double *grads) {
+    if (!b || !grads) return bfail(DESV_EINVAL, "null argument");
+    if (b->forward_only) return bfail(DESV_EINVAL, "a forward-only batch has no gradients");
+    auto t0 = std::chrono::steady_clock::now();
+    if (int r = execute(b, 2)) return r;
+    int64_t goff = 0;
+    for (size_t i = 0; i < b->inst.size(); ++i) {
+        const BInst &in = b->inst[i];
+        for (int64_t k = 0; k < in.n_params; ++k) grads[goff + k] = 0.0;
+        for (size_t s = 0; s < in.slot_param.size(); ++s) grads[goff + in.slot_param[s]] += b->out_host[in.s_out + s];
+        goff += in.n_params;
+    }
+    b->last_us = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
+    return DESV_OK;
+}
+
+int desv_batch_state_device_ptr(desv_batch b, int64_t index, void **out) {
+    if (!b || !out || index < 0 || index >= (int64_t)b->inst.size()) return bfail(DESV_EINVAL, "bad argument");
+    if (!b->built) return bfail(DESV_EINVAL, "batch has not been run");
+    *out = b->inst[index].psi;
     return DESV_OK;
 }
 

@@ -515,6 +515,13 @@ int desv_set_double_buffer(desv_handle h, int enable) {
     return DESV_OK;
 }
 
+int desv_state_device_ptr(desv_handle h, void **out) {
+    if (int rc = check_handle(h)) return rc;
+    if (!out) return fail(DESV_EINVAL, "null output pointer");
+    *out = h->sv;
+    return DESV_OK;
+}
+
 int desv_set_compensated(desv_handle h, int enable) {
     if (int rc = check_handle(h)) return rc;
     h->compensated = enable && h->dtype == DESV_C64 ? 1 : 0;

@@ -24,6 +24,18 @@ if not os.path.exists(LIB_PATH):
         f"libdesv.so not found at {LIB_PATH}; build it with "
         f"`make -C dense-evolution_amd/csrc` (or __graft_entry__.build())")
 
+# One HIP runtime per process: PyTorch-ROCm ships its own libamdhip64.so.7 and
+# HSA runtime under the same sonames as /opt/rocm's.  Whichever loads first
+# serves every later user of that soname, and a process that ends up with the
+# system HIP runtime under torch's HSA runtime sees no GPU from torch.  Loading
+# torch first (when it is installed) makes the engine and torch share torch's
+# runtime, so device tensors, streams and events pass between them.
+if os.environ.get("DESV_NO_TORCH", "") != "1":
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
 lib = C.CDLL(LIB_PATH)
 
 _p = C.c_void_p
@@ -56,6 +68,9 @@ _SIGS = {
     "desv_batch_completion_us": ([_p, _pd], _i),
     "desv_batch_get_states": ([_p, _p], _i),
     "desv_batch_set_forward_only": ([_p, _i], _i),
+    "desv_batch_run_reverse": ([_p, _pd], _i),
+    "desv_batch_state_device_ptr": ([_p, _i64, C.POINTER(_p)], _i),
+    "desv_state_device_ptr": ([_p, C.POINTER(_p)], _i),
     "desv_reset": ([_p], _i),
     "desv_set_state": ([_p, _p], _i),
     "desv_get_state": ([_p, _p], _i),

@@ -135,6 +135,7 @@ class AdjointProgram:
                        tail: Optional[np.ndarray] = None) -> Tuple[float, np.ndarray]:
         """Energy and gradient; `tail` (op-table rows without parameters, e.g.
         a Pauli noise trajectory) runs after the circuit."""
+        self._token = getattr(self, "_token", 0) + 1
         if isinstance(hamiltonian, PauliSumOperator) and (tail is None or len(tail) == 0):
             self.ops_for(theta)  # validates the length
             b = self._prep_batch(theta, hamiltonian, initial_state)
@@ -173,6 +174,26 @@ class AdjointProgram:
                                    _capi.dptr(grad)))
         return energy.value, grad
 
+    def energy_for_backward(self, theta, hamiltonian, initial_state=None) -> Tuple[float, int]:
+        """Energy of a Pauli-sum Hamiltonian from an energies-only run that
+        leaves the reverse sweep's inputs on the device; returns (energy,
+        token).  :meth:`grad_for_backward` with the token then runs only the
+        reverse sweep, if no other evaluation came in between."""
+        self.ops_for(theta)
+        b = self._prep_batch(theta, hamiltonian, initial_state)
+        e = float(b.energies()[0])
+        self._token = getattr(self, "_token", 0) + 1
+        return e, self._token
+
+    def grad_for_backward(self, token, theta, hamiltonian, initial_state=None) -> np.ndarray:
+        """Gradient for the evaluation `token` named: the reverse sweep alone
+        when that run is still the last one on the device, else a full
+        energy + gradient run at `theta`."""
+        if token == getattr(self, "_token", None) and getattr(self, "_bham", None) is hamiltonian:
+            return self._batch.run_reverse()[0]
+        self._token = getattr(self, "_token", 0) + 1
+        return self.value_and_grad(theta, hamiltonian, initial_state)[1]
+
     def energy(self, theta, hamiltonian, initial_state=None, noise=None) -> float:
         """Forward only (no gradient sweep); `noise` (a NoiseSpec) applies one
         trajectory to the final state before the energy."""
@@ -202,10 +223,12 @@ class AdjointProgram:
 
 
 class _BatchState:
-    """Host-transfer view of a batch instance's final state (for DeviceState)."""
+    """A batch instance's final state seen as a DeviceState handle (host
+    transfers and zero-copy device views)."""
 
     def __init__(self, batch, n, dtype):
         self.batch, self.n, self.dtype = batch, n, np.dtype(dtype)
+        self.device = batch.device
 
     @property
     def dim(self):
@@ -213,6 +236,9 @@ class _BatchState:
 
     def download(self):
         return self.batch.get_state(0, self.n)
+
+    def device_ptr(self) -> int:
+        return self.batch.state_device_ptr(0)
 
 
 def _dense_h(hamiltonian, n):
@@ -250,17 +276,32 @@ def _make_autograd():
     import torch
 
     class AdjointEnergy(torch.autograd.Function):
-        """E(theta) with dE/dtheta from one adjoint sweep (computed in forward)."""
+        """E(theta) whose backward runs the adjoint reverse sweep.
+
+        For a Pauli-sum Hamiltonian, forward runs the circuit and the energy
+        only (the run keeps the state copy and lambda = H psi on the device)
+        and backward runs just the reverse sweep over them, so an evaluation
+        whose backward never runs costs no gradient work.  Dense Hamiltonians
+        and noise tails take the one-call adjoint in forward."""
 
         @staticmethod
         def forward(ctx, theta, program, hamiltonian, initial_state, tail):
-            e, g = program.value_and_grad(theta.detach().cpu().numpy(), hamiltonian, initial_state, tail)
-            ctx.save_for_backward(torch.as_tensor(g, dtype=theta.dtype, device=theta.device))
+            th = theta.detach().cpu().numpy()
+            ctx.program, ctx.hamiltonian, ctx.initial_state, ctx.theta = program, hamiltonian, initial_state, th
+            ctx.dtype, ctx.device = theta.dtype, theta.device
+            if isinstance(hamiltonian, PauliSumOperator) and (tail is None or len(tail) == 0):
+                e, ctx.token = program.energy_for_backward(th, hamiltonian, initial_state)
+                ctx.grad = None
+            else:
+                e, ctx.grad = program.value_and_grad(th, hamiltonian, initial_state, tail)
             return torch.as_tensor(e, dtype=theta.dtype, device=theta.device)
 
         @staticmethod
         def backward(ctx, grad_out):
-            (g,) = ctx.saved_tensors
+            g = ctx.grad
+            if g is None:
+                g = ctx.program.grad_for_backward(ctx.token, ctx.theta, ctx.hamiltonian, ctx.initial_state)
+            g = torch.as_tensor(g, dtype=ctx.dtype, device=ctx.device)
             return grad_out * g, None, None, None, None
 
     return AdjointEnergy

@@ -26,6 +26,7 @@ class Batch:
 
     def __init__(self, dtype="complex64", device: int = 0, compensated=None):
         self.code = _capi.C64 if np.dtype(dtype) == np.complex64 else _capi.C128
+        self.device = int(device)
         p = C.c_void_p()
         check(lib.desv_batch_create(self.code, device, C.byref(p)))
         self.ptr = p
@@ -98,6 +99,22 @@ class Batch:
         if ns and all(n == ns[0] for n in ns):
             return out.reshape(len(ns), 1 << ns[0])
         return out
+
+    def run_reverse(self) -> List[np.ndarray]:
+        """Gradients for the last energies-only run (:meth:`energies`), from
+        the psi copy and lambda it left on the device: the reverse sweep alone."""
+        g = np.zeros(max(1, sum(self.n_params)), dtype=np.float64)
+        check(lib.desv_batch_run_reverse(self.ptr, _capi.dptr(g)))
+        out, off = [], 0
+        for k in self.n_params:
+            out.append(g[off:off + k].copy())
+            off += k
+        return out
+
+    def state_device_ptr(self, index: int) -> int:
+        p = C.c_void_p()
+        check(lib.desv_batch_state_device_ptr(self.ptr, index, C.byref(p)))
+        return int(p.value or 0)
 
     def get_state(self, index: int, n_qubits: int) -> np.ndarray:
         out = np.empty(1 << n_qubits, dtype=np.complex64 if self.code == _capi.C64 else np.complex128)

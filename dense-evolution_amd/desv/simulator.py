@@ -124,6 +124,13 @@ class StateHandle:
     def synchronize(self) -> None:
         check(lib.desv_synchronize(self.ptr))
 
+    def device_ptr(self) -> int:
+        """Device address of the amplitudes (changes when a streaming pass
+        swaps the double buffer)."""
+        p = C.c_void_p()
+        check(lib.desv_state_device_ptr(self.ptr, C.byref(p)))
+        return int(p.value or 0)
+
 
 class DeviceState:
     """Array-like view of a device-resident state vector.
@@ -167,6 +174,27 @@ class DeviceState:
 
     def get(self) -> np.ndarray:
         return self._h.download()
+
+    @property
+    def __cuda_array_interface__(self):
+        """Zero-copy device view (version 3 of the array interface, which
+        PyTorch on ROCm reads as well): the amplitudes as a (2^n, 2) float
+        array of (re, im) pairs.  Pending work on the state is finished first.
+        The view aliases the current buffer: a later gate application may move
+        the state to the other half of its double buffer."""
+        if hasattr(self._h, "synchronize"):
+            self._h.synchronize()
+        ptr = self._h.device_ptr()
+        real = np.float32 if self.dtype == np.complex64 else np.float64
+        return {"shape": (self._h.dim, 2), "typestr": np.dtype(real).str, "data": (ptr, False),
+                "version": 3, "strides": None, "stream": None}
+
+    def to_torch(self):
+        """A complex torch tensor on the state's device that aliases the
+        amplitudes (no copy; see ``__cuda_array_interface__``)."""
+        import torch
+        t = torch.as_tensor(self, device=f"cuda:{getattr(self._h, 'device', 0)}")
+        return torch.view_as_complex(t)
 
     def __getitem__(self, key):
         return np.asarray(self)[key]

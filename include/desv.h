@@ -76,6 +76,11 @@ int desv_set_double_buffer(desv_handle h, int enable);
  * the reference (its complex64 path, use_float32, rounds each gate matrix
  * once: circuits/compiler.py:172-179). */
 int desv_set_compensated(desv_handle h, int enable);
+/* Device address of the state (2^n interleaved complex values).  Valid until
+ * the next call that may swap the state buffer (a streaming gate pass with
+ * double buffering) or destroys the handle; for zero-copy views such as
+ * torch tensors.  No counterpart in the reference (JAX owns its buffers). */
+int desv_state_device_ptr(desv_handle h, void **out);
 /* |0...0> */
 int desv_reset(desv_handle h);
 /* Copy 2^n host amplitudes in the handle's dtype (no normalisation). */
@@ -241,6 +246,17 @@ int desv_batch_add(desv_batch b, int n_qubits, const double *ops, int64_t n_ops,
                    const int64_t *param_of_op, int64_t n_params, const uint64_t *xmask,
                    const uint64_t *zmask, const double *coeff, int64_t n_terms,
                    int64_t *index);
+/* Gradients for the last energies-only desv_batch_run (grads == NULL), from
+ * the psi copy and lambda that run left on the device: the
+ * reverse sweep alone, so an energies-only run followed by this call costs one
+ * full run.  DESV_EINVAL if parameters, initial states or the launch shape
+ * changed since that run.  Serves autograd bindings that compute the energy
+ * in forward and the gradient only when backward runs (reference: the
+ * caller's jax.value_and_grad over energy_fn, solvers/autodiff.py:92-169). */
+int desv_batch_run_reverse(desv_batch b, double *grads);
+/* Device address of instance i's final state (valid until the batch is
+ * rebuilt), for zero-copy views such as torch tensors. */
+int desv_batch_state_device_ptr(desv_batch b, int64_t index, void **out);
 /* Update the theta-carrying angles of instance i (n_params values). */
 int desv_batch_set_params(desv_batch b, int64_t index, const double *theta);
 /* Initial state of instance i (2^n amplitudes in the batch dtype), or NULL

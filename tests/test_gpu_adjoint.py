@@ -94,18 +94,45 @@ def test_phase_type_generators():
 
 
 def test_torch_autograd_binding():
+    """torch.autograd.grad through the energy function equals the oracle's
+    adjoint gradient; forward runs the energy only and backward the reverse
+    sweep (a second evaluation in between forces a full rerun, still exact);
+    the returned state is a zero-copy device tensor view."""
     torch = pytest.importorskip("torch")
-    n = 6
-    circ, theta, terms = workloads.vqe_instance(n, 2, 20, seed=9)
+    n = 8
+    circ, theta, terms = workloads.vqe_instance(n, 3, 20, seed=9)
     fn, _ = desv.circuit_to_energy_fn(circ, n)
     H = desv.PauliSumOperator(terms, n)
+    tmpl = O.build_template(circ.ops, n)
+    e_ref, g_ref = O.adjoint_gradient(tmpl, theta, terms, n)
     th = torch.tensor(theta, dtype=torch.float64, requires_grad=True)
     e, sv = fn(th, H)
-    e.backward()
-    e_ref, g_ref = O.adjoint_gradient(O.build_template(circ.ops, n), theta, terms, n)
-    assert abs(float(e) - e_ref) < 1e-10
-    assert np.allclose(th.grad.numpy(), g_ref, atol=1e-10)
-    assert sv.shape == (1 << n,)
+    assert abs(e.item() - e_ref) < 1e-10
+    (g,) = torch.autograd.grad(e, th)
+    assert np.allclose(g.numpy(), g_ref, atol=1e-10)
+    # interleaved evaluations: backward of the first after a second forward
+    theta2 = theta + 0.3
+    e1, _ = fn(th, H)
+    e2, _ = fn(torch.tensor(theta2, requires_grad=True), H)
+    (g1,) = torch.autograd.grad(e1, th)
+    assert np.allclose(g1.numpy(), g_ref, atol=1e-10)
+    # the state as a device tensor aliasing the function's buffer
+    e3, sv = fn(th, H)
+    t = sv.to_torch()
+    assert t.is_cuda and t.dtype == torch.complex128 and t.shape == (1 << n,)
+    assert t.data_ptr() == sv.handle.device_ptr()
+    assert np.allclose(t.cpu().numpy(), np.asarray(sv), atol=0)
+    want = O.run_ops_fast(O.zero_state(n), O.patch_template(tmpl, theta))
+    assert np.allclose(t.cpu().numpy(), want, atol=1e-10)
+    # a simulator's state: writes through the view are seen by the engine
+    sim = desv.DenseSVSimulator(n, use_float32=True)
+    sim.run_circuit([('h', q) for q in range(n)])
+    v = sim.sv.to_torch()
+    assert v.dtype == torch.complex64 and v.data_ptr() == sim.handle.device_ptr()
+    v.mul_(2.0)
+    torch.cuda.synchronize()
+    assert abs(sim.handle.norm2() - 4.0) < 1e-5
+    assert abs(float(e2.detach()) - O.energy(O.patch_template(tmpl, theta2), terms, n)[0]) < 1e-10
 
 
 def test_parameter_shift_baseline_on_gpu():
