@@ -9,6 +9,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -18,6 +19,13 @@
 #include "kernels.h"
 
 using namespace desv;
+
+// Partials of desv_norm2_device, the entry point that allocates nothing: a
+// file-scope device array (each GPU the library runs on holds its own copy
+// until the process ends).  Host code never passes the array's name to a
+// kernel: it uses the per-device address hipGetSymbolAddress returns for it.
+constexpr int kScratchDoubles = 8192;  // 64 KiB: one FP64 partial per workgroup
+static __device__ __align__(256) double scratch[kScratchDoubles];
 
 namespace {
 
@@ -519,6 +527,44 @@ int desv_state_device_ptr(desv_handle h, void **out) {
     if (int rc = check_handle(h)) return rc;
     if (!out) return fail(DESV_EINVAL, "null output pointer");
     *out = h->sv;
+    return DESV_OK;
+}
+
+namespace {
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[64] = {};
+
+// Device address of `scratch` on `device`, looked up once per device with
+// that device current (the caller has made it current).
+int scratch_on(int device, double **out) {
+    if (device < 0 || device >= 64) return fail(DESV_EINVAL, "device index out of range for the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(DESV_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[device] = static_cast<double *>(p);
+    }
+    *out = g_scratch_ptr[device];
+    return DESV_OK;
+}
+}  // namespace
+
+int desv_norm2_device(desv_handle h, double *out_dev) {
+    if (int rc = check_handle(h)) return rc;
+    if (!out_dev) return fail(DESV_EINVAL, "null output pointer");
+    HIP_TRY(hipSetDevice(h->device));
+    const int nblk = reduce_blocks(h->dim);
+    if (nblk > kScratchDoubles)
+        return fail(DESV_EINVAL, "the reduction needs " + std::to_string(nblk) + " partials; the scratch holds " +
+                                     std::to_string(kScratchDoubles));
+    double *part = nullptr;
+    if (int rc = scratch_on(h->device, &part)) return rc;
+    // every workgroup writes its own partial, so nothing needs clearing
+    HIP_TRY(h->dtype == DESV_C64 ? launch_norm2<float>(h->sv, h->dim, part, nblk, h->stream)
+                                 : launch_norm2<double>(h->sv, h->dim, part, nblk, h->stream));
+    HIP_TRY(launch_sum_partials(part, 1, nblk, out_dev, h->stream));
     return DESV_OK;
 }
 

@@ -71,6 +71,7 @@ _SIGS = {
     "desv_batch_run_reverse": ([_p, _pd], _i),
     "desv_batch_state_device_ptr": ([_p, _i64, C.POINTER(_p)], _i),
     "desv_state_device_ptr": ([_p, C.POINTER(_p)], _i),
+    "desv_norm2_device": ([_p, _p], _i),
     "desv_reset": ([_p], _i),
     "desv_set_state": ([_p, _p], _i),
     "desv_get_state": ([_p, _p], _i),

@@ -98,6 +98,15 @@ int desv_scale(desv_handle h, double re, double im);
  * measure (:318-334) and the vdot of statevector_fidelity
  * (utils/measurement.py:71-94). */
 int desv_norm2(desv_handle h, double *out);
+/* desv_norm2 for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops): sum |a_i|^2 in FP64 into *out_dev, a double in
+ * device memory, asynchronously on the handle's stream.  No counterpart in
+ * the reference.  Allocates nothing: the per-workgroup partials live in a
+ * fixed 64 KiB device array (8192 partials, enough for every state size;
+ * larger requests are DESV_EINVAL); the result is bit-identical to
+ * desv_norm2's.  Every call on one GPU shares that array: calls that overlap
+ * on one GPU (other streams, other threads) race. */
+int desv_norm2_device(desv_handle h, double *out_dev);
 int desv_inner(desv_handle a, desv_handle b, double *re, double *im);
 /* |a_i|^2 as float64 into a host buffer of 2^n doubles (raw, not clipped). */
 int desv_probabilities(desv_handle h, double *out_host);
