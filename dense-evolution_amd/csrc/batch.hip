@@ -826,6 +826,8 @@ int desv_batch_configure(desv_batch b, int tile_bits, int fuse_bits, int use_mfm
     if (fuse_bits == 0) fuse_bits = 2;
     if (fuse_bits < 1 || fuse_bits > 5) return bfail(DESV_EINVAL, "fuse_bits must be in [1, 5]");
     const int mf = use_mfma != 0, ug = use_graph != 0;
+    if (mf && b->dtype != DESV_C64)
+        return bfail(DESV_EINVAL, "use_mfma applies to complex64 batches only (complex128 blocks run on VALU)");
     if (b->tile_bits != tile_bits || b->fuse_bits != fuse_bits || b->use_mfma != mf || b->use_graph != ug)
         b->built = false;
     b->tile_bits = tile_bits;

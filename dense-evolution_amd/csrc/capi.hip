@@ -672,7 +672,7 @@ int desv_sample(desv_handle h, const double *uniforms, int64_t n_shots, int64_t 
     HIP_TRY(hipSetDevice(h->device));
     if (int rc = ensure_aux(h, 1)) return rc;  // the CDF (2^n doubles fit in a state buffer)
     const int64_t nb = cdf_blocks(h->dim);
-    if (int rc = ensure_part(h, (size_t)nb + 1)) return rc;
+    if (int rc = ensure_part(h, 2 * ((size_t)nb + 1))) return rc;
     double *cdf = reinterpret_cast<double *>(h->aux[1]);
     hipError_t e = h->dtype == DESV_C64 ? launch_cdf<float>(h->sv, h->dim, cdf, h->d_part, h->stream)
                                         : launch_cdf<double>(h->sv, h->dim, cdf, h->d_part, h->stream);
@@ -832,9 +832,11 @@ int desv_program_compile(desv_program p, const desv_compile_opts *opts) {
         if (opts->tile_bits) o.tile_bits = opts->tile_bits;
         if (opts->low_bits) o.low_bits = opts->low_bits;
         o.per_gate = opts->per_gate;
-        // FP64 MFMA has the same peak as FP64 VALU on MI355X, and its 16x16x4
-        // path did not match the oracle: complex128 blocks stay on VALU.
-        o.use_mfma = dtype == DESV_C64 ? opts->use_mfma : 0;
+        // MFMA blocks are complex64 only: FP64 MFMA has the same MI355X peak as
+        // FP64 VALU (78.6 TF), so complex128 blocks run on VALU
+        if (opts->use_mfma && dtype != DESV_C64)
+            return fail(DESV_EINVAL, "use_mfma applies to complex64 programs only (complex128 blocks run on VALU)");
+        o.use_mfma = opts->use_mfma;
         o.compensated = dtype == DESV_C64 ? opts->compensated : 0;
     }
     if (o.fuse_bits < 1 || o.fuse_bits > 5) return fail(DESV_EINVAL, "fuse_bits must be in [1, 5]");

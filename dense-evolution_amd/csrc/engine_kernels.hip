@@ -2079,49 +2079,134 @@ __global__ void __launch_bounds__(kThreads) k_cdf_totals(const typename VT<T>::c
     if (threadIdx.x == 0) blk[blockIdx.x] = t;
 }
 
-// Second CDF pass (after k_scan_blocks), one 4096-amplitude chunk per
-// workgroup: lane t sums |a|^2 over its 16 consecutive amplitudes, a
-// workgroup inclusive scan gives each lane's prefix `before`, and the
-// normalised CDF is ((before + v) + offset) / total.  Global loads and
-// stores are coalesced through LDS (element e at e + e / 16: lane t's 16
-// elements then fall in distinct banks).
+// The CDF is built from fixed-point weights so that every association of
+// the sums gives the same prefix: q_i = round(|a_i|^2 2^62 / total) as a
+// 64-bit integer (total from the FP64 pass above), chunk totals and prefixes
+// summed in integers, cdf_i = P_i / P_total in FP64.  Integer prefixes make
+// the CDF nondecreasing across chunk boundaries and give a zero-probability
+// amplitude exactly zero width, whatever the chunk and scan layout; the
+// quantisation (2^-62 of the total per amplitude) is far below FP64's
+// resolution of the prefix.
+__device__ inline double cdf_scale(const double *blk, int64_t nb) {
+    const double t = blk[nb];
+    return (t > 0.0 && t <= 1.0e300) ? 4611686018427387904.0 / t : 0.0;  // 2^62 / total
+}
+template <typename C> __device__ inline uint64_t cdf_weight(C a, double scale) {
+    const double2 d = to_d(a);
+    return (uint64_t)((d.x * d.x + d.y * d.y) * scale + 0.5);
+}
+
+__device__ inline uint64_t block_sum_u64(uint64_t v, uint64_t *red) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    uint64_t s = 0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += red[i];
+    return s;
+}
+
+__device__ inline uint64_t block_inclusive_scan_u64(uint64_t v, uint64_t *ts) {
+    ts[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < (int)blockDim.x; off <<= 1) {
+        const uint64_t t = (int)threadIdx.x >= off ? ts[threadIdx.x - off] : 0ull;
+        __syncthreads();
+        ts[threadIdx.x] += t;
+        __syncthreads();
+    }
+    return ts[threadIdx.x];
+}
+
+// qblk[chunk] = sum of the chunk's fixed-point weights (16-byte coalesced
+// loads, as in k_cdf_totals).
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_cdf_qtotals(const typename VT<T>::c *sv, int64_t dim, const double *blk,
+                                                          int64_t nb, uint64_t *qblk) {
+    using C = typename VT<T>::c;
+    constexpr int APU = 16 / (int)sizeof(C);
+    constexpr int per = kScanChunk / APU / kThreads;
+    __shared__ uint64_t red[kThreads / 64];
+    const double scale = cdf_scale(blk, nb);
+    const int64_t c0 = (int64_t)blockIdx.x * kScanChunk;
+    uint64_t acc = 0;
+    if (c0 + kScanChunk <= dim) {
+        const float4 *p = reinterpret_cast<const float4 *>(sv + c0);
+        float4 v[per];
+#pragma unroll
+        for (int k = 0; k < per; ++k) v[k] = p[k * kThreads + threadIdx.x];
+#pragma unroll
+        for (int k = 0; k < per; ++k) {
+            C a[APU];
+            __builtin_memcpy(a, &v[k], sizeof(a));
+#pragma unroll
+            for (int j = 0; j < APU; ++j) acc += cdf_weight(a[j], scale);
+        }
+    } else {
+        for (int64_t e = c0 + threadIdx.x; e < dim; e += kThreads) acc += cdf_weight(sv[e], scale);
+    }
+    const uint64_t t = block_sum_u64(acc, red);
+    if (threadIdx.x == 0) qblk[blockIdx.x] = t;
+}
+
+// qblk[0..nb) <- exclusive prefix sums; qblk[nb] <- total.  One workgroup.
+__global__ void __launch_bounds__(kThreads) k_scan_qblocks(uint64_t *qblk, int64_t nb) {
+    __shared__ uint64_t ts[kThreads];
+    const int64_t seg = (nb + kThreads - 1) / kThreads;
+    const int64_t s0 = (int64_t)threadIdx.x * seg, s1 = s0 + seg < nb ? s0 + seg : nb;
+    uint64_t run = 0;
+    for (int64_t i = s0; i < s1; ++i) run += qblk[i];
+    const uint64_t incl = block_inclusive_scan_u64(run, ts);
+    uint64_t acc = incl - run;
+    for (int64_t i = s0; i < s1; ++i) {
+        const uint64_t x = qblk[i];
+        qblk[i] = acc;
+        acc += x;
+    }
+    if (threadIdx.x == blockDim.x - 1) qblk[nb] = incl;
+}
+
+// Final CDF pass, one 4096-amplitude chunk per workgroup: lane t sums the
+// weights of its 16 consecutive amplitudes, a workgroup scan gives its
+// prefix, and cdf = (prefix + chunk offset) / total.  Global loads and stores
+// are coalesced through LDS (element e at e + e / 16: lane t's 16 elements
+// then fall in distinct banks).
 template <typename T>
 __global__ void __launch_bounds__(kThreads) k_cdf_pass(const typename VT<T>::c *sv, int64_t dim, double *cdf,
-                                                       const double *blk, int64_t nb) {
+                                                       const double *blk, const uint64_t *qblk, int64_t nb) {
     constexpr int per = kScanChunk / kThreads;
-    __shared__ double ts[kThreads];
-    __shared__ double buf[kScanChunk + kScanChunk / per];
+    __shared__ uint64_t ts[kThreads];
+    __shared__ uint64_t buf[kScanChunk + kScanChunk / per];
     auto at = [](int e) { return e + e / per; };
+    const double scale = cdf_scale(blk, nb);
     const int64_t c0 = (int64_t)blockIdx.x * kScanChunk;
     const int t = threadIdx.x;
 #pragma unroll
     for (int k = 0; k < per; ++k) {
         const int e = k * kThreads + t;
-        double p = 0;
-        if (c0 + e < dim) {
-            const double2 a = to_d(sv[c0 + e]);
-            p = a.x * a.x + a.y * a.y;
-        }
-        buf[at(e)] = p;
+        buf[at(e)] = c0 + e < dim ? cdf_weight(sv[c0 + e], scale) : 0ull;
     }
     __syncthreads();
-    double v[per], run = 0;
+    uint64_t v[per], run = 0;
 #pragma unroll
     for (int j = 0; j < per; ++j) {
         run += buf[at(t * per + j)];
         v[j] = run;
     }
-    const double incl = block_inclusive_scan(run, ts);
-    const double before = incl - run;
-    const double off = blk[blockIdx.x], total = blk[nb];
+    const uint64_t incl = block_inclusive_scan_u64(run, ts);
+    const uint64_t base = incl - run + qblk[blockIdx.x];
+    const double total = (double)qblk[nb];
     __syncthreads();  // every lane has read its elements
+    double *out = reinterpret_cast<double *>(buf);
 #pragma unroll
-    for (int j = 0; j < per; ++j) buf[at(t * per + j)] = ((before + v[j]) + off) / total;
+    for (int j = 0; j < per; ++j) out[at(t * per + j)] = (double)(base + v[j]) / total;
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < per; ++k) {
         const int e = k * kThreads + t;
-        if (c0 + e < dim) cdf[c0 + e] = buf[at(e)];
+        if (c0 + e < dim) cdf[c0 + e] = out[at(e)];
     }
 }
 
@@ -2161,11 +2246,15 @@ template <typename T>
 hipError_t launch_cdf(const void *sv, int64_t dim, double *cdf, double *blk, hipStream_t s) {
     const int64_t nb = (dim + kScanChunk - 1) / kScanChunk;
     if (nb > 0x7fffffffll) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_cdf_totals<T>), dim3((unsigned)nb), dim3(kThreads), 0, s, (const typename VT<T>::c *)sv,
-                       dim, blk);
+    using C = typename VT<T>::c;
+    uint64_t *qblk = reinterpret_cast<uint64_t *>(blk + nb + 1);
+    hipLaunchKernelGGL((k_cdf_totals<T>), dim3((unsigned)nb), dim3(kThreads), 0, s, (const C *)sv, dim, blk);
     hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(kThreads), 0, s, blk, nb);
-    hipLaunchKernelGGL((k_cdf_pass<T>), dim3((unsigned)nb), dim3(kThreads), 0, s,
-                       (const typename VT<T>::c *)sv, dim, cdf, blk, nb);
+    hipLaunchKernelGGL((k_cdf_qtotals<T>), dim3((unsigned)nb), dim3(kThreads), 0, s, (const C *)sv, dim,
+                       (const double *)blk, nb, qblk);
+    hipLaunchKernelGGL(k_scan_qblocks, dim3(1), dim3(kThreads), 0, s, qblk, nb);
+    hipLaunchKernelGGL((k_cdf_pass<T>), dim3((unsigned)nb), dim3(kThreads), 0, s, (const C *)sv, dim, cdf,
+                       (const double *)blk, (const uint64_t *)qblk, nb);
     return hipGetLastError();
 }
 int64_t cdf_blocks(int64_t dim) { return (dim + kScanChunk - 1) / kScanChunk; }

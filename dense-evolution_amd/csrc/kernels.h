@@ -152,8 +152,9 @@ hipError_t launch_inner(const void *a, const void *b, int64_t dim, double *parti
 template <typename T>
 hipError_t launch_probs(const void *sv, int64_t dim, double *out, hipStream_t s);
 // Normalised cumulative probabilities of a state into cdf[0..dim) (blk holds
-// cdf_blocks(dim) + 1 doubles of workspace; blk[cdf_blocks(dim)] = the total
-// before normalisation), and the shot search over it.
+// 2 (cdf_blocks(dim) + 1) doubles of workspace: FP64 chunk totals and their
+// scan, blk[cdf_blocks(dim)] = the total before normalisation, then the
+// fixed-point chunk totals), and the shot search over it.
 template <typename T>
 hipError_t launch_cdf(const void *sv, int64_t dim, double *cdf, double *blk, hipStream_t s);
 int64_t cdf_blocks(int64_t dim);

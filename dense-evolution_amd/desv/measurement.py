@@ -3,8 +3,9 @@
 Same behaviour as the reference's ``utils/measurement.py:17-94``: shots are
 drawn with the caller's NumPy generator (the same uniforms its
 ``rng.choice`` draws) and located in the state's cumulative probabilities on
-the GPU (FP64 scan, one binary search per shot); the overlap of two states is
-accumulated in FP64 on the GPU.
+the GPU (fixed-point integer prefix sums, so the CDF is nondecreasing and an
+exactly-zero amplitude is never drawn; one binary search per shot); the
+overlap of two states is accumulated in FP64 on the GPU.
 """
 import ctypes as C
 

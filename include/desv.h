@@ -149,7 +149,8 @@ typedef struct {
     int tile_bits;  /* amplitudes per LDS tile = 2^tile_bits; 0 = default    */
     int low_bits;   /* low index bits always in a tile (coalescing); 0 = dflt*/
     int per_gate;   /* 1: one HBM pass per fused block (no cache blocking)   */
-    int use_mfma;   /* 1: 3-5-qubit blocks run as MFMA GEMMs (complex64)     */
+    int use_mfma;   /* 1: 3-5-qubit blocks run as MFMA GEMMs (complex64 only:
+                       DESV_EINVAL for complex128, whose blocks run on VALU) */
     int dtype;      /* DESV_C64 / DESV_C128: matrix precision of the upload */
     int compensated;/* 1 (complex64): every matrix is uploaded as an FP32 pair
                        hi + lo (lo = the rounding error of hi) and applied lo

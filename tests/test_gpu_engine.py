@@ -236,10 +236,14 @@ def test_brickwork_su4_small(fuse, mfma, dtype):
     for u, a, b in gates:
         prog.add_matrix(u, [a, b])
         want = O.apply_matrix(want, u, [a, b])
+    if mfma and dtype == "complex128":
+        # MFMA blocks are complex64 only; asking for them in complex128 is an error
+        with pytest.raises(ValueError, match="complex64"):
+            prog.compile(dtype, fuse_bits=fuse, use_mfma=True)
+        return
     prog.compile(dtype, fuse_bits=fuse, use_mfma=mfma)
     if mfma:
-        # MFMA blocks are used for complex64 only (complex128 blocks run on VALU)
-        assert (prog.stats()["mfma_blocks"] > 0) == (dtype == "complex64")
+        assert prog.stats()["mfma_blocks"] > 0
     h = desv.StateHandle(n, dtype == "complex128")
     prog.run(h)
     got = h.download()
@@ -589,3 +593,38 @@ def test_entangling_layers_and_random_circuits_on_device(pattern):
     want = O.run_circuit(O.transpile(circ), 5)
     assert np.allclose(sim.get_statevector(), want, atol=1e-12)
     assert abs(sim.get_probabilities().sum() - 1.0) < 1e-9
+
+
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+def test_sampling_never_draws_zero_probability_outcomes(dtype):
+    """Amplitudes that are exactly zero (here at both ends of every
+    4096-amplitude CDF chunk, and after a measurement collapse) get zero
+    width in the device CDF: uniforms placed on and around every cumulative
+    edge never select them, and the draws are ordered with the uniforms."""
+    import ctypes as C
+    from desv import _capi
+    n = 14
+    rng = np.random.default_rng(21)
+    v = (rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)).astype(dtype)
+    v[rng.random(1 << n) < 0.3] = 0
+    v[::4096] = 0
+    v[4095::4096] = 0
+    h = desv.StateHandle(n, dtype == np.complex128)
+    h.upload(v)
+    p = np.abs(v.astype(np.complex128)) ** 2
+    edges = np.cumsum(p) / p.sum()
+    u = np.concatenate([edges, np.nextafter(edges, 0), np.nextafter(edges, 1), rng.random(20000)])
+    u = np.clip(u, 0.0, np.nextafter(1.0, 0))
+    u.sort()
+    out = np.empty(len(u), dtype=np.int64)
+    _capi.check(_capi.lib.desv_sample(h.ptr, _capi.dptr(u), len(u), _capi.i64ptr(out)))
+    assert np.all(p[out] > 0)
+    assert np.all(np.diff(out) >= 0)          # a nondecreasing CDF: sorted uniforms, sorted outcomes
+    # after a collapse half the amplitudes are zero
+    sim = desv.DenseSVSimulator(n, use_float32=dtype == np.complex64)
+    sim.sv = v
+    sim.measure(3, rng=np.random.default_rng(2))
+    pc = np.abs(sim.get_statevector().astype(np.complex128)) ** 2
+    counts = desv.sample_counts(sim.sv, 200000, rng=np.random.default_rng(4))
+    assert all(pc[int(k, 2)] > 0 for k in counts)
+    h.close()
