@@ -59,6 +59,7 @@ struct DevPlan {
     // pool, lo_off complex elements further on
     int comp = 0;
     int32_t lo_off = 0;
+    void *ltab = nullptr;  // per-lane tables of 3-qubit MFMA ops (complex64), or null
 
     void release() {
         if (buf) (void)hipFree(buf);
@@ -70,10 +71,31 @@ struct DevPlan {
 int upload_plan(const Plan &p, int dtype, int device, hipStream_t s, DevPlan &d, bool comp = false) {
     comp = comp && dtype == DESV_C64;
     if (comp && p.mats.size() > (size_t)0x3fffffff) return fail(DESV_EINVAL, "matrix pool too large");
-    const size_t ob = std::max<size_t>(p.ops.size(), 1) * sizeof(DevOp);
+    // matrices in the state precision (complex64: hi, then lo when compensated)
+    const size_t nm = std::max<size_t>(p.mats.size(), 1) * (comp ? 2 : 1);
+    std::vector<float> mf;
+    std::vector<double> md;
+    if (dtype == DESV_C64) {
+        mf.assign(2 * nm, 0.f);
+        round_mats_f32(p, mf.data());
+        if (comp) round_mats_f32_lo(p, mf.data(), mf.data() + 2 * p.mats.size());
+    } else {
+        md.assign(2 * nm, 0.0);
+        for (size_t i = 0; i < p.mats.size(); ++i) {
+            md[2 * i] = p.mats[i].real();
+            md[2 * i + 1] = p.mats[i].imag();
+        }
+    }
+    // ops, with the per-lane tables of 3-qubit MFMA ops (complex64)
+    std::vector<DevOp> ops = p.ops;
+    std::vector<int32_t> lt;
+    if (dtype == DESV_C64)
+        build_lane_tables(p, mf.data(), comp ? mf.data() + 2 * p.mats.size() : nullptr, ops, lt);
+    const size_t ob = std::max<size_t>(ops.size(), 1) * sizeof(DevOp);
     const size_t oba = (ob + 255) & ~size_t(255);
-    const size_t mb = std::max<size_t>(p.mats.size(), 1) * amp_bytes(dtype) * (comp ? 2 : 1);
-    const size_t need = oba + mb;
+    const size_t mb = nm * amp_bytes(dtype);
+    const size_t mba = (mb + 255) & ~size_t(255);
+    const size_t need = oba + mba + lt.size() * sizeof(int32_t);
     HIP_TRY(hipStreamSynchronize(s));  // the previous upload may still be read
     if (d.cap < need || d.device != device) {
         d.release();
@@ -81,22 +103,17 @@ int upload_plan(const Plan &p, int dtype, int device, hipStream_t s, DevPlan &d,
         d.cap = need;
     }
     std::vector<unsigned char> host(need);
-    std::memcpy(host.data(), p.ops.data(), p.ops.size() * sizeof(DevOp));
-    if (dtype == DESV_C64) {
-        float *f = reinterpret_cast<float *>(host.data() + oba);
-        round_mats_f32(p, f);
-        if (comp) round_mats_f32_lo(p, f, f + 2 * p.mats.size());
-    } else {
-        double *f = reinterpret_cast<double *>(host.data() + oba);
-        for (size_t i = 0; i < p.mats.size(); ++i) {
-            f[2 * i] = p.mats[i].real();
-            f[2 * i + 1] = p.mats[i].imag();
-        }
-    }
+    std::memcpy(host.data(), ops.data(), ops.size() * sizeof(DevOp));
+    if (dtype == DESV_C64)
+        std::memcpy(host.data() + oba, mf.data(), mb);
+    else
+        std::memcpy(host.data() + oba, md.data(), mb);
+    if (!lt.empty()) std::memcpy(host.data() + oba + mba, lt.data(), lt.size() * sizeof(int32_t));
     HIP_TRY(hipMemcpyAsync(d.buf, host.data(), need, hipMemcpyHostToDevice, s));
     HIP_TRY(hipStreamSynchronize(s));
     d.ops = reinterpret_cast<DevOp *>(d.buf);
     d.mats = reinterpret_cast<unsigned char *>(d.buf) + oba;
+    d.ltab = lt.empty() ? nullptr : reinterpret_cast<unsigned char *>(d.buf) + oba + mba;
     d.device = device;
     d.dtype = dtype;
     d.comp = comp;
@@ -293,6 +310,7 @@ int run_plan_on(desv_handle h, const Plan &plan, const DevPlan &dp, int nv, void
         t.ops = dp.ops;
         t.mats = dp.mats;
         t.lo_off = dp.lo_off;
+        t.ltab = dp.ltab;
         t.partials = h->d_part;
         t.pstride = pstride;
         t.wg_begin = 0;

@@ -83,8 +83,19 @@ struct DevOp {
     int32_t pos[5];    // tile-local bit of matrix index bit j; for OPK_DIAG a
                        // negative value -1-b means global bit b (outside tile)
     uint32_t cloc;     // controls on tile-local bits
+    int32_t lt = -1;   // 3-qubit MFMA ops: index of the op's per-lane table
+                       // (build_lane_tables), -1: computed in the kernel
     uint64_t cglob;    // controls on global bits outside the tile
 };
+
+// Per-lane constants of a 3-qubit MFMA op for a 256-lane workgroup, four
+// 16-byte words per lane: LDS byte slots (B loads rows g, 4+g; C stores rows
+// 2g, 2g+1), the column-block slot tables of the lane (block g and 64 g), and
+// the stacked real A fragments (hi parts, then lo parts of compensated
+// matrices).  Computed once per program on the host instead of in every
+// workgroup of every stage.
+constexpr int kLaneTableWords = 16;  // int32 per lane
+constexpr int kLaneTableLanes = 256;
 
 constexpr int MAX_HBITS = 24;
 constexpr int MAX_TILE_BITS = 16;
@@ -155,5 +166,11 @@ void round_mats_f32(const Plan &p, float *out);
 // lo[i] = the FP32 rounding of plan.mats[i] - hi[i] (hi from round_mats_f32):
 // hi + lo carries each entry to about 2^-48 relative.
 void round_mats_f32_lo(const Plan &p, const float *hi, float *lo);
+
+// Fill the per-lane tables of every 3-qubit MFMA op of `plan` (complex64
+// matrices hi, lo as uploaded; lo may be null) into `out` and set the ops'
+// `lt` indices in `ops` (a copy of plan.ops).
+void build_lane_tables(const Plan &plan, const float *hi, const float *lo, std::vector<DevOp> &ops,
+                       std::vector<int32_t> &out);
 
 }  // namespace desv

@@ -96,6 +96,7 @@ template <typename X> struct RawOf;
 template <> struct RawOf<float2> { using t = float __attribute__((ext_vector_type(2))); };
 template <> struct RawOf<float4> { using t = float __attribute__((ext_vector_type(4))); };
 template <> struct RawOf<double2> { using t = double __attribute__((ext_vector_type(2))); };
+template <> struct RawOf<int4> { using t = int __attribute__((ext_vector_type(4))); };
 template <typename X> __device__ inline X gload(const DESV_GLOBAL X *p) {
     using R = typename RawOf<X>::t;
     const R r = *(const DESV_GLOBAL R *)p;
@@ -317,6 +318,8 @@ struct BlockSlots {
         lo = slot(spread((wave << 4) | (lane << (4 + kLogWaves)), fixed)) * eb;
         hi = slot(spread(lane << (10 + kLogWaves), fixed)) * eb;
     }
+    struct Given {};
+    __device__ BlockSlots(Given, int lo_, int hi_) : lo(lo_), hi(hi_) {}
     __device__ int operator()(int g) const {
         const int l = __builtin_amdgcn_readlane(lo, g & 63);
         return g < 64 ? l : l ^ __builtin_amdgcn_readlane(hi, g >> 6);
@@ -522,7 +525,8 @@ __device__ inline void op_mfma(char *lds, int vstride, int m, const DevOp &op, M
 // -> R row 4 (l/16) + j, so lane group g holds outputs 2g and 2g+1 as
 // (re, im) register pairs.
 template <int NV, bool CP, typename MP>
-__device__ inline void op_mfma3(char *lds, int vstride, int m, const DevOp &op, MP M, MP ML) {
+__device__ inline void op_mfma3(char *lds, int vstride, int m, const DevOp &op, MP M, MP ML,
+                                const DESV_GLOBAL int4 *lt) {
     constexpr int EB = sizeof(float2);
     const int lane = threadIdx.x & 63;
     const int i16 = lane & 15, g4 = lane >> 4;
@@ -530,22 +534,41 @@ __device__ inline void op_mfma3(char *lds, int vstride, int m, const DevOp &op, 
     const int p0 = op.pos[0], p1 = op.pos[1], p2 = op.pos[2];
     const int fixed = cl | (1 << p0) | (1 << p1) | (1 << p2);
     auto roff = [&](int r) { return ((r & 1) << p0) | ((r >> 1 & 1) << p1) | ((r >> 2 & 1) << p2); };
-    float a[4];
+    float a[4], la[4];  // A fragments; lo parts of the compensated form (accumulated first)
+    int pb0, pb1, py0, py1, blo = 0, bhi = 0;
+    if (lt) {
+        // slots and fragments from the op's host-built lane table
+        const DESV_GLOBAL int4 *w = lt + threadIdx.x * (kLaneTableWords / 4);
+        const int4 w0 = gload(w), w1 = gload(w + 1), w2 = gload(w + 2);
+        pb0 = w0.x, pb1 = w0.y, py0 = w0.z, py1 = w0.w, blo = w1.x, bhi = w1.y;
+        a[0] = __int_as_float(w2.x), a[1] = __int_as_float(w2.y), a[2] = __int_as_float(w2.z),
+        a[3] = __int_as_float(w2.w);
 #pragma unroll
-    for (int ks = 0; ks < 4; ++ks) {
-        const int kk = 4 * ks + g4;
-        const float2 u = gload(M + (i16 >> 1) * 8 + (kk & 7));
-        a[ks] = kk < 8 ? ((i16 & 1) ? u.y : u.x) : ((i16 & 1) ? u.x : -u.y);
-    }
-    float la[4];  // lo parts of the compensated form (accumulated first)
-#pragma unroll
-    for (int ks = 0; ks < 4; ++ks) {
-        la[ks] = 0.f;
+        for (int ks = 0; ks < 4; ++ks) la[ks] = 0.f;
         if constexpr (CP) {
-            const int kk = 4 * ks + g4;
-            const float2 u = gload(ML + (i16 >> 1) * 8 + (kk & 7));
-            la[ks] = kk < 8 ? ((i16 & 1) ? u.y : u.x) : ((i16 & 1) ? u.x : -u.y);
+            const int4 w3 = gload(w + 3);
+            la[0] = __int_as_float(w3.x), la[1] = __int_as_float(w3.y), la[2] = __int_as_float(w3.z),
+            la[3] = __int_as_float(w3.w);
         }
+    } else {
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) {
+            const int kk = 4 * ks + g4;
+            const float2 u = gload(M + (i16 >> 1) * 8 + (kk & 7));
+            a[ks] = kk < 8 ? ((i16 & 1) ? u.y : u.x) : ((i16 & 1) ? u.x : -u.y);
+        }
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) {
+            la[ks] = 0.f;
+            if constexpr (CP) {
+                const int kk = 4 * ks + g4;
+                const float2 u = gload(ML + (i16 >> 1) * 8 + (kk & 7));
+                la[ks] = kk < 8 ? ((i16 & 1) ? u.y : u.x) : ((i16 & 1) ? u.x : -u.y);
+            }
+        }
+        const int pl = spread(i16, fixed) | cl;
+        pb0 = slot(pl | roff(g4)) * EB, pb1 = slot(pl | roff(4 + g4)) * EB;
+        py0 = slot(pl | roff(2 * g4)) * EB, py1 = slot(pl | roff(2 * g4 + 1)) * EB;
     }
     // y = R [Xr; Xi] (+ the lo part first when compensated) for one column block
     auto mv = [&](float2 x0, float2 x1) {
@@ -562,16 +585,13 @@ __device__ inline void op_mfma3(char *lds, int vstride, int m, const DevOp &op, 
         y = __builtin_amdgcn_mfma_f32_16x16x4f32(a[3], x1.y, y, 0, 0, 0);
         return y;
     };
-    const int pl = spread(i16, fixed) | cl;
-    const int pb0 = slot(pl | roff(g4)) * EB, pb1 = slot(pl | roff(4 + g4)) * EB;
-    const int py0 = slot(pl | roff(2 * g4)) * EB, py1 = slot(pl | roff(2 * g4 + 1)) * EB;
     const int G = 1 << (m - __builtin_popcount(fixed));
     if (G >= 16 * kWaves) {
         // Every wave owns G / (16 kWaves) full column blocks: no lane
         // predicates, wave-uniform (scalar) block addresses from a lane
         // table, and the next block's loads issued before this block's stores.
         const int nit = G / (16 * kWaves);
-        const BlockSlots bs(fixed, EB);
+        const BlockSlots bs = lt ? BlockSlots(BlockSlots::Given{}, blo, bhi) : BlockSlots(fixed, EB);
         int Uc = bs(0);
         float2 x0[NV], x1[NV];
 #pragma unroll
@@ -1025,6 +1045,7 @@ k_stage(const StageTask *__restrict__ tasks,
     // lo halves of a compensated program's matrices: the pool's second half
     const DESV_GLOBAL C *matl = CP ? mats + uni(tk.lo_off) : mats;
     const DESV_GLOBAL DevOp *ops = gptr(tk.ops);
+    const DESV_GLOBAL int4 *ltab = tk.ltab ? gptr(reinterpret_cast<const int4 *>(tk.ltab)) : nullptr;
     const bool permuted = uni(tk.st.permuted) != 0;
     const int n_grad = uni(tk.st.n_grad);
 
@@ -1106,7 +1127,10 @@ k_stage(const StageTask *__restrict__ tasks,
                 break;
             case OPK_MFMA:
                 if constexpr (VAR >= 2 && std::is_same<T, float>::value) {
-                    if (op.k == 3) op_mfma3<NV, CP>(lds, vstride, m, op, M, ML);
+                    if (op.k == 3)
+                        op_mfma3<NV, CP>(lds, vstride, m, op, M, ML,
+                                         ltab && op.lt >= 0 ? ltab + (size_t)op.lt * (kLaneTableLanes * kLaneTableWords / 4)
+                                                            : nullptr);
                     else if (op.k == 4) op_mfma<NV, 4, CP>(lds, vstride, m, op, M, ML);
                     else if constexpr (VAR == 3) op_mfma<NV, 5, CP>(lds, vstride, m, op, M, ML);
                 }

@@ -40,6 +40,7 @@ struct StageTask {
     void *sv1;           // lambda (adjoint reverse sweep), else unused
     const DevOp *ops;
     const void *mats;    // complex pool in the state precision
+    const void *ltab;    // per-lane tables of 3-qubit MFMA ops (DevOp::lt), or null
     double *partials;    // gradient partials [slot][pstride]
     int64_t pstride;     // tiles per gradient slot row
     int64_t wg_begin;

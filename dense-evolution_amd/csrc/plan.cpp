@@ -420,6 +420,7 @@ bool lower_stage(const std::vector<Gate> &ops, uint64_t tile, int n, const Compi
     for (const Gate &g : ops) {
         DevOp d;
         std::memset(&d, 0, sizeof(d));
+        d.lt = -1;
         d.k = g.k;
         d.mat = (int)plan.mats.size();
         d.slot = g.grad_slot;
@@ -903,6 +904,54 @@ void round_mats_f32(const Plan &p, float *out) {
     for (size_t i = 0; i < p.mats.size(); ++i) {
         out[2 * i] = (float)p.mats[i].real();
         out[2 * i + 1] = (float)p.mats[i].imag();
+    }
+}
+
+namespace {
+int lt_slot(int i) { return i ^ ((i >> 5) & 31); }
+int lt_spread(int x, int fixed) {
+    for (int f = fixed; f; f &= f - 1) {
+        const int b = __builtin_ctz(f);
+        x = ((x >> b) << (b + 1)) | (x & ((1 << b) - 1));
+    }
+    return x;
+}
+}  // namespace
+
+void build_lane_tables(const Plan &plan, const float *hi, const float *lo, std::vector<DevOp> &ops,
+                       std::vector<int32_t> &out) {
+    constexpr int kLogWaves = 2, EB = 8;  // 4 waves of 64 lanes; complex64 slots
+    out.clear();
+    ops = plan.ops;
+    int32_t next = 0;
+    for (DevOp &op : ops) {
+        if (op.kind != OPK_MFMA || op.k != 3) continue;
+        op.lt = next++;
+        const int p0 = op.pos[0], p1 = op.pos[1], p2 = op.pos[2], cl = (int)op.cloc;
+        const int fixed = cl | (1 << p0) | (1 << p1) | (1 << p2);
+        auto roff = [&](int r) { return ((r & 1) << p0) | ((r >> 1 & 1) << p1) | ((r >> 2 & 1) << p2); };
+        auto frag = [&](const float *m, int i16, int g4, int ks) -> float {
+            const int kk = 4 * ks + g4;
+            const float *u = m + 2 * ((size_t)op.mat + (size_t)(i16 >> 1) * 8 + (kk & 7));
+            return kk < 8 ? ((i16 & 1) ? u[1] : u[0]) : ((i16 & 1) ? u[0] : -u[1]);
+        };
+        for (int t = 0; t < kLaneTableLanes; ++t) {
+            const int lane = t & 63, wave = t >> 6, i16 = lane & 15, g4 = lane >> 4;
+            const int pl = lt_spread(i16, fixed) | cl;
+            int32_t w[kLaneTableWords] = {0};
+            w[0] = lt_slot(pl | roff(g4)) * EB;
+            w[1] = lt_slot(pl | roff(4 + g4)) * EB;
+            w[2] = lt_slot(pl | roff(2 * g4)) * EB;
+            w[3] = lt_slot(pl | roff(2 * g4 + 1)) * EB;
+            w[4] = lt_slot(lt_spread((wave << 4) | (lane << (4 + kLogWaves)), fixed)) * EB;
+            w[5] = lt_slot(lt_spread(lane << (10 + kLogWaves), fixed)) * EB;
+            for (int ks = 0; ks < 4; ++ks) {
+                const float a = frag(hi, i16, g4, ks), b = lo ? frag(lo, i16, g4, ks) : 0.f;
+                std::memcpy(&w[8 + ks], &a, 4);
+                std::memcpy(&w[12 + ks], &b, 4);
+            }
+            out.insert(out.end(), w, w + kLaneTableWords);
+        }
     }
 }
 
