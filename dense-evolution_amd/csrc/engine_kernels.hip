@@ -601,7 +601,9 @@ __device__ inline void op_mfma3(char *lds, int vstride, int m, const DevOp &op, 
         }
         for (int it = 0; it < nit; ++it) {
             const bool more = it + 1 < nit;
-            const int Un = more ? bs(it + 1) : Uc;
+            // a complex64 tile fits LDS up to 2^14 amplitudes, so a wave steps
+            // through at most 32 column blocks and `lo` alone holds their slots
+            const int Un = more ? (nit <= 64 ? __builtin_amdgcn_readlane(bs.lo, it + 1) : bs(it + 1)) : Uc;
             f32x4_t y[NV];
 #pragma unroll
             for (int v = 0; v < NV; ++v) y[v] = mv(x0[v], x1[v]);
@@ -1089,18 +1091,30 @@ k_stage(const StageTask *__restrict__ tasks,
     for (int i = 0; i < PV; ++i) ql[i] = slot(permuted ? perm(le + i) : le + i) * EB;
     __syncthreads();  // jtab / qtab / gacc ready
 
-    // load: chunk j covers amplitudes j*CH .. j*CH + CH - 1 of the tile
+    // load: chunk j covers amplitudes j*CH .. j*CH + CH - 1 of the tile; LB
+    // chunks of loads in flight before their LDS writes
+    constexpr int LB = 8 / NV;
     if (lane_on)
-        for (int j = 0; j < nchunks; ++j) {
-            const uint64_t off = gl | jtab[j];
-            const int S = slot(j * CH) * EB;
+        for (int jb = 0; jb < nchunks; jb += LB) {
+            V x[LB][NV];
 #pragma unroll
-            for (int v = 0; v < NV; ++v) {
-                const V x = gload((const DESV_GLOBAL V *)(svp[v] + off));
-                const C *xc = reinterpret_cast<const C *>(&x);
+            for (int u = 0; u < LB; ++u)
+                if (jb + u < nchunks) {
+                    const uint64_t off = gl | jtab[jb + u];
 #pragma unroll
-                for (int i = 0; i < PV; ++i) lds_st<C>(lds + v * vstride, (sl ^ S) ^ (i * EB), xc[i]);
-            }
+                    for (int v = 0; v < NV; ++v) x[u][v] = gload((const DESV_GLOBAL V *)(svp[v] + off));
+                }
+#pragma unroll
+            for (int u = 0; u < LB; ++u)
+                if (jb + u < nchunks) {
+                    const int S = slot((jb + u) * CH) * EB;
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) {
+                        const C *xc = reinterpret_cast<const C *>(&x[u][v]);
+#pragma unroll
+                        for (int i = 0; i < PV; ++i) lds_st<C>(lds + v * vstride, (sl ^ S) ^ (i * EB), xc[i]);
+                    }
+                }
         }
     __syncthreads();
     int gi = 0;
@@ -1162,19 +1176,29 @@ k_stage(const StageTask *__restrict__ tasks,
         }
         __syncthreads();
     }
-    // store (through the write-back bit permutation when `permuted`)
+    // store (through the write-back bit permutation when `permuted`), LB
+    // chunks of LDS reads before their stores
     if (lane_on)
-        for (int j = 0; j < nchunks; ++j) {
-            const uint64_t off = gl | jtab[j];
-            const int Q = qtab[j];
+        for (int jb = 0; jb < nchunks; jb += LB) {
+            V x[LB][NV];
 #pragma unroll
-            for (int v = 0; v < NV; ++v) {
-                V x;
-                C *xc = reinterpret_cast<C *>(&x);
+            for (int u = 0; u < LB; ++u)
+                if (jb + u < nchunks) {
+                    const int Q = qtab[jb + u];
 #pragma unroll
-                for (int i = 0; i < PV; ++i) xc[i] = lds_ld<C>(lds + v * vstride, ql[i] ^ Q);
-                gstore((DESV_GLOBAL V *)(svp[v] + off), x);
-            }
+                    for (int v = 0; v < NV; ++v) {
+                        C *xc = reinterpret_cast<C *>(&x[u][v]);
+#pragma unroll
+                        for (int i = 0; i < PV; ++i) xc[i] = lds_ld<C>(lds + v * vstride, ql[i] ^ Q);
+                    }
+                }
+#pragma unroll
+            for (int u = 0; u < LB; ++u)
+                if (jb + u < nchunks) {
+                    const uint64_t off = gl | jtab[jb + u];
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) gstore((DESV_GLOBAL V *)(svp[v] + off), x[u][v]);
+                }
         }
     if (n_grad > 0) {
         __syncthreads();
