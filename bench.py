@@ -34,10 +34,49 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(n, gates, budget_s=15.0):
+def host_info():
+    """CPU model, cores this process may use, and RAM of the host."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), model)
+    except OSError:
+        pass
+    try:
+        ram = os.sysconf("SC_PAGE_SIZE") * os.sysconf("SC_PHYS_PAGES") / 2 ** 30
+    except (ValueError, OSError):
+        ram = float("nan")
+    return {"cpu_model": model, "ram_gib": round(ram, 1), "cores_available": len(os.sched_getaffinity(0))}
+
+
+def median_of_runs(step, units, budget_s, runs=5):
+    """The host-baseline protocol: one warm-up run, then the median wall time
+    of `runs` timed runs.  A run is a bounded sample of k consecutive units
+    (gates, evaluations) of the workload, k fixed by the warm-up run, which
+    takes as many units as fit in budget_s / (runs + 1); step(i) processes
+    unit i (taken modulo `units`).  Returns units per second of the median
+    run, k and the run times."""
+    t0 = time.perf_counter()
+    k = 0
+    while k < units and (k == 0 or time.perf_counter() - t0 < budget_s / (runs + 1)):
+        step(k)
+        k += 1
+    times, at = [], k
+    for _ in range(runs):
+        t1 = time.perf_counter()
+        for u in range(k):
+            step((at + u) % units)
+        times.append(time.perf_counter() - t1)
+        at += k
+    med = sorted(times)[len(times) // 2]
+    return k / med, k, times
+
+
+def cpu_baseline(n, gates, budget_s=20.0):
     """The oracle's C + OpenMP restatement (oracle/c) applying the same SU(4)
     gates to a complex64 2^n state on the host, on every core this process may
-    use; a bounded sample (the first gates of the circuit, until `budget_s`)."""
+    use: one warm-up run and the median of 5 runs, each run a bounded sample
+    of consecutive gates of the circuit."""
     import numpy as np
 
     from oracle import sv_oracle_c as OC
@@ -45,38 +84,29 @@ def cpu_baseline(n, gates, budget_s=15.0):
     sv = np.zeros(1 << n, dtype=np.complex64)
     sv[0] = 1
     OC.apply_matrix(sv, np.eye(4), [0, 1], threads=threads)   # page the state in
-    done = 0
-    t0 = time.perf_counter()
-    for u, a, b in gates:
+
+    def step(i):
+        u, a, b = gates[i]
         OC.apply_matrix(sv, u, [a, b], threads=threads)
-        done += 1
-        if time.perf_counter() - t0 > budget_s:
-            break
-    dt = time.perf_counter() - t0
-    return {"value": done / dt, "unit": "gates/s", "cores": threads, "kind": "port",
-            "sample": f"first {done} of the {len(gates)} SU(4) gates at {n} qubits, complex64, "
-                      f"oracle/c (C + OpenMP, one pass over the state per gate), {dt:.1f} s"}
+    rate, k, times = median_of_runs(step, len(gates), budget_s)
+    return {"value": rate, "unit": "gates/s", "cores": threads, "kind": "port",
+            "sample": f"one warm-up run, then the median of 5 runs of {k} consecutive SU(4) gates "
+                      f"(of the {len(gates)}) at {n} qubits, complex64, oracle/c (C + OpenMP, one pass "
+                      f"over the state per gate); run times {[round(t, 2) for t in times]} s",
+            **host_info()}
 
 
 def cpu_baselines_small(budget_s=8.0):
-    """Host baselines of the other workloads, each a bounded sample on the
-    oracle: the C + OpenMP restatement (all host cores) and the NumPy
-    per-gate gather formulation of the reference's kernel (one thread of
-    NumPy), complex64 for gates, complex128 for the VQE energy."""
+    """Host baselines of the other workloads on the oracle, each by the same
+    protocol (one warm-up run, median of 5 runs of a bounded sample): the C +
+    OpenMP restatement (all host cores) and the NumPy per-gate gather
+    formulation of the reference's kernel (one thread of NumPy), complex64 for
+    gates, complex128 for the VQE energy."""
     import numpy as np
     from desv import workloads
     from oracle import sv_oracle as O
     from oracle import sv_oracle_c as OC
     threads = OC.default_threads()
-
-    def timed_rows(run_row, rows, budget):
-        done, t0 = 0, time.perf_counter()
-        for r in rows:
-            run_row(r)
-            done += 1
-            if time.perf_counter() - t0 > budget:
-                break
-        return done, time.perf_counter() - t0
 
     out = {"cores": threads}
     # plain 20-qubit circuit (1580 gates)
@@ -84,36 +114,39 @@ def cpu_baselines_small(budget_s=8.0):
     ops = O.encode_ops(workloads.random_layers_circuit(n, 20, seed=0), n)
     sv = np.zeros(1 << n, dtype=np.complex64)
     sv[0] = 1
-    done, dt = timed_rows(lambda r: OC.run_ops(sv, r[None, :], threads), ops, budget_s)
-    out["plain20_c_gates_per_s"] = done / dt
-    sv2 = O.zero_state(n, np.complex64)
-    box = [sv2]
-    done, dt = timed_rows(lambda r: box.__setitem__(0, O.apply_op_gather(box[0], r)), ops, budget_s / 2)
-    out["plain20_numpy_gather_gates_per_s"] = done / dt
-    # 20-qubit XXZ Trotter (first steps of 200)
+    out["plain20_c_gates_per_s"] = median_of_runs(
+        lambda i: OC.run_ops(sv, ops[i][None, :], threads), len(ops), budget_s)[0]
+    box = [O.zero_state(n, np.complex64)]
+    out["plain20_numpy_gather_gates_per_s"] = median_of_runs(
+        lambda i: box.__setitem__(0, O.apply_op_gather(box[0], ops[i])), len(ops), budget_s / 2)[0]
+    # 20-qubit XXZ Trotter (200 steps)
     circ = workloads.xxz_trotter_circuit(n, 0.05, 200, seed=0)
     xops = O.encode_ops(circ, n)
     sv = np.zeros(1 << n, dtype=np.complex64)
     sv[0] = 1
-    done, dt = timed_rows(lambda r: OC.run_ops(sv, r[None, :], threads), xops, budget_s)
-    out["xxz20_c_gates_per_s"] = done / dt
+    out["xxz20_c_gates_per_s"] = median_of_runs(
+        lambda i: OC.run_ops(sv, xops[i][None, :], threads), len(xops), budget_s)[0]
     out["xxz20_gates"] = len(xops)
     # 20-qubit VQE energy (forward on the C oracle in complex128, energy with
-    # the NumPy Pauli sum): one evaluation; a parameter-shift gradient costs
-    # 2 x 320 of them
+    # the NumPy Pauli sum), one evaluation a unit; a parameter-shift gradient
+    # costs 2 x 320 of them
     vc, theta, terms = workloads.vqe_instance(n, 8, 200, seed=0)
     tmpl = O.build_template(vc.ops, n)
-    t0 = time.perf_counter()
-    sv = np.zeros(1 << n, dtype=np.complex128)
-    sv[0] = 1
-    OC.run_ops(sv, O.patch_template(tmpl, theta), threads)
-    e = O.pauli_sum_expectation(sv, terms, n)
-    dt = time.perf_counter() - t0
-    out["vqe20_energy_evals_per_s"] = 1.0 / dt
-    out["vqe20_param_shift_grads_per_s"] = 1.0 / (dt * 2 * len(theta))
-    out["vqe20_energy"] = float(e)
-    out["sample"] = (f"each figure a bounded sample (about {budget_s:.0f} s): leading gates of the circuit; "
-                     "the VQE figure is one energy evaluation, the gradient figure that time x 640")
+    table = O.patch_template(tmpl, theta)
+    energy = [0.0]
+
+    def vqe_eval(_):
+        st = np.zeros(1 << n, dtype=np.complex128)
+        st[0] = 1
+        OC.run_ops(st, table, threads)
+        energy[0] = O.pauli_sum_expectation(st, terms, n)
+    rate = median_of_runs(vqe_eval, 1, budget_s)[0]
+    out["vqe20_energy_evals_per_s"] = rate
+    out["vqe20_param_shift_grads_per_s"] = rate / (2 * len(theta))
+    out["vqe20_energy"] = float(energy[0])
+    out["sample"] = (f"each figure: one warm-up run, then the median of 5 runs of consecutive gates "
+                     f"(about {budget_s:.0f} s in all); the VQE figure times whole energy evaluations, "
+                     "the gradient figure is that rate / 640")
     return out
 
 
