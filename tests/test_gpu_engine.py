@@ -495,9 +495,11 @@ def test_program_autotune_keeps_parity(rng, tmp_path, monkeypatch):
                                      (16, np.complex128)])
 def test_device_sampling_matches_rng_choice(n, dtype, rng):
     """Shots drawn on the GPU from the caller's generator equal the
-    reference's rng.choice(dim, n_shots, p=probs) with the same seed (the FP64
-    CDF is summed in another order, so a uniform within rounding of a bin
-    edge may land in the neighbouring bin: at most a couple of shots move)."""
+    reference's rng.choice(dim, n_shots, p=probs) with the same seed.  The
+    device CDF is an exact integer prefix sum of the probabilities quantised
+    to 2^-62 of the total, NumPy's a float64 cumsum: a uniform within their
+    rounding of a bin edge may land in the neighbouring bin, so at most a
+    couple of shots move (integer output, parity unpinned at bin edges)."""
     sv = rand_state(n, rng).astype(dtype)
     shots = 20000
     got = desv.sample_counts(sv, shots, rng=np.random.default_rng(7))
