@@ -588,26 +588,37 @@ __device__ inline void op_mfma3(char *lds, int vstride, int m, const DevOp &op, 
     const int G = 1 << (m - __builtin_popcount(fixed));
     if (G >= 16 * kWaves) {
         // Every wave owns G / (16 kWaves) full column blocks: no lane
-        // predicates, wave-uniform (scalar) block addresses from a lane
-        // table, and the next block's loads issued before this block's stores.
+        // predicates and wave-uniform (scalar) block addresses from a lane
+        // table.  Software-pipelined: block it's MFMA chain is issued, then
+        // block it - 1's result (its chain finished meanwhile) is stored and
+        // block it + 1's inputs are loaded, so a wave never waits on the
+        // chain it has just issued.  A complex64 tile fits LDS up to 2^14
+        // amplitudes, so a wave walks at most 32 column blocks and the table's
+        // low half (`lo`) alone holds their slots.
         const int nit = G / (16 * kWaves);
         const BlockSlots bs = lt ? BlockSlots(BlockSlots::Given{}, blo, bhi) : BlockSlots(fixed, EB);
-        int Uc = bs(0);
+        int Uc = __builtin_amdgcn_readlane(bs.lo, 0), Up = Uc;
         float2 x0[NV], x1[NV];
+        f32x4_t yp[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             x0[v] = lds_ld<float2>(lds + v * vstride, pb0 ^ Uc);
             x1[v] = lds_ld<float2>(lds + v * vstride, pb1 ^ Uc);
+            yp[v] = f32x4_t{0.f, 0.f, 0.f, 0.f};
         }
         for (int it = 0; it < nit; ++it) {
-            const bool more = it + 1 < nit;
-            // a complex64 tile fits LDS up to 2^14 amplitudes, so a wave steps
-            // through at most 32 column blocks and `lo` alone holds their slots
-            const int Un = more ? (nit <= 64 ? __builtin_amdgcn_readlane(bs.lo, it + 1) : bs(it + 1)) : Uc;
             f32x4_t y[NV];
 #pragma unroll
             for (int v = 0; v < NV; ++v) y[v] = mv(x0[v], x1[v]);
-            if (more) {
+            if (it > 0) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    lds_st<float2>(lds + v * vstride, py0 ^ Up, make_float2(yp[v][0], yp[v][1]));
+                    lds_st<float2>(lds + v * vstride, py1 ^ Up, make_float2(yp[v][2], yp[v][3]));
+                }
+            }
+            const int Un = __builtin_amdgcn_readlane(bs.lo, (it + 1) & 63);
+            if (it + 1 < nit) {
 #pragma unroll
                 for (int v = 0; v < NV; ++v) {
                     x0[v] = lds_ld<float2>(lds + v * vstride, pb0 ^ Un);
@@ -615,11 +626,14 @@ __device__ inline void op_mfma3(char *lds, int vstride, int m, const DevOp &op, 
                 }
             }
 #pragma unroll
-            for (int v = 0; v < NV; ++v) {
-                lds_st<float2>(lds + v * vstride, py0 ^ Uc, make_float2(y[v][0], y[v][1]));
-                lds_st<float2>(lds + v * vstride, py1 ^ Uc, make_float2(y[v][2], y[v][3]));
-            }
+            for (int v = 0; v < NV; ++v) yp[v] = y[v];
+            Up = Uc;
             Uc = Un;
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            lds_st<float2>(lds + v * vstride, py0 ^ Up, make_float2(yp[v][0], yp[v][1]));
+            lds_st<float2>(lds + v * vstride, py1 ^ Up, make_float2(yp[v][2], yp[v][3]));
         }
         return;
     }
