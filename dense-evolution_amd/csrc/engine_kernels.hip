@@ -383,24 +383,31 @@ __device__ inline void op_mfma(char *lds, int vstride, int m, const DevOp &op, M
     const int G = 1 << (m - __builtin_popcount(fixed));
     if (NV == 1 && G >= 16 * kWaves) {
         // full column blocks for every wave: no lane predicates, block slots
-        // from the lane table, next block's loads before this block's stores
-        // (one vector: with two the prefetched fragments exceed the register
-        // budget of four waves per SIMD)
+        // from the lane table (at most 16 blocks a wave, within its low
+        // half), software-pipelined like the 3-qubit loop: block it's chains
+        // are issued before block it - 1's results are stored and block
+        // it + 1's inputs loaded (one vector: with two the prefetched
+        // fragments exceed the register budget of four waves per SIMD; plain
+        // 5-qubit blocks store each block before the next one's loads, as the
+        // held results would not fit beside their 48 fragment registers)
+        constexpr bool PIPE = !(K == 5 && !CP);
         const int nit = G / (16 * kWaves);
         const BlockSlots bs(fixed, EB);
-        int Uc = bs(0);
+        int Uc = __builtin_amdgcn_readlane(bs.lo, 0), Up = Uc;
         float br[NV][KS], bi[NV][KS];
+        f32x4_t pr[NV][RB], pi[NV][RB];  // the previous block's results
 #pragma unroll
-        for (int v = 0; v < NV; ++v)
+        for (int v = 0; v < NV; ++v) {
 #pragma unroll
             for (int ks = 0; ks < KS; ++ks) {
                 const float2 x = lds_ld<float2>(lds + v * vstride, pb[ks] ^ Uc);
                 br[v][ks] = x.x;
                 bi[v][ks] = x.y;
             }
+#pragma unroll
+            for (int rb = 0; rb < RB; ++rb) pr[v][rb] = pi[v][rb] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+        }
         for (int it = 0; it < nit; ++it) {
-            const bool more = it + 1 < nit;
-            const int Un = more ? bs(it + 1) : Uc;
             f32x4_t yr[NV][RB], yi[NV][RB];
 #pragma unroll
             for (int v = 0; v < NV; ++v)
@@ -425,7 +432,26 @@ __device__ inline void op_mfma(char *lds, int vstride, int m, const DevOp &op, M
                         yi[v][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[rb][ks], br[v][ks], yi[v][rb], 0, 0, 0);
                     }
                 }
-            if (more) {
+            if (PIPE && it > 0) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v)
+#pragma unroll
+                    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+                        for (int j = 0; j < 4; ++j)
+                            lds_st<float2>(lds + v * vstride, py[rb][j] ^ Up, make_float2(pr[v][rb][j], pi[v][rb][j]));
+            }
+            if (!PIPE) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v)
+#pragma unroll
+                    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+                        for (int j = 0; j < 4; ++j)
+                            lds_st<float2>(lds + v * vstride, py[rb][j] ^ Uc, make_float2(yr[v][rb][j], yi[v][rb][j]));
+            }
+            const int Un = __builtin_amdgcn_readlane(bs.lo, (it + 1) & 63);
+            if (it + 1 < nit) {
 #pragma unroll
                 for (int v = 0; v < NV; ++v)
 #pragma unroll
@@ -435,14 +461,26 @@ __device__ inline void op_mfma(char *lds, int vstride, int m, const DevOp &op, M
                         bi[v][ks] = x.y;
                     }
             }
+            if (PIPE) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v)
+#pragma unroll
+                    for (int rb = 0; rb < RB; ++rb) {
+                        pr[v][rb] = yr[v][rb];
+                        pi[v][rb] = yi[v][rb];
+                    }
+            }
+            Up = Uc;
+            Uc = Un;
+        }
+        if (PIPE) {
 #pragma unroll
             for (int v = 0; v < NV; ++v)
 #pragma unroll
                 for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
                     for (int j = 0; j < 4; ++j)
-                        lds_st<float2>(lds + v * vstride, py[rb][j] ^ Uc, make_float2(yr[v][rb][j], yi[v][rb][j]));
-            Uc = Un;
+                        lds_st<float2>(lds + v * vstride, py[rb][j] ^ Up, make_float2(pr[v][rb][j], pi[v][rb][j]));
         }
         return;
     }
