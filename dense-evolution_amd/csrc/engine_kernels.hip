@@ -127,10 +127,24 @@ __device__ inline int spread(int x, int fixed) {
     return x;
 }
 
-template <typename C> __device__ inline C lds_ld(const char *p, int off) {
-    return *reinterpret_cast<const C *>(p + off);
+// The stage tile lives at LDS address 0 (all of the stage kernel's LDS is
+// dynamic, the bookkeeping behind the tiles): tile pointers are plain
+// address-space-3 pointers built from the integer 0, so a slot is the address.
+using lds_t = __attribute__((address_space(3))) char;
+template <typename C> using lds_of = __attribute__((address_space(3))) C;
+template <typename C> __device__ inline C lds_ld(const lds_t *p, int off) {
+    using R = typename RawOf<C>::t;
+    const R r = *reinterpret_cast<const lds_of<R> *>(p + off);
+    C x;
+    __builtin_memcpy(&x, &r, sizeof(C));
+    return x;
 }
-template <typename C> __device__ inline void lds_st(char *p, int off, C v) { *reinterpret_cast<C *>(p + off) = v; }
+template <typename C> __device__ inline void lds_st(lds_t *p, int off, C v) {
+    using R = typename RawOf<C>::t;
+    R r;
+    __builtin_memcpy(&r, &v, sizeof(C));
+    *reinterpret_cast<lds_of<R> *>(p + off) = r;
+}
 
 // Work split of an op whose group enumerates the tile bits not in `fixed`:
 // lane `tid` owns groups tid + 256 j, j < J; lanes with tid >= G idle.
@@ -165,7 +179,7 @@ struct Groups {
 
 // Dense 2^K x 2^K matrix, K = 1, 2, on tile bits pos[] with local controls.
 template <typename T, int NV, int K, bool CP, typename MP>
-__device__ inline void op_dense_lo(char *lds, int vstride, int m, const DevOp &op, MP M, MP ML) {
+__device__ inline void op_dense_lo(lds_t *lds, int vstride, int m, const DevOp &op, MP M, MP ML) {
     using C = typename VT<T>::c;
     constexpr int D = 1 << K;
     constexpr int EB = sizeof(C);
@@ -207,7 +221,7 @@ __device__ inline void op_dense_lo(char *lds, int vstride, int m, const DevOp &o
     };
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
-        char *tb = lds + v * vstride;
+        lds_t *tb = lds + v * vstride;
         int j = 0;
         for (; j + 2 <= g.J; j += 2) {
             const int A0 = g.P ^ g.U(j, EB), A1 = g.P ^ g.U(j + 1, EB);
@@ -237,7 +251,7 @@ __device__ inline void op_dense_lo(char *lds, int vstride, int m, const DevOp &o
 // Dense 2^K x 2^K block on VALU (K = 3..5): one group's amplitudes in
 // registers, rows formed one at a time from the (scalar) matrix row.
 template <typename T, int NV, int K, bool CP, typename MP>
-__device__ inline void op_densek(char *lds, int vstride, int m, const DevOp &op, MP M, MP ML) {
+__device__ inline void op_densek(lds_t *lds, int vstride, int m, const DevOp &op, MP M, MP ML) {
     using C = typename VT<T>::c;
     constexpr int D = 1 << K;
     constexpr int EB = sizeof(C);
@@ -257,7 +271,7 @@ __device__ inline void op_densek(char *lds, int vstride, int m, const DevOp &op,
     if (g.idle()) return;
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
-        char *tb = lds + v * vstride;
+        lds_t *tb = lds + v * vstride;
         for (int j = 0; j < g.J; ++j) {
             const int A = g.P ^ g.U(j, EB);
             C a[D];
@@ -327,7 +341,7 @@ struct BlockSlots {
 };
 
 template <int NV, int K, bool CP, typename MP>
-__device__ inline void op_mfma(char *lds, int vstride, int m, const DevOp &op, MP M, MP ML) {
+__device__ inline void op_mfma(lds_t *lds, int vstride, int m, const DevOp &op, MP M, MP ML) {
     constexpr int D = 1 << K;
     constexpr int RB = D / 16;  // 16-row blocks of U
     constexpr int KS = D / 4;   // 4-wide k slices
@@ -499,7 +513,7 @@ __device__ inline void op_mfma(char *lds, int vstride, int m, const DevOp &op, M
         }
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            char *tb = lds + v * vstride;
+            lds_t *tb = lds + v * vstride;
             float br[UN][KS], bi[UN][KS];
 #pragma unroll
             for (int u = 0; u < UN; ++u)
@@ -563,7 +577,7 @@ __device__ inline void op_mfma(char *lds, int vstride, int m, const DevOp &op, M
 // -> R row 4 (l/16) + j, so lane group g holds outputs 2g and 2g+1 as
 // (re, im) register pairs.
 template <int NV, bool CP, typename MP>
-__device__ inline void op_mfma3(char *lds, int vstride, int m, const DevOp &op, MP M, MP ML,
+__device__ inline void op_mfma3(lds_t *lds, int vstride, int m, const DevOp &op, MP M, MP ML,
                                 const DESV_GLOBAL int4 *lt) {
     constexpr int EB = sizeof(float2);
     const int lane = threadIdx.x & 63;
@@ -688,7 +702,7 @@ __device__ inline void op_mfma3(char *lds, int vstride, int m, const DevOp &op, 
         }
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            char *tb = lds + v * vstride;
+            lds_t *tb = lds + v * vstride;
             float2 x0[UN], x1[UN];
 #pragma unroll
             for (int u = 0; u < UN; ++u) {
@@ -717,7 +731,7 @@ __device__ inline void op_mfma3(char *lds, int vstride, int m, const DevOp &op, 
 // Groups enumerate the tile bits that are neither local targets nor
 // controls; each group holds 2^nl amplitudes with wave-uniform factors.
 template <typename T, int NV, bool CP, typename MP>
-__device__ inline void op_diag(char *lds, int vstride, int m, const DevOp &op, MP Dg, MP DL, uint64_t base) {
+__device__ inline void op_diag(lds_t *lds, int vstride, int m, const DevOp &op, MP Dg, MP DL, uint64_t base) {
     using C = typename VT<T>::c;
     constexpr int EB = sizeof(C);
     int gsub = 0, gmask = 0, lmask = 0;
@@ -750,7 +764,7 @@ __device__ inline void op_diag(char *lds, int vstride, int m, const DevOp &op, M
             const C f = gload(Dg + d), fl = CP ? gload(DL + d) : f;
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
-                char *tb = lds + v * vstride;
+                lds_t *tb = lds + v * vstride;
                 const C a = lds_ld<C>(tb, A ^ o);
                 lds_st<C>(tb, A ^ o, CP ? cfma(f, a, cmul(fl, a)) : cmul(f, a));
             }
@@ -762,7 +776,7 @@ __device__ inline void op_diag(char *lds, int vstride, int m, const DevOp &op, M
 // present in this tile (those whose out-of-tile bits match `base`) are held in
 // registers, and two groups are loaded before either is stored.
 template <typename T, int NV, int K, bool CP, typename MP>
-__device__ inline void op_diag_lo(char *lds, int vstride, int m, const DevOp &op, MP Dg, MP DL, uint64_t base) {
+__device__ inline void op_diag_lo(lds_t *lds, int vstride, int m, const DevOp &op, MP Dg, MP DL, uint64_t base) {
     using C = typename VT<T>::c;
     constexpr int D = 1 << K;
     constexpr int EB = sizeof(C);
@@ -799,7 +813,7 @@ __device__ inline void op_diag_lo(char *lds, int vstride, int m, const DevOp &op
         else return cmul(f[s], a);
     };
     for (int v = 0; v < NV; ++v) {  // not unrolled: one vector's registers at a time
-        char *tb = lds + v * vstride;
+        lds_t *tb = lds + v * vstride;
         int j = 0;
         for (; j + 2 <= g.J; j += 2) {
             const int A0 = g.P ^ g.U(j, EB), A1 = g.P ^ g.U(j + 1, EB);
@@ -828,15 +842,16 @@ __device__ inline void op_diag_lo(char *lds, int vstride, int m, const DevOp &op
 
 // Global phase on the whole tile (order of elements irrelevant).
 template <typename T, int NV, bool CP, typename MP>
-__device__ inline void op_scalar(char *lds, int vstride, int m, MP S, MP SL) {
+__device__ inline void op_scalar(lds_t *lds, int vstride, int m, MP S, MP SL) {
     using C = typename VT<T>::c;
     const C f = gload(S), fl = CP ? gload(SL) : f;
     const int size = 1 << m;
     for (int i = threadIdx.x; i < size; i += blockDim.x)
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            C *tb = reinterpret_cast<C *>(lds + v * vstride);
-            tb[i] = CP ? cfma(f, tb[i], cmul(fl, tb[i])) : cmul(f, tb[i]);
+            lds_t *tb = lds + v * vstride;
+            const C x = lds_ld<C>(tb, i * (int)sizeof(C));
+            lds_st<C>(tb, i * (int)sizeof(C), CP ? cfma(f, x, cmul(fl, x)) : cmul(f, x));
         }
 }
 
@@ -845,14 +860,14 @@ __device__ inline void op_scalar(char *lds, int vstride, int m, MP S, MP SL) {
 // pair's product and inner product in FP32 (gate precision), the sum over the
 // tile in FP64, two groups in flight.
 template <typename MP>
-__device__ inline double op_grad_dense_f32(char *lds, int vstride, int m, const DevOp &op, MP M) {
+__device__ inline double op_grad_dense_f32(lds_t *lds, int vstride, int m, const DevOp &op, MP M) {
     constexpr int EB = sizeof(float2);
     const float2 m00 = gload(M), m01 = gload(M + 1), m10 = gload(M + 2), m11 = gload(M + 3);
     const int cl = (int)op.cloc, tb = 1 << op.pos[0];
     const int o0 = slot(cl) * EB, o1 = slot(cl | tb) * EB;
     const Groups g(m, cl | tb, EB);
     if (g.idle()) return 0.0;
-    const char *ls = lds + vstride;
+    const lds_t *ls = lds + vstride;
     auto pair = [&](int A) {
         const float2 s0 = lds_ld<float2>(lds, A ^ o0), s1 = lds_ld<float2>(lds, A ^ o1);
         const float2 l0 = lds_ld<float2>(ls, A ^ o0), l1 = lds_ld<float2>(ls, A ^ o1);
@@ -871,14 +886,14 @@ __device__ inline double op_grad_dense_f32(char *lds, int vstride, int m, const 
 }
 
 template <typename MP>
-__device__ inline double op_grad_diag_f32(char *lds, int vstride, int m, const DevOp &op, MP Dg) {
+__device__ inline double op_grad_diag_f32(lds_t *lds, int vstride, int m, const DevOp &op, MP Dg) {
     constexpr int EB = sizeof(float2);
     const float2 d0 = gload(Dg), d1 = gload(Dg + 1);
     const int cl = (int)op.cloc, tb = 1 << op.pos[0];
     const int o0 = slot(cl) * EB, o1 = slot(cl | tb) * EB;
     const Groups g(m, cl | tb, EB);
     if (g.idle()) return 0.0;
-    const char *ls = lds + vstride;
+    const lds_t *ls = lds + vstride;
     auto pair = [&](int A) {
         const float2 g0 = cmul(d0, lds_ld<float2>(lds, A ^ o0)), g1 = cmul(d1, lds_ld<float2>(lds, A ^ o1));
         const float2 l0 = lds_ld<float2>(ls, A ^ o0), l1 = lds_ld<float2>(ls, A ^ o1);
@@ -895,7 +910,7 @@ __device__ inline double op_grad_diag_f32(char *lds, int vstride, int m, const D
 }
 
 template <typename T, typename MP>
-__device__ inline double op_grad_dense(char *lds, int vstride, int m, const DevOp &op, MP M) {
+__device__ inline double op_grad_dense(lds_t *lds, int vstride, int m, const DevOp &op, MP M) {
     if constexpr (std::is_same<T, float>::value) return op_grad_dense_f32(lds, vstride, m, op, M);
     using C = typename VT<T>::c;
     constexpr int EB = sizeof(C);
@@ -905,7 +920,7 @@ __device__ inline double op_grad_dense(char *lds, int vstride, int m, const DevO
     const int o0 = slot(cl) * EB, o1 = slot(cl | tb) * EB;
     const Groups g(m, cl | tb, EB);
     if (g.idle()) return 0.0;
-    const char *ls = lds + vstride;
+    const lds_t *ls = lds + vstride;
     double acc = 0;
     for (int j = 0; j < g.J; ++j) {
         const int A = g.P ^ g.U(j, EB);
@@ -920,7 +935,7 @@ __device__ inline double op_grad_dense(char *lds, int vstride, int m, const DevO
 
 // Same for a diagonal generator diag(d0, d1) on tile bit pos[0].
 template <typename T, typename MP>
-__device__ inline double op_grad_diag(char *lds, int vstride, int m, const DevOp &op, MP Dg) {
+__device__ inline double op_grad_diag(lds_t *lds, int vstride, int m, const DevOp &op, MP Dg) {
     if constexpr (std::is_same<T, float>::value) return op_grad_diag_f32(lds, vstride, m, op, Dg);
     using C = typename VT<T>::c;
     constexpr int EB = sizeof(C);
@@ -929,7 +944,7 @@ __device__ inline double op_grad_diag(char *lds, int vstride, int m, const DevOp
     const int o0 = slot(cl) * EB, o1 = slot(cl | tb) * EB;
     const Groups g(m, cl | tb, EB);
     if (g.idle()) return 0.0;
-    const char *ls = lds + vstride;
+    const lds_t *ls = lds + vstride;
     double acc = 0;
     for (int j = 0; j < g.J; ++j) {
         const int A = g.P ^ g.U(j, EB);
@@ -948,7 +963,7 @@ __device__ inline double op_grad_diag(char *lds, int vstride, int m, const DevOp
 // applied to both tiles.  Probe products in the state precision, the tile sum
 // in FP64.
 template <typename T, int KB, bool BD, bool CP, typename MP>
-__device__ inline double op_grad_block_k(char *lds, int vstride, int m, const DevOp &gop, MP G, const DevOp &bop,
+__device__ inline double op_grad_block_k(lds_t *lds, int vstride, int m, const DevOp &gop, MP G, const DevOp &bop,
                                          MP B, MP BL) {
     using C = typename VT<T>::c;
     using R = T;
@@ -983,7 +998,7 @@ __device__ inline double op_grad_block_k(char *lds, int vstride, int m, const De
         const C a0 = cfma(g01, s1, cmul(g00, s0)), a1 = cfma(g11, s1, cmul(g10, s0));
         return l0.x * a0.x + l0.y * a0.y + l1.x * a1.x + l1.y * a1.y;
     };
-    char *lt = lds + vstride;
+    lds_t *lt = lds + vstride;
     double acc = 0;
     for (int j = 0; j < g.J; ++j) {
         const int A = g.P ^ g.U(j, EB);
@@ -1039,7 +1054,7 @@ __device__ inline double op_grad_block_k(char *lds, int vstride, int m, const De
 }
 
 template <typename T, bool CP, typename MP>
-__device__ inline double op_grad_block(char *lds, int vstride, int m, const DevOp &gop, MP G, const DevOp &bop,
+__device__ inline double op_grad_block(lds_t *lds, int vstride, int m, const DevOp &gop, MP G, const DevOp &bop,
                                        MP B, MP BL) {
     if (bop.kind == OPK_DIAG)
         return bop.k == 1 ? op_grad_block_k<T, 1, true, CP>(lds, vstride, m, gop, G, bop, B, BL)
@@ -1063,18 +1078,20 @@ __device__ inline double op_grad_block(char *lds, int vstride, int m, const DevO
 template <typename T, int NV, int VAR, bool CP>
 __global__ void __launch_bounds__(kThreads)
 __attribute__((amdgpu_waves_per_eu(std::is_same<T, float>::value ? (CP && VAR == 3 ? 2 : 4) : 1)))
-k_stage(const StageTask *__restrict__ tasks,
-                                                    const int32_t *__restrict__ wg_task, StageTask single) {
+k_stage(const StageTask *__restrict__ tasks, const int32_t *__restrict__ wg_task, StageTask single, int xoff) {
     using C = typename VT<T>::c;
     using V = typename VT<T>::v;
     constexpr int PV = VT<T>::per_vec;
     constexpr int EB = sizeof(C);
     constexpr int CH = kThreads * PV;  // amplitudes per load/store chunk
+    // All of the kernel's LDS is dynamic, so the tile(s) start at address 0
+    // (a slot is the address: no base add per access) and the bookkeeping
+    // sits behind them, at byte xoff (kStageLdsExtra bytes).
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ double gacc[kMaxGradLDS * (kThreads / 64)];
-    __shared__ uint64_t jtab[256];  // global offset of chunk j
-    __shared__ int qtab[256];       // byte slot of the tile index stored at chunk j
-    __shared__ StageTask tk;
+    StageTask &tk = *reinterpret_cast<StageTask *>(smem + xoff);
+    uint64_t *jtab = reinterpret_cast<uint64_t *>(smem + xoff + kStageLdsTask);       // global offset of chunk j
+    int *qtab = reinterpret_cast<int *>(smem + xoff + kStageLdsTask + 256 * 8);       // byte slot of the tile index stored at chunk j
+    double *gacc = reinterpret_cast<double *>(smem + xoff + kStageLdsTask + 256 * 12);
 
     {
         // copy the task into LDS in 4-byte words, one word per thread
@@ -1089,7 +1106,9 @@ k_stage(const StageTask *__restrict__ tasks,
 
     const int m = uni(tk.st.m), L = uni(tk.st.L);
     const int size = 1 << m;
-    char *lds = reinterpret_cast<char *>(smem);
+    // the tile at address 0 (an integer, not a null-pointer constant)
+    const uint32_t zero = 0;
+    lds_t *lds = reinterpret_cast<lds_t *>((uintptr_t)zero);
     const int vstride = size * EB;
     DESV_GLOBAL C *svp[2] = {gptr_rw(reinterpret_cast<C *>(tk.sv0)), gptr_rw(reinterpret_cast<C *>(tk.sv1))};
     const uint64_t tile_mask = uni64(tk.st.tile_mask);
@@ -1273,7 +1292,10 @@ template <typename T>
 hipError_t launch_stage(int nv, int variant, bool comp, const StageTask *tasks, const int32_t *wg_task,
                         const StageTask &single, int64_t grid, int max_m, hipStream_t s) {
     using C = typename VT<T>::c;
-    const size_t lds = (size_t)nv * ((size_t)1 << max_m) * sizeof(C);
+    // the tile(s), then the bookkeeping
+    const size_t tiles = ((size_t)nv * ((size_t)1 << max_m) * sizeof(C) + 15) / 16 * 16;
+    const size_t lds = tiles + kStageLdsExtra;
+    const int xoff = (int)tiles;
     if (grid <= 0) return hipSuccess;
     if (grid > 0x7fffffffll) return hipErrorInvalidValue;
     if (variant < 0 || variant > 3) return hipErrorInvalidValue;
@@ -1284,9 +1306,9 @@ hipError_t launch_stage(int nv, int variant, bool comp, const StageTask *tasks, 
     if (comp && !std::is_same<T, float>::value) return hipErrorInvalidValue;
     const dim3 g((unsigned)grid);
 #define DESV_ST(NVV, VV) \
-    hipLaunchKernelGGL((k_stage<T, NVV, VV, false>), g, dim3(kThreads), lds, s, tasks, wg_task, single)
+    hipLaunchKernelGGL((k_stage<T, NVV, VV, false>), g, dim3(kThreads), lds, s, tasks, wg_task, single, xoff)
 #define DESV_STC(NVV, VV) \
-    hipLaunchKernelGGL((k_stage<float, NVV, VV, true>), g, dim3(kThreads), lds, s, tasks, wg_task, single)
+    hipLaunchKernelGGL((k_stage<float, NVV, VV, true>), g, dim3(kThreads), lds, s, tasks, wg_task, single, xoff)
     if constexpr (std::is_same<T, float>::value) {
         if (comp) {
             if (nv == 1) {
@@ -1973,19 +1995,19 @@ size_t stage_lds_limit(int device) {
         (const void *)k_stage<float, 1, 2, true>,   (const void *)k_stage<float, 1, 3, true>,
         (const void *)k_stage<float, 2, 0, true>,   (const void *)k_stage<float, 2, 1, true>,
         (const void *)k_stage<float, 2, 2, true>,   (const void *)k_stage<float, 2, 3, true>};
-    // leave room for the kernels' own (static) LDS arrays, as compiled
+    // the stage kernels' bookkeeping sits behind the tiles in dynamic LDS
+    // (they have no static LDS): the tiles get what is left of it
     size_t static_lds = 0;
     for (const void *f : fns) {
         hipFuncAttributes a;
         if (hipFuncGetAttributes(&a, f) == hipSuccess) static_lds = std::max(static_lds, (size_t)a.sharedSizeBytes);
         else (void)hipGetLastError();
     }
-    static_lds = std::max(static_lds, kMaxGradLDS * (kThreads / 64) * sizeof(double) + 256 * sizeof(uint64_t) +
-                                          256 * sizeof(int) + sizeof(StageTask));
-    static_lds = (static_lds + 255) & ~(size_t)255;
+    static_lds = (static_lds + kStageLdsExtra + 255) & ~(size_t)255;
     lim -= static_lds;
     for (const void *f : fns)
-        if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lim) != hipSuccess) {
+        if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(lim + kStageLdsExtra)) !=
+            hipSuccess) {
             (void)hipGetLastError();
             lim = 64 * 1024 - static_lds;
         }

@@ -46,6 +46,12 @@ struct StageTask {
     int64_t wg_begin;
 };
 
+// Stage kernel LDS behind the tile(s): the task copy, the chunk tables
+// (256 global offsets, 256 slots) and the per-wave gradient partials.
+constexpr int kStageLdsTask = (int)((sizeof(StageTask) + 15) / 16 * 16);
+constexpr int kStageLdsExtra = kStageLdsTask + 256 * 8 + 256 * 4 + kMaxGradLDS * (kThreads / 64) * 8;
+
+
 // lambda = H psi, psi2 = psi and FP64 partials of Re<psi|lambda>, for a chunk of
 // one instance's amplitudes per workgroup.
 // Amplitudes per workgroup of the tiled Pauli matvec (one aligned tile in
