@@ -60,6 +60,10 @@ struct DevPlan {
     int comp = 0;
     int32_t lo_off = 0;
     void *ltab = nullptr;  // per-lane tables of 3-qubit MFMA ops (complex64), or null
+    // register groups (complex64): device words and fragments, per-stage offsets
+    int32_t *rwords = nullptr;
+    float *rfrags = nullptr;
+    std::vector<int32_t> rstage_word, rstage_frag;
 
     void release() {
         if (buf) (void)hipFree(buf);
@@ -68,8 +72,10 @@ struct DevPlan {
     }
 };
 
-int upload_plan(const Plan &p, int dtype, int device, hipStream_t s, DevPlan &d, bool comp = false) {
+int upload_plan(const Plan &p, int dtype, int device, hipStream_t s, DevPlan &d, bool comp = false,
+                bool regs = false) {
     comp = comp && dtype == DESV_C64;
+    regs = regs && dtype == DESV_C64;
     if (comp && p.mats.size() > (size_t)0x3fffffff) return fail(DESV_EINVAL, "matrix pool too large");
     // matrices in the state precision (complex64: hi, then lo when compensated)
     const size_t nm = std::max<size_t>(p.mats.size(), 1) * (comp ? 2 : 1);
@@ -91,11 +97,15 @@ int upload_plan(const Plan &p, int dtype, int device, hipStream_t s, DevPlan &d,
     std::vector<int32_t> lt;
     if (dtype == DESV_C64)
         build_lane_tables(p, mf.data(), comp ? mf.data() + 2 * p.mats.size() : nullptr, ops, lt);
+    RegProgram rp;
+    if (regs) build_reg_program(p, comp, rp);
     const size_t ob = std::max<size_t>(ops.size(), 1) * sizeof(DevOp);
     const size_t oba = (ob + 255) & ~size_t(255);
     const size_t mb = nm * amp_bytes(dtype);
     const size_t mba = (mb + 255) & ~size_t(255);
-    const size_t need = oba + mba + lt.size() * sizeof(int32_t);
+    const size_t lb = (lt.size() * sizeof(int32_t) + 255) & ~size_t(255);
+    const size_t rwb = (rp.words.size() * sizeof(int32_t) + 255) & ~size_t(255);
+    const size_t need = oba + mba + lb + rwb + rp.frags.size() * sizeof(float);
     HIP_TRY(hipStreamSynchronize(s));  // the previous upload may still be read
     if (d.cap < need || d.device != device) {
         d.release();
@@ -109,11 +119,19 @@ int upload_plan(const Plan &p, int dtype, int device, hipStream_t s, DevPlan &d,
     else
         std::memcpy(host.data() + oba, md.data(), mb);
     if (!lt.empty()) std::memcpy(host.data() + oba + mba, lt.data(), lt.size() * sizeof(int32_t));
+    if (!rp.words.empty()) std::memcpy(host.data() + oba + mba + lb, rp.words.data(), rp.words.size() * sizeof(int32_t));
+    if (!rp.frags.empty())
+        std::memcpy(host.data() + oba + mba + lb + rwb, rp.frags.data(), rp.frags.size() * sizeof(float));
     HIP_TRY(hipMemcpyAsync(d.buf, host.data(), need, hipMemcpyHostToDevice, s));
     HIP_TRY(hipStreamSynchronize(s));
+    unsigned char *base = reinterpret_cast<unsigned char *>(d.buf);
     d.ops = reinterpret_cast<DevOp *>(d.buf);
-    d.mats = reinterpret_cast<unsigned char *>(d.buf) + oba;
-    d.ltab = lt.empty() ? nullptr : reinterpret_cast<unsigned char *>(d.buf) + oba + mba;
+    d.mats = base + oba;
+    d.ltab = lt.empty() ? nullptr : base + oba + mba;
+    d.rwords = rp.words.empty() ? nullptr : reinterpret_cast<int32_t *>(base + oba + mba + lb);
+    d.rfrags = rp.words.empty() ? nullptr : reinterpret_cast<float *>(base + oba + mba + lb + rwb);
+    d.rstage_word = rp.words.empty() ? std::vector<int32_t>() : rp.stage_word;
+    d.rstage_frag = rp.stage_frag;
     d.device = device;
     d.dtype = dtype;
     d.comp = comp;
@@ -314,8 +332,15 @@ int run_plan_on(desv_handle h, const Plan &plan, const DevPlan &dp, int nv, void
         t.partials = h->d_part;
         t.pstride = pstride;
         t.wg_begin = 0;
+        int variant = st.heavy;
+        if (nv == 1 && si - 1 < dp.rstage_word.size() && dp.rstage_word[si - 1] >= 0) {
+            // register groups (the stage index si - 1: si was advanced above)
+            t.rprog = dp.rwords + dp.rstage_word[si - 1];
+            t.rfrag = dp.rfrags + (size_t)dp.rstage_frag[si - 1] * 256;
+            variant = 4;
+        }
         hipError_t e = h->dtype == DESV_C64
-                           ? launch_stage<float>(nv, st.heavy, dp.comp != 0, nullptr, nullptr, t, st.n_tiles, st.m,
+                           ? launch_stage<float>(nv, variant, dp.comp != 0, nullptr, nullptr, t, st.n_tiles, st.m,
                                                  h->stream)
                            : launch_stage<double>(nv, st.heavy, false, nullptr, nullptr, t, st.n_tiles, st.m, h->stream);
         if (e != hipSuccess) return fail(DESV_EDEVICE, std::string("stage launch: ") + hipGetErrorString(e));
@@ -797,6 +822,7 @@ int desv_default_opts(int dtype, desv_compile_opts *opts) {
     opts->use_mfma = o.use_mfma;
     opts->dtype = dtype;
     opts->compensated = o.compensated;
+    opts->reg_groups = o.reg_groups;
     return DESV_OK;
 }
 
@@ -856,6 +882,9 @@ int desv_program_compile(desv_program p, const desv_compile_opts *opts) {
             return fail(DESV_EINVAL, "use_mfma applies to complex64 programs only (complex128 blocks run on VALU)");
         o.use_mfma = opts->use_mfma;
         o.compensated = dtype == DESV_C64 ? opts->compensated : 0;
+        if (opts->reg_groups && !(dtype == DESV_C64 && opts->use_mfma))
+            return fail(DESV_EINVAL, "reg_groups applies to complex64 programs with use_mfma");
+        o.reg_groups = opts->reg_groups;
     }
     if (o.fuse_bits < 1 || o.fuse_bits > 5) return fail(DESV_EINVAL, "fuse_bits must be in [1, 5]");
     const size_t lim = stage_lds_limit(dev);
@@ -919,6 +948,29 @@ int desv_program_plan_dump(desv_program p, uint64_t *tile_masks, int64_t *stage_
     return DESV_OK;
 }
 
+int desv_program_reg_sizes(desv_program p, int compensated, int64_t *n_words, int64_t *n_frag_floats) {
+    if (!p) return fail(DESV_EINVAL, "null program");
+    if (!p->compiled) return fail(DESV_EINVAL, "program is not compiled");
+    RegProgram rp;
+    build_reg_program(p->plan, compensated != 0, rp);
+    if (n_words) *n_words = (int64_t)rp.words.size();
+    if (n_frag_floats) *n_frag_floats = (int64_t)rp.frags.size();
+    return DESV_OK;
+}
+
+int desv_program_reg_dump(desv_program p, int compensated, int32_t *stage_word, int32_t *stage_frag,
+                          int32_t *words, float *frags) {
+    if (!p) return fail(DESV_EINVAL, "null program");
+    if (!p->compiled) return fail(DESV_EINVAL, "program is not compiled");
+    RegProgram rp;
+    build_reg_program(p->plan, compensated != 0, rp);
+    if (stage_word) std::copy(rp.stage_word.begin(), rp.stage_word.end(), stage_word);
+    if (stage_frag) std::copy(rp.stage_frag.begin(), rp.stage_frag.end(), stage_frag);
+    if (words) std::copy(rp.words.begin(), rp.words.end(), words);
+    if (frags) std::copy(rp.frags.begin(), rp.frags.end(), frags);
+    return DESV_OK;
+}
+
 namespace {
 // Compile with defaults if needed and upload the plan for this state.
 int prepare_program(desv_handle h, desv_program p) {
@@ -933,7 +985,8 @@ int prepare_program(desv_handle h, desv_program p) {
         if (rc) return rc;
     }
     if (p->dev.dtype != h->dtype || p->dev.device != h->device) {
-        int rc = upload_plan(p->plan, h->dtype, h->device, h->stream, p->dev, p->opts.compensated != 0);
+        int rc = upload_plan(p->plan, h->dtype, h->device, h->stream, p->dev, p->opts.compensated != 0,
+                             p->opts.reg_groups != 0 && p->opts.use_mfma != 0);
         if (rc) return rc;
     }
     return DESV_OK;

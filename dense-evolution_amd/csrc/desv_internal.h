@@ -100,6 +100,42 @@ constexpr int kLaneTableLanes = 256;
 constexpr int MAX_HBITS = 24;
 constexpr int MAX_TILE_BITS = 16;
 
+// Register groups: a complex64 forward stage of 2^12-amplitude tiles whose
+// ops are all uncontrolled dense or diagonal blocks of at most 3 bits can run
+// with the tile in registers instead of LDS: each of the 4 waves holds 1024
+// amplitudes (64 lanes x 16 registers), so a tile-local bit sits at one of 12
+// positions -- lane bits 0..5, register bits 0..3 (amplitude v[j] of a lane,
+// bit r of j) or wave bits 0..1.  A group reads the tile from LDS in a layout
+// (which bit sits where), runs its ops on the registers and writes the tile
+// back in the layout it ends in; no op of a group touches its wave bits, so
+// a group needs no barrier until its write-back.  Every op of a group is a
+// 3-bit block on MFMA (blocks of 1-2 bits are widened by identity factors on
+// other bits) whose targets sit at lane bits 4, 5 and one register bit; the
+// host inserts lane/register bit exchanges in front of each op to get them
+// there.  Stage words (int32):
+//   [0] groups, [1] MFMA ops, then per group:
+//   [0] micro-ops, [1..6] tile bit at lane bits 0..5 (read layout),
+//   [7..8] tile bit at wave bits 0..1, [9..24] byte slot of register j (read
+//   layout), [25..30] tile bit at lane bits 0..5 (write layout), [31..46]
+//   byte slot of register j (write layout), then one word per micro-op:
+//   RG_XLANE | r << 4 | b << 8   exchange register bit r with lane bit b
+//   RG_MFMA3 | s << 4            3-bit block: targets at lane bits 4, 5 and
+//                                register bit s; A fragments from the next
+//                                fragment block of the stage
+// Fragment blocks: 64 lanes x 4 floats (the stacked real A fragments of
+// ks = 0..3), hi parts, then (compensated programs) lo parts, per MFMA op.
+constexpr int kRegTileBits = 12;
+constexpr int kRegGroupHeader = 47;
+enum RegCode : int32_t { RG_XLANE = 1, RG_MFMA3 = 2 };
+struct RegProgram {
+    std::vector<int32_t> words;       // all register stages' words
+    std::vector<int32_t> stage_word;  // per stage: offset into words, -1 = LDS op path
+    std::vector<int32_t> stage_frag;  // per stage: index of its first fragment block
+    std::vector<float> frags;         // fragment blocks, 256 floats each
+    int blocks_per_op = 1;            // 2 for compensated programs (hi, lo)
+    int64_t groups = 0, exchanges = 0, mfma_ops = 0;
+};
+
 struct DevStage {
     int32_t m;         // tile bits (amplitudes per tile = 2^m)
     int32_t L;         // low contiguous global bits inside the tile
@@ -148,6 +184,7 @@ struct CompileOpts {
     int nv = 1;            // vectors per tile (2 for the adjoint reverse sweep)
     int remap = 1;         // let stages permute which qubits sit on the low bits
     int window = 4096;     // gates scanned ahead when choosing a tile
+    int reg_groups = 0;    // complex64 MFMA programs: register-group stages (RegProgram)
 };
 
 // Build the device plan for a gate list (in application order).
@@ -172,5 +209,9 @@ void round_mats_f32_lo(const Plan &p, const float *hi, float *lo);
 // `lt` indices in `ops` (a copy of plan.ops).
 void build_lane_tables(const Plan &plan, const float *hi, const float *lo, std::vector<DevOp> &ops,
                        std::vector<int32_t> &out);
+
+// Register-group programs of every eligible stage of `plan` (see RegProgram;
+// fragments rounded from plan.mats, with lo parts when `comp`).
+void build_reg_program(const Plan &plan, bool comp, RegProgram &out);
 
 }  // namespace desv

@@ -110,9 +110,16 @@ template <typename X> __device__ inline void gstore(DESV_GLOBAL X *p, const X &x
     __builtin_memcpy(&r, &x, sizeof(X));
     *(DESV_GLOBAL R *)p = r;
 }
-__device__ inline DevOp gload_op(const DESV_GLOBAL DevOp *p) {
+// Constant-address-space view of a wave-uniform pointer to data no kernel of
+// the launch writes (op descriptors): every load through it is a scalar load,
+// whatever the compiler can prove about aliasing stores.
+#define DESV_CONST __attribute__((address_space(4)))
+template <typename P> __device__ inline const DESV_CONST P *cptr(const P *p) {
+    return (const DESV_CONST P *)uni_ptr(p);
+}
+__device__ inline DevOp gload_op(const DESV_CONST DevOp *p) {
     constexpr int W = (int)(sizeof(DevOp) / 4);
-    const DESV_GLOBAL uint32_t *w = (const DESV_GLOBAL uint32_t *)p;
+    const DESV_CONST uint32_t *w = (const DESV_CONST uint32_t *)p;
     uint32_t buf[W];
 #pragma unroll
     for (int i = 0; i < W; ++i) buf[i] = w[i];
@@ -576,9 +583,25 @@ __device__ inline void op_mfma(lds_t *lds, int vstride, int m, const DevOp &op, 
 // parts for ks < 2 and imaginary parts for ks >= 2; accumulator lane l, reg j
 // -> R row 4 (l/16) + j, so lane group g holds outputs 2g and 2g+1 as
 // (re, im) register pairs.
+// This lane's words of a 3-qubit MFMA op's host-built lane table (slots,
+// block slots, hi and lo A fragments), or none (`on` false: the op derives
+// them in the kernel).  The stage kernel requests an op's words while the
+// op before it runs, so their latency hides under that op.
+struct LaneWords {
+    int4 w0, w1, w2, w3;
+    bool on;
+};
+template <bool CP> __device__ inline LaneWords load_lane_words(const DESV_GLOBAL int4 *lt) {
+    LaneWords r;
+    const DESV_GLOBAL int4 *w = lt + threadIdx.x * (kLaneTableWords / 4);
+    r.w0 = gload(w), r.w1 = gload(w + 1), r.w2 = gload(w + 2);
+    r.w3 = CP ? gload(w + 3) : make_int4(0, 0, 0, 0);
+    r.on = true;
+    return r;
+}
+
 template <int NV, bool CP, typename MP>
-__device__ inline void op_mfma3(lds_t *lds, int vstride, int m, const DevOp &op, MP M, MP ML,
-                                const DESV_GLOBAL int4 *lt) {
+__device__ inline void op_mfma3(lds_t *lds, int vstride, int m, const DevOp &op, MP M, MP ML, const LaneWords &lw) {
     constexpr int EB = sizeof(float2);
     const int lane = threadIdx.x & 63;
     const int i16 = lane & 15, g4 = lane >> 4;
@@ -588,19 +611,16 @@ __device__ inline void op_mfma3(lds_t *lds, int vstride, int m, const DevOp &op,
     auto roff = [&](int r) { return ((r & 1) << p0) | ((r >> 1 & 1) << p1) | ((r >> 2 & 1) << p2); };
     float a[4], la[4];  // A fragments; lo parts of the compensated form (accumulated first)
     int pb0, pb1, py0, py1, blo = 0, bhi = 0;
-    if (lt) {
+    if (lw.on) {
         // slots and fragments from the op's host-built lane table
-        const DESV_GLOBAL int4 *w = lt + threadIdx.x * (kLaneTableWords / 4);
-        const int4 w0 = gload(w), w1 = gload(w + 1), w2 = gload(w + 2);
-        pb0 = w0.x, pb1 = w0.y, py0 = w0.z, py1 = w0.w, blo = w1.x, bhi = w1.y;
-        a[0] = __int_as_float(w2.x), a[1] = __int_as_float(w2.y), a[2] = __int_as_float(w2.z),
-        a[3] = __int_as_float(w2.w);
+        pb0 = lw.w0.x, pb1 = lw.w0.y, py0 = lw.w0.z, py1 = lw.w0.w, blo = lw.w1.x, bhi = lw.w1.y;
+        a[0] = __int_as_float(lw.w2.x), a[1] = __int_as_float(lw.w2.y), a[2] = __int_as_float(lw.w2.z),
+        a[3] = __int_as_float(lw.w2.w);
 #pragma unroll
         for (int ks = 0; ks < 4; ++ks) la[ks] = 0.f;
         if constexpr (CP) {
-            const int4 w3 = gload(w + 3);
-            la[0] = __int_as_float(w3.x), la[1] = __int_as_float(w3.y), la[2] = __int_as_float(w3.z),
-            la[3] = __int_as_float(w3.w);
+            la[0] = __int_as_float(lw.w3.x), la[1] = __int_as_float(lw.w3.y), la[2] = __int_as_float(lw.w3.z),
+            la[3] = __int_as_float(lw.w3.w);
         }
     } else {
 #pragma unroll
@@ -648,7 +668,7 @@ __device__ inline void op_mfma3(lds_t *lds, int vstride, int m, const DevOp &op,
         // amplitudes, so a wave walks at most 32 column blocks and the table's
         // low half (`lo`) alone holds their slots.
         const int nit = G / (16 * kWaves);
-        const BlockSlots bs = lt ? BlockSlots(BlockSlots::Given{}, blo, bhi) : BlockSlots(fixed, EB);
+        const BlockSlots bs = lw.on ? BlockSlots(BlockSlots::Given{}, blo, bhi) : BlockSlots(fixed, EB);
         int Uc = __builtin_amdgcn_readlane(bs.lo, 0), Up = Uc;
         float2 x0[NV], x1[NV];
         f32x4_t yp[NV];
@@ -1066,6 +1086,165 @@ __device__ inline double op_grad_block(lds_t *lds, int vstride, int m, const Dev
 }  // namespace
 
 // ---------------------------------------------------------------------------
+// Register groups (complex64 forward stages of 2^12-amplitude tiles).
+//
+// Each wave holds 1024 amplitudes of the tile: lane l, register j (v[j]) holds
+// tile index sum_i lane bit i << lpos[i] | sum_r (j >> r & 1) << rpos[r] |
+// wave bits; a group reads them from LDS, exchanges lane and register bits in
+// registers, applies its 3-bit blocks on MFMA and writes the tile back in the
+// layout it ends in (the host tracks the layouts and emits the exchanges).
+// ---------------------------------------------------------------------------
+template <int CTRL, int BM> __device__ inline float dppf(float old, float src) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(src), CTRL, 0xf, BM, false));
+}
+
+// Exchange register bit R with lane bit B: the amplitude at (lane bit B = x,
+// register bit R = y) moves to (lane bit B = y, register bit R = x).  For
+// each register pair (a = v[j], b = v[j | 2^R]) lanes with bit B set take
+// b's value from lane l - 2^B into a, lanes with it clear take a's value from
+// lane l + 2^B into b.  Lane bits 4, 5: one v_permlane16/32_swap per float;
+// lane bits 2, 3: DPP row shifts with bank masks; lane bits 0, 1: quad
+// permutations and a select.
+template <int R, int B> __device__ inline void rg_xlane(float2 (&v)[16]) {
+    const int lane = threadIdx.x & 63;
+    const bool up = (lane >> B) & 1;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        if (j >> R & 1) continue;
+        float2 &a = v[j];
+        float2 &b = v[j | (1 << R)];
+        if constexpr (B == 5 || B == 4) {
+            uint32_t ax = __float_as_uint(a.x), ay = __float_as_uint(a.y);
+            uint32_t bx = __float_as_uint(b.x), by = __float_as_uint(b.y);
+            if constexpr (B == 5) {
+                const auto x = __builtin_amdgcn_permlane32_swap(ax, bx, false, false);
+                const auto y = __builtin_amdgcn_permlane32_swap(ay, by, false, false);
+                ax = x[0], bx = x[1], ay = y[0], by = y[1];
+            } else {
+                const auto x = __builtin_amdgcn_permlane16_swap(ax, bx, false, false);
+                const auto y = __builtin_amdgcn_permlane16_swap(ay, by, false, false);
+                ax = x[0], bx = x[1], ay = y[0], by = y[1];
+            }
+            a = make_float2(__uint_as_float(ax), __uint_as_float(ay));
+            b = make_float2(__uint_as_float(bx), __uint_as_float(by));
+        } else if constexpr (B >= 2) {
+            // banks (4-lane groups of a 16-lane row) with the bit set / clear
+            constexpr int SHR = 0x110 + (1 << B), SHL = 0x100 + (1 << B);
+            constexpr int BM1 = B == 2 ? 0xa : 0xc, BM0 = B == 2 ? 0x5 : 0x3;
+            const float2 a0 = a;
+            a = make_float2(dppf<SHR, BM1>(a.x, b.x), dppf<SHR, BM1>(a.y, b.y));
+            b = make_float2(dppf<SHL, BM0>(b.x, a0.x), dppf<SHL, BM0>(b.y, a0.y));
+        } else {
+            constexpr int QP = B == 0 ? 0xb1 : 0x4e;  // lane l reads lane l ^ 2^B
+            auto qp = [](float x) { return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), B == 0 ? 0xb1 : 0x4e, 0xf, 0xf, true)); };
+            const float2 ta = make_float2(qp(b.x), qp(b.y));
+            const float2 tb = make_float2(qp(a.x), qp(a.y));
+            a = up ? ta : a;
+            b = up ? b : tb;
+        }
+    }
+}
+
+// One 3-bit block: targets at lane bits 4, 5 (the MFMA row groups) and
+// register bit S; columns are lane bits 0..3 and the other register bits.
+// Register pair (v[j], v[j | 2^S]) is the B operand [Re x0, Re x1, Im x0,
+// Im x1] of one stacked real 16x16 product; the host orders the matrix rows
+// so the accumulator comes back as (Re y0, Im y0, Re y1, Im y1) in the same
+// layout.  Compensated: lo fragments accumulated first.
+template <int S, bool CP>
+__device__ inline void rg_mfma3(float2 (&v)[16], const float4 &fa, const float4 &fl) {
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        if (j >> S & 1) continue;
+        const float2 x0 = v[j], x1 = v[j | (1 << S)];
+        f32x4_t y = f32x4_t{0.f, 0.f, 0.f, 0.f};
+        if constexpr (CP) {
+            y = __builtin_amdgcn_mfma_f32_16x16x4f32(fl.x, x0.x, y, 0, 0, 0);
+            y = __builtin_amdgcn_mfma_f32_16x16x4f32(fl.y, x1.x, y, 0, 0, 0);
+            y = __builtin_amdgcn_mfma_f32_16x16x4f32(fl.z, x0.y, y, 0, 0, 0);
+            y = __builtin_amdgcn_mfma_f32_16x16x4f32(fl.w, x1.y, y, 0, 0, 0);
+        }
+        y = __builtin_amdgcn_mfma_f32_16x16x4f32(fa.x, x0.x, y, 0, 0, 0);
+        y = __builtin_amdgcn_mfma_f32_16x16x4f32(fa.y, x1.x, y, 0, 0, 0);
+        y = __builtin_amdgcn_mfma_f32_16x16x4f32(fa.z, x0.y, y, 0, 0, 0);
+        y = __builtin_amdgcn_mfma_f32_16x16x4f32(fa.w, x1.y, y, 0, 0, 0);
+        v[j] = make_float2(y[0], y[1]);
+        v[j | (1 << S)] = make_float2(y[2], y[3]);
+    }
+}
+
+// byte slot of this lane's part of a layout (lane and wave bits)
+__device__ inline int rg_lane_slot(const DESV_CONST int32_t *lpos, const DESV_CONST int32_t *wpos) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int idx = 0;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) idx |= ((lane >> i) & 1) << lpos[i];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) idx |= ((wave >> i) & 1) << wpos[i];
+    return slot(idx) * (int)sizeof(float2);
+}
+
+// `na`, `nl`: the first MFMA op's fragments (hi, lo), already requested.
+template <bool CP>
+__device__ inline void run_reg_groups(lds_t *lds, const DESV_CONST int32_t *rp, const DESV_GLOBAL float4 *fr,
+                                      float4 na, float4 nl) {
+    const int lane = threadIdx.x & 63;
+    const int ng = rp[0], nq = rp[1];
+    constexpr int FB = CP ? 2 : 1;  // fragment blocks per MFMA op
+    // fragments of MFMA op q + 1 are requested while op q runs
+    int q = 0, w = 2;
+    float2 v[16];
+    for (int g = 0; g < ng; ++g) {
+        const DESV_CONST int32_t *h = rp + w;
+        const int nmu = h[0];
+        {
+            const int lb = rg_lane_slot(h + 1, h + 7);
+#pragma unroll
+            for (int j = 0; j < 16; ++j) v[j] = lds_ld<float2>(lds, lb ^ h[9 + j]);
+        }
+        for (int u = 0; u < nmu; ++u) {
+            const int code = h[kRegGroupHeader + u];
+            const int a1 = (code >> 4) & 15;
+            if ((code & 15) == RG_XLANE) {
+                const int b = (code >> 8) & 15;
+                switch (a1 * 8 + b) {
+#define DESV_RX(R)                                             \
+    case R * 8 + 0: rg_xlane<R, 0>(v); break;                  \
+    case R * 8 + 1: rg_xlane<R, 1>(v); break;                  \
+    case R * 8 + 2: rg_xlane<R, 2>(v); break;                  \
+    case R * 8 + 3: rg_xlane<R, 3>(v); break;                  \
+    case R * 8 + 4: rg_xlane<R, 4>(v); break;                  \
+    case R * 8 + 5: rg_xlane<R, 5>(v); break;
+                    DESV_RX(0) DESV_RX(1) DESV_RX(2) DESV_RX(3)
+#undef DESV_RX
+                    default: break;
+                }
+            } else {
+                const float4 fa = na, fl = nl;
+                if (q + 1 < nq) {
+                    na = gload(fr + (size_t)(q + 1) * FB * 64 + lane);
+                    if constexpr (CP) nl = gload(fr + (size_t)(q + 1) * FB * 64 + 64 + lane);
+                }
+                ++q;
+                switch (a1) {
+                    case 0: rg_mfma3<0, CP>(v, fa, fl); break;
+                    case 1: rg_mfma3<1, CP>(v, fa, fl); break;
+                    case 2: rg_mfma3<2, CP>(v, fa, fl); break;
+                    default: rg_mfma3<3, CP>(v, fa, fl); break;
+                }
+            }
+        }
+        {
+            const int lb = rg_lane_slot(h + 25, h + 7);
+#pragma unroll
+            for (int j = 0; j < 16; ++j) lds_st<float2>(lds, lb ^ h[31 + j], v[j]);
+        }
+        __syncthreads();
+        w += kRegGroupHeader + nmu;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Stage kernel
 // ---------------------------------------------------------------------------
 // VAR selects the compiled micro-ops (register budget and occupancy):
@@ -1117,7 +1296,7 @@ k_stage(const StageTask *__restrict__ tasks, const int32_t *__restrict__ wg_task
     const DESV_GLOBAL C *mats = gptr(reinterpret_cast<const C *>(tk.mats));
     // lo halves of a compensated program's matrices: the pool's second half
     const DESV_GLOBAL C *matl = CP ? mats + uni(tk.lo_off) : mats;
-    const DESV_GLOBAL DevOp *ops = gptr(tk.ops);
+    const DESV_CONST DevOp *ops = cptr(tk.ops);
     const DESV_GLOBAL int4 *ltab = tk.ltab ? gptr(reinterpret_cast<const int4 *>(tk.ltab)) : nullptr;
     const bool permuted = uni(tk.st.permuted) != 0;
     const int n_grad = uni(tk.st.n_grad);
@@ -1157,6 +1336,18 @@ k_stage(const StageTask *__restrict__ tasks, const int32_t *__restrict__ wg_task
     const bool lane_on = le < size;
     const uint64_t gl = lane_on ? base | goff(le) : 0;
     const int sl = slot(le) * EB;
+    // register groups: the first MFMA op's fragments, requested before the tile loads
+    float4 rfa = make_float4(0.f, 0.f, 0.f, 0.f), rfl = rfa;
+    const DESV_CONST int32_t *rprog = nullptr;
+    const DESV_GLOBAL float4 *rfrag = nullptr;
+    if constexpr (VAR == 4) {
+        rprog = cptr(tk.rprog);
+        rfrag = gptr(reinterpret_cast<const float4 *>(tk.rfrag));
+        if (rprog[1] > 0) {
+            rfa = gload(rfrag + (threadIdx.x & 63));
+            if constexpr (CP) rfl = gload(rfrag + 64 + (threadIdx.x & 63));
+        }
+    }
     int ql[PV];
 #pragma unroll
     for (int i = 0; i < PV; ++i) ql[i] = slot(permuted ? perm(le + i) : le + i) * EB;
@@ -1190,8 +1381,29 @@ k_stage(const StageTask *__restrict__ tasks, const int32_t *__restrict__ wg_task
     __syncthreads();
     int gi = 0;
     const int ob = uni(tk.st.op_begin), oe = uni(tk.st.op_end);
+    if constexpr (VAR == 4) {
+        if constexpr (NV == 1 && std::is_same<T, float>::value) run_reg_groups<CP>(lds, rprog, rfrag, rfa, rfl);
+    } else {
+    // lane-table words of op o (3-qubit MFMA ops), requested one op ahead
+    constexpr bool PF = VAR >= 2 && std::is_same<T, float>::value;
+    auto lane_words = [&](int o) {
+        LaneWords w;
+        w.on = false;
+        if constexpr (PF) {
+            if (o < oe && ltab) {
+                const DESV_CONST DevOp *p = ops + o;
+                const int lt = p->lt;
+                if (p->kind == OPK_MFMA && p->k == 3 && lt >= 0)
+                    w = load_lane_words<CP>(ltab + (size_t)lt * (kLaneTableLanes * kLaneTableWords / 4));
+            }
+        }
+        return w;
+    };
+    LaneWords pf = lane_words(ob);
     for (int o = ob; o < oe; ++o) {
         const DevOp op = gload_op(ops + o);
+        const LaneWords lw = pf;
+        if constexpr (PF) pf = lane_words(o + 1);
         const bool is_grad = op.kind == OPK_GRAD || op.kind == OPK_GRADDIAG;
         if ((base & op.cglob) != op.cglob) {
             gi += is_grad;
@@ -1213,9 +1425,7 @@ k_stage(const StageTask *__restrict__ tasks, const int32_t *__restrict__ wg_task
             case OPK_MFMA:
                 if constexpr (VAR >= 2 && std::is_same<T, float>::value) {
                     if (op.k == 3)
-                        op_mfma3<NV, CP>(lds, vstride, m, op, M, ML,
-                                         ltab && op.lt >= 0 ? ltab + (size_t)op.lt * (kLaneTableLanes * kLaneTableWords / 4)
-                                                            : nullptr);
+                        op_mfma3<NV, CP>(lds, vstride, m, op, M, ML, lw);
                     else if (op.k == 4) op_mfma<NV, 4, CP>(lds, vstride, m, op, M, ML);
                     else if constexpr (VAR == 3) op_mfma<NV, 5, CP>(lds, vstride, m, op, M, ML);
                 }
@@ -1233,6 +1443,7 @@ k_stage(const StageTask *__restrict__ tasks, const int32_t *__restrict__ wg_task
                         const DevOp nx = gload_op(ops + o + 1);
                         part = op_grad_block<T, CP>(lds, vstride, m, op, M, nx, mats + nx.mat, matl + nx.mat);
                         ++o;
+                        if constexpr (PF) pf = lane_words(o + 1);  // the block was op o + 1
                     } else {
                         part = op.kind == OPK_GRAD ? op_grad_dense<T>(lds, vstride, m, op, M)
                                                    : op_grad_diag<T>(lds, vstride, m, op, M);
@@ -1246,6 +1457,7 @@ k_stage(const StageTask *__restrict__ tasks, const int32_t *__restrict__ wg_task
             default: break;
         }
         __syncthreads();
+    }
     }
     // store (through the write-back bit permutation when `permuted`), LB
     // chunks of LDS reads before their stores
@@ -1298,7 +1510,9 @@ hipError_t launch_stage(int nv, int variant, bool comp, const StageTask *tasks, 
     const int xoff = (int)tiles;
     if (grid <= 0) return hipSuccess;
     if (grid > 0x7fffffffll) return hipErrorInvalidValue;
-    if (variant < 0 || variant > 3) return hipErrorInvalidValue;
+    if (variant < 0 || variant > 4) return hipErrorInvalidValue;
+    // register groups: complex64 forward stages only
+    if (variant == 4 && (!std::is_same<T, float>::value || nv != 1)) return hipErrorInvalidValue;
     // complex128 never uses MFMA blocks: its variants 2 and 3 would only
     // differ from 1 by omitting k = 4, 5 VALU blocks
     if (!std::is_same<T, float>::value && variant >= 2) variant = 1;
@@ -1310,6 +1524,10 @@ hipError_t launch_stage(int nv, int variant, bool comp, const StageTask *tasks, 
 #define DESV_STC(NVV, VV) \
     hipLaunchKernelGGL((k_stage<float, NVV, VV, true>), g, dim3(kThreads), lds, s, tasks, wg_task, single, xoff)
     if constexpr (std::is_same<T, float>::value) {
+        if (variant == 4) {
+            if (comp) DESV_STC(1, 4); else DESV_ST(1, 4);
+            return hipGetLastError();
+        }
         if (comp) {
             if (nv == 1) {
                 if (variant == 0) DESV_STC(1, 0); else if (variant == 1) DESV_STC(1, 1);

@@ -41,6 +41,8 @@ struct StageTask {
     const DevOp *ops;
     const void *mats;    // complex pool in the state precision
     const void *ltab;    // per-lane tables of 3-qubit MFMA ops (DevOp::lt), or null
+    const int32_t *rprog;  // register-group words of this stage (RegProgram), or null
+    const void *rfrag;     // this stage's first register-group fragment block
     double *partials;    // gradient partials [slot][pstride]
     int64_t pstride;     // tiles per gradient slot row
     int64_t wg_begin;
@@ -132,8 +134,9 @@ hipError_t launch_direct(const void *in, void *out, int n, const DirectOp &op, c
 // or, if tasks is null, `single`.
 template <typename T>
 // `variant`: DevStage::heavy (0 light, 1 VALU blocks k = 3..5, 2 MFMA blocks
-// k = 3, 4, 3 MFMA blocks up to k = 5).  `comp`: compensated matrices (the lo
-// pool at mats + lo_off; complex64 only).
+// k = 3, 4, 3 MFMA blocks up to k = 5), or 4: register groups (complex64,
+// nv = 1, tasks carry `rprog`).  `comp`: compensated matrices (the lo pool at
+// mats + lo_off; complex64 only).
 hipError_t launch_stage(int nv, int variant, bool comp, const StageTask *tasks, const int32_t *wg_task,
                         const StageTask &single, int64_t grid, int max_m, hipStream_t s);
 template <typename T>

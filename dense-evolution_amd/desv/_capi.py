@@ -51,7 +51,8 @@ _pi = C.POINTER(C.c_int)
 class CompileOpts(C.Structure):
     """``desv_compile_opts``."""
     _fields_ = [("fuse_bits", _i), ("tile_bits", _i), ("low_bits", _i),
-                ("per_gate", _i), ("use_mfma", _i), ("dtype", _i), ("compensated", _i)]
+                ("per_gate", _i), ("use_mfma", _i), ("dtype", _i), ("compensated", _i),
+                ("reg_groups", _i)]
 
 
 _SIGS = {
@@ -99,6 +100,8 @@ _SIGS = {
     "desv_run_program_timed": ([_p, _p, C.POINTER(C.c_float)], _i),
     "desv_program_plan_sizes": ([_p, _pi64, _pi64, _pi64], _i),
     "desv_program_plan_dump": ([_p, _pu64, _pi64, _pi64, _pd], _i),
+    "desv_program_reg_sizes": ([_p, _i, _pi64, _pi64], _i),
+    "desv_program_reg_dump": ([_p, _i, _pi, _pi, _pi, C.POINTER(C.c_float)], _i),
     "desv_pauli_expect": ([_p, _pu64, _pu64, _i64, _pd], _i),
     "desv_pauli_matvec": ([_p, _p, _pu64, _pu64, _pd, _pd, _i64], _i),
     "desv_pauli_sum_expect": ([_p, _pu64, _pu64, _pd, _i64, _pd], _i),

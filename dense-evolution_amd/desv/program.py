@@ -54,11 +54,13 @@ class Program:
 
     def compile(self, dtype="complex64", fuse_bits: Optional[int] = None, tile_bits: Optional[int] = None,
                 low_bits: Optional[int] = None, per_gate: bool = False, use_mfma: bool = False,
-                compensated: bool = False) -> "Program":
+                compensated: bool = False, reg_groups: bool = False) -> "Program":
         """Compile for a state of `dtype`.  ``compensated`` (complex64):
         matrices uploaded as FP32 hi + lo pairs and applied lo part first, for
         long programs whose repeated matrices would otherwise accumulate their
-        FP32 rounding coherently; about twice the gate arithmetic."""
+        FP32 rounding coherently; about twice the gate arithmetic.
+        ``reg_groups`` (complex64 with ``use_mfma``): eligible stages keep the
+        tile in registers between LDS round trips (``desv_compile_opts``)."""
         code = _capi.C64 if np.dtype(dtype) == np.complex64 else _capi.C128
         o = _capi.CompileOpts()
         check(lib.desv_default_opts(code, C.byref(o)))
@@ -71,6 +73,7 @@ class Program:
         o.per_gate = int(bool(per_gate))
         o.use_mfma = int(bool(use_mfma))
         o.compensated = int(bool(compensated))
+        o.reg_groups = int(bool(reg_groups))
         o.dtype = code
         check(lib.desv_program_compile(self.ptr, C.byref(o)))
         return self
@@ -92,6 +95,22 @@ class Program:
         check(lib.desv_program_plan_dump(self.ptr, _capi.u64ptr(tm), _capi.i64ptr(so), _capi.i64ptr(ops),
                                          _capi.dptr(mats)))
         return tm, so, ops, mats.view(np.complex128)
+
+    def reg_program(self, compensated: bool = False):
+        """The register-group program the upload would build: (stage_word
+        i32[S] (-1: LDS path), stage_frag i32[S], words i32[W], frags
+        f32[F/256, 64, 4])."""
+        w, f = C.c_int64(), C.c_int64()
+        check(lib.desv_program_reg_sizes(self.ptr, int(compensated), C.byref(w), C.byref(f)))
+        s, o, m = C.c_int64(), C.c_int64(), C.c_int64()
+        check(lib.desv_program_plan_sizes(self.ptr, C.byref(s), C.byref(o), C.byref(m)))
+        sw = np.zeros(max(1, s.value), dtype=np.int32)
+        sf = np.zeros(max(1, s.value), dtype=np.int32)
+        words = np.zeros(max(1, w.value), dtype=np.int32)
+        frags = np.zeros(max(1, f.value), dtype=np.float32)
+        check(lib.desv_program_reg_dump(self.ptr, int(compensated), _capi.iptr(sw), _capi.iptr(sf),
+                                        _capi.iptr(words), frags.ctypes.data_as(C.POINTER(C.c_float))))
+        return sw[:s.value], sf[:s.value], words[:w.value], frags[:f.value].reshape(-1, 64, 4)
 
     def run(self, target) -> None:
         h = target.handle if not isinstance(target, StateHandle) else target

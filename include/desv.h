@@ -160,6 +160,14 @@ typedef struct {
                        counterpart in the reference, whose complex64 path
                        rounds each gate matrix once (circuits/compiler.py:
                        172-179).                                              */
+    int reg_groups; /* 1 (complex64 with use_mfma): stages of 2^12-amplitude
+                       tiles whose blocks are all uncontrolled dense/diagonal
+                       blocks of at most 3 qubits keep the tile in registers
+                       between LDS round trips (groups of blocks, lane and
+                       register bits exchanged in registers, every block on
+                       MFMA); other stages are unchanged.  Same results as the
+                       LDS path to FP32 rounding.  No counterpart in the
+                       reference.                                             */
 } desv_compile_opts;
 
 /* Fill `opts` with the defaults for a dtype. */
@@ -173,6 +181,15 @@ int desv_program_compile(desv_program p, const desv_compile_opts *opts);
  * ops after fusion, mfma_blocks = ops run on MFMA. */
 int desv_program_stats(desv_program p, int64_t *gates, int64_t *passes,
                        int64_t *blocks, int64_t *mfma_blocks);
+/* Register-group program of a compiled program (introspection, no device
+ * needed; see desv_compile_opts.reg_groups): sizes, then per stage the offset
+ * of its words (-1: the stage runs its blocks from LDS) and of its first
+ * fragment block, the words and the fragment floats (256 per block: 64 lanes
+ * x 4 stacked real A fragments; compensated programs hold hi then lo blocks).
+ * No counterpart in the reference; used by the CPU replay of the layouts. */
+int desv_program_reg_sizes(desv_program p, int compensated, int64_t *n_words, int64_t *n_frag_floats);
+int desv_program_reg_dump(desv_program p, int compensated, int32_t *stage_word, int32_t *stage_frag,
+                          int32_t *words, float *frags);
 int desv_run_program(desv_handle h, desv_program p);
 /* Same as desv_run_program, with a HIP event recorded on the handle's stream
  * before every stage and after the last; waits for the run and writes each

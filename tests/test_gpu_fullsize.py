@@ -44,11 +44,12 @@ def test_copy_converts_precision(rng):
         h.close()
 
 
-# launch shapes of the 30-qubit benchmark: the autotuner's pick on MI355X
-# (fusion 3, 3-qubit blocks on MFMA, 2^12-amplitude tiles: 14 passes, 184 ops),
-# fusion 4 on MFMA, and the same pick with compensated matrices
+# launch shapes of the 30-qubit benchmark: fusion 3, 3-qubit blocks on MFMA,
+# 2^12-amplitude tiles (14 passes, 184 ops), fusion 4 on MFMA, the first with
+# compensated matrices and with register-group stages
 SHAPES_30 = [dict(fuse_bits=3, tile_bits=12, use_mfma=True), dict(fuse_bits=4, use_mfma=True),
-             dict(fuse_bits=3, tile_bits=12, use_mfma=True, compensated=True)]
+             dict(fuse_bits=3, tile_bits=12, use_mfma=True, compensated=True),
+             dict(fuse_bits=3, tile_bits=12, use_mfma=True, reg_groups=True)]
 
 
 @pytest.mark.parametrize("shape", SHAPES_30)

@@ -1,0 +1,316 @@
+"""Register-group stages, replayed on the CPU.
+
+A complex64 stage of 2^12-amplitude tiles whose blocks are all uncontrolled
+dense or diagonal blocks of at most 3 bits can keep the tile in registers
+(`desv_compile_opts.reg_groups`, `run_reg_groups` in
+dense-evolution_amd/csrc/engine_kernels.hip): 4 waves x 64 lanes x 16
+registers hold the tile; a group reads it from the swizzled LDS tile in a
+layout (which tile bit sits at which lane, register or wave bit), exchanges
+lane and register bits, runs each block as a stacked real 16x16 MFMA product
+and writes the tile back in the layout it ends in.  The host
+(`plan.cpp::build_reg_program`) chooses the groups, layouts, exchanges and the
+row/column order of every block's A fragments.
+
+This module restates the kernel's semantics lane by lane in NumPy -- the LDS
+slot addressing, the instruction-level behaviour of v_permlane16/32_swap and
+of the DPP row shifts, bank masks and quad permutations (as measured on
+MI355X: lane l of a row_shr:n result reads lane l - n of its 16-lane row,
+permlane32_swap exchanges the upper half of its first operand with the lower
+half of its second, permlane16_swap the odd rows of the first with the even
+rows of the second), and the 16x16x4 MFMA operand layout -- and replays the
+dumped register programs.  The result must equal the oracle.  With the
+compensated fragments (hi + lo) the replay carries the matrices to about
+2^-48, so a wrong layout, exchange, fragment order or slot shows up far above
+the tolerance.
+"""
+import numpy as np
+import pytest
+
+import desv
+from desv import workloads
+from oracle import sv_oracle as O
+
+HDR = 47
+RG_XLANE, RG_MFMA3 = 1, 2
+
+
+def slot(i):
+    return i ^ ((i >> 5) & 31)
+
+
+LANE = np.arange(64)
+
+
+# ---- instruction-level restatements --------------------------------------
+
+def permlane32_swap(x, y):
+    return np.concatenate([x[:32], y[:32]]), np.concatenate([x[32:], y[32:]])
+
+
+def permlane16_swap(x, y):
+    nx, ny = x.copy(), y.copy()
+    for r in (0, 2):
+        # odd rows (1, 3) of x <-> even rows (0, 2) of y
+        nx[(r + 1) * 16:(r + 2) * 16] = y[r * 16:(r + 1) * 16]
+        ny[r * 16:(r + 1) * 16] = x[(r + 1) * 16:(r + 2) * 16]
+    return nx, ny
+
+
+def dpp_row_shift(old, src, n, bank_mask):
+    """row_shr:n (n > 0) or row_shl:-n (n < 0) with a bank mask: enabled lanes
+    whose source lane lies in the same 16-lane row read it, others keep old."""
+    out = old.copy()
+    for l in range(64):
+        if not (bank_mask >> ((l % 16) // 4)) & 1:
+            continue
+        s = l - n
+        if s // 16 == l // 16 and 0 <= s < 64:
+            out[l] = src[s]
+    return out
+
+
+def quad_perm_xor(src, b):
+    return src[LANE ^ (1 << b)]
+
+
+def xlane(V, r, b):
+    """V: (64, 16) complex register file of one wave."""
+    for j in range(16):
+        if (j >> r) & 1:
+            continue
+        a, c = V[:, j].copy(), V[:, j | (1 << r)].copy()
+        if b == 5:
+            na, nc = permlane32_swap(a, c)
+        elif b == 4:
+            na, nc = permlane16_swap(a, c)
+        elif b >= 2:
+            bm1, bm0 = (0xa, 0x5) if b == 2 else (0xc, 0x3)
+            na = dpp_row_shift(a, c, 1 << b, bm1)
+            nc = dpp_row_shift(c, a, -(1 << b), bm0)
+        else:
+            up = ((LANE >> b) & 1).astype(bool)
+            ta, tc = quad_perm_xor(c, b), quad_perm_xor(a, b)
+            na = np.where(up, ta, a)
+            nc = np.where(up, c, tc)
+        V[:, j], V[:, j | (1 << r)] = na, nc
+
+
+def mfma3(V, s, frag):
+    """frag: (64, 4) A fragments (lane, ks) of the stacked real 16x16 matrix."""
+    A = np.zeros((16, 16))
+    for l in range(64):
+        for ks in range(4):
+            A[l % 16, 4 * ks + l // 16] = frag[l, ks]
+    for j in range(16):
+        if (j >> s) & 1:
+            continue
+        x0, x1 = V[:, j], V[:, j | (1 << s)]
+        Bk = [x0.real, x1.real, x0.imag, x1.imag]
+        Bm = np.zeros((16, 16))  # B[k][n]: k = 4 ks + l // 16, n = l % 16
+        for ks in range(4):
+            for l in range(64):
+                Bm[4 * ks + l // 16, l % 16] = Bk[ks][l]
+        D = A @ Bm
+        y = np.array([[D[4 * (l // 16) + jj, l % 16] for jj in range(4)] for l in range(64)])
+        V[:, j] = y[:, 0] + 1j * y[:, 1]
+        V[:, j | (1 << s)] = y[:, 2] + 1j * y[:, 3]
+
+
+def lane_slot(lpos, wpos, wave):
+    idx = np.zeros(64, dtype=np.int64)
+    for i in range(6):
+        idx |= ((LANE >> i) & 1) << lpos[i]
+    for i in range(2):
+        idx |= ((wave >> i) & 1) << wpos[i]
+    return slot(idx) * 8
+
+
+def run_groups(lds, words, frags, fb):
+    """lds: complex array indexed by LDS element (slot); words: the stage's
+    register words; frags: this stage's fragment blocks (fb per MFMA op)."""
+    ng, nq = int(words[0]), int(words[1])
+    w, q = 2, 0
+    for _ in range(ng):
+        h = words[w:]
+        nmu = int(h[0])
+        for wave in range(4):
+            lb = lane_slot(h[1:7], h[7:9], wave)
+            V = np.zeros((64, 16), dtype=np.complex128)
+            for j in range(16):
+                V[:, j] = lds[(lb ^ int(h[9 + j])) // 8]
+            qq = q
+            for u in range(nmu):
+                code = int(h[HDR + u])
+                a1 = (code >> 4) & 15
+                if code & 15 == RG_XLANE:
+                    xlane(V, a1, (code >> 8) & 15)
+                else:
+                    assert code & 15 == RG_MFMA3
+                    fr = frags[qq * fb].astype(np.float64)
+                    if fb == 2:
+                        fr = fr + frags[qq * fb + 1].astype(np.float64)
+                    mfma3(V, a1, fr)
+                    qq += 1
+            lbo = lane_slot(h[25:31], h[7:9], wave)
+            for j in range(16):
+                lds[(lbo ^ int(h[31 + j])) // 8] = V[:, j]
+        q = qq
+        w += HDR + nmu
+    assert q == nq
+
+
+def goff_fn(tm, n, m):
+    tbits = [b for b in range(n) if (tm >> b) & 1]
+    L = 0
+    while L < n and (tm >> L) & 1:
+        L += 1
+    hbits = [b for b in tbits if b >= L]
+
+    def goff(e):
+        o = e & ((1 << L) - 1)
+        r, j = e >> L, 0
+        while r:
+            if r & 1:
+                o |= 1 << hbits[j]
+            r >>= 1
+            j += 1
+        return o
+    return goff
+
+
+def replay(prog, sv, n, compensated):
+    """Replay a compiled program: register stages through run_groups, the
+    others through the LDS replay of test_tile_layout."""
+    from test_tile_layout import run_stage
+    tm, so, ops, mats = prog.plan()
+    sw, sf, words, frags = prog.reg_program(compensated)
+    fb = 2 if compensated else 1
+    nreg = 0
+    for s in range(len(tm)):
+        if sw[s] < 0:
+            sv = run_stage(sv, n, int(tm[s]), so[s], ops, mats, 2)
+            continue
+        nreg += 1
+        m = bin(int(tm[s])).count("1")
+        assert m == 12
+        goff = goff_fn(int(tm[s]), n, m)
+        sperm = [int(x) for x in so[s][2:2 + m]]
+
+        def perm(e):
+            o, j = 0, 0
+            while e:
+                if e & 1:
+                    o |= 1 << sperm[j]
+                e >>= 1
+                j += 1
+            return o
+        free = [b for b in range(n) if not (int(tm[s]) >> b) & 1]
+        gidx = np.array([goff(i) for i in range(1 << m)])
+        pidx = np.array([perm(e) for e in range(1 << m)])
+        sl = slot(np.arange(1 << m))
+        out = sv.copy()
+        for t in range(1 << (n - m)):
+            base = 0
+            for i, b in enumerate(free):
+                if (t >> i) & 1:
+                    base |= 1 << b
+            lds = np.zeros(1 << m, dtype=np.complex128)
+            lds[sl] = sv[base | gidx]
+            run_groups(lds, words[sw[s]:], frags[sf[s]:], fb)
+            T = lds[sl]
+            out[base | gidx] = T[pidx]
+        sv = out
+    return sv, nreg
+
+
+def _rand_state(n, rng):
+    sv = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    return sv / np.linalg.norm(sv)
+
+
+@pytest.mark.parametrize("n,depth,fuse", [(13, 6, 3), (14, 5, 3), (13, 6, 2)])
+def test_brickwork_register_stages_replay(n, depth, fuse, rng):
+    gates = workloads.brickwork_su4(n, depth, seed=3)
+    prog = desv.Program(n)
+    for u, a, b in gates:
+        prog.add_matrix(u, [a, b])
+    prog.compile("complex64", fuse_bits=fuse, tile_bits=12, use_mfma=True, reg_groups=True)
+    sv0 = _rand_state(n, rng)
+    got, nreg = replay(prog, sv0, n, compensated=True)
+    assert nreg >= 1
+    want = sv0.copy()
+    for u, a, b in gates:
+        want = O.apply_matrix(want, u, [a, b])
+    assert np.abs(got - want).max() < 1e-10
+    # the hi fragments alone: FP32 matrix rounding only
+    got32, _ = replay(prog, sv0, n, compensated=False)
+    assert np.abs(got32 - want).max() < 1e-5
+
+
+def _mixed_ops(n, count, rng, controlled):
+    rows = []
+    for _ in range(count):
+        u = rng.random()
+        if controlled and u < 0.2:
+            a, b = rng.choice(n, 2, replace=False)
+            rows.append([int(rng.choice([20, 21, 22, 24, 25])), a, b, rng.uniform(-np.pi, np.pi)])
+        elif u < 0.45:
+            a, b = rng.choice(n, 2, replace=False)
+            rows.append([23, a, b, 0.0])  # swap: dense 2-qubit, uncontrolled
+        else:
+            rows.append([int(rng.integers(1, 14)), int(rng.integers(0, n)), 0, rng.uniform(-np.pi, np.pi)])
+    return np.array(rows, dtype=np.float64)
+
+
+@pytest.mark.parametrize("fuse,controlled", [(1, False), (2, False), (3, False), (3, True)])
+def test_mixed_blocks_register_stages_replay(fuse, controlled, rng):
+    n = 13
+    ops = _mixed_ops(n, 120, rng, controlled)
+    prog = desv.Program(n).add_ops(ops)
+    prog.compile("complex64", fuse_bits=fuse, tile_bits=12, use_mfma=True, reg_groups=True)
+    sv0 = _rand_state(n, rng)
+    got, nreg = replay(prog, sv0, n, compensated=True)
+    if not controlled:
+        assert nreg >= 1
+    assert np.abs(got - O.run_ops_fast(sv0, ops)).max() < 1e-10
+
+
+def test_register_program_words_are_consistent():
+    n = 14
+    gates = workloads.brickwork_su4(n, 8, seed=5)
+    prog = desv.Program(n)
+    for u, a, b in gates:
+        prog.add_matrix(u, [a, b])
+    prog.compile("complex64", fuse_bits=3, tile_bits=12, use_mfma=True, reg_groups=True)
+    sw, sf, words, frags = prog.reg_program(False)
+    tm, so, ops, _ = prog.plan()
+    for s in range(len(tm)):
+        if sw[s] < 0:
+            continue
+        w = words[sw[s]:]
+        ng, nq, i, q = int(w[0]), int(w[1]), 2, 0
+        for _ in range(ng):
+            h = w[i:]
+            lin = list(h[1:7]) + list(h[7:9])
+            lout = list(h[25:31]) + list(h[7:9])
+            # every layout places 8 distinct tile bits on lanes and waves; the
+            # register slots enumerate the other 4
+            assert len(set(lin)) == 8 and len(set(lout)) == 8
+            regs_in = sorted({int(x) // 8 for x in h[9:25]})
+            assert len(regs_in) == 16
+            for u in range(int(h[0])):
+                code = int(h[HDR + u])
+                assert code & 15 in (RG_XLANE, RG_MFMA3)
+                q += code & 15 == RG_MFMA3
+            i += HDR + int(h[0])
+        assert q == nq
+        assert sf[s] + nq <= len(frags)
+
+
+def test_reg_groups_option_rules():
+    p = desv.Program(12)
+    p.add_matrix(np.eye(4), [0, 1])
+    with pytest.raises(Exception):
+        p.compile("complex64", use_mfma=False, reg_groups=True)
+    with pytest.raises(Exception):
+        p.compile("complex128", reg_groups=True)
