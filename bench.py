@@ -4,8 +4,9 @@ Workload (one step): depth-24 brickwork of Haar-random SU(4) gates on 30
 qubits (348 gates, seed 0), applied to a resident 8 GiB complex64 state.
 The circuit is compiled once in the launch shape the program autotuner
 measures fastest on this GPU (fusion width 2-5, 3-5-qubit blocks on MFMA or
-VALU, tile of 2^11-2^13 amplitudes; cache-blocked LDS stages); a step is one
-replay of the compiled program.
+VALU, tile of 2^11-2^13 amplitudes; cache-blocked stages with the tile in LDS,
+in registers between LDS round trips, or one whole tile per wave in
+registers); a step is one replay of the compiled program.
 
 Printed JSON line: gates/s as `value`; the whole-job HBM rate; the roofline
 of the dominant kernel (algorithmic FLOPs or bytes per launch / its average
@@ -653,6 +654,8 @@ def main():
     ap.add_argument("--low-bits", type=int, default=0)
     ap.add_argument("--per-gate", action="store_true")
     ap.add_argument("--no-mfma", action="store_true", help="fused blocks on VALU instead of MFMA")
+    ap.add_argument("--reg-groups", type=int, default=0,
+                    help="with --fuse: 1 = register-group stages, 2 = one tile per wave in registers")
     ap.add_argument("--in-place", action="store_true",
                     help="streaming passes update the state in place (no second buffer)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -700,6 +703,8 @@ def main():
         kw = dict(fuse_bits=args.fuse if args.fuse is not None else 4, per_gate=args.per_gate, use_mfma=mfma)
         if args.tile_bits:
             kw["tile_bits"] = args.tile_bits
+        if args.reg_groups:
+            kw["reg_groups"] = args.reg_groups
     if args.low_bits:
         kw["low_bits"] = args.low_bits
     prog.compile("complex64", **kw)
@@ -760,6 +765,7 @@ def main():
     avg_ms = d["ms"] / d["launches"]
     workload_key = (f"brickwork{n}_depth{args.depth}_fuse{fuse}{'_mfma' if mfma and fuse >= 3 else ''}"
                     f"{'_tile' + str(kw['tile_bits']) if kw.get('tile_bits') else ''}"
+                    f"{['', '_groups', '_waves'][kw.get('reg_groups', 0)]}"
                     f"{'_pergate' if args.per_gate else ''}")
     traffic = _load_traffic(dom, workload_key)
     if dom == "k_direct":
@@ -796,7 +802,7 @@ def main():
                                f"{len(gates)} gates, complex64 state resident in HBM",
                    "qubits": n, "depth": args.depth, "gates": len(gates),
                    "fuse_bits": fuse, "tile_bits": kw.get("tile_bits"), "per_gate": args.per_gate,
-                   "mfma": mfma, "shape_from_autotuner": tuned is not None,
+                   "mfma": mfma, "reg_groups": kw.get("reg_groups", 0), "shape_from_autotuner": tuned is not None,
                    "hbm_passes_per_step": st["passes"], "device_ops_per_step": st["blocks"],
                    "mfma_blocks_per_step": st["mfma_blocks"]},
         # whole-job HBM rate: algorithmic bytes of the passes executed / time

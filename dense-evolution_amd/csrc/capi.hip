@@ -64,6 +64,7 @@ struct DevPlan {
     int32_t *rwords = nullptr;
     float *rfrags = nullptr;
     std::vector<int32_t> rstage_word, rstage_frag;
+    int rmode = 0;  // 1 register groups, 2 wave tiles
 
     void release() {
         if (buf) (void)hipFree(buf);
@@ -73,9 +74,9 @@ struct DevPlan {
 };
 
 int upload_plan(const Plan &p, int dtype, int device, hipStream_t s, DevPlan &d, bool comp = false,
-                bool regs = false) {
+                int regs = 0) {
     comp = comp && dtype == DESV_C64;
-    regs = regs && dtype == DESV_C64;
+    if (dtype != DESV_C64) regs = 0;
     if (comp && p.mats.size() > (size_t)0x3fffffff) return fail(DESV_EINVAL, "matrix pool too large");
     // matrices in the state precision (complex64: hi, then lo when compensated)
     const size_t nm = std::max<size_t>(p.mats.size(), 1) * (comp ? 2 : 1);
@@ -98,7 +99,8 @@ int upload_plan(const Plan &p, int dtype, int device, hipStream_t s, DevPlan &d,
     if (dtype == DESV_C64)
         build_lane_tables(p, mf.data(), comp ? mf.data() + 2 * p.mats.size() : nullptr, ops, lt);
     RegProgram rp;
-    if (regs) build_reg_program(p, comp, rp);
+    if (regs == 1) build_reg_program(p, comp, rp);
+    if (regs == 2) build_wave_program(p, comp, rp);
     const size_t ob = std::max<size_t>(ops.size(), 1) * sizeof(DevOp);
     const size_t oba = (ob + 255) & ~size_t(255);
     const size_t mb = nm * amp_bytes(dtype);
@@ -132,6 +134,7 @@ int upload_plan(const Plan &p, int dtype, int device, hipStream_t s, DevPlan &d,
     d.rfrags = rp.words.empty() ? nullptr : reinterpret_cast<float *>(base + oba + mba + lb + rwb);
     d.rstage_word = rp.words.empty() ? std::vector<int32_t>() : rp.stage_word;
     d.rstage_frag = rp.stage_frag;
+    d.rmode = rp.words.empty() ? 0 : regs;
     d.device = device;
     d.dtype = dtype;
     d.comp = comp;
@@ -334,9 +337,14 @@ int run_plan_on(desv_handle h, const Plan &plan, const DevPlan &dp, int nv, void
         t.wg_begin = 0;
         int variant = st.heavy;
         if (nv == 1 && si - 1 < dp.rstage_word.size() && dp.rstage_word[si - 1] >= 0) {
-            // register groups (the stage index si - 1: si was advanced above)
+            // register-resident tile (the stage index si - 1: si was advanced above)
             t.rprog = dp.rwords + dp.rstage_word[si - 1];
             t.rfrag = dp.rfrags + (size_t)dp.rstage_frag[si - 1] * 256;
+            if (dp.rmode == 2) {
+                const hipError_t e = launch_wave_stage(st.m, dp.comp != 0, t, h->stream);
+                if (e != hipSuccess) return fail(DESV_EDEVICE, std::string("wave stage launch: ") + hipGetErrorString(e));
+                continue;
+            }
             variant = 4;
         }
         hipError_t e = h->dtype == DESV_C64
@@ -884,6 +892,7 @@ int desv_program_compile(desv_program p, const desv_compile_opts *opts) {
         o.compensated = dtype == DESV_C64 ? opts->compensated : 0;
         if (opts->reg_groups && !(dtype == DESV_C64 && opts->use_mfma))
             return fail(DESV_EINVAL, "reg_groups applies to complex64 programs with use_mfma");
+        if (opts->reg_groups < 0 || opts->reg_groups > 2) return fail(DESV_EINVAL, "reg_groups must be 0, 1 or 2");
         o.reg_groups = opts->reg_groups;
     }
     if (o.fuse_bits < 1 || o.fuse_bits > 5) return fail(DESV_EINVAL, "fuse_bits must be in [1, 5]");
@@ -952,7 +961,8 @@ int desv_program_reg_sizes(desv_program p, int compensated, int64_t *n_words, in
     if (!p) return fail(DESV_EINVAL, "null program");
     if (!p->compiled) return fail(DESV_EINVAL, "program is not compiled");
     RegProgram rp;
-    build_reg_program(p->plan, compensated != 0, rp);
+    if (p->opts.reg_groups == 2) build_wave_program(p->plan, compensated != 0, rp);
+    else build_reg_program(p->plan, compensated != 0, rp);
     if (n_words) *n_words = (int64_t)rp.words.size();
     if (n_frag_floats) *n_frag_floats = (int64_t)rp.frags.size();
     return DESV_OK;
@@ -963,7 +973,8 @@ int desv_program_reg_dump(desv_program p, int compensated, int32_t *stage_word, 
     if (!p) return fail(DESV_EINVAL, "null program");
     if (!p->compiled) return fail(DESV_EINVAL, "program is not compiled");
     RegProgram rp;
-    build_reg_program(p->plan, compensated != 0, rp);
+    if (p->opts.reg_groups == 2) build_wave_program(p->plan, compensated != 0, rp);
+    else build_reg_program(p->plan, compensated != 0, rp);
     if (stage_word) std::copy(rp.stage_word.begin(), rp.stage_word.end(), stage_word);
     if (stage_frag) std::copy(rp.stage_frag.begin(), rp.stage_frag.end(), stage_frag);
     if (words) std::copy(rp.words.begin(), rp.words.end(), words);
@@ -986,7 +997,7 @@ int prepare_program(desv_handle h, desv_program p) {
     }
     if (p->dev.dtype != h->dtype || p->dev.device != h->device) {
         int rc = upload_plan(p->plan, h->dtype, h->device, h->stream, p->dev, p->opts.compensated != 0,
-                             p->opts.reg_groups != 0 && p->opts.use_mfma != 0);
+                             p->opts.use_mfma ? p->opts.reg_groups : 0);
         if (rc) return rc;
     }
     return DESV_OK;

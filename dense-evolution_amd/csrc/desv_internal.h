@@ -184,7 +184,7 @@ struct CompileOpts {
     int nv = 1;            // vectors per tile (2 for the adjoint reverse sweep)
     int remap = 1;         // let stages permute which qubits sit on the low bits
     int window = 4096;     // gates scanned ahead when choosing a tile
-    int reg_groups = 0;    // complex64 MFMA programs: register-group stages (RegProgram)
+    int reg_groups = 0;    // complex64 MFMA programs: 1 register-group stages, 2 wave-tile stages (RegProgram)
 };
 
 // Build the device plan for a gate list (in application order).
@@ -213,5 +213,15 @@ void build_lane_tables(const Plan &plan, const float *hi, const float *lo, std::
 // Register-group programs of every eligible stage of `plan` (see RegProgram;
 // fragments rounded from plan.mats, with lo parts when `comp`).
 void build_reg_program(const Plan &plan, bool comp, RegProgram &out);
+// Wave-tile programs: every eligible stage of 2^11- or 2^12-amplitude tiles
+// runs one tile per wave, loaded from HBM into registers and stored back
+// without LDS.  Stage words: [0] micro-ops, [1] MFMA ops, [2 .. 2+m) global
+// bit of each position (lane bits 0..5, register bits 0..m-7) in the load
+// layout, [2+m .. 2+2m) the same for the store layout, then the micro-ops
+// (RG_XLANE / RG_MFMA3 as for groups).  The load layout holds tile bit 0 at
+// register bit 0 and tile bits 1..4 at lane bits 0..3; the store layout
+// store-order bit 0 at register bit 0 and store-order bits 1..4 at lane bits
+// 0..3 (16-byte accesses, 256 contiguous bytes per 16 lanes).
+void build_wave_program(const Plan &plan, bool comp, RegProgram &out);
 
 }  // namespace desv

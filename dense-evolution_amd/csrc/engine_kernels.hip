@@ -1094,9 +1094,6 @@ __device__ inline double op_grad_block(lds_t *lds, int vstride, int m, const Dev
 // registers, applies its 3-bit blocks on MFMA and writes the tile back in the
 // layout it ends in (the host tracks the layouts and emits the exchanges).
 // ---------------------------------------------------------------------------
-template <int CTRL, int BM> __device__ inline float dppf(float old, float src) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(src), CTRL, 0xf, BM, false));
-}
 
 // Exchange register bit R with lane bit B: the amplitude at (lane bit B = x,
 // register bit R = y) moves to (lane bit B = y, register bit R = x).  For
@@ -1105,11 +1102,57 @@ template <int CTRL, int BM> __device__ inline float dppf(float old, float src) {
 // lane l + 2^B into b.  Lane bits 4, 5: one v_permlane16/32_swap per float;
 // lane bits 2, 3: DPP row shifts with bank masks; lane bits 0, 1: quad
 // permutations and a select.
-template <int R, int B> __device__ inline void rg_xlane(float2 (&v)[16]) {
-    const int lane = threadIdx.x & 63;
-    const bool up = (lane >> B) & 1;
+// In-place DPP exchange of one float of a register pair (a: register bit
+// clear, b: set) across lane bit B.  Written as one asm block with a single
+// temporary so a large register-resident tile is updated in place; the
+// leading s_nop covers the 2 wait states a DPP source needs after a VALU
+// write, and the caller ends the exchange with s_nop before the results are
+// read by MFMA.
+template <int B> __device__ inline void rg_xchg(float &a, float &b) {
+    float t;
+    if constexpr (B == 2) {
+        asm volatile("s_nop 1\n\tv_mov_b32 %2, %0\n\t"
+                     "v_mov_b32_dpp %0, %1 row_shr:4 row_mask:0xf bank_mask:0xa\n\ts_nop 1\n\t"
+                     "v_mov_b32_dpp %1, %2 row_shl:4 row_mask:0xf bank_mask:0x5"
+                     : "+v"(a), "+v"(b), "=&v"(t));
+    } else if constexpr (B == 3) {
+        asm volatile("s_nop 1\n\tv_mov_b32 %2, %0\n\t"
+                     "v_mov_b32_dpp %0, %1 row_shr:8 row_mask:0xf bank_mask:0xc\n\ts_nop 1\n\t"
+                     "v_mov_b32_dpp %1, %2 row_shl:8 row_mask:0xf bank_mask:0x3"
+                     : "+v"(a), "+v"(b), "=&v"(t));
+    } else if constexpr (B == 1) {
+        // vcc = lanes with lane bit 1 set: b' = vcc ? b : a[l ^ 2]; then
+        // vcc = clear: a' = vcc ? a : b[l ^ 2]
+        asm volatile("s_nop 1\n\tv_mov_b32 %2, %1\n\t"
+                     "s_mov_b32 vcc_lo, 0xcccccccc\n\ts_mov_b32 vcc_hi, 0xcccccccc\n\t"
+                     "v_cndmask_b32_dpp %1, %0, %1, vcc quad_perm:[2,3,0,1] row_mask:0xf bank_mask:0xf\n\t"
+                     "s_not_b64 vcc, vcc\n\ts_nop 1\n\t"
+                     "v_cndmask_b32_dpp %0, %2, %0, vcc quad_perm:[2,3,0,1] row_mask:0xf bank_mask:0xf"
+                     : "+v"(a), "+v"(b), "=&v"(t)
+                     :
+                     : "vcc");
+    } else {
+        asm volatile("s_nop 1\n\tv_mov_b32 %2, %1\n\t"
+                     "s_mov_b32 vcc_lo, 0xaaaaaaaa\n\ts_mov_b32 vcc_hi, 0xaaaaaaaa\n\t"
+                     "v_cndmask_b32_dpp %1, %0, %1, vcc quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf\n\t"
+                     "s_not_b64 vcc, vcc\n\ts_nop 1\n\t"
+                     "v_cndmask_b32_dpp %0, %2, %0, vcc quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf"
+                     : "+v"(a), "+v"(b), "=&v"(t)
+                     :
+                     : "vcc");
+    }
+}
+
+// Exchange register bit R with lane bit B: the amplitude at (lane bit B = x,
+// register bit R = y) moves to (lane bit B = y, register bit R = x).  For
+// each register pair (a = v[j], b = v[j | 2^R]) lanes with bit B set take
+// b's value from lane l ^ 2^B into a, lanes with it clear take a's value from
+// lane l ^ 2^B into b.  Lane bits 4, 5: one v_permlane16/32_swap per float;
+// lane bits 2, 3: DPP row shifts with bank masks; lane bits 0, 1: quad
+// permutations in a masked select.
+template <int R, int B, int NJ> __device__ inline void rg_xlane(float2 (&v)[NJ]) {
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
+    for (int j = 0; j < NJ; ++j) {
         if (j >> R & 1) continue;
         float2 &a = v[j];
         float2 &b = v[j | (1 << R)];
@@ -1127,22 +1170,12 @@ template <int R, int B> __device__ inline void rg_xlane(float2 (&v)[16]) {
             }
             a = make_float2(__uint_as_float(ax), __uint_as_float(ay));
             b = make_float2(__uint_as_float(bx), __uint_as_float(by));
-        } else if constexpr (B >= 2) {
-            // banks (4-lane groups of a 16-lane row) with the bit set / clear
-            constexpr int SHR = 0x110 + (1 << B), SHL = 0x100 + (1 << B);
-            constexpr int BM1 = B == 2 ? 0xa : 0xc, BM0 = B == 2 ? 0x5 : 0x3;
-            const float2 a0 = a;
-            a = make_float2(dppf<SHR, BM1>(a.x, b.x), dppf<SHR, BM1>(a.y, b.y));
-            b = make_float2(dppf<SHL, BM0>(b.x, a0.x), dppf<SHL, BM0>(b.y, a0.y));
         } else {
-            constexpr int QP = B == 0 ? 0xb1 : 0x4e;  // lane l reads lane l ^ 2^B
-            auto qp = [](float x) { return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), B == 0 ? 0xb1 : 0x4e, 0xf, 0xf, true)); };
-            const float2 ta = make_float2(qp(b.x), qp(b.y));
-            const float2 tb = make_float2(qp(a.x), qp(a.y));
-            a = up ? ta : a;
-            b = up ? b : tb;
+            rg_xchg<B>(a.x, b.x);
+            rg_xchg<B>(a.y, b.y);
         }
     }
+    if constexpr (B < 4) asm volatile("s_nop 4");
 }
 
 // One 3-bit block: targets at lane bits 4, 5 (the MFMA row groups) and
@@ -1151,25 +1184,119 @@ template <int R, int B> __device__ inline void rg_xlane(float2 (&v)[16]) {
 // Im x1] of one stacked real 16x16 product; the host orders the matrix rows
 // so the accumulator comes back as (Re y0, Im y0, Re y1, Im y1) in the same
 // layout.  Compensated: lo fragments accumulated first.
-template <int S, bool CP>
-__device__ inline void rg_mfma3(float2 (&v)[16], const float4 &fa, const float4 &fl) {
+template <int S, bool CP, int NJ>
+__device__ inline void rg_mfma3(float2 (&v)[NJ], const float4 &fa, const float4 &fl) {
+    // the pairs in batches of IL independent accumulator chains, issued
+    // interleaved so a wave has IL MFMAs in flight instead of waiting on each
+    // chain's previous product
+    constexpr int NP = NJ / 2, IL = NP >= 4 ? 4 : NP;
+    auto pj = [](int p) { return ((p >> S) << (S + 1)) | (p & ((1 << S) - 1)); };  // p-th pair's low register
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
-        if (j >> S & 1) continue;
-        const float2 x0 = v[j], x1 = v[j | (1 << S)];
-        f32x4_t y = f32x4_t{0.f, 0.f, 0.f, 0.f};
+    for (int p0 = 0; p0 < NP; p0 += IL) {
+        f32x4_t y[IL];
+#pragma unroll
+        for (int i = 0; i < IL; ++i) y[i] = f32x4_t{0.f, 0.f, 0.f, 0.f};
         if constexpr (CP) {
-            y = __builtin_amdgcn_mfma_f32_16x16x4f32(fl.x, x0.x, y, 0, 0, 0);
-            y = __builtin_amdgcn_mfma_f32_16x16x4f32(fl.y, x1.x, y, 0, 0, 0);
-            y = __builtin_amdgcn_mfma_f32_16x16x4f32(fl.z, x0.y, y, 0, 0, 0);
-            y = __builtin_amdgcn_mfma_f32_16x16x4f32(fl.w, x1.y, y, 0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < IL; ++i) y[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(fl.x, v[pj(p0 + i)].x, y[i], 0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < IL; ++i)
+                y[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(fl.y, v[pj(p0 + i) | (1 << S)].x, y[i], 0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < IL; ++i) y[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(fl.z, v[pj(p0 + i)].y, y[i], 0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < IL; ++i)
+                y[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(fl.w, v[pj(p0 + i) | (1 << S)].y, y[i], 0, 0, 0);
         }
-        y = __builtin_amdgcn_mfma_f32_16x16x4f32(fa.x, x0.x, y, 0, 0, 0);
-        y = __builtin_amdgcn_mfma_f32_16x16x4f32(fa.y, x1.x, y, 0, 0, 0);
-        y = __builtin_amdgcn_mfma_f32_16x16x4f32(fa.z, x0.y, y, 0, 0, 0);
-        y = __builtin_amdgcn_mfma_f32_16x16x4f32(fa.w, x1.y, y, 0, 0, 0);
-        v[j] = make_float2(y[0], y[1]);
-        v[j | (1 << S)] = make_float2(y[2], y[3]);
+#pragma unroll
+        for (int i = 0; i < IL; ++i) y[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa.x, v[pj(p0 + i)].x, y[i], 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < IL; ++i)
+            y[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa.y, v[pj(p0 + i) | (1 << S)].x, y[i], 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < IL; ++i) y[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa.z, v[pj(p0 + i)].y, y[i], 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < IL; ++i)
+            y[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa.w, v[pj(p0 + i) | (1 << S)].y, y[i], 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < IL; ++i) {
+            v[pj(p0 + i)] = make_float2(y[i][0], y[i][1]);
+            v[pj(p0 + i) | (1 << S)] = make_float2(y[i][2], y[i][3]);
+        }
+    }
+}
+
+// Runs micro-ops mu[0 .. nmu) on the lane's registers v[0 .. NJ) (NJ = 2^NR
+// amplitudes, NR register bits).  `q`: index of the next MFMA op of the
+// stage, whose fragments `na` (hi), `nl` (lo) are already requested; the
+// following op's fragments are requested while an op runs.
+template <bool CP, int NJ>
+__device__ inline void rg_run_micro(float2 (&v)[NJ], const DESV_CONST int32_t *mu, int nmu,
+                                    const DESV_GLOBAL float4 *fr, int &q, int nq, float4 &na, float4 &nl) {
+    constexpr int NR = NJ == 16 ? 4 : NJ == 32 ? 5 : 6;
+    static_assert((1 << NR) == NJ, "register bits");
+    constexpr int FB = CP ? 2 : 1;  // fragment blocks per MFMA op
+    const int lane = threadIdx.x & 63;
+    for (int u = 0; u < nmu; ++u) {
+        const int code = mu[u];
+        const int a1 = (code >> 4) & 15;
+        if ((code & 15) == RG_XLANE) {
+            const int b = (code >> 8) & 15;
+            switch (a1 * 8 + b) {
+#define DESV_RX(R)                                             \
+    case R * 8 + 0: rg_xlane<R, 0>(v); break;                  \
+    case R * 8 + 1: rg_xlane<R, 1>(v); break;                  \
+    case R * 8 + 2: rg_xlane<R, 2>(v); break;                  \
+    case R * 8 + 3: rg_xlane<R, 3>(v); break;                  \
+    case R * 8 + 4: rg_xlane<R, 4>(v); break;                  \
+    case R * 8 + 5: rg_xlane<R, 5>(v); break;
+                DESV_RX(0) DESV_RX(1) DESV_RX(2) DESV_RX(3)
+#undef DESV_RX
+                default:
+                    if constexpr (NR >= 5) {
+                        switch (a1 * 8 + b) {
+#define DESV_RX(R)                                             \
+    case R * 8 + 0: rg_xlane<R, 0>(v); break;                  \
+    case R * 8 + 1: rg_xlane<R, 1>(v); break;                  \
+    case R * 8 + 2: rg_xlane<R, 2>(v); break;                  \
+    case R * 8 + 3: rg_xlane<R, 3>(v); break;                  \
+    case R * 8 + 4: rg_xlane<R, 4>(v); break;                  \
+    case R * 8 + 5: rg_xlane<R, 5>(v); break;
+                            DESV_RX(4)
+                            default:
+                                if constexpr (NR >= 6) {
+                                    switch (a1 * 8 + b) {
+                                        DESV_RX(5)
+                                        default: break;
+                                    }
+                                }
+                                break;
+#undef DESV_RX
+                        }
+                    }
+                    break;
+            }
+        } else {
+            const float4 fa = na, fl = nl;
+            if (q + 1 < nq) {
+                na = gload(fr + (size_t)(q + 1) * FB * 64 + lane);
+                if constexpr (CP) nl = gload(fr + (size_t)(q + 1) * FB * 64 + 64 + lane);
+            }
+            ++q;
+            switch (a1) {
+                case 0: rg_mfma3<0, CP>(v, fa, fl); break;
+                case 1: rg_mfma3<1, CP>(v, fa, fl); break;
+                case 2: rg_mfma3<2, CP>(v, fa, fl); break;
+                case 3: rg_mfma3<3, CP>(v, fa, fl); break;
+                default:
+                    if constexpr (NR >= 5) {
+                        if (a1 == 4) rg_mfma3<4, CP>(v, fa, fl);
+                        if constexpr (NR >= 6)
+                            if (a1 == 5) rg_mfma3<5, CP>(v, fa, fl);
+                    }
+                    break;
+            }
+        }
     }
 }
 
@@ -1188,10 +1315,7 @@ __device__ inline int rg_lane_slot(const DESV_CONST int32_t *lpos, const DESV_CO
 template <bool CP>
 __device__ inline void run_reg_groups(lds_t *lds, const DESV_CONST int32_t *rp, const DESV_GLOBAL float4 *fr,
                                       float4 na, float4 nl) {
-    const int lane = threadIdx.x & 63;
     const int ng = rp[0], nq = rp[1];
-    constexpr int FB = CP ? 2 : 1;  // fragment blocks per MFMA op
-    // fragments of MFMA op q + 1 are requested while op q runs
     int q = 0, w = 2;
     float2 v[16];
     for (int g = 0; g < ng; ++g) {
@@ -1202,38 +1326,7 @@ __device__ inline void run_reg_groups(lds_t *lds, const DESV_CONST int32_t *rp, 
 #pragma unroll
             for (int j = 0; j < 16; ++j) v[j] = lds_ld<float2>(lds, lb ^ h[9 + j]);
         }
-        for (int u = 0; u < nmu; ++u) {
-            const int code = h[kRegGroupHeader + u];
-            const int a1 = (code >> 4) & 15;
-            if ((code & 15) == RG_XLANE) {
-                const int b = (code >> 8) & 15;
-                switch (a1 * 8 + b) {
-#define DESV_RX(R)                                             \
-    case R * 8 + 0: rg_xlane<R, 0>(v); break;                  \
-    case R * 8 + 1: rg_xlane<R, 1>(v); break;                  \
-    case R * 8 + 2: rg_xlane<R, 2>(v); break;                  \
-    case R * 8 + 3: rg_xlane<R, 3>(v); break;                  \
-    case R * 8 + 4: rg_xlane<R, 4>(v); break;                  \
-    case R * 8 + 5: rg_xlane<R, 5>(v); break;
-                    DESV_RX(0) DESV_RX(1) DESV_RX(2) DESV_RX(3)
-#undef DESV_RX
-                    default: break;
-                }
-            } else {
-                const float4 fa = na, fl = nl;
-                if (q + 1 < nq) {
-                    na = gload(fr + (size_t)(q + 1) * FB * 64 + lane);
-                    if constexpr (CP) nl = gload(fr + (size_t)(q + 1) * FB * 64 + 64 + lane);
-                }
-                ++q;
-                switch (a1) {
-                    case 0: rg_mfma3<0, CP>(v, fa, fl); break;
-                    case 1: rg_mfma3<1, CP>(v, fa, fl); break;
-                    case 2: rg_mfma3<2, CP>(v, fa, fl); break;
-                    default: rg_mfma3<3, CP>(v, fa, fl); break;
-                }
-            }
-        }
+        rg_run_micro<CP>(v, h + kRegGroupHeader, nmu, fr, q, nq, na, nl);
         {
             const int lb = rg_lane_slot(h + 25, h + 7);
 #pragma unroll
@@ -1242,6 +1335,104 @@ __device__ inline void run_reg_groups(lds_t *lds, const DESV_CONST int32_t *rp, 
         __syncthreads();
         w += kRegGroupHeader + nmu;
     }
+}
+
+// Wave-tile stage: each 64-lane workgroup holds one tile of 2^m amplitudes
+// (m = 6 + NR: 64 lanes x 2^NR registers) from its HBM load to its HBM store,
+// with no LDS and no barrier, so a CU's waves drift apart and one wave's HBM
+// traffic overlaps other waves' MFMA work.  The load layout puts tile bit 0
+// at register bit 0 and tile bits 1..4 at lane bits 0..3 (each 16-byte load
+// is an amplitude pair, 16 lanes read 256 contiguous bytes), the store layout
+// does the same for the store order; the host's exchanges move bits between.
+// waves per SIMD the register budget is sized for (2^NR amplitudes per lane)
+#ifndef DESV_WSTAGE_WAVES5
+#define DESV_WSTAGE_WAVES5 4
+#endif
+#ifndef DESV_WSTAGE_WAVES6
+#define DESV_WSTAGE_WAVES6 2
+#endif
+template <int NR, bool CP>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NR <= 5 ? DESV_WSTAGE_WAVES5 : DESV_WSTAGE_WAVES6)))
+k_wstage(StageTask tk) {
+    constexpr int NJ = 1 << NR, M = 6 + NR;
+    const int lane = threadIdx.x;
+    const int64_t t = blockIdx.x;
+    if (t >= uni64((uint64_t)tk.st.n_tiles)) return;
+    const DESV_CONST int32_t *wp = cptr(tk.rprog);
+    const DESV_GLOBAL float4 *fr = gptr(reinterpret_cast<const float4 *>(tk.rfrag));
+    const int nmu = wp[0], nq = wp[1];
+    float4 na = make_float4(0.f, 0.f, 0.f, 0.f), nl = na;
+    if (nq > 0) {
+        na = gload(fr + lane);
+        if constexpr (CP) nl = gload(fr + 64 + lane);
+    }
+    // tile base: the tile number's bits deposited into the non-tile bits
+    const uint64_t tile_mask = uni64(tk.st.tile_mask);
+    const int n = uni(tk.n);
+    const uint64_t all = (n >= 64) ? ~0ull : ((1ull << n) - 1);
+    uint64_t base = 0;
+    {
+        uint64_t tt = (uint64_t)t, fm = ~tile_mask & all;
+        while (fm) {
+            const uint64_t lb = fm & (~fm + 1);
+            if (tt & 1) base |= lb;
+            tt >>= 1;
+            fm ^= lb;
+        }
+    }
+    DESV_GLOBAL float4 *sv = gptr_rw(reinterpret_cast<float4 *>(tk.sv0));
+    // amplitude offset of this lane (lane bits) and of register pair p
+    // (register bits 1.., wave-uniform) in a layout given as global bits
+    auto lane_off = [&](const DESV_CONST int32_t *gb) {
+        uint64_t o = base;
+#pragma unroll
+        for (int i = 0; i < 6; ++i)
+            if ((lane >> i) & 1) o |= 1ull << gb[i];
+        return o;
+    };
+    auto reg_off = [&](const DESV_CONST int32_t *gb, int p) {
+        uint64_t o = 0;
+#pragma unroll
+        for (int r = 1; r < NR; ++r)
+            if ((p >> (r - 1)) & 1) o |= 1ull << gb[6 + r];
+        return o;
+    };
+    float2 v[NJ];
+    {
+        const DESV_CONST int32_t *gb = wp + 2;
+        const uint64_t lo = lane_off(gb);
+#pragma unroll
+        for (int p = 0; p < NJ / 2; ++p) {
+            const float4 x = gload(sv + ((lo | reg_off(gb, p)) >> 1));
+            v[2 * p] = make_float2(x.x, x.y);
+            v[2 * p + 1] = make_float2(x.z, x.w);
+        }
+    }
+    int q = 0;
+    rg_run_micro<CP>(v, wp + 2 + 2 * M, nmu, fr, q, nq, na, nl);
+    {
+        const DESV_CONST int32_t *gb = wp + 2 + M;
+        const uint64_t lo = lane_off(gb);
+#pragma unroll
+        for (int p = 0; p < NJ / 2; ++p)
+            gstore(sv + ((lo | reg_off(gb, p)) >> 1), make_float4(v[2 * p].x, v[2 * p].y, v[2 * p + 1].x, v[2 * p + 1].y));
+    }
+}
+
+hipError_t launch_wave_stage(int m, bool comp, const StageTask &t, hipStream_t s) {
+    if (t.st.n_tiles <= 0) return hipSuccess;
+    if (t.st.n_tiles > 0x7fffffffll) return hipErrorInvalidValue;
+    const dim3 g((unsigned)t.st.n_tiles);
+    if (m == 11) {
+        if (comp) hipLaunchKernelGGL((k_wstage<5, true>), g, dim3(64), 0, s, t);
+        else hipLaunchKernelGGL((k_wstage<5, false>), g, dim3(64), 0, s, t);
+    } else if (m == 12) {
+        if (comp) hipLaunchKernelGGL((k_wstage<6, true>), g, dim3(64), 0, s, t);
+        else hipLaunchKernelGGL((k_wstage<6, false>), g, dim3(64), 0, s, t);
+    } else {
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------

@@ -964,15 +964,15 @@ void round_mats_f32_lo(const Plan &p, const float *hi, float *lo) {
 
 
 // ---------------------------------------------------------------------------
-// Register groups (RegProgram): grouping, layouts, exchanges and fragments
+// Register-resident tiles (RegProgram): groups, wave stages, layouts,
+// exchanges and fragments
 // ---------------------------------------------------------------------------
 namespace {
 
-// Positions of a tile bit in the register layout.
-constexpr int kPosR0 = 6, kPosW0 = 10, kNPos = 12;
+// Positions of a tile bit: lane bits 0..5, then nr register bits, then wave bits.
+constexpr int kPosR0 = 6, kMaxPos = 14;
 inline bool pos_is_col(int q) { return q < 4; }             // lane bits 0..3: MFMA columns
 inline bool pos_is_h(int q) { return q == 4 || q == 5; }   // lane bits 4, 5: MFMA row groups
-inline bool pos_is_r(int q) { return q >= kPosR0 && q < kPosW0; }
 
 struct RegCore {
     int k = 0;
@@ -1006,10 +1006,13 @@ bool reg_core_of(const DevOp &d, RegCore &c) {
     return true;
 }
 
-// Layout of the 12 tile bits: at[q] = tile bit at position q, where[b] = position of tile bit b.
+// Layout of a tile's bits over positions: at[q] = tile bit at position q,
+// where[b] = position of tile bit b.
 struct RegLayout {
-    int at[kNPos];
-    int where[kNPos];
+    int nr = 4;
+    int at[kMaxPos];
+    int where[kMaxPos];
+    bool is_r(int q) const { return q >= kPosR0 && q < kPosR0 + nr; }
     void set(int q, int b) {
         at[q] = b;
         where[b] = q;
@@ -1019,9 +1022,9 @@ struct RegLayout {
         set(q1, b2);
         set(q2, b1);
     }
-    int32_t reg_slot(int j) const {  // byte slot of the register part of v[j]
+    int32_t reg_slot(int j) const {  // LDS byte slot of the register part of v[j]
         int idx = 0;
-        for (int r = 0; r < 4; ++r)
+        for (int r = 0; r < nr; ++r)
             if (j >> r & 1) idx |= 1 << at[kPosR0 + r];
         return lt_slot(idx) * 8;
     }
@@ -1041,6 +1044,160 @@ int reg_cost(const RegLayout &L, const int *T) {
     return 2 * col + std::max(0, 2 - h);
 }
 
+// Emits the exchanges and MFMA micro-ops of cores[b, e) on layout L (and
+// their fragment blocks); `avail` = tile bits an op's identity factors may
+// use (the layout's lane and register bits).
+void schedule_ops(const Plan &plan, const std::vector<RegCore> &cores, int b, int e, const std::vector<int> &avail,
+                  RegLayout &L, std::vector<int32_t> &mu, RegProgram &out) {
+    auto next_use = [&](int bit, int i) {
+        for (int j = i + 1; j < e; ++j)
+            for (int t = 0; t < cores[j].k; ++t)
+                if (cores[j].t[t] == bit) return j;
+        return 1 << 30;
+    };
+    auto xlane = [&](int qr, int qb) {
+        mu.push_back(RG_XLANE | (qr - kPosR0) << 4 | qb << 8);
+        L.swap_pos(qr, qb);
+        out.exchanges++;
+    };
+    for (int i = b; i < e; ++i) {
+        const RegCore &c = cores[i];
+        // targets: the core bits, widened to 3 by the cheapest extra bits
+        int T[3] = {c.t[0], c.t[1], c.t[2]};
+        if (c.k < 3) {
+            int best = 1 << 30;
+            int cand[3] = {T[0], T[1], T[2]};
+            const int na = (int)avail.size();
+            for (int a = 0; a < na; ++a) {
+                const int ba = avail[a];
+                bool in = false;
+                for (int j = 0; j < c.k; ++j) in |= c.t[j] == ba;
+                if (in) continue;
+                if (c.k == 2) {
+                    cand[2] = ba;
+                    const int cst = reg_cost(L, cand);
+                    if (cst < best) {
+                        best = cst;
+                        std::copy(cand, cand + 3, T);
+                    }
+                    continue;
+                }
+                for (int a2 = a + 1; a2 < na; ++a2) {
+                    const int bb = avail[a2];
+                    if (bb == c.t[0]) continue;
+                    cand[1] = ba;
+                    cand[2] = bb;
+                    const int cst = reg_cost(L, cand);
+                    if (cst < best) {
+                        best = cst;
+                        std::copy(cand, cand + 3, T);
+                    }
+                }
+            }
+        }
+        auto is_t = [&](int bit) { return bit == T[0] || bit == T[1] || bit == T[2]; };
+        // targets on column lane bits move to register bits (the displaced
+        // non-target with the latest next use goes there)
+        for (int j = 0; j < 3; ++j) {
+            const int q = L.where[T[j]];
+            if (!pos_is_col(q)) continue;
+            int pick = -1, far = -1;
+            for (int r = 0; r < L.nr; ++r) {
+                const int bit = L.at[kPosR0 + r];
+                if (is_t(bit)) continue;
+                const int nu = next_use(bit, i);
+                if (nu > far) {
+                    far = nu;
+                    pick = r;
+                }
+            }
+            xlane(kPosR0 + pick, q);
+        }
+        // two targets at lane bits 4, 5
+        for (;;) {
+            int h = 0;
+            for (int j = 0; j < 3; ++j) h += pos_is_h(L.where[T[j]]);
+            if (h >= 2) break;
+            int tr = -1;
+            for (int j = 0; j < 3 && tr < 0; ++j)
+                if (L.is_r(L.where[T[j]])) tr = j;
+            int hq = -1, far = -1;
+            for (int q = 4; q <= 5; ++q) {
+                if (is_t(L.at[q])) continue;
+                const int nu = next_use(L.at[q], i);
+                if (nu > far) {
+                    far = nu;
+                    hq = q;
+                }
+            }
+            xlane(L.where[T[tr]], hq);
+        }
+        int s = -1;
+        for (int j = 0; j < 3; ++j)
+            if (L.is_r(L.where[T[j]])) s = L.where[T[j]] - kPosR0;
+        mu.push_back(RG_MFMA3 | s << 4);
+        out.mfma_ops++;
+        // fragments: the 8x8 block over (T[0], T[1], T[2]) as the stacked
+        // real 16x16 matrix in this layout's row/column order
+        auto tl = [&](int sel, int g) {
+            int idx = 0;
+            for (int j = 0; j < 3; ++j) {
+                const int q = L.where[T[j]];
+                const int v = q == 4 ? (g & 1) : q == 5 ? (g >> 1 & 1) : sel;
+                idx |= v << j;
+            }
+            return idx;
+        };
+        const int kc = c.k, mk = (1 << kc) - 1;
+        auto U8 = [&](int o, int in) -> cd {
+            if ((o >> kc) != (in >> kc)) return 0.0;
+            const int ro = o & mk, ri = in & mk;
+            if (c.diag) return ro == ri ? plan.mats[(size_t)c.mat + ro] : cd(0.0);
+            return plan.mats[(size_t)c.mat + (size_t)ro * (1 << kc) + ri];
+        };
+        float R[16][16], Rl[16][16];
+        for (int m = 0; m < 16; ++m)
+            for (int kk = 0; kk < 16; ++kk) {
+                const int go = m >> 2, jj = m & 3, ks = kk >> 2, gi = kk & 3;
+                const cd u = U8(tl(jj >> 1, go), tl(ks & 1, gi));
+                const bool re_o = !(jj & 1), re_i = ks < 2;
+                const double val = re_o ? (re_i ? u.real() : -u.imag()) : (re_i ? u.imag() : u.real());
+                R[m][kk] = (float)val;
+                Rl[m][kk] = (float)(val - (double)R[m][kk]);
+            }
+        for (int blk = 0; blk < out.blocks_per_op; ++blk)
+            for (int lane = 0; lane < 64; ++lane)
+                for (int ks = 0; ks < 4; ++ks)
+                    out.frags.push_back(blk ? Rl[lane & 15][4 * ks + (lane >> 4)] : R[lane & 15][4 * ks + (lane >> 4)]);
+    }
+}
+
+bool stage_cores(const Plan &plan, const DevStage &st, std::vector<RegCore> &cores) {
+    cores.clear();
+    for (int i = st.op_begin; i < st.op_end; ++i) {
+        RegCore c;
+        if (!reg_core_of(plan.ops[i], c)) return false;
+        cores.push_back(c);
+    }
+    return true;
+}
+
+uint32_t core_mask(const RegCore &c) {
+    uint32_t m = 0;
+    for (int j = 0; j < c.k; ++j) m |= 1u << c.t[j];
+    return m;
+}
+
+// Tile bits ordered by first use in cores[b, e) (never used: last, by bit).
+std::vector<int> by_first_use(const std::vector<RegCore> &cores, int b, int e, const std::vector<int> &bits) {
+    std::vector<int> first(64, 1 << 30);
+    for (int i = e - 1; i >= b; --i)
+        for (int j = 0; j < cores[i].k; ++j) first[cores[i].t[j]] = i;
+    std::vector<int> order = bits;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return first[x] < first[y]; });
+    return order;
+}
+
 }  // namespace
 
 void build_reg_program(const Plan &plan, bool comp, RegProgram &out) {
@@ -1049,30 +1206,19 @@ void build_reg_program(const Plan &plan, bool comp, RegProgram &out) {
     const size_t ns = plan.stages.size();
     out.stage_word.assign(ns, -1);
     out.stage_frag.assign(ns, 0);
+    std::vector<RegCore> cores;
     for (size_t si = 0; si < ns; ++si) {
         const DevStage &st = plan.stages[si];
         out.stage_frag[si] = (int32_t)(out.frags.size() / 256);
         if (st.m != kRegTileBits || st.n_grad || st.op_end <= st.op_begin) continue;
-        std::vector<RegCore> cores;
-        bool ok = true;
-        for (int i = st.op_begin; i < st.op_end && ok; ++i) {
-            RegCore c;
-            ok = reg_core_of(plan.ops[i], c);
-            cores.push_back(c);
-        }
-        if (!ok) continue;
+        if (!stage_cores(plan, st, cores)) continue;
         const int nops = (int)cores.size();
-        auto cmask = [&](int i) {
-            uint32_t m = 0;
-            for (int j = 0; j < cores[i].k; ++j) m |= 1u << cores[i].t[j];
-            return m;
-        };
         // groups: maximal runs of ops whose targets leave two tile bits free (the wave bits)
         std::vector<std::pair<int, int>> groups;
         for (int i = 0; i < nops;) {
             uint32_t U = 0;
             int e = i;
-            while (e < nops && popc(U | cmask(e)) <= kRegTileBits - 2) U |= cmask(e++);
+            while (e < nops && popc(U | core_mask(cores[e])) <= kRegTileBits - 2) U |= core_mask(cores[e++]);
             groups.push_back({i, e});
             i = e;
         }
@@ -1080,164 +1226,126 @@ void build_reg_program(const Plan &plan, bool comp, RegProgram &out) {
         out.stage_word[si] = (int32_t)w0;
         out.words.push_back((int32_t)groups.size());
         out.words.push_back(0);  // MFMA ops, filled below
-        int nmfma = 0;
+        const int64_t q0 = out.mfma_ops;
         for (const auto &gr : groups) {
             uint32_t U = 0;
-            for (int i = gr.first; i < gr.second; ++i) U |= cmask(i);
+            for (int i = gr.first; i < gr.second; ++i) U |= core_mask(cores[i]);
             // wave bits: the two highest bits no op of the group touches
             int wb[2], nw = 0;
             for (int b = kRegTileBits - 1; b >= 0 && nw < 2; --b)
                 if (!(U >> b & 1)) wb[nw++] = b;
-            // first use of each other bit within the group
-            std::vector<int> order;
-            std::vector<int> first(kRegTileBits, 1 << 30);
-            for (int i = gr.second - 1; i >= gr.first; --i)
-                for (int j = 0; j < cores[i].k; ++j) first[cores[i].t[j]] = i;
+            std::vector<int> bits;
             for (int b = 0; b < kRegTileBits; ++b)
-                if (b != wb[0] && b != wb[1]) order.push_back(b);
-            std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return first[a] < first[b]; });
+                if (b != wb[0] && b != wb[1]) bits.push_back(b);
+            const std::vector<int> order = by_first_use(cores, gr.first, gr.second, bits);
             // initial layout: earliest-used bits at lane bits 4, 5, then the
             // register bits, the latest-used at the column lane bits
             RegLayout L;
+            L.nr = 4;
             const int seq[10] = {4, 5, kPosR0, kPosR0 + 1, kPosR0 + 2, kPosR0 + 3, 0, 1, 2, 3};
             for (int i = 0; i < 10; ++i) L.set(seq[i], order[i]);
-            L.set(kPosW0, wb[0]);
-            L.set(kPosW0 + 1, wb[1]);
+            L.set(kPosR0 + 4, wb[0]);
+            L.set(kPosR0 + 5, wb[1]);
             const RegLayout L_in = L;
-            auto next_use = [&](int b, int i) {
-                for (int j = i + 1; j < gr.second; ++j)
-                    for (int t = 0; t < cores[j].k; ++t)
-                        if (cores[j].t[t] == b) return j;
-                return 1 << 30;
-            };
             std::vector<int32_t> mu;
-            for (int i = gr.first; i < gr.second; ++i) {
-                const RegCore &c = cores[i];
-                // targets: the core bits, widened to 3 by the cheapest extra bits
-                int T[3] = {c.t[0], c.t[1], c.t[2]};
-                if (c.k < 3) {
-                    int best = 1 << 30;
-                    int cand[3] = {T[0], T[1], T[2]};
-                    for (int a = 0; a < 10; ++a) {
-                        const int ba = order[a];
-                        bool in = false;
-                        for (int j = 0; j < c.k; ++j) in |= c.t[j] == ba;
-                        if (in) continue;
-                        if (c.k == 2) {
-                            cand[2] = ba;
-                            const int cst = reg_cost(L, cand);
-                            if (cst < best) {
-                                best = cst;
-                                std::copy(cand, cand + 3, T);
-                            }
-                            continue;
-                        }
-                        for (int b2 = a + 1; b2 < 10; ++b2) {
-                            const int bb = order[b2];
-                            if (bb == c.t[0]) continue;
-                            cand[1] = ba;
-                            cand[2] = bb;
-                            const int cst = reg_cost(L, cand);
-                            if (cst < best) {
-                                best = cst;
-                                std::copy(cand, cand + 3, T);
-                            }
-                        }
-                    }
-                }
-                auto is_t = [&](int b) { return b == T[0] || b == T[1] || b == T[2]; };
-                // targets on column lane bits move to register bits (the
-                // displaced non-target with the latest next use goes there)
-                for (int j = 0; j < 3; ++j) {
-                    const int q = L.where[T[j]];
-                    if (!pos_is_col(q)) continue;
-                    int pick = -1, far = -1;
-                    for (int r = 0; r < 4; ++r) {
-                        const int b = L.at[kPosR0 + r];
-                        if (is_t(b)) continue;
-                        const int nu = next_use(b, i);
-                        if (nu > far) {
-                            far = nu;
-                            pick = r;
-                        }
-                    }
-                    mu.push_back(RG_XLANE | pick << 4 | q << 8);
-                    L.swap_pos(kPosR0 + pick, q);
-                    out.exchanges++;
-                }
-                // two targets at lane bits 4, 5
-                for (;;) {
-                    int h = 0;
-                    for (int j = 0; j < 3; ++j) h += pos_is_h(L.where[T[j]]);
-                    if (h >= 2) break;
-                    int tr = -1;
-                    for (int j = 0; j < 3 && tr < 0; ++j)
-                        if (pos_is_r(L.where[T[j]])) tr = j;
-                    int hq = -1, far = -1;
-                    for (int q = 4; q <= 5; ++q) {
-                        if (is_t(L.at[q])) continue;
-                        const int nu = next_use(L.at[q], i);
-                        if (nu > far) {
-                            far = nu;
-                            hq = q;
-                        }
-                    }
-                    const int qr = L.where[T[tr]];
-                    mu.push_back(RG_XLANE | (qr - kPosR0) << 4 | hq << 8);
-                    L.swap_pos(qr, hq);
-                    out.exchanges++;
-                }
-                int s = -1;
-                for (int j = 0; j < 3; ++j)
-                    if (pos_is_r(L.where[T[j]])) s = L.where[T[j]] - kPosR0;
-                mu.push_back(RG_MFMA3 | s << 4);
-                ++nmfma;
-                out.mfma_ops++;
-                // fragments: the 8x8 block over (T[0], T[1], T[2]) as the
-                // stacked real 16x16 matrix in this layout's row/column order
-                auto tl = [&](int sel, int g) {
-                    int idx = 0;
-                    for (int j = 0; j < 3; ++j) {
-                        const int q = L.where[T[j]];
-                        const int v = q == 4 ? (g & 1) : q == 5 ? (g >> 1 & 1) : sel;
-                        idx |= v << j;
-                    }
-                    return idx;
-                };
-                const int kc = c.k, mk = (1 << kc) - 1;
-                auto U8 = [&](int o, int in) -> cd {
-                    if ((o >> kc) != (in >> kc)) return 0.0;
-                    const int ro = o & mk, ri = in & mk;
-                    if (c.diag) return ro == ri ? plan.mats[(size_t)c.mat + ro] : cd(0.0);
-                    return plan.mats[(size_t)c.mat + (size_t)ro * (1 << kc) + ri];
-                };
-                float R[16][16], Rl[16][16];
-                for (int m = 0; m < 16; ++m)
-                    for (int kk = 0; kk < 16; ++kk) {
-                        const int go = m >> 2, jj = m & 3, ks = kk >> 2, gi = kk & 3;
-                        const cd u = U8(tl(jj >> 1, go), tl(ks & 1, gi));
-                        const bool re_o = !(jj & 1), re_i = ks < 2;
-                        const double val = re_o ? (re_i ? u.real() : -u.imag()) : (re_i ? u.imag() : u.real());
-                        R[m][kk] = (float)val;
-                        Rl[m][kk] = (float)(val - (double)R[m][kk]);
-                    }
-                for (int blk = 0; blk < out.blocks_per_op; ++blk)
-                    for (int lane = 0; lane < 64; ++lane)
-                        for (int ks = 0; ks < 4; ++ks)
-                            out.frags.push_back(blk ? Rl[lane & 15][4 * ks + (lane >> 4)]
-                                                    : R[lane & 15][4 * ks + (lane >> 4)]);
-            }
+            schedule_ops(plan, cores, gr.first, gr.second, order, L, mu, out);
             out.groups++;
             out.words.push_back((int32_t)mu.size());
             for (int q = 0; q < 6; ++q) out.words.push_back(L_in.at[q]);
-            out.words.push_back(L_in.at[kPosW0]);
-            out.words.push_back(L_in.at[kPosW0 + 1]);
+            out.words.push_back(L_in.at[kPosR0 + 4]);
+            out.words.push_back(L_in.at[kPosR0 + 5]);
             for (int j = 0; j < 16; ++j) out.words.push_back(L_in.reg_slot(j));
             for (int q = 0; q < 6; ++q) out.words.push_back(L.at[q]);
             for (int j = 0; j < 16; ++j) out.words.push_back(L.reg_slot(j));
             out.words.insert(out.words.end(), mu.begin(), mu.end());
         }
-        out.words[w0 + 1] = nmfma;
+        out.words[w0 + 1] = (int32_t)(out.mfma_ops - q0);
+    }
+}
+
+void build_wave_program(const Plan &plan, bool comp, RegProgram &out) {
+    out = RegProgram();
+    out.blocks_per_op = comp ? 2 : 1;
+    const size_t ns = plan.stages.size();
+    out.stage_word.assign(ns, -1);
+    out.stage_frag.assign(ns, 0);
+    std::vector<RegCore> cores;
+    for (size_t si = 0; si < ns; ++si) {
+        const DevStage &st = plan.stages[si];
+        out.stage_frag[si] = (int32_t)(out.frags.size() / 256);
+        const int m = st.m;
+        if ((m != 11 && m != 12) || st.L < 5 || st.n_grad) continue;
+        if (!stage_cores(plan, st, cores)) continue;
+        const int nops = (int)cores.size();
+        // global bit of tile-local bit b, and of store-order bit j
+        auto gbit = [&](int b) { return b < st.L ? b : st.hbits[b - st.L]; };
+        int sperm[MAX_TILE_BITS];
+        for (int j = 0; j < m; ++j) sperm[j] = st.sperm[j];
+        RegLayout L;
+        L.nr = m - 6;
+        // load layout (coalesced 16-byte loads, 256 contiguous bytes per 16
+        // lanes): tile bit 0 at register bit 0 (the amplitude pair of one
+        // load), tile bits 1..4 at lane bits 0..3; the other bits by first
+        // use: lane bits 4, 5, then register bits 1..
+        L.set(kPosR0, 0);
+        for (int q = 0; q < 4; ++q) L.set(q, 1 + q);
+        std::vector<int> rest, all;
+        for (int b = 0; b < m; ++b) all.push_back(b);
+        for (int b = 5; b < m; ++b) rest.push_back(b);
+        rest = by_first_use(cores, 0, nops, rest);
+        {
+            int i = 0;
+            L.set(4, rest[i++]);
+            L.set(5, rest[i++]);
+            for (int r = 1; r < L.nr; ++r) L.set(kPosR0 + r, rest[i++]);
+        }
+        const RegLayout L_in = L;
+        std::vector<int32_t> mu;
+        const int64_t q0 = out.mfma_ops;
+        schedule_ops(plan, cores, 0, nops, all, L, mu, out);
+        // store layout: store-order bit 0 at register bit 0, store-order bits
+        // 1..4 at lane bits 0..3 (any order)
+        auto xlane = [&](int qr, int qb) {
+            mu.push_back(RG_XLANE | (qr - kPosR0) << 4 | qb << 8);
+            L.swap_pos(qr, qb);
+            out.exchanges++;
+        };
+        {
+            const int b0 = sperm[0], q = L.where[b0];
+            if (L.is_r(q) && q != kPosR0) {
+                xlane(q, 4);
+                xlane(kPosR0, 4);
+            } else if (!L.is_r(q)) {
+                xlane(kPosR0, q);
+            }
+        }
+        auto in_set = [&](int b) {
+            for (int j = 1; j <= 4; ++j)
+                if (sperm[j] == b) return true;
+            return false;
+        };
+        for (int j = 1; j <= 4; ++j) {
+            const int b = sperm[j];
+            if (pos_is_col(L.where[b])) continue;
+            int qc = -1;
+            for (int q = 0; q < 4 && qc < 0; ++q)
+                if (!in_set(L.at[q])) qc = q;
+            if (pos_is_h(L.where[b])) xlane(kPosR0 + 1, L.where[b]);
+            xlane(L.where[b], qc);
+        }
+        const size_t w0 = out.words.size();
+        out.stage_word[si] = (int32_t)w0;
+        out.words.push_back((int32_t)mu.size());
+        out.words.push_back((int32_t)(out.mfma_ops - q0));
+        // global bit of each position, load then store layout
+        for (int q = 0; q < 6 + L.nr; ++q) out.words.push_back(gbit(L_in.at[q]));
+        for (int q = 0; q < 6 + L.nr; ++q) {
+            int j = 0;
+            while (sperm[j] != L.at[q]) ++j;
+            out.words.push_back(gbit(j));
+        }
+        out.words.insert(out.words.end(), mu.begin(), mu.end());
+        out.groups++;
     }
 }
 

@@ -59,8 +59,10 @@ class Program:
         matrices uploaded as FP32 hi + lo pairs and applied lo part first, for
         long programs whose repeated matrices would otherwise accumulate their
         FP32 rounding coherently; about twice the gate arithmetic.
-        ``reg_groups`` (complex64 with ``use_mfma``): eligible stages keep the
-        tile in registers between LDS round trips (``desv_compile_opts``)."""
+        ``reg_groups`` (complex64 with ``use_mfma``): 1 = eligible stages keep
+        the tile in registers between LDS round trips, 2 = eligible stages of
+        2^11- or 2^12-amplitude tiles hold one tile per wave in registers from
+        HBM load to HBM store (``desv_compile_opts``)."""
         code = _capi.C64 if np.dtype(dtype) == np.complex64 else _capi.C128
         o = _capi.CompileOpts()
         check(lib.desv_default_opts(code, C.byref(o)))
@@ -73,9 +75,10 @@ class Program:
         o.per_gate = int(bool(per_gate))
         o.use_mfma = int(bool(use_mfma))
         o.compensated = int(bool(compensated))
-        o.reg_groups = int(bool(reg_groups))
+        o.reg_groups = int(reg_groups)
         o.dtype = code
         check(lib.desv_program_compile(self.ptr, C.byref(o)))
+        self._reg_mode = int(reg_groups) if use_mfma and code == _capi.C64 else 0
         return self
 
     def stats(self) -> Dict[str, int]:
@@ -131,7 +134,17 @@ class Program:
         amplitude).  FLOPs: N 2^k 8 per dense k-qubit block, 6 N per diagonal
         or scalar op; bytes: one read and one write of the state per stage."""
         tm, so, ops, _ = self.plan()
-        return [_stage_kernel(so[i], ops, self.n) for i in range(len(tm))]
+        mode = getattr(self, "_reg_mode", 0)
+        regs = self.reg_program()[0] if mode else np.full(len(tm), -1)
+        out = []
+        for i in range(len(tm)):
+            name, flops, mflops = _stage_kernel(so[i], ops, self.n)
+            if regs[i] >= 0 and name != "k_direct":
+                # register-resident tile: every block on MFMA
+                name = "k_wstage" if mode == 2 else "k_stage<regs>"
+                mflops = flops
+            out.append((name, flops, mflops))
+        return out
 
 
 _OPK_DENSE1, _OPK_DENSE2, _OPK_DENSEK, _OPK_DIAG, _OPK_SCALAR, _OPK_GRAD, _OPK_GRADDIAG, _OPK_MFMA = range(8)

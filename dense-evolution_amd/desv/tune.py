@@ -83,6 +83,10 @@ def program_candidates(dtype: str, n: int) -> List[Dict]:
     out = [{"fuse_bits": f, "tile_bits": t, "use_mfma": False} for f in (2, 3) for t in tiles]
     if c64:
         out += [{"fuse_bits": f, "tile_bits": t, "use_mfma": True} for f in (3, 4, 5) for t in tiles]
+        # register-resident tiles: groups (2^12 tiles), whole tiles per wave (2^11, 2^12)
+        out += [{"fuse_bits": 3, "tile_bits": 12, "use_mfma": True, "reg_groups": 1}] if 12 in tiles else []
+        out += [{"fuse_bits": f, "tile_bits": t, "use_mfma": True, "reg_groups": 2}
+                for f in (2, 3) for t in (11, 12) if t in tiles]
     return out
 
 

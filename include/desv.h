@@ -165,9 +165,12 @@ typedef struct {
                        blocks of at most 3 qubits keep the tile in registers
                        between LDS round trips (groups of blocks, lane and
                        register bits exchanged in registers, every block on
-                       MFMA); other stages are unchanged.  Same results as the
-                       LDS path to FP32 rounding.  No counterpart in the
-                       reference.                                             */
+                       MFMA); other stages are unchanged.  2: eligible stages
+                       of 2^11- or 2^12-amplitude tiles (tile_bits 11 or 12)
+                       hold one whole tile per wave in registers from its HBM
+                       load to its HBM store (no LDS, no barriers).  Same
+                       results as the LDS path to FP32 rounding.  No
+                       counterpart in the reference.                          */
 } desv_compile_opts;
 
 /* Fill `opts` with the defaults for a dtype. */

@@ -46,10 +46,14 @@ def test_copy_converts_precision(rng):
 
 # launch shapes of the 30-qubit benchmark: fusion 3, 3-qubit blocks on MFMA,
 # 2^12-amplitude tiles (14 passes, 184 ops), fusion 4 on MFMA, the first with
-# compensated matrices and with register-group stages
+# compensated matrices, with register-group stages and with one tile per wave
+# in registers (2^12 and 2^11 tiles)
 SHAPES_30 = [dict(fuse_bits=3, tile_bits=12, use_mfma=True), dict(fuse_bits=4, use_mfma=True),
              dict(fuse_bits=3, tile_bits=12, use_mfma=True, compensated=True),
-             dict(fuse_bits=3, tile_bits=12, use_mfma=True, reg_groups=True)]
+             dict(fuse_bits=3, tile_bits=12, use_mfma=True, reg_groups=1),
+             dict(fuse_bits=3, tile_bits=12, use_mfma=True, reg_groups=2),
+             dict(fuse_bits=3, tile_bits=11, use_mfma=True, reg_groups=2),
+             dict(fuse_bits=3, tile_bits=12, use_mfma=True, reg_groups=2, compensated=True)]
 
 
 @pytest.mark.parametrize("shape", SHAPES_30)

@@ -1,9 +1,11 @@
-"""Register-group stages on the GPU against the oracle.
+"""Register-resident stages on the GPU against the oracle.
 
 Stages of 2^12-amplitude complex64 tiles whose blocks are all uncontrolled
 dense or diagonal blocks of at most 3 qubits keep the tile in registers
 (`reg_groups=True`); their layouts and exchanges are replayed on the CPU in
-test_reg_groups.py.  Here the kernels run: brickwork circuits of Haar SU(4)
+test_reg_groups.py; so are the wave-tile stages (`reg_groups=2`: one 2^11-
+or 2^12-amplitude tile per wave, loaded from HBM into registers and stored
+back, no LDS).  Here the kernels run: brickwork circuits of Haar SU(4)
 gates (plain and compensated matrices) and mixed op tables (1-2-qubit dense
 and diagonal gates, with and without controls, so register and LDS stages
 alternate) against the oracle, and the register path against the LDS path of
@@ -81,6 +83,51 @@ def test_register_and_lds_paths_agree():
     prog, _ = _brickwork(n, 10, seed=7)
     outs = []
     for rg in (False, True):
+        prog.compile("complex64", fuse_bits=3, tile_bits=12, use_mfma=True, reg_groups=rg)
+        h = desv.StateHandle(n, False)
+        prog.run(h)
+        outs.append(h.download().astype(np.complex128))
+        h.close()
+    assert np.abs(outs[0] - outs[1]).max() < 1e-6
+
+
+@pytest.mark.parametrize("n,tile,fuse,comp", [(13, 11, 3, False), (14, 12, 3, True), (15, 12, 2, False),
+                                              (16, 11, 3, True), (17, 12, 3, False)])
+def test_brickwork_wave_stages_against_oracle(n, tile, fuse, comp):
+    prog, want = _brickwork(n, 8, seed=n + 1)
+    prog.compile("complex64", fuse_bits=fuse, tile_bits=tile, use_mfma=True, reg_groups=2, compensated=comp)
+    sw = prog.reg_program(comp)[0]
+    assert (sw >= 0).all()
+    h = desv.StateHandle(n, False)
+    prog.run(h)
+    got = h.download().astype(np.complex128)
+    h.close()
+    assert np.abs(got - want).max() < 2e-6 * 2 ** (-n / 2) * 64
+    assert abs(np.vdot(want, got)) ** 2 >= 1 - 1e-6
+
+
+@pytest.mark.parametrize("fuse,controlled", [(1, False), (2, False), (3, True)])
+def test_mixed_wave_stages_against_oracle(fuse, controlled, rng):
+    n = 14
+    ops = _mixed_ops(n, 150, rng, controlled)
+    prog = desv.Program(n).add_ops(ops)
+    prog.compile("complex64", fuse_bits=fuse, tile_bits=11, use_mfma=True, reg_groups=2)
+    sv0 = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    sv0 /= np.linalg.norm(sv0)
+    h = desv.StateHandle(n, False)
+    h.upload(sv0)
+    prog.run(h)
+    got = h.download().astype(np.complex128)
+    h.close()
+    want = O.run_ops_fast(sv0, ops)
+    assert abs(np.vdot(want, got)) ** 2 >= 1 - 1e-6
+
+
+def test_wave_and_lds_paths_agree():
+    n = 16
+    prog, _ = _brickwork(n, 10, seed=9)
+    outs = []
+    for rg in (0, 2):
         prog.compile("complex64", fuse_bits=3, tile_bits=12, use_mfma=True, reg_groups=rg)
         h = desv.StateHandle(n, False)
         prog.run(h)

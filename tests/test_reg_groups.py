@@ -74,8 +74,8 @@ def quad_perm_xor(src, b):
 
 
 def xlane(V, r, b):
-    """V: (64, 16) complex register file of one wave."""
-    for j in range(16):
+    """V: (64, NJ) complex register file of one wave."""
+    for j in range(V.shape[1]):
         if (j >> r) & 1:
             continue
         a, c = V[:, j].copy(), V[:, j | (1 << r)].copy()
@@ -101,7 +101,7 @@ def mfma3(V, s, frag):
     for l in range(64):
         for ks in range(4):
             A[l % 16, 4 * ks + l // 16] = frag[l, ks]
-    for j in range(16):
+    for j in range(V.shape[1]):
         if (j >> s) & 1:
             continue
         x0, x1 = V[:, j], V[:, j | (1 << s)]
@@ -314,3 +314,94 @@ def test_reg_groups_option_rules():
         p.compile("complex64", use_mfma=False, reg_groups=True)
     with pytest.raises(Exception):
         p.compile("complex128", reg_groups=True)
+
+
+# ---- wave-tile stages (reg_groups=2): one tile per wave, HBM <-> registers --
+
+def run_micro(V, mu, frags, fb, q):
+    for code in mu:
+        code = int(code)
+        a1 = (code >> 4) & 15
+        if code & 15 == RG_XLANE:
+            xlane(V, a1, (code >> 8) & 15)
+        else:
+            assert code & 15 == RG_MFMA3
+            fr = frags[q * fb].astype(np.float64)
+            if fb == 2:
+                fr = fr + frags[q * fb + 1].astype(np.float64)
+            mfma3(V, a1, fr)
+            q += 1
+    return q
+
+
+def replay_wave(prog, sv, n, compensated):
+    from test_tile_layout import run_stage
+    tm, so, ops, mats = prog.plan()
+    sw, sf, words, frags = prog.reg_program(compensated)
+    fb = 2 if compensated else 1
+    nwave = 0
+    for s in range(len(tm)):
+        if sw[s] < 0:
+            sv = run_stage(sv, n, int(tm[s]), so[s], ops, mats, 2)
+            continue
+        nwave += 1
+        m = bin(int(tm[s])).count("1")
+        nr = m - 6
+        w = words[sw[s]:]
+        nmu, nq = int(w[0]), int(w[1])
+        gl, gs = [int(x) for x in w[2:2 + m]], [int(x) for x in w[2 + m:2 + 2 * m]]
+        mu = w[2 + 2 * m:2 + 2 * m + nmu]
+        # coalescing constraints of the kernel's 16-byte accesses
+        assert gl[6] == 0 and gs[6] == 0
+        assert sorted(gl[:4]) == [1, 2, 3, 4] and sorted(gs[:4]) == [1, 2, 3, 4]
+        free = [b for b in range(n) if not (int(tm[s]) >> b) & 1]
+
+        def index(gb):
+            idx = np.zeros((64, 1 << nr), dtype=np.int64)
+            for i in range(6):
+                idx |= ((np.arange(64)[:, None] >> i) & 1) << gb[i]
+            for r in range(nr):
+                idx |= ((np.arange(1 << nr)[None, :] >> r) & 1) << gb[6 + r]
+            return idx
+        il, is_ = index(gl), index(gs)
+        out = sv.copy()
+        for t in range(1 << (n - m)):
+            base = 0
+            for i, b in enumerate(free):
+                if (t >> i) & 1:
+                    base |= 1 << b
+            V = sv[base | il].astype(np.complex128)
+            q = run_micro(V, mu, frags[sf[s]:], fb, 0)
+            assert q == nq
+            out[base | is_] = V
+        sv = out
+    return sv, nwave
+
+
+@pytest.mark.parametrize("n,depth,tile,fuse", [(13, 6, 11, 3), (14, 5, 12, 3), (13, 6, 12, 2), (15, 4, 11, 3)])
+def test_brickwork_wave_stages_replay(n, depth, tile, fuse, rng):
+    gates = workloads.brickwork_su4(n, depth, seed=11)
+    prog = desv.Program(n)
+    for u, a, b in gates:
+        prog.add_matrix(u, [a, b])
+    prog.compile("complex64", fuse_bits=fuse, tile_bits=tile, use_mfma=True, reg_groups=2)
+    sv0 = _rand_state(n, rng)
+    got, nw = replay_wave(prog, sv0, n, compensated=True)
+    assert nw == len(prog.plan()[0])  # every stage, restore passes included
+    want = sv0.copy()
+    for u, a, b in gates:
+        want = O.apply_matrix(want, u, [a, b])
+    assert np.abs(got - want).max() < 1e-10
+
+
+@pytest.mark.parametrize("fuse,controlled", [(2, False), (3, True)])
+def test_mixed_blocks_wave_stages_replay(fuse, controlled, rng):
+    n = 13
+    ops = _mixed_ops(n, 120, rng, controlled)
+    prog = desv.Program(n).add_ops(ops)
+    prog.compile("complex64", fuse_bits=fuse, tile_bits=11, use_mfma=True, reg_groups=2)
+    sv0 = _rand_state(n, rng)
+    got, nw = replay_wave(prog, sv0, n, compensated=True)
+    if not controlled:
+        assert nw >= 1
+    assert np.abs(got - O.run_ops_fast(sv0, ops)).max() < 1e-10
