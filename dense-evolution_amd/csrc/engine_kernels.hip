@@ -1095,49 +1095,50 @@ __device__ inline double op_grad_block(lds_t *lds, int vstride, int m, const Dev
 // layout it ends in (the host tracks the layouts and emits the exchanges).
 // ---------------------------------------------------------------------------
 
-// Exchange register bit R with lane bit B: the amplitude at (lane bit B = x,
-// register bit R = y) moves to (lane bit B = y, register bit R = x).  For
-// each register pair (a = v[j], b = v[j | 2^R]) lanes with bit B set take
-// b's value from lane l - 2^B into a, lanes with it clear take a's value from
-// lane l + 2^B into b.  Lane bits 4, 5: one v_permlane16/32_swap per float;
-// lane bits 2, 3: DPP row shifts with bank masks; lane bits 0, 1: quad
-// permutations and a select.
-// In-place DPP exchange of one float of a register pair (a: register bit
-// clear, b: set) across lane bit B.  Written as one asm block with a single
-// temporary so a large register-resident tile is updated in place; the
-// leading s_nop covers the 2 wait states a DPP source needs after a VALU
-// write, and the caller ends the exchange with s_nop before the results are
-// read by MFMA.
-template <int B> __device__ inline void rg_xchg(float &a, float &b) {
-    float t;
+// In-place DPP exchange of one register pair (a: register bit clear, b: set;
+// both floats) across lane bit B, as one asm block with two temporaries, so a
+// large register-resident tile is updated in place.  Each DPP source written
+// inside the block is written at least two instructions before it is read;
+// the caller puts s_nop around the whole exchange for the sources written
+// before it and for the MFMA reads after it.
+template <int B> __device__ inline void rg_xchg(float2 &a, float2 &b) {
+    float t0, t1;
     if constexpr (B == 2) {
-        asm volatile("s_nop 1\n\tv_mov_b32 %2, %0\n\t"
-                     "v_mov_b32_dpp %0, %1 row_shr:4 row_mask:0xf bank_mask:0xa\n\ts_nop 1\n\t"
-                     "v_mov_b32_dpp %1, %2 row_shl:4 row_mask:0xf bank_mask:0x5"
-                     : "+v"(a), "+v"(b), "=&v"(t));
+        asm volatile("v_mov_b32 %4, %0\n\tv_mov_b32 %5, %1\n\t"
+                     "v_mov_b32_dpp %0, %2 row_shr:4 row_mask:0xf bank_mask:0xa\n\t"
+                     "v_mov_b32_dpp %1, %3 row_shr:4 row_mask:0xf bank_mask:0xa\n\t"
+                     "v_mov_b32_dpp %2, %4 row_shl:4 row_mask:0xf bank_mask:0x5\n\t"
+                     "v_mov_b32_dpp %3, %5 row_shl:4 row_mask:0xf bank_mask:0x5"
+                     : "+v"(a.x), "+v"(a.y), "+v"(b.x), "+v"(b.y), "=&v"(t0), "=&v"(t1));
     } else if constexpr (B == 3) {
-        asm volatile("s_nop 1\n\tv_mov_b32 %2, %0\n\t"
-                     "v_mov_b32_dpp %0, %1 row_shr:8 row_mask:0xf bank_mask:0xc\n\ts_nop 1\n\t"
-                     "v_mov_b32_dpp %1, %2 row_shl:8 row_mask:0xf bank_mask:0x3"
-                     : "+v"(a), "+v"(b), "=&v"(t));
+        asm volatile("v_mov_b32 %4, %0\n\tv_mov_b32 %5, %1\n\t"
+                     "v_mov_b32_dpp %0, %2 row_shr:8 row_mask:0xf bank_mask:0xc\n\t"
+                     "v_mov_b32_dpp %1, %3 row_shr:8 row_mask:0xf bank_mask:0xc\n\t"
+                     "v_mov_b32_dpp %2, %4 row_shl:8 row_mask:0xf bank_mask:0x3\n\t"
+                     "v_mov_b32_dpp %3, %5 row_shl:8 row_mask:0xf bank_mask:0x3"
+                     : "+v"(a.x), "+v"(a.y), "+v"(b.x), "+v"(b.y), "=&v"(t0), "=&v"(t1));
     } else if constexpr (B == 1) {
         // vcc = lanes with lane bit 1 set: b' = vcc ? b : a[l ^ 2]; then
         // vcc = clear: a' = vcc ? a : b[l ^ 2]
-        asm volatile("s_nop 1\n\tv_mov_b32 %2, %1\n\t"
+        asm volatile("v_mov_b32 %4, %2\n\tv_mov_b32 %5, %3\n\t"
                      "s_mov_b32 vcc_lo, 0xcccccccc\n\ts_mov_b32 vcc_hi, 0xcccccccc\n\t"
-                     "v_cndmask_b32_dpp %1, %0, %1, vcc quad_perm:[2,3,0,1] row_mask:0xf bank_mask:0xf\n\t"
-                     "s_not_b64 vcc, vcc\n\ts_nop 1\n\t"
-                     "v_cndmask_b32_dpp %0, %2, %0, vcc quad_perm:[2,3,0,1] row_mask:0xf bank_mask:0xf"
-                     : "+v"(a), "+v"(b), "=&v"(t)
+                     "v_cndmask_b32_dpp %2, %0, %2, vcc quad_perm:[2,3,0,1] row_mask:0xf bank_mask:0xf\n\t"
+                     "v_cndmask_b32_dpp %3, %1, %3, vcc quad_perm:[2,3,0,1] row_mask:0xf bank_mask:0xf\n\t"
+                     "s_not_b64 vcc, vcc\n\t"
+                     "v_cndmask_b32_dpp %0, %4, %0, vcc quad_perm:[2,3,0,1] row_mask:0xf bank_mask:0xf\n\t"
+                     "v_cndmask_b32_dpp %1, %5, %1, vcc quad_perm:[2,3,0,1] row_mask:0xf bank_mask:0xf"
+                     : "+v"(a.x), "+v"(a.y), "+v"(b.x), "+v"(b.y), "=&v"(t0), "=&v"(t1)
                      :
                      : "vcc");
     } else {
-        asm volatile("s_nop 1\n\tv_mov_b32 %2, %1\n\t"
+        asm volatile("v_mov_b32 %4, %2\n\tv_mov_b32 %5, %3\n\t"
                      "s_mov_b32 vcc_lo, 0xaaaaaaaa\n\ts_mov_b32 vcc_hi, 0xaaaaaaaa\n\t"
-                     "v_cndmask_b32_dpp %1, %0, %1, vcc quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf\n\t"
-                     "s_not_b64 vcc, vcc\n\ts_nop 1\n\t"
-                     "v_cndmask_b32_dpp %0, %2, %0, vcc quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf"
-                     : "+v"(a), "+v"(b), "=&v"(t)
+                     "v_cndmask_b32_dpp %2, %0, %2, vcc quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf\n\t"
+                     "v_cndmask_b32_dpp %3, %1, %3, vcc quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf\n\t"
+                     "s_not_b64 vcc, vcc\n\t"
+                     "v_cndmask_b32_dpp %0, %4, %0, vcc quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf\n\t"
+                     "v_cndmask_b32_dpp %1, %5, %1, vcc quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf"
+                     : "+v"(a.x), "+v"(a.y), "+v"(b.x), "+v"(b.y), "=&v"(t0), "=&v"(t1)
                      :
                      : "vcc");
     }
@@ -1151,6 +1152,7 @@ template <int B> __device__ inline void rg_xchg(float &a, float &b) {
 // lane bits 2, 3: DPP row shifts with bank masks; lane bits 0, 1: quad
 // permutations in a masked select.
 template <int R, int B, int NJ> __device__ inline void rg_xlane(float2 (&v)[NJ]) {
+    if constexpr (B < 4) asm volatile("s_nop 1");
 #pragma unroll
     for (int j = 0; j < NJ; ++j) {
         if (j >> R & 1) continue;
@@ -1171,8 +1173,7 @@ template <int R, int B, int NJ> __device__ inline void rg_xlane(float2 (&v)[NJ])
             a = make_float2(__uint_as_float(ax), __uint_as_float(ay));
             b = make_float2(__uint_as_float(bx), __uint_as_float(by));
         } else {
-            rg_xchg<B>(a.x, b.x);
-            rg_xchg<B>(a.y, b.y);
+            rg_xchg<B>(a, b);
         }
     }
     if constexpr (B < 4) asm volatile("s_nop 4");
@@ -1189,7 +1190,7 @@ __device__ inline void rg_mfma3(float2 (&v)[NJ], const float4 &fa, const float4 
     // the pairs in batches of IL independent accumulator chains, issued
     // interleaved so a wave has IL MFMAs in flight instead of waiting on each
     // chain's previous product
-    constexpr int NP = NJ / 2, IL = NP >= 4 ? 4 : NP;
+    constexpr int NP = NJ / 2, IL = NP >= 32 ? 2 : NP >= 4 ? 4 : NP;
     auto pj = [](int p) { return ((p >> S) << (S + 1)) | (p & ((1 << S) - 1)); };  // p-th pair's low register
 #pragma unroll
     for (int p0 = 0; p0 < NP; p0 += IL) {
@@ -1346,7 +1347,7 @@ __device__ inline void run_reg_groups(lds_t *lds, const DESV_CONST int32_t *rp, 
 // does the same for the store order; the host's exchanges move bits between.
 // waves per SIMD the register budget is sized for (2^NR amplitudes per lane)
 #ifndef DESV_WSTAGE_WAVES5
-#define DESV_WSTAGE_WAVES5 4
+#define DESV_WSTAGE_WAVES5 3
 #endif
 #ifndef DESV_WSTAGE_WAVES6
 #define DESV_WSTAGE_WAVES6 2
