@@ -448,15 +448,16 @@ def suite_parity30(n=30, depth=24, shape=None):
             "norm_c64_minus_1": n64 - 1, "norm_c128_minus_1": n128 - 1}
 
 
-def suite_brickwork32(n=32, depth=24):
-    """The same circuit structure at 32 qubits (32 GiB complex64 state)."""
+def suite_brickwork32(n=32, depth=24, shape=None):
+    """The same circuit structure at 32 qubits (32 GiB complex64 state), in the
+    headline's launch shape."""
     import desv
     from desv import workloads
     gates = workloads.brickwork_su4(n, depth, seed=0)
     prog = desv.Program(n)
     for u, a, b in gates:
         prog.add_matrix(u, [a, b])
-    prog.compile("complex64", fuse_bits=4, use_mfma=True)
+    prog.compile("complex64", **dict(shape or dict(fuse_bits=4, use_mfma=True)))
     h = desv.StateHandle(n, complex128=False)
     prog.run(h)
     h.reset()
@@ -580,7 +581,7 @@ def suite_readout(n=30, shots=1 << 20):
             "measure_ms": t_measure * 1e3, "noise": noise}
 
 
-def suite_brickwork34(n=34, depth=24):
+def suite_brickwork34(n=34, depth=24, shape=None):
     """The brickwork at 34 qubits: a 128 GiB complex64 state, the largest
     power of two that fits one MI355X's 288 GB (the state is updated in
     place; the fused program has no out-of-place streaming stage).  Skipped
@@ -591,7 +592,7 @@ def suite_brickwork34(n=34, depth=24):
     prog = desv.Program(n)
     for u, a, b in gates:
         prog.add_matrix(u, [a, b])
-    prog.compile("complex64", fuse_bits=3, use_mfma=True, tile_bits=12)
+    prog.compile("complex64", **dict(shape or dict(fuse_bits=3, use_mfma=True, tile_bits=12)))
     try:
         h = desv.StateHandle(n, complex128=False)
     except MemoryError as e:
@@ -829,7 +830,7 @@ def main():
     for name in [x for x in args.suite.split(",") if x]:
         log(f"suite {name} ...")
         t0 = time.perf_counter()
-        extras[name] = fns[name](shape=kw) if name == "parity30" else fns[name]()
+        extras[name] = fns[name](shape=kw) if name in ("parity30", "brickwork32", "brickwork34") else fns[name]()
         log(f"suite {name}: {time.perf_counter() - t0:.1f} s {extras[name]}")
     if extras:
         out["extras"] = extras

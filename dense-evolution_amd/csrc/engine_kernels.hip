@@ -1280,8 +1280,11 @@ __device__ inline void rg_run_micro(float2 (&v)[NJ], const DESV_CONST int32_t *m
         } else {
             const float4 fa = na, fl = nl;
             if (q + 1 < nq) {
-                na = gload(fr + (size_t)(q + 1) * FB * 64 + lane);
-                if constexpr (CP) nl = gload(fr + (size_t)(q + 1) * FB * 64 + 64 + lane);
+                // wave-uniform base + 32-bit lane offset: no 64-bit per-lane
+                // address held across the micro-op loop
+                const uint32_t i0 = (uint32_t)((q + 1) * FB * 64) + (uint32_t)lane;
+                na = gload(fr + i0);
+                if constexpr (CP) nl = gload(fr + (i0 + 64u));
             }
             ++q;
             switch (a1) {

@@ -1048,12 +1048,15 @@ int reg_cost(const RegLayout &L, const int *T) {
 // their fragment blocks); `avail` = tile bits an op's identity factors may
 // use (the layout's lane and register bits).
 void schedule_ops(const Plan &plan, const std::vector<RegCore> &cores, int b, int e, const std::vector<int> &avail,
-                  RegLayout &L, std::vector<int32_t> &mu, RegProgram &out) {
+                  RegLayout &L, std::vector<int32_t> &mu, RegProgram &out, uint32_t col_final = 0) {
+    // index of the next op using `bit` after op i; bits no later op uses rank
+    // last, those the final layout wants on column lane bits (`col_final`)
+    // after the others, so they are the first displaced onto column bits
     auto next_use = [&](int bit, int i) {
         for (int j = i + 1; j < e; ++j)
             for (int t = 0; t < cores[j].k; ++t)
                 if (cores[j].t[t] == bit) return j;
-        return 1 << 30;
+        return (1 << 30) + (int)(col_final >> bit & 1);
     };
     auto xlane = [&](int qr, int qb) {
         mu.push_back(RG_XLANE | (qr - kPosR0) << 4 | qb << 8);
@@ -1302,7 +1305,9 @@ void build_wave_program(const Plan &plan, bool comp, RegProgram &out) {
         const RegLayout L_in = L;
         std::vector<int32_t> mu;
         const int64_t q0 = out.mfma_ops;
-        schedule_ops(plan, cores, 0, nops, all, L, mu, out);
+        uint32_t col_final = 0;
+        for (int j = 1; j <= 4; ++j) col_final |= 1u << sperm[j];
+        schedule_ops(plan, cores, 0, nops, all, L, mu, out, col_final);
         // store layout: store-order bit 0 at register bit 0, store-order bits
         // 1..4 at lane bits 0..3 (any order)
         auto xlane = [&](int qr, int qb) {
