@@ -63,7 +63,7 @@ struct DevPlan {
     // register groups (complex64): device words and fragments, per-stage offsets
     int32_t *rwords = nullptr;
     float *rfrags = nullptr;
-    std::vector<int32_t> rstage_word, rstage_frag;
+    std::vector<int32_t> rstage_word, rstage_frag, rstage_nq;
     int rmode = 0;  // 1 register groups, 2 wave tiles
 
     void release() {
@@ -134,6 +134,9 @@ int upload_plan(const Plan &p, int dtype, int device, hipStream_t s, DevPlan &d,
     d.rfrags = rp.words.empty() ? nullptr : reinterpret_cast<float *>(base + oba + mba + lb + rwb);
     d.rstage_word = rp.words.empty() ? std::vector<int32_t>() : rp.stage_word;
     d.rstage_frag = rp.stage_frag;
+    d.rstage_nq.assign(rp.stage_word.size(), 0);
+    for (size_t i = 0; i < rp.stage_word.size(); ++i)
+        if (rp.stage_word[i] >= 0) d.rstage_nq[i] = rp.words[(size_t)rp.stage_word[i] + 1];
     d.rmode = rp.words.empty() ? 0 : regs;
     d.device = device;
     d.dtype = dtype;
@@ -341,7 +344,7 @@ int run_plan_on(desv_handle h, const Plan &plan, const DevPlan &dp, int nv, void
             t.rprog = dp.rwords + dp.rstage_word[si - 1];
             t.rfrag = dp.rfrags + (size_t)dp.rstage_frag[si - 1] * 256;
             if (dp.rmode == 2) {
-                const hipError_t e = launch_wave_stage(st.m, dp.comp != 0, t, h->stream);
+                const hipError_t e = launch_wave_stage(st.m, dp.comp != 0, dp.rstage_nq[si - 1], t, h->stream);
                 if (e != hipSuccess) return fail(DESV_EDEVICE, std::string("wave stage launch: ") + hipGetErrorString(e));
                 continue;
             }

@@ -126,7 +126,10 @@ constexpr int MAX_TILE_BITS = 16;
 // ks = 0..3), hi parts, then (compensated programs) lo parts, per MFMA op.
 constexpr int kRegTileBits = 12;
 constexpr int kRegGroupHeader = 47;
-enum RegCode : int32_t { RG_XLANE = 1, RG_MFMA3 = 2 };
+enum RegCode : int32_t { RG_XLANE = 1, RG_MFMA3 = 2, RG_MFMA3T = 3 };
+// Wave-tile stages keep their fragment blocks in LDS (one copy per workgroup
+// of 4 waves): at most this many 1 KiB blocks per stage.
+constexpr int kWaveMaxFragBlocks = 64;
 struct RegProgram {
     std::vector<int32_t> words;       // all register stages' words
     std::vector<int32_t> stage_word;  // per stage: offset into words, -1 = LDS op path
@@ -215,10 +218,13 @@ void build_lane_tables(const Plan &plan, const float *hi, const float *lo, std::
 void build_reg_program(const Plan &plan, bool comp, RegProgram &out);
 // Wave-tile programs: every eligible stage of 2^11- or 2^12-amplitude tiles
 // runs one tile per wave, loaded from HBM into registers and stored back
-// without LDS.  Stage words: [0] micro-ops, [1] MFMA ops, [2 .. 2+m) global
-// bit of each position (lane bits 0..5, register bits 0..m-7) in the load
-// layout, [2+m .. 2+2m) the same for the store layout, then the micro-ops
-// (RG_XLANE / RG_MFMA3 as for groups).  The load layout holds tile bit 0 at
+// (the stage's A fragments in LDS).  Stage words: [0] micro-ops, [1] MFMA
+// ops, [2 .. 2+m) global bit of each position (lane bits 0..5, register bits
+// 0..m-7) in the load layout, [2+m .. 2+2m) the same for the store layout,
+// then the micro-ops: RG_XLANE as for groups, RG_MFMA3T | x << 4 | y << 8 |
+// z << 12 a 3-qubit block on register bits x < y < z in the 4-block MFMA
+// form (plan.cpp, schedule_ops_t; fragment block = 16 rows x 16 k-steps,
+// row-major).  The load layout holds tile bit 0 at
 // register bit 0 and tile bits 1..4 at lane bits 0..3; the store layout
 // store-order bit 0 at register bit 0 and store-order bits 1..4 at lane bits
 // 0..3 (16-byte accesses, 256 contiguous bytes per 16 lanes).

@@ -1341,42 +1341,168 @@ __device__ inline void run_reg_groups(lds_t *lds, const DESV_CONST int32_t *rp, 
     }
 }
 
-// Wave-tile stage: each 64-lane workgroup holds one tile of 2^m amplitudes
-// (m = 6 + NR: 64 lanes x 2^NR registers) from its HBM load to its HBM store,
-// with no LDS and no barrier, so a CU's waves drift apart and one wave's HBM
+// One 3-qubit block of a wave tile in the 4-block MFMA form: targets on
+// register bits X < Y < Z; for each group of 8 amplitudes (the other
+// register bits), 16 k-steps of v_mfma_f32_16x16x1_4b_f32, k-step k reading
+// float (k & 1) of the register with bits X, Y, Z = k >> 1 as the lane's B
+// value and the fragment a[k] (row lane % 16 of the stacked real matrix) as
+// its A value; accumulator register r goes back to the register k = r came
+// from.  The instruction returns input lane group b's outputs in
+// accumulator registers 4b .. 4b + 3 and output rows 4g' .. 4g' + 3 in lane
+// group g', so the op also exchanges lane bits 4, 5 with register bits Y, Z
+// (the host plans for it).  IL groups are in flight at a time.
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+#ifndef DESV_WT_IL
+#define DESV_WT_IL 1
+#endif
+#ifndef DESV_WT_W5
+#define DESV_WT_W5 3
+#endif
+#ifndef DESV_WT_W6
+#define DESV_WT_W6 3
+#endif
+// A register copy the compiler keeps (an identity DPP move): without it the
+// register allocator merges the tile registers into the accumulators'
+// 16-register tuples, whose grouping differs from block to block, and spills.
+__device__ inline float rg_copy(float x) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, x), 0xe4, 0xf, 0xf, false));
+}
+template <int X, int Y, int Z, int NR>
+__device__ constexpr int rg_treg(int g, int t) {
+    int j = 0, gi = 0;
+    for (int b = 0; b < NR; ++b) {
+        if (b == X) j |= (t & 1) << b;
+        else if (b == Y) j |= ((t >> 1) & 1) << b;
+        else if (b == Z) j |= ((t >> 2) & 1) << b;
+        else j |= ((g >> gi++) & 1) << b;
+    }
+    return j;
+}
+__device__ inline float &rg_f(float2 &x, int c) { return c ? x.y : x.x; }
+template <int X, int Y, int Z, bool CP, int NJ>
+__device__ inline void rg_mfma3t(float2 (&v)[NJ], const float (&a)[16], const float (&al)[16]) {
+    constexpr int NR = NJ == 32 ? 5 : 6, NG = NJ / 8, IL = DESV_WT_IL;
+#pragma unroll
+    for (int g0 = 0; g0 < NG; g0 += IL) {
+        f32x16_t d[IL];
+#pragma unroll
+        for (int i = 0; i < IL; ++i) d[i] = f32x16_t{};
+        if constexpr (CP) {
+#pragma unroll
+            for (int k = 0; k < 16; ++k)
+#pragma unroll
+                for (int i = 0; i < IL; ++i)
+                    d[i] = __builtin_amdgcn_mfma_f32_16x16x1f32(al[k], rg_f(v[rg_treg<X, Y, Z, NR>(g0 + i, k >> 1)], k & 1),
+                                                                d[i], 0, 0, 0);
+        }
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+#pragma unroll
+            for (int i = 0; i < IL; ++i)
+                d[i] = __builtin_amdgcn_mfma_f32_16x16x1f32(a[k], rg_f(v[rg_treg<X, Y, Z, NR>(g0 + i, k >> 1)], k & 1),
+                                                            d[i], 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < IL; ++i)
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+                rg_f(v[rg_treg<X, Y, Z, NR>(g0 + i, r >> 1)], r & 1) = rg_copy(d[i][r]);
+    }
+}
+
+// Micro-ops of a wave tile: exchanges and 4-block MFMA blocks.  `fl` = the
+// stage's fragment blocks in LDS (FB = 1 or 2 blocks of 16 x 16 floats per
+// op); a lane reads row lane % 16 of each.
+template <bool CP, int NJ>
+__device__ inline void rg_run_wave(float2 (&v)[NJ], const DESV_CONST int32_t *mu, int nmu, const lds_t *fl) {
+    constexpr int NR = NJ == 32 ? 5 : 6, FB = CP ? 2 : 1;
+    const int row = (threadIdx.x & 15) * 16 * (int)sizeof(float);
+    int q = 0;
+    for (int u = 0; u < nmu; ++u) {
+        const int code = mu[u];
+        if ((code & 15) == RG_XLANE) {
+            const int a1 = (code >> 4) & 15, b = (code >> 8) & 15;
+            switch (a1 * 8 + b) {
+#define DESV_RX(R)                                             \
+    case R * 8 + 0: rg_xlane<R, 0>(v); break;                  \
+    case R * 8 + 1: rg_xlane<R, 1>(v); break;                  \
+    case R * 8 + 2: rg_xlane<R, 2>(v); break;                  \
+    case R * 8 + 3: rg_xlane<R, 3>(v); break;                  \
+    case R * 8 + 4: rg_xlane<R, 4>(v); break;                  \
+    case R * 8 + 5: rg_xlane<R, 5>(v); break;
+                DESV_RX(0) DESV_RX(1) DESV_RX(2) DESV_RX(3) DESV_RX(4)
+                default:
+                    if constexpr (NR >= 6) {
+                        switch (a1 * 8 + b) {
+                            DESV_RX(5)
+                            default: break;
+                        }
+                    }
+                    break;
+#undef DESV_RX
+            }
+        } else {
+            float a[16], al[16];
+            const lds_t *f = fl + (size_t)q * FB * 1024 + row;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const float4 x = lds_ld<float4>(f, 16 * c);
+                a[4 * c] = x.x, a[4 * c + 1] = x.y, a[4 * c + 2] = x.z, a[4 * c + 3] = x.w;
+                if constexpr (CP) {
+                    const float4 y = lds_ld<float4>(f + 1024, 16 * c);
+                    al[4 * c] = y.x, al[4 * c + 1] = y.y, al[4 * c + 2] = y.z, al[4 * c + 3] = y.w;
+                } else {
+                    al[4 * c] = al[4 * c + 1] = al[4 * c + 2] = al[4 * c + 3] = 0.f;
+                }
+            }
+            ++q;
+            switch (code >> 4) {
+#define DESV_RT(X, Y, Z) \
+    case X | Y << 4 | Z << 8: rg_mfma3t<X, Y, Z, CP>(v, a, al); break;
+                DESV_RT(0, 1, 2) DESV_RT(0, 1, 3) DESV_RT(0, 1, 4) DESV_RT(0, 2, 3) DESV_RT(0, 2, 4)
+                DESV_RT(0, 3, 4) DESV_RT(1, 2, 3) DESV_RT(1, 2, 4) DESV_RT(1, 3, 4) DESV_RT(2, 3, 4)
+                default:
+                    if constexpr (NR >= 6) {
+                        switch (code >> 4) {
+                            DESV_RT(0, 1, 5) DESV_RT(0, 2, 5) DESV_RT(0, 3, 5) DESV_RT(0, 4, 5) DESV_RT(1, 2, 5)
+                            DESV_RT(1, 3, 5) DESV_RT(1, 4, 5) DESV_RT(2, 3, 5) DESV_RT(2, 4, 5) DESV_RT(3, 4, 5)
+                            default: break;
+                        }
+                    }
+                    break;
+#undef DESV_RT
+            }
+        }
+    }
+}
+
+// Wave-tile stage: each of the 4 waves of a workgroup holds one tile of 2^m
+// amplitudes (m = 6 + NR: 64 lanes x 2^NR registers) from its HBM load to its
+// HBM store, with no barrier after the workgroup has copied the stage's
+// fragment blocks into LDS, so a CU's waves drift apart and one wave's HBM
 // traffic overlaps other waves' MFMA work.  The load layout puts tile bit 0
 // at register bit 0 and tile bits 1..4 at lane bits 0..3 (each 16-byte load
 // is an amplitude pair, 16 lanes read 256 contiguous bytes), the store layout
-// does the same for the store order; the host's exchanges move bits between.
-// waves per SIMD the register budget is sized for (2^NR amplitudes per lane)
-#ifndef DESV_WSTAGE_WAVES5
-#define DESV_WSTAGE_WAVES5 3
-#endif
-#ifndef DESV_WSTAGE_WAVES6
-#define DESV_WSTAGE_WAVES6 2
-#endif
+// does the same for the store order; the host's exchanges and the 4-block
+// MFMA form move bits between.
 template <int NR, bool CP>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NR <= 5 ? DESV_WSTAGE_WAVES5 : DESV_WSTAGE_WAVES6)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NR <= 5 ? DESV_WT_W5 : CP ? 2 : DESV_WT_W6)))
 k_wstage(StageTask tk) {
     constexpr int NJ = 1 << NR, M = 6 + NR;
-    const int lane = threadIdx.x;
-    const int64_t t = blockIdx.x;
-    if (t >= uni64((uint64_t)tk.st.n_tiles)) return;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const uint32_t zero = 0;
+    lds_t *lds = reinterpret_cast<lds_t *>((uintptr_t)zero);
+    const int lane = threadIdx.x & 63;
+    const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const bool live = t < uni64((uint64_t)tk.st.n_tiles);
     const DESV_CONST int32_t *wp = cptr(tk.rprog);
-    const DESV_GLOBAL float4 *fr = gptr(reinterpret_cast<const float4 *>(tk.rfrag));
     const int nmu = wp[0], nq = wp[1];
-    float4 na = make_float4(0.f, 0.f, 0.f, 0.f), nl = na;
-    if (nq > 0) {
-        na = gload(fr + lane);
-        if constexpr (CP) nl = gload(fr + 64 + lane);
-    }
     // tile base: the tile number's bits deposited into the non-tile bits
     const uint64_t tile_mask = uni64(tk.st.tile_mask);
     const int n = uni(tk.n);
     const uint64_t all = (n >= 64) ? ~0ull : ((1ull << n) - 1);
     uint64_t base = 0;
     {
-        uint64_t tt = (uint64_t)t, fm = ~tile_mask & all;
+        uint64_t tt = (uint64_t)uni64((uint64_t)t), fm = ~tile_mask & all;
         while (fm) {
             const uint64_t lb = fm & (~fm + 1);
             if (tt & 1) base |= lb;
@@ -1402,7 +1528,7 @@ k_wstage(StageTask tk) {
         return o;
     };
     float2 v[NJ];
-    {
+    if (live) {
         const DESV_CONST int32_t *gb = wp + 2;
         const uint64_t lo = lane_off(gb);
 #pragma unroll
@@ -1412,8 +1538,27 @@ k_wstage(StageTask tk) {
             v[2 * p + 1] = make_float2(x.z, x.w);
         }
     }
-    int q = 0;
-    rg_run_micro<CP>(v, wp + 2 + 2 * M, nmu, fr, q, nq, na, nl);
+    {
+        // the stage's fragment blocks into LDS, 4 float4s a thread at a time
+        const DESV_GLOBAL float4 *fr = gptr(reinterpret_cast<const float4 *>(tk.rfrag));
+        const int cnt = nq * (CP ? 2 : 1) * 64;
+        for (int i0 = 0; i0 < cnt; i0 += 4 * 256) {
+            float4 x[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const int i = i0 + c * 256 + (int)threadIdx.x;
+                if (i < cnt) x[c] = gload(fr + i);
+            }
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const int i = i0 + c * 256 + (int)threadIdx.x;
+                if (i < cnt) lds_st<float4>(lds, i * 16, x[c]);
+            }
+        }
+    }
+    __syncthreads();
+    if (!live) return;
+    rg_run_wave<CP>(v, wp + 2 + 2 * M, nmu, lds);
     {
         const DESV_CONST int32_t *gb = wp + 2 + M;
         const uint64_t lo = lane_off(gb);
@@ -1423,16 +1568,18 @@ k_wstage(StageTask tk) {
     }
 }
 
-hipError_t launch_wave_stage(int m, bool comp, const StageTask &t, hipStream_t s) {
+hipError_t launch_wave_stage(int m, bool comp, int nq, const StageTask &t, hipStream_t s) {
     if (t.st.n_tiles <= 0) return hipSuccess;
-    if (t.st.n_tiles > 0x7fffffffll) return hipErrorInvalidValue;
-    const dim3 g((unsigned)t.st.n_tiles);
+    if (t.st.n_tiles > 0x1ffffffffll) return hipErrorInvalidValue;
+    if (nq < 0 || nq * (comp ? 2 : 1) > kWaveMaxFragBlocks) return hipErrorInvalidValue;
+    const dim3 g((unsigned)((t.st.n_tiles + 3) / 4));
+    const size_t lds = (size_t)nq * (comp ? 2 : 1) * 1024;
     if (m == 11) {
-        if (comp) hipLaunchKernelGGL((k_wstage<5, true>), g, dim3(64), 0, s, t);
-        else hipLaunchKernelGGL((k_wstage<5, false>), g, dim3(64), 0, s, t);
+        if (comp) hipLaunchKernelGGL((k_wstage<5, true>), g, dim3(256), lds, s, t);
+        else hipLaunchKernelGGL((k_wstage<5, false>), g, dim3(256), lds, s, t);
     } else if (m == 12) {
-        if (comp) hipLaunchKernelGGL((k_wstage<6, true>), g, dim3(64), 0, s, t);
-        else hipLaunchKernelGGL((k_wstage<6, false>), g, dim3(64), 0, s, t);
+        if (comp) hipLaunchKernelGGL((k_wstage<6, true>), g, dim3(256), lds, s, t);
+        else hipLaunchKernelGGL((k_wstage<6, false>), g, dim3(256), lds, s, t);
     } else {
         return hipErrorInvalidValue;
     }

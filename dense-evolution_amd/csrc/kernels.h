@@ -139,10 +139,11 @@ template <typename T>
 // mats + lo_off; complex64 only).
 hipError_t launch_stage(int nv, int variant, bool comp, const StageTask *tasks, const int32_t *wg_task,
                         const StageTask &single, int64_t grid, int max_m, hipStream_t s);
-// Wave-tile stage (complex64, nv = 1): one 2^m-amplitude tile per 64-lane
-// workgroup held in registers, m = 11 or 12; `t.rprog` = the stage's wave
-// program words, `t.rfrag` its first fragment block.
-hipError_t launch_wave_stage(int m, bool comp, const StageTask &t, hipStream_t s);
+// Wave-tile stage (complex64, nv = 1): one 2^m-amplitude tile per wave held
+// in registers (4 waves a workgroup), m = 11 or 12; `t.rprog` = the stage's
+// wave program words, `t.rfrag` its first fragment block, `nq` its MFMA ops
+// (their fragment blocks are staged in LDS).
+hipError_t launch_wave_stage(int m, bool comp, int nq, const StageTask &t, hipStream_t s);
 template <typename T>
 hipError_t launch_matvec_tasks(const MatvecTask *tasks, int ntasks, int64_t grid, hipStream_t s);
 template <typename T>

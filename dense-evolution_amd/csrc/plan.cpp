@@ -1175,6 +1175,123 @@ void schedule_ops(const Plan &plan, const std::vector<RegCore> &cores, int b, in
     }
 }
 
+// Wave-tile form of the ops (RG_MFMA3T): a 3-qubit block with all three
+// targets on register bits x < y < z, as 16 k-steps of the 4-block
+// v_mfma_f32_16x16x1_4b_f32 per group of 8 amplitudes: k-step k reads float
+// (k & 1) of the register with bits x, y, z = k >> 1 (the B operand of the
+// lane's own column), and accumulator register r returns to the register
+// that k = r was read from.  The instruction's layout moves two bits
+// between lanes and registers: the outputs of input lane group b (lane bits
+// 4, 5) come back in accumulator registers 4b .. 4b + 3, and output rows 4g'
+// .. 4g' + 3 land in lane group g'.  So after the op the target o0 sits at
+// register bit x, the old occupants of lane bits 4, 5 at register bits y, z,
+// and the targets o1, o2 at lane bits 4, 5 (the host orders the matrix rows
+// as m = re/im + 2 o0 + 4 o1 + 8 o2 and its columns as k = re/im + 2 t_x +
+// 4 t_y + 8 t_z).  A target on a lane bit is first exchanged with a register
+// bit (one permlane swap for lane bits 4, 5, a DPP exchange for 0..3).
+void schedule_ops_t(const Plan &plan, const std::vector<RegCore> &cores, int b, int e, const std::vector<int> &avail,
+                    RegLayout &L, std::vector<int32_t> &mu, RegProgram &out, uint32_t col_final) {
+    auto next_use = [&](int bit, int i) {
+        for (int j = i + 1; j < e; ++j)
+            for (int t = 0; t < cores[j].k; ++t)
+                if (cores[j].t[t] == bit) return j;
+        return (1 << 30) + (int)(col_final >> bit & 1);
+    };
+    auto pos_cost = [&](int q) { return L.is_r(q) ? 0 : pos_is_h(q) ? 1 : 3; };
+    for (int i = b; i < e; ++i) {
+        const RegCore &c = cores[i];
+        int T[3] = {c.t[0], c.t[1], c.t[2]};
+        if (c.k < 3) {
+            // identity factors on the cheapest bits (register-resident first;
+            // among those, the bits used soonest: they stay register-resident)
+            long best = 1l << 60;
+            const int na = (int)avail.size();
+            for (int a = 0; a < na; ++a) {
+                const int ba = avail[a];
+                bool in = false;
+                for (int j = 0; j < c.k; ++j) in |= c.t[j] == ba;
+                if (in) continue;
+                for (int a2 = (c.k == 2 ? a : a + 1); a2 < na; ++a2) {
+                    const int bb = avail[a2];
+                    if (c.k == 1 && (bb == c.t[0] || bb == ba)) continue;
+                    int cand[3] = {c.t[0], c.k == 2 ? c.t[1] : ba, c.k == 2 ? ba : bb};
+                    if (c.k == 2 && a2 != a) break;
+                    long cst = 0;
+                    for (int j = 0; j < 3; ++j) cst += pos_cost(L.where[cand[j]]);
+                    cst = cst * 4096 + std::min(next_use(cand[2], i), 4095) + (c.k == 1 ? std::min(next_use(cand[1], i), 4095) : 0);
+                    if (cst < best) {
+                        best = cst;
+                        std::copy(cand, cand + 3, T);
+                    }
+                }
+            }
+        }
+        auto is_t = [&](int bit) { return bit == T[0] || bit == T[1] || bit == T[2]; };
+        // targets on lane bits move to register bits (displacing the
+        // non-target used latest)
+        for (int j = 0; j < 3; ++j) {
+            const int q = L.where[T[j]];
+            if (L.is_r(q)) continue;
+            int pick = -1, far = -1;
+            for (int r = 0; r < L.nr; ++r) {
+                const int bit = L.at[kPosR0 + r];
+                if (is_t(bit)) continue;
+                const int nu = next_use(bit, i);
+                if (nu > far) {
+                    far = nu;
+                    pick = r;
+                }
+            }
+            mu.push_back(RG_XLANE | pick << 4 | q << 8);
+            L.swap_pos(kPosR0 + pick, q);
+            out.exchanges++;
+        }
+        int rp[3];
+        for (int j = 0; j < 3; ++j) rp[j] = L.where[T[j]] - kPosR0;
+        std::sort(rp, rp + 3);
+        const int tx = L.at[kPosR0 + rp[0]], ty = L.at[kPosR0 + rp[1]], tz = L.at[kPosR0 + rp[2]];
+        // o0 (stays at register bit x): the target used soonest
+        int o[3] = {T[0], T[1], T[2]};
+        std::sort(o, o + 3, [&](int u, int w) { return next_use(u, i) < next_use(w, i); });
+        const int c1 = L.at[4], c2 = L.at[5];
+        mu.push_back(RG_MFMA3T | rp[0] << 4 | rp[1] << 8 | rp[2] << 12);
+        out.mfma_ops++;
+        auto tbit = [&](int bit) {
+            for (int j = 0; j < 3; ++j)
+                if (T[j] == bit) return j;
+            return -1;
+        };
+        const int jx = tbit(tx), jy = tbit(ty), jz = tbit(tz), j0 = tbit(o[0]), j1 = tbit(o[1]), j2 = tbit(o[2]);
+        const int kc = c.k, mk = (1 << kc) - 1;
+        auto U8 = [&](int oo, int in) -> cd {
+            if ((oo >> kc) != (in >> kc)) return 0.0;
+            const int ro = oo & mk, ri = in & mk;
+            if (c.diag) return ro == ri ? plan.mats[(size_t)c.mat + ro] : cd(0.0);
+            return plan.mats[(size_t)c.mat + (size_t)ro * (1 << kc) + ri];
+        };
+        float R[16][16], Rl[16][16];
+        for (int m = 0; m < 16; ++m)
+            for (int k = 0; k < 16; ++k) {
+                const int oi = ((m >> 1 & 1) << j0) | ((m >> 2 & 1) << j1) | ((m >> 3 & 1) << j2);
+                const int ii = ((k >> 1 & 1) << jx) | ((k >> 2 & 1) << jy) | ((k >> 3 & 1) << jz);
+                const cd u = U8(oi, ii);
+                const bool re_o = !(m & 1), re_i = !(k & 1);
+                const double val = re_o ? (re_i ? u.real() : -u.imag()) : (re_i ? u.imag() : u.real());
+                R[m][k] = (float)val;
+                Rl[m][k] = (float)(val - (double)R[m][k]);
+            }
+        for (int blk = 0; blk < out.blocks_per_op; ++blk)
+            for (int m = 0; m < 16; ++m)
+                for (int k = 0; k < 16; ++k) out.frags.push_back(blk ? Rl[m][k] : R[m][k]);
+        // layout after the op
+        L.set(kPosR0 + rp[0], o[0]);
+        L.set(kPosR0 + rp[1], c1);
+        L.set(kPosR0 + rp[2], c2);
+        L.set(4, o[1]);
+        L.set(5, o[2]);
+    }
+}
+
 bool stage_cores(const Plan &plan, const DevStage &st, std::vector<RegCore> &cores) {
     cores.clear();
     for (int i = st.op_begin; i < st.op_end; ++i) {
@@ -1297,17 +1414,27 @@ void build_wave_program(const Plan &plan, bool comp, RegProgram &out) {
         for (int b = 5; b < m; ++b) rest.push_back(b);
         rest = by_first_use(cores, 0, nops, rest);
         {
+            // the first-used bits on register bits 1.. (every op reads its
+            // targets from registers), the next two at lane bits 4, 5 (the
+            // first op moves them into registers)
             int i = 0;
+            for (int r = 1; r < L.nr; ++r) L.set(kPosR0 + r, rest[i++]);
             L.set(4, rest[i++]);
             L.set(5, rest[i++]);
-            for (int r = 1; r < L.nr; ++r) L.set(kPosR0 + r, rest[i++]);
         }
         const RegLayout L_in = L;
         std::vector<int32_t> mu;
         const int64_t q0 = out.mfma_ops;
+        const size_t f0 = out.frags.size();
         uint32_t col_final = 0;
         for (int j = 1; j <= 4; ++j) col_final |= 1u << sperm[j];
-        schedule_ops(plan, cores, 0, nops, all, L, mu, out, col_final);
+        schedule_ops_t(plan, cores, 0, nops, all, L, mu, out, col_final);
+        if ((out.mfma_ops - q0) * out.blocks_per_op > kWaveMaxFragBlocks) {
+            // the stage's fragments would not fit the kernel's LDS: LDS op path
+            out.mfma_ops = q0;
+            out.frags.resize(f0);
+            continue;
+        }
         // store layout: store-order bit 0 at register bit 0, store-order bits
         // 1..4 at lane bits 0..3 (any order)
         auto xlane = [&](int qr, int qb) {

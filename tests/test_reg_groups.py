@@ -31,7 +31,7 @@ from desv import workloads
 from oracle import sv_oracle as O
 
 HDR = 47
-RG_XLANE, RG_MFMA3 = 1, 2
+RG_XLANE, RG_MFMA3, RG_MFMA3T = 1, 2, 3
 
 
 def slot(i):
@@ -325,13 +325,63 @@ def run_micro(V, mu, frags, fb, q):
         if code & 15 == RG_XLANE:
             xlane(V, a1, (code >> 8) & 15)
         else:
-            assert code & 15 == RG_MFMA3
             fr = frags[q * fb].astype(np.float64)
             if fb == 2:
                 fr = fr + frags[q * fb + 1].astype(np.float64)
-            mfma3(V, a1, fr)
+            if code & 15 == RG_MFMA3:
+                mfma3(V, a1, fr)
+            else:
+                assert code & 15 == RG_MFMA3T
+                mfma3t(V, a1, (code >> 8) & 15, (code >> 12) & 15, fr.reshape(16, 16))
             q += 1
     return q
+
+
+def mfma_4b(a, b):
+    """v_mfma_f32_16x16x1_4b_f32 (one k-step, as measured on MI355X): four
+    16x16 outer products, block bb from lanes 16 bb .. 16 bb + 15 of both
+    operands (A_bb[m] = a[16 bb + m], B_bb[n] = b[16 bb + n]); D_bb[m][n] is
+    returned at lane 16 (m // 4) + n, accumulator register 4 bb + m % 4."""
+    D = np.zeros((64, 16))
+    for bb in range(4):
+        for m in range(16):
+            for n in range(16):
+                D[16 * (m // 4) + n, 4 * bb + m % 4] = a[16 * bb + m] * b[16 * bb + n]
+    return D
+
+
+def treg(x, y, z, nr, g, t):
+    """Register holding target bits x, y, z = t's bits 0, 1, 2 in group g
+    (g's bits fill the other register bits in increasing order)."""
+    j, gi = 0, 0
+    for b in range(nr):
+        if b == x:
+            j |= (t & 1) << b
+        elif b == y:
+            j |= ((t >> 1) & 1) << b
+        elif b == z:
+            j |= ((t >> 2) & 1) << b
+        else:
+            j |= ((g >> gi) & 1) << b
+            gi += 1
+    return j
+
+
+def mfma3t(V, x, y, z, frag):
+    """3-bit block in the 4-block form: per group of 8 registers, k-step k
+    takes float k & 1 of register treg(k >> 1) as every lane's B value and
+    frag[lane % 16][k] as its A value; accumulator register r is written to
+    float r & 1 of register treg(r >> 1)."""
+    nr = int(np.log2(V.shape[1]))
+    assert x < y < z < nr
+    a = [frag[LANE % 16, k] for k in range(16)]
+    for g in range(V.shape[1] // 8):
+        D = np.zeros((64, 16))
+        for k in range(16):
+            xk = V[:, treg(x, y, z, nr, g, k >> 1)]
+            D += mfma_4b(a[k], xk.imag if k & 1 else xk.real)
+        for t in range(8):
+            V[:, treg(x, y, z, nr, g, t)] = D[:, 2 * t] + 1j * D[:, 2 * t + 1]
 
 
 def replay_wave(prog, sv, n, compensated):
