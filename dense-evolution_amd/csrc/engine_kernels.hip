@@ -1352,6 +1352,9 @@ __device__ inline void run_reg_groups(lds_t *lds, const DESV_CONST int32_t *rp, 
 // group g', so the op also exchanges lane bits 4, 5 with register bits Y, Z
 // (the host plans for it).  IL groups are in flight at a time.
 typedef float f32x16_t __attribute__((ext_vector_type(16)));
+// Build knobs of the wave-tile kernel: MFMA chains in flight per wave (1: 166
+// VGPRs at 64 amplitudes per lane; 2 needs 256 and spills) and waves per SIMD
+// at 32 / 64 amplitudes per lane.
 #ifndef DESV_WT_IL
 #define DESV_WT_IL 1
 #endif
@@ -1361,11 +1364,30 @@ typedef float f32x16_t __attribute__((ext_vector_type(16)));
 #ifndef DESV_WT_W6
 #define DESV_WT_W6 3
 #endif
-// A register copy the compiler keeps (an identity DPP move): without it the
-// register allocator merges the tile registers into the accumulators'
-// 16-register tuples, whose grouping differs from block to block, and spills.
-__device__ inline float rg_copy(float x) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, x), 0xe4, 0xf, 0xf, false));
+// The accumulator's 8 register pairs into the tile registers o0..o7, one
+// 64-bit v_pk_mov_b32 each, in one asm block: the compiler neither merges the
+// tile registers into the accumulators' 16-register tuples (whose grouping
+// differs from block to block; it spilled when it did) nor tracks MFMA
+// hazards into inline asm, so the block waits out the MFMA result latency
+// itself (16 wait states before the first read) and leaves 4 after its last
+// write for the MFMA, permlane or DPP reads that follow.
+__device__ inline void rg_copy8(float2 &o0, float2 &o1, float2 &o2, float2 &o3, float2 &o4, float2 &o5,
+                                float2 &o6, float2 &o7, const f32x16_t &d) {
+    const float2 p0 = make_float2(d[0], d[1]), p1 = make_float2(d[2], d[3]), p2 = make_float2(d[4], d[5]),
+                 p3 = make_float2(d[6], d[7]), p4 = make_float2(d[8], d[9]), p5 = make_float2(d[10], d[11]),
+                 p6 = make_float2(d[12], d[13]), p7 = make_float2(d[14], d[15]);
+    asm volatile("s_nop 7\n\ts_nop 7\n\t"
+                 "v_pk_mov_b32 %0, %8, %8 op_sel:[0,1]\n\t"
+                 "v_pk_mov_b32 %1, %9, %9 op_sel:[0,1]\n\t"
+                 "v_pk_mov_b32 %2, %10, %10 op_sel:[0,1]\n\t"
+                 "v_pk_mov_b32 %3, %11, %11 op_sel:[0,1]\n\t"
+                 "v_pk_mov_b32 %4, %12, %12 op_sel:[0,1]\n\t"
+                 "v_pk_mov_b32 %5, %13, %13 op_sel:[0,1]\n\t"
+                 "v_pk_mov_b32 %6, %14, %14 op_sel:[0,1]\n\t"
+                 "v_pk_mov_b32 %7, %15, %15 op_sel:[0,1]\n\t"
+                 "s_nop 3"
+                 : "=&v"(o0), "=&v"(o1), "=&v"(o2), "=&v"(o3), "=&v"(o4), "=&v"(o5), "=&v"(o6), "=&v"(o7)
+                 : "v"(p0), "v"(p1), "v"(p2), "v"(p3), "v"(p4), "v"(p5), "v"(p6), "v"(p7));
 }
 template <int X, int Y, int Z, int NR>
 __device__ constexpr int rg_treg(int g, int t) {
@@ -1402,10 +1424,12 @@ __device__ inline void rg_mfma3t(float2 (&v)[NJ], const float (&a)[16], const fl
                 d[i] = __builtin_amdgcn_mfma_f32_16x16x1f32(a[k], rg_f(v[rg_treg<X, Y, Z, NR>(g0 + i, k >> 1)], k & 1),
                                                             d[i], 0, 0, 0);
 #pragma unroll
-        for (int i = 0; i < IL; ++i)
-#pragma unroll
-            for (int r = 0; r < 16; ++r)
-                rg_f(v[rg_treg<X, Y, Z, NR>(g0 + i, r >> 1)], r & 1) = rg_copy(d[i][r]);
+        for (int i = 0; i < IL; ++i) {
+            rg_copy8(v[rg_treg<X, Y, Z, NR>(g0 + i, 0)], v[rg_treg<X, Y, Z, NR>(g0 + i, 1)],
+                     v[rg_treg<X, Y, Z, NR>(g0 + i, 2)], v[rg_treg<X, Y, Z, NR>(g0 + i, 3)],
+                     v[rg_treg<X, Y, Z, NR>(g0 + i, 4)], v[rg_treg<X, Y, Z, NR>(g0 + i, 5)],
+                     v[rg_treg<X, Y, Z, NR>(g0 + i, 6)], v[rg_treg<X, Y, Z, NR>(g0 + i, 7)], d[i]);
+        }
     }
 }
 
