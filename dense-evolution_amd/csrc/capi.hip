@@ -63,8 +63,9 @@ struct DevPlan {
     // register groups (complex64): device words and fragments, per-stage offsets
     int32_t *rwords = nullptr;
     float *rfrags = nullptr;
-    std::vector<int32_t> rstage_word, rstage_frag, rstage_nq;
+    std::vector<int32_t> rstage_word, rstage_frag, rstage_nq;  // plan.stages, then plan.fb_stages
     int rmode = 0;  // 1 register groups, 2 wave tiles
+    int relabel_ok = 0;  // every relabelling stage has a wave program
 
     void release() {
         if (buf) (void)hipFree(buf);
@@ -138,6 +139,7 @@ int upload_plan(const Plan &p, int dtype, int device, hipStream_t s, DevPlan &d,
     for (size_t i = 0; i < rp.stage_word.size(); ++i)
         if (rp.stage_word[i] >= 0) d.rstage_nq[i] = rp.words[(size_t)rp.stage_word[i] + 1];
     d.rmode = rp.words.empty() ? 0 : regs;
+    d.relabel_ok = regs == 2 && p.fb_from >= 0 && rp.relabel_ok && !rp.words.empty();
     d.device = device;
     d.dtype = dtype;
     d.comp = comp;
@@ -302,10 +304,23 @@ void *ensure_alt(desv_handle h) {
 // `evs` (optional, stages + 1 events) are recorded before every stage and
 // after the last one.
 int run_plan_on(desv_handle h, const Plan &plan, const DevPlan &dp, int nv, void *sv0, void *sv1,
-                int64_t pstride, hipEvent_t *evs = nullptr) {
+                int64_t pstride, hipEvent_t *evs = nullptr, size_t *n_run = nullptr) {
     const bool own_state = nv == 1 && sv0 == h->sv;
+    // the stages to run (with their register-program index): a relabelled
+    // tail needs the second buffer, else its in-place form runs
+    const size_t S = plan.stages.size();
+    const bool relabel = plan.fb_from >= 0 && dp.relabel_ok && dp.rmode == 2 && own_state && ensure_alt(h);
+    std::vector<std::pair<const DevStage *, size_t>> list;
+    const size_t head = plan.fb_from < 0 || relabel ? S : (size_t)plan.fb_from;
+    for (size_t i = 0; i < head; ++i) list.emplace_back(&plan.stages[i], i);
+    if (plan.fb_from >= 0 && !relabel)
+        for (size_t k = 0; k < plan.fb_stages.size(); ++k) list.emplace_back(&plan.fb_stages[k], S + k);
+    if (n_run) *n_run = list.size();
     size_t si = 0;
-    for (const DevStage &st : plan.stages) {
+    for (const auto &entry : list) {
+        const DevStage &st = *entry.first;
+        const size_t ri = entry.second;
+        const bool relabelled = relabel && ri < S && !plan.relabel.empty() && plan.relabel[ri].store_mask;
         if (evs) HIP_TRY(hipEventRecord(evs[si], h->stream));
         ++si;
         DirectOp dop;
@@ -339,17 +354,24 @@ int run_plan_on(desv_handle h, const Plan &plan, const DevPlan &dp, int nv, void
         t.pstride = pstride;
         t.wg_begin = 0;
         int variant = st.heavy;
-        if (nv == 1 && si - 1 < dp.rstage_word.size() && dp.rstage_word[si - 1] >= 0) {
-            // register-resident tile (the stage index si - 1: si was advanced above)
-            t.rprog = dp.rwords + dp.rstage_word[si - 1];
-            t.rfrag = dp.rfrags + (size_t)dp.rstage_frag[si - 1] * 256;
+        if (nv == 1 && ri < dp.rstage_word.size() && dp.rstage_word[ri] >= 0) {
+            // register-resident tile
+            t.rprog = dp.rwords + dp.rstage_word[ri];
+            t.rfrag = dp.rfrags + (size_t)dp.rstage_frag[ri] * 256;
             if (dp.rmode == 2) {
-                const hipError_t e = launch_wave_stage(st.m, dp.comp != 0, dp.rstage_nq[si - 1], t, h->stream);
+                if (relabelled) t.sv1 = h->alt;
+                const hipError_t e = launch_wave_stage(st.m, dp.comp != 0, dp.rstage_nq[ri], t, h->stream);
                 if (e != hipSuccess) return fail(DESV_EDEVICE, std::string("wave stage launch: ") + hipGetErrorString(e));
+                if (relabelled) {
+                    h->alt = sv0;
+                    h->sv = t.sv1;
+                    sv0 = h->sv;
+                }
                 continue;
             }
             variant = 4;
         }
+        if (relabelled) return fail(DESV_EINVAL, "internal: relabelling stage without a wave program");
         hipError_t e = h->dtype == DESV_C64
                            ? launch_stage<float>(nv, variant, dp.comp != 0, nullptr, nullptr, t, st.n_tiles, st.m,
                                                  h->stream)
@@ -902,8 +924,19 @@ int desv_program_compile(desv_program p, const desv_compile_opts *opts) {
     const size_t lim = stage_lds_limit(dev);
     if (((size_t)1 << std::min(o.tile_bits, p->n)) * amp_bytes(dtype) > lim)
         return fail(DESV_EINVAL, "tile_bits exceeds the LDS of this device");
+    // wave-tile programs end with an out-of-place relabelling stage instead of
+    // restore passes when that stage can run as a wave tile
+    o.relabel = o.reg_groups == 2;
     std::string err;
     if (!build_plan(p->gates, p->n, o, p->plan, err)) return fail(DESV_EINVAL, err);
+    if (p->plan.fb_from >= 0) {
+        RegProgram rp;
+        build_wave_program(p->plan, o.compensated != 0, rp);
+        if (!rp.relabel_ok) {
+            o.relabel = 0;
+            if (!build_plan(p->gates, p->n, o, p->plan, err)) return fail(DESV_EINVAL, err);
+        }
+    }
     p->plan.n_gates = p->n_logical;
     p->opts = o;
     p->compiled = true;
@@ -978,10 +1011,51 @@ int desv_program_reg_dump(desv_program p, int compensated, int32_t *stage_word, 
     RegProgram rp;
     if (p->opts.reg_groups == 2) build_wave_program(p->plan, compensated != 0, rp);
     else build_reg_program(p->plan, compensated != 0, rp);
-    if (stage_word) std::copy(rp.stage_word.begin(), rp.stage_word.end(), stage_word);
-    if (stage_frag) std::copy(rp.stage_frag.begin(), rp.stage_frag.end(), stage_frag);
+    // entries of plan.stages (a relabelled plan's in-place tail follows them
+    // in `words` and `frags`: desv_program_fallback_dump)
+    const size_t S = p->plan.stages.size();
+    if (stage_word) std::copy(rp.stage_word.begin(), rp.stage_word.begin() + (long)S, stage_word);
+    if (stage_frag) std::copy(rp.stage_frag.begin(), rp.stage_frag.begin() + (long)S, stage_frag);
     if (words) std::copy(rp.words.begin(), rp.words.end(), words);
     if (frags) std::copy(rp.frags.begin(), rp.frags.end(), frags);
+    return DESV_OK;
+}
+
+int desv_program_tail_dump(desv_program p, int compensated, int64_t *fb_from, int64_t *n_fb, uint64_t *store_masks,
+                           int32_t *omaps, uint64_t *fb_tile_masks, int64_t *fb_stage_ops, int32_t *fb_stage_word,
+                           int32_t *fb_stage_frag) {
+    if (!p) return fail(DESV_EINVAL, "null program");
+    if (!p->compiled) return fail(DESV_EINVAL, "program is not compiled");
+    const Plan &pl = p->plan;
+    const size_t S = pl.stages.size(), F = pl.fb_stages.size();
+    if (fb_from) *fb_from = pl.fb_from;
+    if (n_fb) *n_fb = (int64_t)F;
+    for (size_t s = 0; s < S; ++s) {
+        const StageRelabel *r = pl.relabel.empty() ? nullptr : &pl.relabel[s];
+        if (store_masks) store_masks[s] = r ? r->store_mask : 0;
+        if (omaps)
+            for (int i = 0; i < 64; ++i)
+                omaps[64 * s + i] = r && r->store_mask && i < (int)r->omap.size() ? r->omap[i] : -1;
+    }
+    for (size_t k = 0; k < F; ++k) {
+        const DevStage &st = pl.fb_stages[k];
+        if (fb_tile_masks) fb_tile_masks[k] = st.tile_mask;
+        if (fb_stage_ops) {
+            int64_t *so = fb_stage_ops + 18 * k;
+            so[0] = st.op_begin;
+            so[1] = st.op_end;
+            for (int j = 0; j < 16; ++j) so[2 + j] = j < st.m ? st.sperm[j] : -1;
+        }
+    }
+    if ((fb_stage_word || fb_stage_frag) && F) {
+        RegProgram rp;
+        if (p->opts.reg_groups == 2) build_wave_program(pl, compensated != 0, rp);
+        for (size_t k = 0; k < F; ++k) {
+            const bool has = rp.stage_word.size() == S + F;
+            if (fb_stage_word) fb_stage_word[k] = has ? rp.stage_word[S + k] : -1;
+            if (fb_stage_frag) fb_stage_frag[k] = has ? rp.stage_frag[S + k] : 0;
+        }
+    }
     return DESV_OK;
 }
 
@@ -1015,8 +1089,11 @@ int desv_run_program(desv_handle h, desv_program p) {
 int desv_run_program_timed(desv_handle h, desv_program p, float *stage_ms) {
     if (!stage_ms) return fail(DESV_EINVAL, "null stage_ms");
     if (int rc = prepare_program(h, p)) return rc;
+    // the in-place tail of a relabelled plan may run instead (more stages):
+    // its extra stages' times are added to the last entry
     const size_t S = p->plan.stages.size();
-    std::vector<hipEvent_t> evs(S + 1, nullptr);
+    const size_t cap = std::max(S, (p->plan.fb_from >= 0 ? (size_t)p->plan.fb_from : S) + p->plan.fb_stages.size());
+    std::vector<hipEvent_t> evs(cap + 1, nullptr);
     int rc = DESV_OK;
     for (auto &e : evs)
         if (hipEventCreate(&e) != hipSuccess) {
@@ -1024,11 +1101,16 @@ int desv_run_program_timed(desv_handle h, desv_program p, float *stage_ms) {
             rc = fail(DESV_EDEVICE, "hipEventCreate failed");
             break;
         }
-    if (!rc) rc = run_plan_on(h, p->plan, p->dev, 1, h->sv, nullptr, 0, evs.data());
-    if (!rc && hipEventSynchronize(evs[S]) != hipSuccess) rc = fail(DESV_EDEVICE, "hipEventSynchronize failed");
-    for (size_t i = 0; !rc && i < S; ++i)
-        if (hipEventElapsedTime(&stage_ms[i], evs[i], evs[i + 1]) != hipSuccess)
+    size_t ran = 0;
+    if (!rc) rc = run_plan_on(h, p->plan, p->dev, 1, h->sv, nullptr, 0, evs.data(), &ran);
+    if (!rc && hipEventSynchronize(evs[ran]) != hipSuccess) rc = fail(DESV_EDEVICE, "hipEventSynchronize failed");
+    for (size_t i = 0; i < S; ++i) stage_ms[i] = 0.f;
+    for (size_t i = 0; !rc && i < ran; ++i) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, evs[i], evs[i + 1]) != hipSuccess)
             rc = fail(DESV_EDEVICE, "hipEventElapsedTime failed");
+        if (S) stage_ms[std::min(i, S - 1)] += ms;
+    }
     for (auto &e : evs)
         if (e) (void)hipEventDestroy(e);
     return rc;

@@ -137,6 +137,9 @@ struct RegProgram {
     std::vector<float> frags;         // fragment blocks, 256 floats each
     int blocks_per_op = 1;            // 2 for compensated programs (hi, lo)
     int64_t groups = 0, exchanges = 0, mfma_ops = 0;
+    // wave programs only: entries past plan.stages.size() belong to
+    // plan.fb_stages; relabel_ok = every relabelling stage has a program
+    bool relabel_ok = true;
 };
 
 struct DevStage {
@@ -166,6 +169,18 @@ struct Recipe {
     uint64_t bits = 0;
 };
 
+// A stage that writes out of place (into the second state buffer) with its
+// outer bits relabelled: store-order bit j of the tile goes to the j-th
+// lowest bit of `store_mask` (taking tile-local bit sperm[j] as usual), and
+// the i-th lowest bit outside the tile (load side) goes to bit omap[i].  One
+// such stage ends a program in the identity layout with no restore passes:
+// only the new low bits need to be inside its tile, the other qubits move
+// with the tile addresses.
+struct StageRelabel {
+    uint64_t store_mask = 0;  // 0: an ordinary in-place stage
+    std::vector<int> omap;
+};
+
 struct Plan {
     int n = 0;
     std::vector<Recipe> recipes;  // one per op (empty for restore-only stages)
@@ -175,6 +190,13 @@ struct Plan {
     std::vector<DevStage> stages;
     std::vector<DevOp> ops;
     std::vector<cd> mats;  // pooled matrices (complex128), rounded at upload
+    // Out-of-place tail (CompileOpts::relabel): `relabel` has one entry per
+    // stage when some stage relabels; such a plan also keeps the in-place
+    // form of its tail, `fb_stages`, which replaces stages[fb_from ..] when no
+    // second buffer is available.  fb_from = -1: no relabelled stages.
+    std::vector<StageRelabel> relabel;
+    int fb_from = -1;
+    std::vector<DevStage> fb_stages;
 };
 
 struct CompileOpts {
@@ -188,6 +210,7 @@ struct CompileOpts {
     int remap = 1;         // let stages permute which qubits sit on the low bits
     int window = 4096;     // gates scanned ahead when choosing a tile
     int reg_groups = 0;    // complex64 MFMA programs: 1 register-group stages, 2 wave-tile stages (RegProgram)
+    int relabel = 0;       // end with an out-of-place relabelling stage instead of restore passes (Plan::relabel)
 };
 
 // Build the device plan for a gate list (in application order).
@@ -224,10 +247,13 @@ void build_reg_program(const Plan &plan, bool comp, RegProgram &out);
 // then the micro-ops: RG_XLANE as for groups, RG_MFMA3T | x << 4 | y << 8 |
 // z << 12 a 3-qubit block on register bits x < y < z in the 4-block MFMA
 // form (plan.cpp, schedule_ops_t; fragment block = 16 rows x 16 k-steps,
-// row-major).  The load layout holds tile bit 0 at
+// row-major); then 0 for an in-place stage, or 1 and the n - m destination
+// bits of the outer bits (StageRelabel::omap) for a relabelling one.  The
+// load layout holds tile bit 0 at
 // register bit 0 and tile bits 1..4 at lane bits 0..3; the store layout
 // store-order bit 0 at register bit 0 and store-order bits 1..4 at lane bits
-// 0..3 (16-byte accesses, 256 contiguous bytes per 16 lanes).
+// 0..3 (16-byte accesses, 256 contiguous bytes per 16 lanes).  Programs are
+// built for plan.stages, then plan.fb_stages.
 void build_wave_program(const Plan &plan, bool comp, RegProgram &out);
 
 }  // namespace desv

@@ -1507,7 +1507,8 @@ __device__ inline void rg_run_wave(float2 (&v)[NJ], const DESV_CONST int32_t *mu
 // at register bit 0 and tile bits 1..4 at lane bits 0..3 (each 16-byte load
 // is an amplitude pair, 16 lanes read 256 contiguous bytes), the store layout
 // does the same for the store order; the host's exchanges and the 4-block
-// MFMA form move bits between.
+// MFMA form move bits between.  A relabelling stage (flag word after the
+// micro-ops) stores into tk.sv1 with its outer bits relabelled.
 template <int NR, bool CP>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NR <= 5 ? DESV_WT_W5 : CP ? 2 : DESV_WT_W6)))
 k_wstage(StageTask tk) {
@@ -1584,11 +1585,22 @@ k_wstage(StageTask tk) {
     if (!live) return;
     rg_run_wave<CP>(v, wp + 2 + 2 * M, nmu, lds);
     {
+        // a relabelling stage writes out of place, its outer bits moved to
+        // the listed destination bits
+        const DESV_CONST int32_t *rl = wp + 2 + 2 * M + nmu;
+        DESV_GLOBAL float4 *dst = sv;
+        if (rl[0]) {
+            dst = gptr_rw(reinterpret_cast<float4 *>(tk.sv1));
+            uint64_t tt = (uint64_t)uni64((uint64_t)t);
+            base = 0;
+            for (int i = 0; i < n - M; ++i, tt >>= 1)
+                if (tt & 1) base |= 1ull << rl[1 + i];
+        }
         const DESV_CONST int32_t *gb = wp + 2 + M;
         const uint64_t lo = lane_off(gb);
 #pragma unroll
         for (int p = 0; p < NJ / 2; ++p)
-            gstore(sv + ((lo | reg_off(gb, p)) >> 1), make_float4(v[2 * p].x, v[2 * p].y, v[2 * p + 1].x, v[2 * p + 1].y));
+            gstore(dst + ((lo | reg_off(gb, p)) >> 1), make_float4(v[2 * p].x, v[2 * p].y, v[2 * p + 1].x, v[2 * p + 1].y));
     }
 }
 

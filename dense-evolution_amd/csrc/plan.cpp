@@ -630,6 +630,65 @@ void store_perm(uint64_t P, uint64_t S, const std::vector<int> &pi, const std::v
     }
 }
 
+// Relabelling store of a stage whose tile is P under layout pi (logical u at
+// physical pi[u]) into the identity layout: store-order bit j (the j-th lowest
+// logical bit of the tile) takes tile-local bit sperm[j]; the i-th lowest
+// physical bit outside P goes to the logical index of its occupant.
+void identity_store(uint64_t P, const std::vector<int> &pi, int n, DevStage &st, StageRelabel &r) {
+    std::vector<int> inv(n), lpos(n, -1);
+    for (int u = 0; u < n; ++u) inv[pi[u]] = u;
+    for (int b = 0, c = 0; b < n; ++b)
+        if (P >> b & 1) lpos[b] = c++;
+    r.store_mask = 0;
+    for (int b = 0; b < n; ++b)
+        if (P >> b & 1) r.store_mask |= 1ull << inv[b];
+    int j = 0;
+    for (int u = 0; u < n; ++u)
+        if (r.store_mask >> u & 1) st.sperm[j++] = lpos[pi[u]];
+    st.permuted = 1;
+    r.omap.clear();
+    for (int b = 0; b < n; ++b)
+        if (!(P >> b & 1)) r.omap.push_back(inv[b]);
+}
+
+// Replace the restore passes (stages n_compute ..) by relabelling stages: the
+// last compute stage itself when its tile holds the qubits of the low bits
+// 0 .. L-1, else one restore pass whose tile holds the low bits' occupants
+// and those qubits.  The in-place stages are kept as the fallback.
+void relabel_tail(Plan &plan, int n, int m, int L, const CompileOpts &o, const std::vector<int> &last_pi,
+                  uint64_t last_S, const std::vector<int> &end_pi, size_t n_compute) {
+    const size_t n_restore = plan.stages.size() - n_compute;
+    const bool last_has_low = (last_S & low_mask(L)) == low_mask(L);
+    if (!last_has_low && n_restore < 2) return;  // one relabelling pass would save nothing
+    const size_t from = last_has_low ? n_compute - 1 : n_compute;
+    plan.fb_from = (int)from;
+    plan.fb_stages.assign(plan.stages.begin() + (long)from, plan.stages.end());
+    plan.stages.resize(from);
+    plan.relabel.assign(from, StageRelabel());
+    StageRelabel r;
+    if (last_has_low) {
+        DevStage st = plan.fb_stages[0];
+        identity_store(st.tile_mask, last_pi, n, st, r);
+        plan.stages.push_back(st);
+    } else {
+        uint64_t P = low_mask(L);
+        for (int u = 0; u < L; ++u) P |= 1ull << end_pi[u];
+        P = fill_tile(P, m, n);
+        std::string err;
+        if (!lower_stage({}, P, n, o, plan, err)) {
+            // cannot happen for a tile the plan's other stages already use;
+            // keep the in-place tail
+            plan.stages.insert(plan.stages.end(), plan.fb_stages.begin(), plan.fb_stages.end());
+            plan.fb_stages.clear();
+            plan.relabel.clear();
+            plan.fb_from = -1;
+            return;
+        }
+        identity_store(P, end_pi, n, plan.stages.back(), r);
+    }
+    plan.relabel.push_back(r);
+}
+
 }  // namespace
 
 bool build_plan(const std::vector<Gate> &gates, int n, const CompileOpts &o, Plan &plan,
@@ -671,6 +730,8 @@ bool build_plan(const std::vector<Gate> &gates, int n, const CompileOpts &o, Pla
     std::vector<int> rem(gates.size());
     for (size_t i = 0; i < gates.size(); ++i) rem[i] = (int)i;
     const int window = std::max(64, o.window);
+    std::vector<int> last_pi;  // layout before the last compute stage
+    uint64_t last_S = 0;       // its tile (logical bits)
     while (!rem.empty()) {
         uint64_t S = 0;  // logical bits in the tile
         for (int p = 0; p < L; ++p) S |= 1ull << inv[p];
@@ -720,6 +781,10 @@ bool build_plan(const std::vector<Gate> &gates, int n, const CompileOpts &o, Pla
             uint64_t T = 0;
             for (int gi : take) T |= gates[gi].tile_mask();
             for (int p = 0; p < L; ++p) T |= 1ull << inv[p];
+            // a relabelling last stage needs the qubits of the low bits
+            // (0 .. L-1) in its tile to write the identity layout coalesced
+            if (o.relabel)
+                for (int u = 0; u < L && popc(T) < m; ++u) T |= 1ull << u;
             if (popc(T) <= m) {
                 for (bool grew = true; grew && popc(T) < m;) {
                     grew = false;
@@ -813,10 +878,14 @@ bool build_plan(const std::vector<Gate> &gates, int n, const CompileOpts &o, Pla
         int sperm[MAX_TILE_BITS];
         store_perm(P, S, pi, pi2, n, sperm);
         if (!lower_stage(phys, P, n, o, plan, err, sperm)) return false;
+        last_pi = pi;
+        last_S = S;
         pi = pi2;
         for (int u = 0; u < n; ++u) inv[pi[u]] = u;
         rem.swap(defer);
     }
+    const std::vector<int> end_pi = pi;
+    const size_t n_compute = plan.stages.size();
     // restore the identity layout: whole cycles of pi per pass
     for (int guard = 0; guard < 4 * n; ++guard) {
         bool ident = true;
@@ -880,6 +949,8 @@ bool build_plan(const std::vector<Gate> &gates, int n, const CompileOpts &o, Pla
             err = "internal: layout restore failed";
             return false;
         }
+    if (o.relabel && m < n && n_compute > 0 && plan.stages.size() > n_compute)
+        relabel_tail(plan, n, m, L, o, last_pi, last_S, end_pi, n_compute);
     return true;
 }
 
@@ -1386,19 +1457,33 @@ void build_reg_program(const Plan &plan, bool comp, RegProgram &out) {
 void build_wave_program(const Plan &plan, bool comp, RegProgram &out) {
     out = RegProgram();
     out.blocks_per_op = comp ? 2 : 1;
-    const size_t ns = plan.stages.size();
-    out.stage_word.assign(ns, -1);
-    out.stage_frag.assign(ns, 0);
+    const size_t ns = plan.stages.size(), nall = ns + plan.fb_stages.size();
+    out.stage_word.assign(nall, -1);
+    out.stage_frag.assign(nall, 0);
     std::vector<RegCore> cores;
-    for (size_t si = 0; si < ns; ++si) {
-        const DevStage &st = plan.stages[si];
+    int pending = 0;  // relabelling stages without a program
+    for (size_t si = 0; si < nall; ++si) {
+        const DevStage &st = si < ns ? plan.stages[si] : plan.fb_stages[si - ns];
+        const StageRelabel *rl = si < ns && !plan.relabel.empty() && plan.relabel[si].store_mask
+                                     ? &plan.relabel[si] : nullptr;
+        pending += rl ? 1 : 0;
         out.stage_frag[si] = (int32_t)(out.frags.size() / 256);
         const int m = st.m;
         if ((m != 11 && m != 12) || st.L < 5 || st.n_grad) continue;
         if (!stage_cores(plan, st, cores)) continue;
         const int nops = (int)cores.size();
-        // global bit of tile-local bit b, and of store-order bit j
+        // global bit of tile-local bit b, and of store-order bit j (the j-th
+        // lowest bit of the tile, or of the store mask of a relabelling stage)
         auto gbit = [&](int b) { return b < st.L ? b : st.hbits[b - st.L]; };
+        int sbits[MAX_TILE_BITS];
+        for (int j = 0, c = 0; j < m; ++j) {
+            if (!rl) {
+                sbits[j] = gbit(j);
+                continue;
+            }
+            while (!(rl->store_mask >> c & 1)) ++c;
+            sbits[j] = c++;
+        }
         int sperm[MAX_TILE_BITS];
         for (int j = 0; j < m; ++j) sperm[j] = st.sperm[j];
         RegLayout L;
@@ -1474,11 +1559,17 @@ void build_wave_program(const Plan &plan, bool comp, RegProgram &out) {
         for (int q = 0; q < 6 + L.nr; ++q) {
             int j = 0;
             while (sperm[j] != L.at[q]) ++j;
-            out.words.push_back(gbit(j));
+            out.words.push_back(sbits[j]);
         }
         out.words.insert(out.words.end(), mu.begin(), mu.end());
+        out.words.push_back(rl ? 1 : 0);
+        if (rl) {
+            out.words.insert(out.words.end(), rl->omap.begin(), rl->omap.end());
+            pending--;
+        }
         out.groups++;
     }
+    out.relabel_ok = pending == 0;
 }
 
 }  // namespace desv

@@ -102,6 +102,7 @@ _SIGS = {
     "desv_program_plan_dump": ([_p, _pu64, _pi64, _pi64, _pd], _i),
     "desv_program_reg_sizes": ([_p, _i, _pi64, _pi64], _i),
     "desv_program_reg_dump": ([_p, _i, _pi, _pi, _pi, C.POINTER(C.c_float)], _i),
+    "desv_program_tail_dump": ([_p, _i, _pi64, _pi64, _pu64, _pi, _pu64, _pi64, _pi, _pi], _i),
     "desv_pauli_expect": ([_p, _pu64, _pu64, _i64, _pd], _i),
     "desv_pauli_matvec": ([_p, _p, _pu64, _pu64, _pd, _pd, _i64], _i),
     "desv_pauli_sum_expect": ([_p, _pu64, _pu64, _pd, _i64, _pd], _i),

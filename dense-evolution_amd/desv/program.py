@@ -82,9 +82,17 @@ class Program:
         return self
 
     def stats(self) -> Dict[str, int]:
+        """gates, passes (HBM sweeps per run), blocks, mfma_blocks, and
+        passes_in_place: the sweeps on a handle without a second buffer (a
+        relabelled tail then runs in its in-place form)."""
         g, p, b, m = (C.c_int64() for _ in range(4))
         check(lib.desv_program_stats(self.ptr, C.byref(g), C.byref(p), C.byref(b), C.byref(m)))
-        return {"gates": g.value, "passes": p.value, "blocks": b.value, "mfma_blocks": m.value}
+        fb_from, n_fb = C.c_int64(), C.c_int64()
+        check(lib.desv_program_tail_dump(self.ptr, 0, C.byref(fb_from), C.byref(n_fb), None, None, None, None,
+                                         None, None))
+        in_place = p.value if fb_from.value < 0 else fb_from.value + n_fb.value
+        return {"gates": g.value, "passes": p.value, "blocks": b.value, "mfma_blocks": m.value,
+                "passes_in_place": in_place}
 
     def plan(self):
         """(tile_masks u64[S], stage_ops i64[S,18] = (begin, end, sperm[16]),
@@ -114,6 +122,30 @@ class Program:
         check(lib.desv_program_reg_dump(self.ptr, int(compensated), _capi.iptr(sw), _capi.iptr(sf),
                                         _capi.iptr(words), frags.ctypes.data_as(C.POINTER(C.c_float))))
         return sw[:s.value], sf[:s.value], words[:w.value], frags[:f.value].reshape(-1, 64, 4)
+
+    def tail(self, compensated: bool = False):
+        """The relabelled tail of a wave-tile program (``desv_program_tail_dump``):
+        dict with fb_from (-1: none), store_masks u64[S], omaps i32[S, 64], and
+        for the in-place form of the tail fb_tile_masks u64[F], fb_stage_ops
+        i64[F, 18], fb_stage_word / fb_stage_frag i32[F] (offsets into
+        ``reg_program``'s words and fragment blocks)."""
+        s, o, m = C.c_int64(), C.c_int64(), C.c_int64()
+        check(lib.desv_program_plan_sizes(self.ptr, C.byref(s), C.byref(o), C.byref(m)))
+        fb_from, n_fb = C.c_int64(), C.c_int64()
+        check(lib.desv_program_tail_dump(self.ptr, int(compensated), C.byref(fb_from), C.byref(n_fb), None, None,
+                                         None, None, None, None))
+        S, F = s.value, n_fb.value
+        sm = np.zeros(max(1, S), dtype=np.uint64)
+        om = np.zeros((max(1, S), 64), dtype=np.int32)
+        ftm = np.zeros(max(1, F), dtype=np.uint64)
+        fso = np.zeros((max(1, F), 18), dtype=np.int64)
+        fsw = np.zeros(max(1, F), dtype=np.int32)
+        fsf = np.zeros(max(1, F), dtype=np.int32)
+        check(lib.desv_program_tail_dump(self.ptr, int(compensated), C.byref(fb_from), C.byref(n_fb),
+                                         _capi.u64ptr(sm), _capi.iptr(om), _capi.u64ptr(ftm), _capi.i64ptr(fso),
+                                         _capi.iptr(fsw), _capi.iptr(fsf)))
+        return {"fb_from": fb_from.value, "store_masks": sm[:S], "omaps": om[:S], "fb_tile_masks": ftm[:F],
+                "fb_stage_ops": fso[:F], "fb_stage_word": fsw[:F], "fb_stage_frag": fsf[:F]}
 
     def run(self, target) -> None:
         h = target.handle if not isinstance(target, StateHandle) else target

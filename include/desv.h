@@ -193,6 +193,23 @@ int desv_program_stats(desv_program p, int64_t *gates, int64_t *passes,
 int desv_program_reg_sizes(desv_program p, int compensated, int64_t *n_words, int64_t *n_frag_floats);
 int desv_program_reg_dump(desv_program p, int compensated, int32_t *stage_word, int32_t *stage_frag,
                           int32_t *words, float *frags);
+/* Relabelled tail of a wave-tile program (introspection, no device needed).
+ * With reg_groups = 2 the compiler ends a program with a stage that writes
+ * into the second state buffer with every qubit moved to its home bit (the
+ * qubits outside the tile move with the tile addresses), instead of in-place
+ * restore passes; the in-place form of that tail is kept and runs when the
+ * handle has no second buffer (desv_set_double_buffer(h, 0)).  Outputs:
+ * fb_from = first stage the in-place tail replaces (-1: none), n_fb = its
+ * stages; per stage of the program store_masks[s] (0: in place, else the bits
+ * its tile is written to) and omaps[64 s ..] (destination bit of each bit
+ * outside the tile, lowest first, -1 padded); per in-place tail stage its
+ * tile mask, (op_begin, op_end, sperm[16]) as desv_program_plan_dump, and
+ * the offsets of its words and first fragment block in the arrays of
+ * desv_program_reg_dump.  Any pointer may be null.  No counterpart in the
+ * reference; used by the CPU replay. */
+int desv_program_tail_dump(desv_program p, int compensated, int64_t *fb_from, int64_t *n_fb, uint64_t *store_masks,
+                           int32_t *omaps, uint64_t *fb_tile_masks, int64_t *fb_stage_ops, int32_t *fb_stage_word,
+                           int32_t *fb_stage_frag);
 int desv_run_program(desv_handle h, desv_program p);
 /* Same as desv_run_program, with a HIP event recorded on the handle's stream
  * before every stage and after the last; waits for the run and writes each

@@ -62,11 +62,22 @@ def test_brickwork_30q_c64_against_c128(shape):
     prog = brickwork_program(n, 24)
     h64 = desv.StateHandle(n, False)
     prog.compile("complex64", **shape)
+    waves = shape.get("reg_groups") == 2
     if shape.get("tile_bits") == 12 and shape["fuse_bits"] == 3:
         st = prog.stats()
-        assert (st["passes"], st["blocks"]) == (14, 184) and st["mfma_blocks"] > 0
+        # wave tiles end with one out-of-place relabelling pass instead of
+        # two in-place restore passes
+        assert (st["passes"], st["blocks"]) == (13 if waves else 14, 184) and st["mfma_blocks"] > 0
     prog.run(h64)
     n64 = h64.norm2()
+    if waves:
+        # the in-place form of the tail (no second buffer) gives the same state
+        assert prog.tail(shape.get("compensated", False))["fb_from"] >= 0
+        hb = desv.StateHandle(n, False)
+        hb.set_double_buffer(False)
+        prog.run(hb)
+        assert abs(h64.inner(hb) - n64) < 1e-9
+        hb.close()
     h128 = desv.StateHandle(n, True)
     prog.compile("complex128")
     prog.run(h128)

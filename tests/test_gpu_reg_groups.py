@@ -134,3 +134,30 @@ def test_wave_and_lds_paths_agree():
         outs.append(h.download().astype(np.complex128))
         h.close()
     assert np.abs(outs[0] - outs[1]).max() < 1e-6
+
+
+@pytest.mark.parametrize("n,count,tile,fuse,comp", [(14, 120, 12, 3, False), (16, 120, 12, 2, True)])
+def test_relabelled_tail_against_oracle(n, count, tile, fuse, comp):
+    """A wave-tile program ending with an out-of-place relabelling stage (the
+    qubits outside its tile move with the tile addresses), run twice in a row
+    (the handle's state buffer swaps each time), and its in-place form on a
+    handle without a second buffer, against the oracle."""
+    rng = np.random.default_rng(n * 1000 + count)
+    ops = _mixed_ops(n, count, rng, False)
+    prog = desv.Program(n).add_ops(ops)
+    prog.compile("complex64", fuse_bits=fuse, tile_bits=tile, use_mfma=True, reg_groups=2, compensated=comp)
+    assert prog.tail(comp)["fb_from"] >= 0
+    sv0 = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    sv0 /= np.linalg.norm(sv0)
+    want1 = O.run_ops_fast(sv0, ops)
+    want2 = O.run_ops_fast(want1, ops)
+    for double_buffer in (True, False):
+        h = desv.StateHandle(n, False)
+        h.set_double_buffer(double_buffer)
+        h.upload(sv0)
+        for want in (want1, want2):
+            prog.run(h)
+            got = h.download().astype(np.complex128)
+            assert np.abs(got - want).max() < 2e-6 * 2 ** (-n / 2) * 64
+            assert abs(np.vdot(want, got)) ** 2 >= 1 - 1e-6
+        h.close()

@@ -367,6 +367,26 @@ def treg(x, y, z, nr, g, t):
     return j
 
 
+# mfma_4b as index arrays: output (lane, register) of block bb, row m, column
+# n, and the operand lanes it reads
+_BB, _M, _N = np.meshgrid(np.arange(4), np.arange(16), np.arange(16), indexing="ij")
+_OUT_L, _OUT_R = 16 * (_M // 4) + _N, 4 * _BB + _M % 4
+_IN_A, _IN_B = 16 * _BB + _M, 16 * _BB + _N
+
+
+def mfma_4b_chain(Ak, Bk):
+    """sum_k mfma_4b(Ak[:, k], Bk[:, k]) (Ak, Bk: (64, K) operand lanes)."""
+    D = np.zeros((64, 16))
+    D[_OUT_L, _OUT_R] = np.einsum("...k,...k->...", Ak[_IN_A], Bk[_IN_B])
+    return D
+
+
+def test_mfma_4b_chain_matches_the_instruction(rng):
+    Ak, Bk = rng.standard_normal((64, 5)), rng.standard_normal((64, 5))
+    want = sum(mfma_4b(Ak[:, k], Bk[:, k]) for k in range(5))
+    assert np.allclose(mfma_4b_chain(Ak, Bk), want, rtol=0, atol=1e-12)
+
+
 def mfma3t(V, x, y, z, frag):
     """3-bit block in the 4-block form: per group of 8 registers, k-step k
     takes float k & 1 of register treg(k >> 1) as every lane's B value and
@@ -374,37 +394,66 @@ def mfma3t(V, x, y, z, frag):
     float r & 1 of register treg(r >> 1)."""
     nr = int(np.log2(V.shape[1]))
     assert x < y < z < nr
-    a = [frag[LANE % 16, k] for k in range(16)]
+    Ak = frag[LANE % 16, :]  # (64, 16): lane's A value at k-step k
     for g in range(V.shape[1] // 8):
-        D = np.zeros((64, 16))
-        for k in range(16):
-            xk = V[:, treg(x, y, z, nr, g, k >> 1)]
-            D += mfma_4b(a[k], xk.imag if k & 1 else xk.real)
+        regs = [treg(x, y, z, nr, g, t) for t in range(8)]
+        Bk = np.empty((64, 16))
         for t in range(8):
-            V[:, treg(x, y, z, nr, g, t)] = D[:, 2 * t] + 1j * D[:, 2 * t + 1]
+            Bk[:, 2 * t], Bk[:, 2 * t + 1] = V[:, regs[t]].real, V[:, regs[t]].imag
+        D = mfma_4b_chain(Ak, Bk)
+        for t in range(8):
+            V[:, regs[t]] = D[:, 2 * t] + 1j * D[:, 2 * t + 1]
 
 
-def replay_wave(prog, sv, n, compensated):
+def wave_stages(prog, compensated, fallback=False):
+    """The stages a wave-tile program runs, as (tile mask, stage_ops row,
+    word offset, first fragment block, store mask or 0, omap): the
+    relabelled tail (the default, with a second state buffer) or its in-place
+    form (``fallback``)."""
+    tm, so, _, _ = prog.plan()
+    sw, sf, _, _ = prog.reg_program(compensated)
+    t = prog.tail(compensated)
+    out = []
+    head = len(tm) if (t["fb_from"] < 0 or not fallback) else t["fb_from"]
+    for s in range(head):
+        out.append((int(tm[s]), so[s], int(sw[s]), int(sf[s]), int(t["store_masks"][s]), t["omaps"][s]))
+    if fallback and t["fb_from"] >= 0:
+        for k in range(len(t["fb_tile_masks"])):
+            out.append((int(t["fb_tile_masks"][k]), t["fb_stage_ops"][k], int(t["fb_stage_word"][k]),
+                        int(t["fb_stage_frag"][k]), 0, None))
+    return out
+
+
+def replay_wave(prog, sv, n, compensated, fallback=False):
     from test_tile_layout import run_stage
-    tm, so, ops, mats = prog.plan()
-    sw, sf, words, frags = prog.reg_program(compensated)
+    _, _, ops, mats = prog.plan()
+    _, _, words, frags = prog.reg_program(compensated)
     fb = 2 if compensated else 1
     nwave = 0
-    for s in range(len(tm)):
-        if sw[s] < 0:
-            sv = run_stage(sv, n, int(tm[s]), so[s], ops, mats, 2)
+    for tmask, srow, sw, sf, smask, omap in wave_stages(prog, compensated, fallback):
+        if sw < 0:
+            assert smask == 0  # a relabelling stage always runs as a wave tile
+            sv = run_stage(sv, n, tmask, srow, ops, mats, 2)
             continue
         nwave += 1
-        m = bin(int(tm[s])).count("1")
+        m = bin(tmask).count("1")
         nr = m - 6
-        w = words[sw[s]:]
+        w = words[sw:]
         nmu, nq = int(w[0]), int(w[1])
         gl, gs = [int(x) for x in w[2:2 + m]], [int(x) for x in w[2 + m:2 + 2 * m]]
         mu = w[2 + 2 * m:2 + 2 * m + nmu]
+        flag = int(w[2 + 2 * m + nmu])
+        assert flag == (1 if smask else 0)
+        if smask:
+            om = [int(x) for x in w[3 + 2 * m + nmu:3 + 2 * m + nmu + n - m]]
+            assert om == [int(x) for x in omap[:n - m]] and all(x == -1 for x in omap[n - m:])
+            # every bit goes somewhere exactly once: tile bits to the store mask
+            assert sorted(om + [b for b in range(n) if (smask >> b) & 1]) == list(range(n))
+            assert sorted(gs) == [b for b in range(n) if (smask >> b) & 1]
         # coalescing constraints of the kernel's 16-byte accesses
         assert gl[6] == 0 and gs[6] == 0
         assert sorted(gl[:4]) == [1, 2, 3, 4] and sorted(gs[:4]) == [1, 2, 3, 4]
-        free = [b for b in range(n) if not (int(tm[s]) >> b) & 1]
+        free = [b for b in range(n) if not (tmask >> b) & 1]
 
         def index(gb):
             idx = np.zeros((64, 1 << nr), dtype=np.int64)
@@ -414,16 +463,18 @@ def replay_wave(prog, sv, n, compensated):
                 idx |= ((np.arange(1 << nr)[None, :] >> r) & 1) << gb[6 + r]
             return idx
         il, is_ = index(gl), index(gs)
-        out = sv.copy()
+        out = sv.copy() if not smask else np.full_like(sv, np.nan)
         for t in range(1 << (n - m)):
-            base = 0
+            base = sbase = 0
             for i, b in enumerate(free):
                 if (t >> i) & 1:
                     base |= 1 << b
+                    sbase |= 1 << (om[i] if smask else b)
             V = sv[base | il].astype(np.complex128)
-            q = run_micro(V, mu, frags[sf[s]:], fb, 0)
+            q = run_micro(V, mu, frags[sf:], fb, 0)
             assert q == nq
-            out[base | is_] = V
+            out[sbase | is_] = V
+        assert not np.isnan(out.real).any()
         sv = out
     return sv, nwave
 
@@ -436,12 +487,13 @@ def test_brickwork_wave_stages_replay(n, depth, tile, fuse, rng):
         prog.add_matrix(u, [a, b])
     prog.compile("complex64", fuse_bits=fuse, tile_bits=tile, use_mfma=True, reg_groups=2)
     sv0 = _rand_state(n, rng)
-    got, nw = replay_wave(prog, sv0, n, compensated=True)
-    assert nw == len(prog.plan()[0])  # every stage, restore passes included
     want = sv0.copy()
     for u, a, b in gates:
         want = O.apply_matrix(want, u, [a, b])
-    assert np.abs(got - want).max() < 1e-10
+    for fallback in (False, True):
+        got, nw = replay_wave(prog, sv0, n, compensated=True, fallback=fallback)
+        assert nw == len(wave_stages(prog, True, fallback))  # every stage, restore passes included
+        assert np.abs(got - want).max() < 1e-10
 
 
 @pytest.mark.parametrize("fuse,controlled", [(2, False), (3, True)])
@@ -451,7 +503,34 @@ def test_mixed_blocks_wave_stages_replay(fuse, controlled, rng):
     prog = desv.Program(n).add_ops(ops)
     prog.compile("complex64", fuse_bits=fuse, tile_bits=11, use_mfma=True, reg_groups=2)
     sv0 = _rand_state(n, rng)
-    got, nw = replay_wave(prog, sv0, n, compensated=True)
-    if not controlled:
-        assert nw >= 1
-    assert np.abs(got - O.run_ops_fast(sv0, ops)).max() < 1e-10
+    want = O.run_ops_fast(sv0, ops)
+    for fallback in (False, True):
+        got, nw = replay_wave(prog, sv0, n, compensated=True, fallback=fallback)
+        if not controlled:
+            assert nw >= 1
+        assert np.abs(got - want).max() < 1e-10
+
+
+@pytest.mark.parametrize("n,count,tile,fuse", [(14, 120, 12, 3), (16, 120, 12, 2)])
+def test_relabelled_tail_replay(n, count, tile, fuse):
+    """Programs whose in-place form needs two or more restore passes end with
+    one out-of-place stage that writes every qubit to its home bit (the
+    qubits outside its tile move with the tile addresses); both that tail and
+    the kept in-place form replay to the oracle's state."""
+    rng = np.random.default_rng(n * 1000 + count)
+    ops = _mixed_ops(n, count, rng, False)
+    prog = desv.Program(n).add_ops(ops)
+    prog.compile("complex64", fuse_bits=fuse, tile_bits=tile, use_mfma=True, reg_groups=2)
+    t = prog.tail(True)
+    assert t["fb_from"] >= 0 and len(t["fb_tile_masks"]) >= 2
+    assert (t["store_masks"] != 0).sum() == 1 and t["store_masks"][-1] != 0
+    assert len(wave_stages(prog, True, False)) < len(wave_stages(prog, True, True))
+    sv0 = _rand_state(n, rng)
+    want = O.run_ops_fast(sv0, ops)
+    for fallback in (False, True):
+        got, _ = replay_wave(prog, sv0, n, compensated=True, fallback=fallback)
+        assert np.abs(got - want).max() < 1e-10
+    # without wave tiles (or with register groups) no stage relabels
+    for rg in (0, 1):
+        prog.compile("complex64", fuse_bits=fuse, tile_bits=tile, use_mfma=True, reg_groups=rg)
+        assert prog.tail()["fb_from"] == -1
