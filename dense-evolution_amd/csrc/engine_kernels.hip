@@ -1517,7 +1517,8 @@ k_wstage(StageTask tk) {
     const uint32_t zero = 0;
     lds_t *lds = reinterpret_cast<lds_t *>((uintptr_t)zero);
     const int lane = threadIdx.x & 63;
-    const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    // the wave's tile number, wave-uniform (kept in scalar registers)
+    const int64_t t = (int64_t)uni64((uint64_t)((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)));
     const bool live = t < uni64((uint64_t)tk.st.n_tiles);
     const DESV_CONST int32_t *wp = cptr(tk.rprog);
     const int nmu = wp[0], nq = wp[1];
@@ -1527,7 +1528,7 @@ k_wstage(StageTask tk) {
     const uint64_t all = (n >= 64) ? ~0ull : ((1ull << n) - 1);
     uint64_t base = 0;
     {
-        uint64_t tt = (uint64_t)uni64((uint64_t)t), fm = ~tile_mask & all;
+        uint64_t tt = (uint64_t)t, fm = ~tile_mask & all;
         while (fm) {
             const uint64_t lb = fm & (~fm + 1);
             if (tt & 1) base |= lb;
@@ -1591,7 +1592,7 @@ k_wstage(StageTask tk) {
         DESV_GLOBAL float4 *dst = sv;
         if (rl[0]) {
             dst = gptr_rw(reinterpret_cast<float4 *>(tk.sv1));
-            uint64_t tt = (uint64_t)uni64((uint64_t)t);
+            uint64_t tt = (uint64_t)t;
             base = 0;
             for (int i = 0; i < n - M; ++i, tt >>= 1)
                 if (tt & 1) base |= 1ull << rl[1 + i];
