@@ -927,6 +927,9 @@ int desv_program_compile(desv_program p, const desv_compile_opts *opts) {
     // wave-tile programs end with an out-of-place relabelling stage instead of
     // restore passes when that stage can run as a wave tile
     o.relabel = o.reg_groups == 2;
+    // wave-tile programs (deep circuits) also take the cheapest of 64
+    // randomised planner passes (about 1.5 ms each for 348 gates on 30 qubits)
+    o.search = o.reg_groups == 2 ? 64 : 0;
     std::string err;
     if (!build_plan(p->gates, p->n, o, p->plan, err)) return fail(DESV_EINVAL, err);
     if (p->plan.fb_from >= 0) {

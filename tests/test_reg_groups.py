@@ -534,3 +534,24 @@ def test_relabelled_tail_replay(n, count, tile, fuse):
     for rg in (0, 1):
         prog.compile("complex64", fuse_bits=fuse, tile_bits=tile, use_mfma=True, reg_groups=rg)
         assert prog.tail()["fb_from"] == -1
+
+
+def test_wave_programs_take_the_cheapest_planner_pass():
+    """Wave-tile programs are compiled by the cheapest of the planner's
+    deterministic pass and 64 randomised ones (fixed seeds): the same program
+    always gets the same plan, and the 30-qubit depth-24 brickwork needs at
+    most the deterministic pass's 13 sweeps (12 compute stages and one
+    relabelling pass)."""
+    gates = workloads.brickwork_su4(30, 24, seed=0)
+    prog = desv.Program(30)
+    for u, a, b in gates:
+        prog.add_matrix(u, [a, b])
+    dumps = []
+    for _ in range(2):
+        prog.compile("complex64", fuse_bits=3, tile_bits=12, use_mfma=True, reg_groups=2)
+        tm, so, ops, _ = prog.plan()
+        dumps.append((tm.copy(), so.copy(), ops.copy()))
+    assert all(np.array_equal(a, b) for a, b in zip(*dumps))
+    st = prog.stats()
+    assert st["passes"] <= 13 and st["blocks"] <= 184
+    assert st["passes_in_place"] >= st["passes"]
