@@ -856,6 +856,7 @@ int desv_default_opts(int dtype, desv_compile_opts *opts) {
     opts->dtype = dtype;
     opts->compensated = o.compensated;
     opts->reg_groups = o.reg_groups;
+    opts->plan_search = -1;
     return DESV_OK;
 }
 
@@ -927,9 +928,13 @@ int desv_program_compile(desv_program p, const desv_compile_opts *opts) {
     // wave-tile programs end with an out-of-place relabelling stage instead of
     // restore passes when that stage can run as a wave tile
     o.relabel = o.reg_groups == 2;
-    // wave-tile programs (deep circuits) also take the cheapest of 64
-    // randomised planner passes (about 1.5 ms each for 348 gates on 30 qubits)
-    o.search = o.reg_groups == 2 ? 64 : 0;
+    // wave-tile programs (deep circuits) also take the cheapest of up to 64
+    // randomised planner passes (about 1.5 ms each for 348 gates on 30
+    // qubits; the default budget keeps a long program's compile near 4 of
+    // its greedy passes' time beyond 512 gates x 64 passes)
+    const int64_t ng = (int64_t)p->gates.size();
+    const int auto_search = o.reg_groups == 2 ? (int)std::max<int64_t>(4, std::min<int64_t>(64, 32768 / std::max<int64_t>(ng, 1))) : 0;
+    o.search = !opts || opts->plan_search < 0 ? auto_search : opts->plan_search;
     std::string err;
     if (!build_plan(p->gates, p->n, o, p->plan, err)) return fail(DESV_EINVAL, err);
     if (p->plan.fb_from >= 0) {

@@ -52,7 +52,7 @@ class CompileOpts(C.Structure):
     """``desv_compile_opts``."""
     _fields_ = [("fuse_bits", _i), ("tile_bits", _i), ("low_bits", _i),
                 ("per_gate", _i), ("use_mfma", _i), ("dtype", _i), ("compensated", _i),
-                ("reg_groups", _i)]
+                ("reg_groups", _i), ("plan_search", _i)]
 
 
 _SIGS = {

@@ -54,7 +54,8 @@ class Program:
 
     def compile(self, dtype="complex64", fuse_bits: Optional[int] = None, tile_bits: Optional[int] = None,
                 low_bits: Optional[int] = None, per_gate: bool = False, use_mfma: bool = False,
-                compensated: bool = False, reg_groups: bool = False) -> "Program":
+                compensated: bool = False, reg_groups: bool = False,
+                plan_search: Optional[int] = None) -> "Program":
         """Compile for a state of `dtype`.  ``compensated`` (complex64):
         matrices uploaded as FP32 hi + lo pairs and applied lo part first, for
         long programs whose repeated matrices would otherwise accumulate their
@@ -62,7 +63,9 @@ class Program:
         ``reg_groups`` (complex64 with ``use_mfma``): 1 = eligible stages keep
         the tile in registers between LDS round trips, 2 = eligible stages of
         2^11- or 2^12-amplitude tiles hold one tile per wave in registers from
-        HBM load to HBM store (``desv_compile_opts``)."""
+        HBM load to HBM store (``desv_compile_opts``).  ``plan_search``:
+        randomised planner passes tried beside the greedy one (None: the
+        library default, 0: greedy only)."""
         code = _capi.C64 if np.dtype(dtype) == np.complex64 else _capi.C128
         o = _capi.CompileOpts()
         check(lib.desv_default_opts(code, C.byref(o)))
@@ -76,6 +79,8 @@ class Program:
         o.use_mfma = int(bool(use_mfma))
         o.compensated = int(bool(compensated))
         o.reg_groups = int(reg_groups)
+        if plan_search is not None:
+            o.plan_search = int(plan_search)
         o.dtype = code
         check(lib.desv_program_compile(self.ptr, C.byref(o)))
         self._reg_mode = int(reg_groups) if use_mfma and code == _capi.C64 else 0

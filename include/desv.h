@@ -171,6 +171,12 @@ typedef struct {
                        load to its HBM store (no LDS, no barriers).  Same
                        results as the LDS path to FP32 rounding.  No
                        counterpart in the reference.                          */
+    int plan_search;/* randomised planner passes tried beside the greedy one
+                       (the cheapest plan by estimated time is kept; fixed
+                       seeds, so a program always gets the same plan): -1 =
+                       default (reg_groups 2: 64 passes for programs of up
+                       to 512 gates, fewer for longer ones, at least 4),
+                       0 = greedy only.  No counterpart in the reference.   */
 } desv_compile_opts;
 
 /* Fill `opts` with the defaults for a dtype. */
