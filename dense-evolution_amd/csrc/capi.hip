@@ -746,9 +746,52 @@ int desv_sample(desv_handle h, const double *uniforms, int64_t n_shots, int64_t 
     if (n_shots < 1) return fail(DESV_EINVAL, "n_shots must be >= 1, got " + std::to_string(n_shots));
     if (!uniforms || !out) return fail(DESV_EINVAL, "null sample buffers");
     HIP_TRY(hipSetDevice(h->device));
-    if (int rc = ensure_aux(h, 1)) return rc;  // the CDF (2^n doubles fit in a state buffer)
     const int64_t nb = cdf_blocks(h->dim);
-    if (int rc = ensure_part(h, 2 * ((size_t)nb + 1))) return rc;
+    if (int rc = ensure_part(h, 2 * ((size_t)nb + 1) + (size_t)sample_scan_words(nb))) return rc;
+    {
+        // Two-level sampling (chunk totals, then the chunks that hold shots):
+        // the fixed-point weights use the fixed scale 2^62, which keeps the
+        // 64-bit sums exact for a total probability below 4 (and within a
+        // factor 16 of the CDF path's quantisation above 1/16).
+        double *blk = h->d_part;
+        uint64_t *qblk = reinterpret_cast<uint64_t *>(blk + nb + 1);
+        uint64_t *sums = qblk + nb + 1;
+        hipError_t e = h->dtype == DESV_C64 ? launch_sample_totals<float>(h->sv, h->dim, blk, qblk, sums, h->stream)
+                                            : launch_sample_totals<double>(h->sv, h->dim, blk, qblk, sums, h->stream);
+        if (e != hipSuccess) return fail(DESV_EDEVICE, hipGetErrorString(e));
+        double total = 0;
+        HIP_TRY(hipMemcpyAsync(&total, blk + nb, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        if (!(total > 0) || !std::isfinite(total))
+            return fail(DESV_EINVAL, "statevector has zero or non-finite total probability");
+        if (total >= 0.0625 && total < 3.5) {
+            const size_t S = (size_t)n_shots, B = (size_t)nb;
+            auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+            const size_t o_u = 0, o_out = o_u + al(S * 8), o_tgt = o_out + al(S * 8), o_chunk = o_tgt + al(S * 8),
+                         o_bucket = o_chunk + al(S * 4), o_counts = o_bucket + al(S * 8),
+                         o_offsets = o_counts + al(B * 4), bytes = o_offsets + al((B + 1) * 8);
+            unsigned char *w = nullptr;
+            e = hipMalloc(&w, bytes);
+            if (e != hipSuccess) return fail(DESV_ENOMEM, std::string("sample scratch: ") + hipGetErrorString(e));
+            SampleScratch sc{reinterpret_cast<uint64_t *>(w + o_tgt), reinterpret_cast<uint32_t *>(w + o_chunk),
+                             reinterpret_cast<int64_t *>(w + o_bucket), reinterpret_cast<uint32_t *>(w + o_counts),
+                             reinterpret_cast<uint64_t *>(w + o_offsets), sums};
+            double *du = reinterpret_cast<double *>(w + o_u);
+            int64_t *dout = reinterpret_cast<int64_t *>(w + o_out);
+            e = hipMemcpyAsync(du, uniforms, S * sizeof(double), hipMemcpyHostToDevice, h->stream);
+            if (e == hipSuccess)
+                e = h->dtype == DESV_C64
+                        ? launch_sample_chunks<float>(h->sv, h->dim, qblk, du, n_shots, sc, dout, h->stream)
+                        : launch_sample_chunks<double>(h->sv, h->dim, qblk, du, n_shots, sc, dout, h->stream);
+            if (e == hipSuccess) e = hipMemcpyAsync(out, dout, S * sizeof(int64_t), hipMemcpyDeviceToHost, h->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+            (void)hipFree(w);
+            if (e != hipSuccess) return fail(DESV_EDEVICE, hipGetErrorString(e));
+            return DESV_OK;
+        }
+    }
+    // far from normalised: the CDF path, whose fixed-point scale follows the total
+    if (int rc = ensure_aux(h, 1)) return rc;  // the CDF (2^n doubles fit in a state buffer)
     double *cdf = reinterpret_cast<double *>(h->aux[1]);
     hipError_t e = h->dtype == DESV_C64 ? launch_cdf<float>(h->sv, h->dim, cdf, h->d_part, h->stream)
                                         : launch_cdf<double>(h->sv, h->dim, cdf, h->d_part, h->stream);

@@ -2985,6 +2985,247 @@ hipError_t launch_sample_search(const double *cdf, int64_t dim, const double *u,
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Two-level shot sampling (no CDF array): one pass gives every chunk's FP64
+// total and its fixed-point total with the fixed scale 2^62 (|a|^2 <= 1 per
+// amplitude, so a state of total probability below 4 cannot overflow the
+// 64-bit sums); a scan of the chunk totals; each shot's target T = floor(u
+// P_total) picks the first chunk whose inclusive prefix exceeds T; shots are
+// bucketed by chunk (counting sort), and one workgroup per chunk that holds
+// shots rebuilds the chunk's integer prefix in LDS and binary-searches every
+// shot's first index with prefix > T.  The state is read twice at most, the
+// weights and prefixes are the integer ones of the CDF path, so an amplitude
+// of probability zero has zero width.
+// ---------------------------------------------------------------------------
+constexpr double kSampleScale = 4611686018427387904.0;  // 2^62
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_sample_totals(const typename VT<T>::c *sv, int64_t dim, double *blk,
+                                                            uint64_t *qblk) {
+    using C = typename VT<T>::c;
+    constexpr int APU = 16 / (int)sizeof(C);
+    constexpr int per = kScanChunk / APU / kThreads;
+    __shared__ double red[kThreads / 64];
+    __shared__ uint64_t qred[kThreads / 64];
+    const int64_t c0 = (int64_t)blockIdx.x * kScanChunk;
+    double acc = 0;
+    uint64_t qacc = 0;
+    auto add = [&](C a) {
+        const double2 d = to_d(a);
+        const double p = d.x * d.x + d.y * d.y;
+        acc += p;
+        qacc += (uint64_t)(p * kSampleScale + 0.5);
+    };
+    if (c0 + kScanChunk <= dim) {
+        const float4 *p = reinterpret_cast<const float4 *>(sv + c0);
+        float4 v[per];
+#pragma unroll
+        for (int k = 0; k < per; ++k) v[k] = p[k * kThreads + threadIdx.x];
+#pragma unroll
+        for (int k = 0; k < per; ++k) {
+            C a[APU];
+            __builtin_memcpy(a, &v[k], sizeof(a));
+#pragma unroll
+            for (int j = 0; j < APU; ++j) add(a[j]);
+        }
+    } else {
+        for (int64_t e = c0 + threadIdx.x; e < dim; e += kThreads) add(sv[e]);
+    }
+    const double t = block_sum(acc, red);
+    const uint64_t q = block_sum_u64(qacc, qred);
+    if (threadIdx.x == 0) {
+        blk[blockIdx.x] = t;
+        qblk[blockIdx.x] = q;
+    }
+}
+
+// Per shot: target T = floor(u * P_total) (P_total = qblk[nb]; u clamped to
+// [0, 1)), its chunk (the first c with qblk[c + 1] > T, qblk holding the
+// exclusive prefixes and the total at nb) and the chunk's shot count.
+__global__ void k_sample_locate(const double *u, int64_t shots, const uint64_t *qblk, int64_t nb, uint64_t *tgt,
+                                uint32_t *chunk, uint32_t *counts) {
+    const uint64_t total = qblk[nb];
+    for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < shots; s += (int64_t)gridDim.x * blockDim.x) {
+        double x = u[s];
+        x = x > 0.0 ? (x < 1.0 ? x : 0.99999999999999989) : 0.0;  // NaN -> 0
+        uint64_t t = (uint64_t)(x * (double)total);
+        if (t >= total) t = total - 1;
+        int64_t lo = 0, hi = nb - 1;  // first c with incl(c) > t; incl(nb - 1) = total > t
+        while (lo < hi) {
+            const int64_t mid = lo + ((hi - lo) >> 1);
+            if (qblk[mid + 1] > t) hi = mid;
+            else lo = mid + 1;
+        }
+        tgt[s] = t;
+        chunk[s] = (uint32_t)lo;
+        atomicAdd(&counts[lo], 1u);
+    }
+}
+
+// bucket[offsets[c] + k] = the k-th shot (in arrival order) of chunk c;
+// cursor[] starts at zero.
+__global__ void k_sample_scatter(const uint32_t *chunk, int64_t shots, const uint64_t *offsets, uint32_t *cursor,
+                                 int64_t *bucket) {
+    for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < shots; s += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t c = chunk[s];
+        bucket[offsets[c] + atomicAdd(&cursor[c], 1u)] = s;
+    }
+}
+
+// One workgroup per chunk: the chunk's inclusive integer prefix in LDS
+// (lane t sums its 16 consecutive weights, a workgroup scan, then the
+// per-element prefixes), then for each of the chunk's shots the first index
+// whose prefix (plus the chunk's offset) exceeds the shot's target.
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_sample_chunks(const typename VT<T>::c *sv, int64_t dim,
+                                                            const uint64_t *qblk, const uint64_t *offsets,
+                                                            const int64_t *bucket, const uint64_t *tgt,
+                                                            int64_t *out) {
+    constexpr int per = kScanChunk / kThreads;
+    __shared__ uint64_t ts[kThreads];
+    __shared__ uint64_t buf[kScanChunk + kScanChunk / per];
+    const uint64_t b0 = offsets[blockIdx.x], b1 = offsets[blockIdx.x + 1];
+    if (b0 == b1) return;  // no shot in this chunk (uniform over the workgroup)
+    auto at = [](int e) { return e + e / per; };
+    const int64_t c0 = (int64_t)blockIdx.x * kScanChunk;
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < per; ++k) {
+        const int e = k * kThreads + t;
+        buf[at(e)] = c0 + e < dim ? cdf_weight(sv[c0 + e], kSampleScale) : 0ull;
+    }
+    __syncthreads();
+    uint64_t v[per], run = 0;
+#pragma unroll
+    for (int j = 0; j < per; ++j) {
+        run += buf[at(t * per + j)];
+        v[j] = run;
+    }
+    const uint64_t incl = block_inclusive_scan_u64(run, ts);
+    const uint64_t base = incl - run + qblk[blockIdx.x];
+    __syncthreads();  // every lane has read its elements
+#pragma unroll
+    for (int j = 0; j < per; ++j) buf[at(t * per + j)] = base + v[j];
+    __syncthreads();
+    const int64_t n_here = dim - c0 < kScanChunk ? dim - c0 : kScanChunk;
+    for (uint64_t b = b0 + t; b < b1; b += kThreads) {
+        const int64_t s = bucket[b];
+        const uint64_t x = tgt[s];
+        int lo = 0, hi = (int)n_here - 1;  // first i with prefix(i) > x (the chunk was chosen so that one exists)
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (buf[at(mid)] > x) hi = mid;
+            else lo = mid + 1;
+        }
+        out[s] = c0 + lo;
+    }
+}
+
+// Exclusive prefix sums of x[0..n) (u32 or u64) into out[0..n), out[n] =
+// the total, over 4096-entry tiles in parallel: tile sums, one workgroup's
+// scan of the tile sums, then every tile's scan with its offset (x may be
+// out: each workgroup reads its whole tile before writing it).
+constexpr int kScanTile = 4096;
+template <typename X>
+__global__ void __launch_bounds__(kThreads) k_tile_sums(const X *x, int64_t n, uint64_t *sums) {
+    constexpr int per = kScanTile / kThreads;
+    __shared__ uint64_t red[kThreads / 64];
+    const int64_t b = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x;
+    uint64_t acc = 0;
+#pragma unroll
+    for (int k = 0; k < per; ++k)
+        if (b + k * kThreads < n) acc += x[b + k * kThreads];
+    const uint64_t t = block_sum_u64(acc, red);
+    if (threadIdx.x == 0) sums[blockIdx.x] = t;
+}
+template <typename X>
+__global__ void __launch_bounds__(kThreads) k_tile_scan(const X *x, int64_t n, const uint64_t *offs, int64_t ntiles,
+                                                        uint64_t *out) {
+    constexpr int per = kScanTile / kThreads;
+    __shared__ uint64_t ts[kThreads];
+    __shared__ uint64_t buf[kScanTile + kScanTile / per];
+    auto at = [](int e) { return e + e / per; };
+    const int64_t c0 = (int64_t)blockIdx.x * kScanTile;
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < per; ++k) {
+        const int e = k * kThreads + t;
+        buf[at(e)] = c0 + e < n ? (uint64_t)x[c0 + e] : 0ull;
+    }
+    __syncthreads();
+    uint64_t v[per], run = 0;
+#pragma unroll
+    for (int j = 0; j < per; ++j) {
+        v[j] = run;  // exclusive within the lane's 16
+        run += buf[at(t * per + j)];
+    }
+    const uint64_t incl = block_inclusive_scan_u64(run, ts);
+    const uint64_t base = incl - run + offs[blockIdx.x];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < per; ++j) buf[at(t * per + j)] = base + v[j];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < per; ++k) {
+        const int e = k * kThreads + t;
+        if (c0 + e < n) out[c0 + e] = buf[at(e)];
+    }
+    if (blockIdx.x == ntiles - 1 && t == kThreads - 1) out[n] = offs[ntiles];
+}
+// sum of x[0..n) into *out (one workgroup, coalesced)
+__global__ void __launch_bounds__(kThreads) k_sum_f64(const double *x, int64_t n, double *out) {
+    __shared__ double red[kThreads / 64];
+    double acc = 0;
+    for (int64_t i = threadIdx.x; i < n; i += kThreads) acc += x[i];
+    const double t = block_sum(acc, red);
+    if (threadIdx.x == 0) *out = t;
+}
+// tiles: (n + kScanTile - 1) / kScanTile <= kThreads * 1024; sums holds
+// tiles + 1 entries
+template <typename X>
+hipError_t launch_exclusive_scan(const X *x, int64_t n, uint64_t *sums, uint64_t *out, hipStream_t s) {
+    const int64_t tiles = (n + kScanTile - 1) / kScanTile;
+    if (tiles < 1 || tiles > 0x7fffffffll) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_tile_sums<X>), dim3((unsigned)tiles), dim3(kThreads), 0, s, x, n, sums);
+    hipLaunchKernelGGL(k_scan_qblocks, dim3(1), dim3(kThreads), 0, s, sums, tiles);
+    hipLaunchKernelGGL((k_tile_scan<X>), dim3((unsigned)tiles), dim3(kThreads), 0, s, x, n, (const uint64_t *)sums,
+                       tiles, out);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_sample_totals(const void *sv, int64_t dim, double *blk, uint64_t *qblk, uint64_t *sums,
+                                hipStream_t s) {
+    const int64_t nb = (dim + kScanChunk - 1) / kScanChunk;
+    if (nb > 0x7fffffffll) return hipErrorInvalidValue;
+    using C = typename VT<T>::c;
+    hipLaunchKernelGGL((k_sample_totals<T>), dim3((unsigned)nb), dim3(kThreads), 0, s, (const C *)sv, dim, blk, qblk);
+    hipLaunchKernelGGL(k_sum_f64, dim3(1), dim3(kThreads), 0, s, (const double *)blk, nb, blk + nb);
+    return launch_exclusive_scan<uint64_t>(qblk, nb, sums, qblk, s);
+}
+
+template <typename T>
+hipError_t launch_sample_chunks(const void *sv, int64_t dim, const uint64_t *qblk, const double *u, int64_t shots,
+                                SampleScratch w, int64_t *out, hipStream_t s) {
+    const int64_t nb = (dim + kScanChunk - 1) / kScanChunk;
+    if (nb > 0x7fffffffll || shots <= 0) return shots <= 0 ? hipSuccess : hipErrorInvalidValue;
+    using C = typename VT<T>::c;
+    const int64_t grid = std::min<int64_t>((shots + kThreads - 1) / kThreads, 1 << 14);
+    hipError_t e = hipMemsetAsync(w.counts, 0, (size_t)nb * sizeof(uint32_t), s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_sample_locate, dim3((unsigned)grid), dim3(kThreads), 0, s, u, shots, qblk, nb, w.tgt,
+                       w.chunk, w.counts);
+    e = launch_exclusive_scan<uint32_t>(w.counts, nb, w.sums, w.offsets, s);
+    if (e != hipSuccess) return e;
+    e = hipMemsetAsync(w.counts, 0, (size_t)nb * sizeof(uint32_t), s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_sample_scatter, dim3((unsigned)grid), dim3(kThreads), 0, s, (const uint32_t *)w.chunk, shots,
+                       (const uint64_t *)w.offsets, w.counts, w.bucket);
+    hipLaunchKernelGGL((k_sample_chunks<T>), dim3((unsigned)nb), dim3(kThreads), 0, s, (const C *)sv, dim, qblk,
+                       (const uint64_t *)w.offsets, (const int64_t *)w.bucket, (const uint64_t *)w.tgt, out);
+    return hipGetLastError();
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kThreads) k_qubit_probs(const typename VT<T>::c *sv, int64_t dim, int bit,
                                                           double *partials) {
@@ -3521,6 +3762,10 @@ hipError_t launch_fill_random(void *sv, int64_t dim, uint64_t seed, hipStream_t 
     template hipError_t launch_inner<T>(const void *, const void *, int64_t, double *, int, hipStream_t);     \
     template hipError_t launch_probs<T>(const void *, int64_t, double *, hipStream_t);                        \
     template hipError_t launch_cdf<T>(const void *, int64_t, double *, double *, hipStream_t);                \
+    template hipError_t launch_sample_totals<T>(const void *, int64_t, double *, uint64_t *, uint64_t *,       \
+                                                hipStream_t);                                                  \
+    template hipError_t launch_sample_chunks<T>(const void *, int64_t, const uint64_t *, const double *,      \
+                                                int64_t, SampleScratch, int64_t *, hipStream_t);               \
     template hipError_t launch_qubit_probs<T>(const void *, int64_t, int, double *, int, hipStream_t);        \
     template hipError_t launch_collapse<T>(void *, int64_t, int, int, double, hipStream_t);                   \
     template hipError_t launch_pauli_expect<T>(const void *, int, const PauliItem *, int, const uint64_t *,   \

@@ -180,6 +180,28 @@ hipError_t launch_apply1q_probs(void *sv, int64_t dim, int bit, const double *m,
                                 hipStream_t s);
 hipError_t launch_sample_search(const double *cdf, int64_t dim, const double *u, int64_t shots, int64_t *out,
                                 hipStream_t s);
+// Two-level sampling without a CDF array.  launch_sample_totals: blk (nb + 1
+// doubles) <- FP64 chunk totals, blk[nb] = the total; qblk (nb + 1) <- the
+// exclusive prefixes of the fixed-point chunk totals (weights round(|a|^2
+// 2^62), valid while the total is below 4), qblk[nb] = their total; sums:
+// sample_scan_words(nb) words of scan workspace.  launch_sample_chunks:
+// out[s] = the first index whose inclusive fixed-point prefix exceeds
+// floor(u[s] qblk[nb]).
+struct SampleScratch {
+    uint64_t *tgt;      // shots
+    uint32_t *chunk;    // shots
+    int64_t *bucket;    // shots
+    uint32_t *counts;   // nb
+    uint64_t *offsets;  // nb + 1
+    uint64_t *sums;     // sample_scan_words(nb)
+};
+inline int64_t sample_scan_words(int64_t nb) { return (nb + 4095) / 4096 + 1; }
+template <typename T>
+hipError_t launch_sample_totals(const void *sv, int64_t dim, double *blk, uint64_t *qblk, uint64_t *sums,
+                                hipStream_t s);
+template <typename T>
+hipError_t launch_sample_chunks(const void *sv, int64_t dim, const uint64_t *qblk, const double *u, int64_t shots,
+                                SampleScratch w, int64_t *out, hipStream_t s);
 template <typename T>
 hipError_t launch_qubit_probs(const void *sv, int64_t dim, int bit, double *partials, int nblk, hipStream_t s);
 template <typename T>

@@ -511,6 +511,24 @@ def test_device_sampling_matches_rng_choice(n, dtype, rng):
     assert sum(got.values()) == shots and moved <= 4
 
 
+@pytest.mark.parametrize("scale", [0.05, 1.0, 4.0])
+def test_device_sampling_of_unnormalised_states(scale, rng):
+    """Shots are drawn from |a|^2 / total whatever the total: states within
+    [1/16, 3.5) of unit norm take the two-level path (chunk totals, then the
+    chunks that hold shots), others the full-CDF path whose fixed-point scale
+    follows the total; both against rng.choice with the same uniforms."""
+    n = 13
+    sv = rand_state(n, rng) * np.sqrt(scale)
+    shots = 20000
+    got = desv.sample_counts(sv.astype(np.complex64), shots, rng=np.random.default_rng(11))
+    p = np.abs(sv.astype(np.complex64).astype(np.complex128)) ** 2
+    p /= p.sum()
+    vals, freqs = np.unique(np.random.default_rng(11).choice(1 << n, size=shots, p=p), return_counts=True)
+    want = {format(int(v), f'0{n}b'): int(f) for v, f in zip(vals, freqs)}
+    moved = sum(abs(got.get(k, 0) - want.get(k, 0)) for k in set(got) | set(want))
+    assert sum(got.values()) == shots and moved <= 4
+
+
 def test_device_sampling_on_device_state_and_errors():
     sim = desv.DenseSVSimulator(3)
     sim.run_circuit([('h', 0), ('cx', 0, 1), ('cx', 1, 2)])
