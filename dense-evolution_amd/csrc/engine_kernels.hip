@@ -3696,6 +3696,50 @@ __global__ void __launch_bounds__(kThreads) k_combine(BasisPtrs in, BasisPtrs ou
     }
 }
 
+// The same for at most 2 outputs, streaming over the inputs: each workgroup
+// takes E x 256 consecutive elements (E per lane, 256 apart) of every input
+// in turn, so each input is read in E KiB (complex64) or 2E KiB (complex128)
+// contiguous runs per workgroup instead of 256 elements, and the lane keeps
+// only its outputs' accumulators.  An output may alias an input: the lane
+// loads all of its elements' inputs before storing their outputs.
+template <typename T, int E>
+__global__ void __launch_bounds__(kThreads) k_combine_stream(BasisPtrs in, BasisPtrs out, int m, int p,
+                                                             const double2 *__restrict__ coef, int64_t dim) {
+    using C = typename VT<T>::c;
+    const int64_t tile = (int64_t)E * kThreads;
+    for (int64_t t0 = (int64_t)blockIdx.x * tile; t0 < dim; t0 += (int64_t)gridDim.x * tile) {
+        double2 acc[2][E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) acc[0][e] = acc[1][e] = make_double2(0.0, 0.0);
+        for (int j = 0; j < m; ++j) {
+            const C *src = static_cast<const C *>(in.p[j]);
+            const double2 c0 = coef[j * p], c1 = p > 1 ? coef[j * p + 1] : make_double2(0.0, 0.0);
+            double2 x[E];
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const int64_t i = t0 + e * kThreads + threadIdx.x;
+                x[e] = i < dim ? to_d(src[i]) : make_double2(0.0, 0.0);
+            }
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                acc[0][e] = cfma(c0, x[e], acc[0][e]);
+                acc[1][e] = cfma(c1, x[e], acc[1][e]);
+            }
+        }
+        for (int l = 0; l < p; ++l)
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const int64_t i = t0 + e * kThreads + threadIdx.x;
+                if (i < dim) {
+                    C r;
+                    r.x = (T)acc[l][e].x;
+                    r.y = (T)acc[l][e].y;
+                    static_cast<C *>(const_cast<void *>(out.p[l]))[i] = r;
+                }
+            }
+    }
+}
+
 // Amplitude i = (u0 - 1/2) + i (u1 - 1/2), u from SplitMix64 of (seed, i):
 // a reproducible start vector with no host transfer (not normalised).
 __device__ inline uint64_t splitmix64(uint64_t z) {
@@ -3728,11 +3772,19 @@ hipError_t launch_inner_many(const void *w, const BasisPtrs &vs, int m, int64_t 
     return hipGetLastError();
 }
 
+#ifndef DESV_COMBINE_E
+#define DESV_COMBINE_E 2
+#endif
 template <typename T>
 hipError_t launch_combine(const BasisPtrs &in, const BasisPtrs &out, int m, int p, const double2 *coef, int64_t dim,
                           hipStream_t s) {
     const dim3 g(ew_grid(dim)), b(kThreads);
-    if (m <= 8)
+    if (p <= 2 && DESV_COMBINE_E > 0) {
+        constexpr int E = DESV_COMBINE_E > 0 ? DESV_COMBINE_E : 1;
+        const int64_t tiles = (dim + (int64_t)E * kThreads - 1) / ((int64_t)E * kThreads);
+        const dim3 gs((unsigned)std::min<int64_t>(std::max<int64_t>(tiles, 1), 65536));
+        hipLaunchKernelGGL((k_combine_stream<T, E>), gs, b, 0, s, in, out, m, p, coef, dim);
+    } else if (m <= 8)
         hipLaunchKernelGGL((k_combine<T, 8>), g, b, 0, s, in, out, m, p, coef, dim);
     else if (m <= 16)
         hipLaunchKernelGGL((k_combine<T, 16>), g, b, 0, s, in, out, m, p, coef, dim);
