@@ -3172,6 +3172,18 @@ __global__ void __launch_bounds__(kThreads) k_tile_scan(const X *x, int64_t n, c
     }
     if (blockIdx.x == ntiles - 1 && t == kThreads - 1) out[n] = offs[ntiles];
 }
+// out[w] = sum of x over 4096-entry tile w (a fixed summation order)
+__global__ void __launch_bounds__(kThreads) k_tile_sums_f64(const double *x, int64_t n, double *out) {
+    constexpr int per = kScanTile / kThreads;
+    __shared__ double red[kThreads / 64];
+    const int64_t b = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x;
+    double acc = 0;
+#pragma unroll
+    for (int k = 0; k < per; ++k)
+        if (b + k * kThreads < n) acc += x[b + k * kThreads];
+    const double t = block_sum(acc, red);
+    if (threadIdx.x == 0) out[blockIdx.x] = t;
+}
 // sum of x[0..n) into *out (one workgroup, coalesced)
 __global__ void __launch_bounds__(kThreads) k_sum_f64(const double *x, int64_t n, double *out) {
     __shared__ double red[kThreads / 64];
@@ -3200,7 +3212,12 @@ hipError_t launch_sample_totals(const void *sv, int64_t dim, double *blk, uint64
     if (nb > 0x7fffffffll) return hipErrorInvalidValue;
     using C = typename VT<T>::c;
     hipLaunchKernelGGL((k_sample_totals<T>), dim3((unsigned)nb), dim3(kThreads), 0, s, (const C *)sv, dim, blk, qblk);
-    hipLaunchKernelGGL(k_sum_f64, dim3(1), dim3(kThreads), 0, s, (const double *)blk, nb, blk + nb);
+    // the FP64 total through per-tile sums (the scan workspace holds them
+    // until the fixed-point scan below needs it)
+    const int64_t tiles = (nb + kScanTile - 1) / kScanTile;
+    double *tsum = reinterpret_cast<double *>(sums);
+    hipLaunchKernelGGL(k_tile_sums_f64, dim3((unsigned)tiles), dim3(kThreads), 0, s, (const double *)blk, nb, tsum);
+    hipLaunchKernelGGL(k_sum_f64, dim3(1), dim3(kThreads), 0, s, (const double *)tsum, tiles, blk + nb);
     return launch_exclusive_scan<uint64_t>(qblk, nb, sums, qblk, s);
 }
 
