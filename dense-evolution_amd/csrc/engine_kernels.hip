@@ -1565,7 +1565,10 @@ k_wstage(StageTask tk) {
         }
     }
     {
-        // the stage's fragment blocks into LDS, 4 float4s a thread at a time
+        // the stage's fragment blocks into LDS, 4 float4s a thread at a time;
+        // the loads are unconditional (indices clamped into the table), so
+        // the wait before each store counts only the loads in flight instead
+        // of draining the tile loads issued above at every step
         const DESV_GLOBAL float4 *fr = gptr(reinterpret_cast<const float4 *>(tk.rfrag));
         const int cnt = nq * (CP ? 2 : 1) * 64;
         for (int i0 = 0; i0 < cnt; i0 += 4 * 256) {
@@ -1573,7 +1576,7 @@ k_wstage(StageTask tk) {
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
                 const int i = i0 + c * 256 + (int)threadIdx.x;
-                if (i < cnt) x[c] = gload(fr + i);
+                x[c] = gload(fr + (i < cnt ? i : cnt - 1));
             }
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
