@@ -1401,21 +1401,39 @@ __device__ constexpr int rg_treg(int g, int t) {
     return j;
 }
 __device__ inline float &rg_f(float2 &x, int c) { return c ? x.y : x.x; }
+// 16 A values (one fragment row) from LDS
+__device__ inline void rg_frag_row(const lds_t *f, float (&a)[16]) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const float4 x = lds_ld<float4>(f, 16 * c);
+        a[4 * c] = x.x, a[4 * c + 1] = x.y, a[4 * c + 2] = x.z, a[4 * c + 3] = x.w;
+    }
+}
+// Compensated blocks (CP) read their hi and lo rows from LDS (`f`, the lo
+// block 1 KiB after the hi one) for every group instead of holding both, so
+// the 64-amplitude tile, one accumulator and one row fit 3 waves per SIMD.
 template <int X, int Y, int Z, bool CP, int NJ>
-__device__ inline void rg_mfma3t(float2 (&v)[NJ], const float (&a)[16], const float (&al)[16]) {
+__device__ inline void rg_mfma3t(float2 (&v)[NJ], const float (&a0)[16], const lds_t *f) {
     constexpr int NR = NJ == 32 ? 5 : 6, NG = NJ / 8, IL = DESV_WT_IL;
 #pragma unroll
     for (int g0 = 0; g0 < NG; g0 += IL) {
         f32x16_t d[IL];
 #pragma unroll
         for (int i = 0; i < IL; ++i) d[i] = f32x16_t{};
+        float a[16];
         if constexpr (CP) {
+            float al[16];
+            rg_frag_row(f + 1024, al);
 #pragma unroll
             for (int k = 0; k < 16; ++k)
 #pragma unroll
                 for (int i = 0; i < IL; ++i)
                     d[i] = __builtin_amdgcn_mfma_f32_16x16x1f32(al[k], rg_f(v[rg_treg<X, Y, Z, NR>(g0 + i, k >> 1)], k & 1),
                                                                 d[i], 0, 0, 0);
+            rg_frag_row(f, a);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 16; ++k) a[k] = a0[k];
         }
 #pragma unroll
         for (int k = 0; k < 16; ++k)
@@ -1465,23 +1483,13 @@ __device__ inline void rg_run_wave(float2 (&v)[NJ], const DESV_CONST int32_t *mu
 #undef DESV_RX
             }
         } else {
-            float a[16], al[16];
+            float a[16];
             const lds_t *f = fl + (size_t)q * FB * 1024 + row;
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                const float4 x = lds_ld<float4>(f, 16 * c);
-                a[4 * c] = x.x, a[4 * c + 1] = x.y, a[4 * c + 2] = x.z, a[4 * c + 3] = x.w;
-                if constexpr (CP) {
-                    const float4 y = lds_ld<float4>(f + 1024, 16 * c);
-                    al[4 * c] = y.x, al[4 * c + 1] = y.y, al[4 * c + 2] = y.z, al[4 * c + 3] = y.w;
-                } else {
-                    al[4 * c] = al[4 * c + 1] = al[4 * c + 2] = al[4 * c + 3] = 0.f;
-                }
-            }
+            if constexpr (!CP) rg_frag_row(f, a);
             ++q;
             switch (code >> 4) {
 #define DESV_RT(X, Y, Z) \
-    case X | Y << 4 | Z << 8: rg_mfma3t<X, Y, Z, CP>(v, a, al); break;
+    case X | Y << 4 | Z << 8: rg_mfma3t<X, Y, Z, CP>(v, a, f); break;
                 DESV_RT(0, 1, 2) DESV_RT(0, 1, 3) DESV_RT(0, 1, 4) DESV_RT(0, 2, 3) DESV_RT(0, 2, 4)
                 DESV_RT(0, 3, 4) DESV_RT(1, 2, 3) DESV_RT(1, 2, 4) DESV_RT(1, 3, 4) DESV_RT(2, 3, 4)
                 default:
@@ -1510,7 +1518,7 @@ __device__ inline void rg_run_wave(float2 (&v)[NJ], const DESV_CONST int32_t *mu
 // MFMA form move bits between.  A relabelling stage (flag word after the
 // micro-ops) stores into tk.sv1 with its outer bits relabelled.
 template <int NR, bool CP>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NR <= 5 ? DESV_WT_W5 : CP ? 2 : DESV_WT_W6)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NR <= 5 ? DESV_WT_W5 : DESV_WT_W6)))
 k_wstage(StageTask tk) {
     constexpr int NJ = 1 << NR, M = 6 + NR;
     extern __shared__ __align__(16) unsigned char smem[];
