@@ -168,7 +168,7 @@ typedef struct {
                        MFMA); other stages are unchanged.  2: eligible stages
                        of 2^11- or 2^12-amplitude tiles (tile_bits 11 or 12)
                        hold one whole tile per wave in registers from its HBM
-                       load to its HBM store (no LDS, no barriers).  Same
+                       load to its HBM store (the tile never passes through LDS).  Same
                        results as the LDS path to FP32 rounding.  No
                        counterpart in the reference.                          */
     int plan_search;/* randomised planner passes tried beside the greedy one
