@@ -3342,14 +3342,24 @@ __global__ void __launch_bounds__(kThreads) k_apply1q_probs(typename VT<T>::c *s
 // wave is one contiguous run; the products per amplitude are those of the
 // generic kernel and the populations are accumulated in FP64 the same way.
 template <bool B0, bool NT>
+#ifndef DESV_A1Q_GPT
+#define DESV_A1Q_GPT 8
+#endif
 __global__ void __launch_bounds__(kThreads) k_apply1q_probs_c64(float4 *sv, int64_t units, int bit, float2 a00,
                                                                 float2 a01, float2 a10, float2 a11, int pbit,
-                                                                double *partials, int nblk) {
-    constexpr int GPT = 4;
+                                                                double *partials, int nblk, int xcd) {
+    constexpr int GPT = DESV_A1Q_GPT;
     __shared__ double red[kThreads / 64];
     const int64_t tb = 1ll << bit;
     const int lane = threadIdx.x & 63;
-    const int64_t wave0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    // workgroups are dealt to the XCDs round-robin: XCD x takes the x-th
+    // slice of the runs (one L2 per XCD sees one contiguous range)
+    int64_t bid = blockIdx.x;
+    if (xcd > 1) {
+        const int64_t per = (int64_t)gridDim.x / xcd;
+        if (bid < per * xcd) bid = (bid % xcd) * per + bid / xcd;
+    }
+    const int64_t wave0 = (bid * blockDim.x + threadIdx.x) >> 6;
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
     double s0 = 0, s1 = 0;
     auto pop = [&](int64_t e, float2 w) {
@@ -3415,6 +3425,9 @@ __global__ void __launch_bounds__(kThreads) k_apply1q_probs_c64(float4 *sv, int6
     }
 }
 
+#ifndef DESV_A1Q_XCD
+#define DESV_A1Q_XCD direct_xcd()
+#endif
 template <typename T>
 hipError_t launch_apply1q_probs(void *sv, int64_t dim, int bit, const double *m, int pbit, double *partials, int nblk,
                                 hipStream_t s) {
@@ -3426,13 +3439,13 @@ hipError_t launch_apply1q_probs(void *sv, int64_t dim, int bit, const double *m,
             float4 *p = (float4 *)sv;
             if (bit == 0)
                 hipLaunchKernelGGL((k_apply1q_probs_c64<true, false>), dim3(nblk), dim3(kThreads), 0, s, p, dim / 2,
-                                   bit, a00, a01, a10, a11, pbit, partials, nblk);
+                                   bit, a00, a01, a10, a11, pbit, partials, nblk, DESV_A1Q_XCD);
             else if (direct_shape(false, true, bit).nt)
                 hipLaunchKernelGGL((k_apply1q_probs_c64<false, true>), dim3(nblk), dim3(kThreads), 0, s, p, dim / 4,
-                                   bit, a00, a01, a10, a11, pbit, partials, nblk);
+                                   bit, a00, a01, a10, a11, pbit, partials, nblk, DESV_A1Q_XCD);
             else
                 hipLaunchKernelGGL((k_apply1q_probs_c64<false, false>), dim3(nblk), dim3(kThreads), 0, s, p, dim / 4,
-                                   bit, a00, a01, a10, a11, pbit, partials, nblk);
+                                   bit, a00, a01, a10, a11, pbit, partials, nblk, DESV_A1Q_XCD);
             return hipGetLastError();
         }
     }
