@@ -116,8 +116,12 @@ int desv_qubit_probs(desv_handle h, int qubit, double *p0, double *p1);
  * total) (utils/measurement.py:17-68): the caller draws the uniforms in
  * [0, 1) (numpy's choice draws rng.random(n_shots)); out[s] is the basis
  * index whose normalised cumulative probability first exceeds uniforms[s]
- * (searchsorted side='right').  The CDF is built on the device in FP64.
- * DESV_EINVAL if the total probability is zero or not finite. */
+ * (searchsorted side='right').  The cumulative probabilities are exact
+ * integer prefix sums of the probabilities quantised to 2^-62 (of the
+ * total), so an amplitude of probability zero is never drawn; the device
+ * reads the state once for chunk totals and once more for the chunks that
+ * hold shots, without building a full CDF (states far from unit norm: a full
+ * FP64 CDF).  DESV_EINVAL if the total probability is zero or not finite. */
 int desv_sample(desv_handle h, const double *uniforms, int64_t n_shots, int64_t *out);
 /* Zero the amplitudes whose `qubit` differs from `outcome`, scale the rest. */
 int desv_collapse(desv_handle h, int qubit, int outcome, double scale);
