@@ -855,6 +855,142 @@ int desv_apply1q_probs(desv_handle h, int qubit, const double *m, int next_qubit
     return DESV_OK;
 }
 
+int desv_apply_monomial_layer(desv_handle h, int k, const int *qubits, const double *mats, int n_pop,
+                              const int *pop_qubits, double *pops) {
+    if (int rc = check_handle(h)) return rc;
+    const int n = h->n;
+    if (k < 0 || k > n || (k > 0 && (!qubits || !mats))) return fail(DESV_EINVAL, "bad operator list");
+    if (pops && (n_pop < 0 || n_pop > std::min(n, kMonoMaxPops) || (n_pop > 0 && !pop_qubits)))
+        return fail(DESV_EINVAL, "population qubits: at most " + std::to_string(kMonoMaxPops));
+    if (k == 0 && !pops) return DESV_OK;
+    const int pvb = h->dtype == DESV_C64 ? 1 : 0;
+    uint64_t F = 0, used = 0;
+    struct Fac {
+        int bit;
+        double2 c[2];
+    };
+    std::vector<Fac> fac;
+    for (int i = 0; i < k; ++i) {
+        const int q = qubits[i];
+        if (q < 0 || q >= n) return fail(DESV_EINVAL, "Qubit " + std::to_string(q) + " out of range");
+        const int bit = n - 1 - q;
+        if (used >> bit & 1) return fail(DESV_EINVAL, "qubits of a layer must differ");
+        used |= 1ull << bit;
+        const double *m = mats + 8 * i;
+        int flip = -1;
+        Fac f{bit, {make_double2(0, 0), make_double2(0, 0)}};
+        for (int r = 0; r < 2; ++r) {
+            int col = -1;
+            for (int c = 0; c < 2; ++c)
+                if (m[4 * r + 2 * c] != 0.0 || m[4 * r + 2 * c + 1] != 0.0) {
+                    if (col >= 0) return fail(DESV_EINVAL, "operator on qubit " + std::to_string(q) +
+                                                               " has two nonzero entries in a row");
+                    col = c;
+                }
+            if (col < 0) continue;
+            if (flip >= 0 && flip != (r ^ col))
+                return fail(DESV_EINVAL, "operator on qubit " + std::to_string(q) + " maps both rows to one column");
+            flip = r ^ col;
+            f.c[r] = make_double2(m[4 * r + 2 * col], m[4 * r + 2 * col + 1]);
+        }
+        if (flip > 0) F |= 1ull << bit;
+        fac.push_back(f);
+    }
+    std::vector<int> pbit(pops ? n_pop : 0);
+    uint64_t pused = 0;
+    for (size_t j = 0; j < pbit.size(); ++j) {
+        const int q = pop_qubits[j];
+        if (q < 0 || q >= n) return fail(DESV_EINVAL, "Qubit " + std::to_string(q) + " out of range");
+        pbit[j] = n - 1 - q;
+        if (pused >> pbit[j] & 1) return fail(DESV_EINVAL, "population qubits must differ");
+        pused |= 1ull << pbit[j];
+    }
+    HIP_TRY(hipSetDevice(h->device));
+    MonoTask t{};
+    const int nvb = n - pvb;
+    t.nvec = h->dim >> pvb;
+    t.f0 = pvb ? (int)(F & 1) : 0;
+    t.fv = (int64_t)(F >> pvb);
+    t.hb = t.fv ? 63 - __builtin_clzll((uint64_t)t.fv) : -1;
+    t.lb = (t.fv && t.hb < 6) ? 7 : 6;
+    t.lamp = t.lb + pvb;
+    t.store = k > 0;
+    std::vector<double2> tab;
+    for (const Fac &f : fac) {
+        if (f.bit < t.lamp) {
+            t.lbit[t.nlow] = f.bit;
+            t.lc[t.nlow][0] = f.c[0];
+            t.lc[t.nlow][1] = f.c[1];
+            ++t.nlow;
+        } else {
+            t.ntab = std::max(t.ntab, (f.bit - t.lamp) / 8 + 1);
+        }
+    }
+    if (t.ntab > kMonoTables) return fail(DESV_EINVAL, "too many qubits for a layer");
+    if (t.ntab) {
+        tab.assign((size_t)t.ntab * 256, make_double2(1.0, 0.0));
+        for (const Fac &f : fac) {
+            if (f.bit < t.lamp) continue;
+            const int kk = (f.bit - t.lamp) / 8, sh = (f.bit - t.lamp) % 8;
+            for (int by = 0; by < 256; ++by) {
+                double2 &e = tab[(size_t)kk * 256 + by];
+                const double2 c = f.c[(by >> sh) & 1];
+                e = make_double2(e.x * c.x - e.y * c.y, e.x * c.y + e.y * c.x);
+            }
+        }
+        if (int rc = ensure_small(h, tab.size() * sizeof(double2))) return rc;
+        HIP_TRY(hipMemcpyAsync(h->d_small, tab.data(), tab.size() * sizeof(double2), hipMemcpyHostToDevice,
+                               h->stream));
+        t.tab = (const double2 *)h->d_small;
+    }
+    const int hbv = (t.fv && t.hb >= t.lb) ? t.hb : -1;
+    t.hbit_amp = hbv >= 0 ? 1ull << (hbv + pvb) : 0;
+    for (int b : pbit) {
+        if (b < t.lamp || b - pvb == hbv) t.plmask |= 1ull << b;
+        else t.phmask |= 1ull << (b - pvb);
+    }
+    for (int b = t.lb; b < nvb; ++b)
+        if (b != hbv && !(t.phmask >> b & 1)) t.rmask |= 1ull << b;
+    t.rstep1 = t.rmask & (~t.rmask + 1);
+    t.nruns = 1ll << __builtin_popcountll(t.rmask);
+    t.npl = __builtin_popcountll(t.plmask);
+    for (int i = 0, r = 0; i < 64; ++i)
+        if (t.phmask >> i & 1) {
+            if ((uint64_t)t.fv >> i & 1) t.fph |= 1u << r;
+            ++r;
+        }
+    const int ns = t.fv ? 2 : 1;
+    const int gy = 1 << __builtin_popcountll(t.phmask);
+    const int64_t want = std::max<int64_t>(1, 8192 / gy);
+    int64_t gx = std::min<int64_t>(want, std::max<int64_t>(1, (t.nruns + 15) / 16));
+    if (pops) gx = std::min<int64_t>(gx, std::max<int64_t>(1, (1ll << 22) / ((int64_t)ns << n_pop)));
+    if (pops) {
+        if (int rc = ensure_part(h, ((size_t)1 << n_pop) * gx * ns)) return rc;
+        t.partials = h->d_part;
+    }
+    HIP_TRY(h->dtype == DESV_C64 ? launch_mono_layer<float>(h->sv, t, (int)gx, gy, h->stream)
+                                 : launch_mono_layer<double>(h->sv, t, (int)gx, gy, h->stream));
+    if (!pops) {
+        if (t.ntab) HIP_TRY(hipStreamSynchronize(h->stream));  // the host table outlives the copy
+        return DESV_OK;
+    }
+    std::vector<double> comp((size_t)1 << n_pop);
+    if (int rc = sum_to_host(h, 1 << n_pop, (int)gx * ns, comp.data())) return rc;
+    // composite bin: plmask bits (ascending) below, phmask bits (ascending) above
+    std::vector<int> at(pbit.size());
+    for (size_t j = 0; j < pbit.size(); ++j) {
+        const int b = pbit[j];
+        if (t.plmask >> b & 1) at[j] = __builtin_popcountll(t.plmask & ((1ull << b) - 1));
+        else at[j] = t.npl + __builtin_popcountll(t.phmask & ((1ull << (b - pvb)) - 1));
+    }
+    for (size_t c = 0; c < comp.size(); ++c) {
+        size_t idx = 0;
+        for (size_t j = 0; j < pbit.size(); ++j) idx |= (size_t)((c >> at[j]) & 1) << j;
+        pops[idx] = comp[c];
+    }
+    return DESV_OK;
+}
+
 int desv_collapse(desv_handle h, int qubit, int outcome, double scale) {
     if (int rc = check_handle(h)) return rc;
     if (qubit < 0 || qubit >= h->n) return fail(DESV_EINVAL, "Qubit " + std::to_string(qubit) + " out of range");

@@ -3455,6 +3455,153 @@ hipError_t launch_apply1q_probs(void *sv, int64_t dim, int bit, const double *m,
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Layers of single-qubit monomial operators (noise trajectories, reference
+// noise/kraus_channels.py:104-240): out[x] = coef(x) in[x ^ F], one streaming
+// pass in place, and the FP64 populations of the result over up to 12 bits.
+// A lane owns vector v and, when F reaches past the vector, its partner
+// v ^ fv: both are read before either is written.  Index bits below t.lamp
+// (amplitude units) are fixed per lane for the whole launch; the bits above
+// are uniform across a wave's run of 64 vectors, so their coefficients come
+// from byte-indexed tables and the population bits among them from blockIdx.y.
+// ---------------------------------------------------------------------------
+__device__ inline uint64_t pdep64(uint64_t x, uint64_t mask) {
+    uint64_t r = 0;
+    for (uint64_t m = mask; m && x; m &= m - 1, x >>= 1)
+        if (x & 1) r |= m & (~m + 1);
+    return r;
+}
+__device__ inline uint32_t pext64(uint64_t x, uint64_t mask) {
+    uint32_t r = 0;
+    int k = 0;
+    for (uint64_t m = mask; m; m &= m - 1, ++k)
+        if (x & m & (~m + 1)) r |= 1u << k;
+    return r;
+}
+
+template <typename T, bool PAIR, bool POPS>
+__global__ void __launch_bounds__(kThreads) k_mono_layer(typename VT<T>::v *sv, const MonoTask t) {
+    using V = typename VT<T>::v;
+    constexpr int PV = VT<T>::per_vec, PVB = PV == 2 ? 1 : 0, NS = PAIR ? 2 : 1, GPT = 4;
+    __shared__ double acc_l[POPS ? kThreads * NS * PV : 1];
+    __shared__ int bin_l[POPS ? kThreads * NS * PV : 1];
+    const int lane = threadIdx.x & 63;
+    const int64_t vlo = (PAIR && t.hb < 6) ? (int64_t)insert_zero(lane, t.hb) : (int64_t)lane;
+    const int64_t wlo = vlo ^ (t.fv & ((1ll << t.lb) - 1));
+    // per-lane factors of the low bits, and the population bins of the lane part
+    double2 cl[NS * PV];
+    int bl[NS * PV];
+    double acc[NS * PV];
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+#pragma unroll
+        for (int e = 0; e < PV; ++e) {
+            const int64_t a = ((s ? wlo : vlo) << PVB) | e;
+            double2 c = make_double2(1.0, 0.0);
+            for (int i = 0; i < t.nlow; ++i) c = cmul(c, t.lc[i][(a >> t.lbit[i]) & 1]);
+            cl[s * PV + e] = c;
+            acc[s * PV + e] = 0.0;
+            bl[s * PV + e] = POPS ? (int)pext64((uint64_t)a | (s ? t.hbit_amp : 0), t.plmask) : 0;
+        }
+    const uint64_t hy = pdep64(blockIdx.y, t.phmask);
+    const int64_t wave = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6)));
+    const int64_t nw = (int64_t)gridDim.x * (kThreads / 64);
+    const uint64_t nr = ~t.rmask;
+    const uint64_t step = pdep64((uint64_t)(nw * GPT), t.rmask);
+    uint64_t hd = pdep64((uint64_t)(wave * GPT), t.rmask);
+    for (int64_t r0 = wave * GPT; r0 < t.nruns; r0 += nw * GPT, hd = ((hd | nr) + step) & t.rmask) {
+        int64_t iv[GPT];
+        bool on[GPT];
+        V a[GPT], b[GPT];
+        uint64_t h = hd;
+#pragma unroll
+        for (int j = 0; j < GPT; ++j) {
+            iv[j] = (int64_t)(h | hy) | vlo;
+            on[j] = r0 + j < t.nruns && iv[j] < t.nvec;
+            if (on[j]) {
+                a[j] = sv[iv[j]];
+                if constexpr (PAIR) b[j] = sv[iv[j] ^ t.fv];
+            }
+            h = ((h | nr) + t.rstep1) & t.rmask;
+        }
+#pragma unroll
+        for (int j = 0; j < GPT; ++j) {
+            if (!on[j]) continue;
+            double2 ch[NS];
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                const uint64_t amp = (uint64_t)(s ? (iv[j] ^ t.fv) : iv[j]) << PVB;
+                double2 c = make_double2(1.0, 0.0);
+                for (int k = 0; k < t.ntab; ++k) c = cmul(c, t.tab[k * 256 + ((amp >> (t.lamp + 8 * k)) & 255)]);
+                ch[s] = c;
+            }
+            V outv[NS];
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                // the v outputs read the partner (or v itself), the w outputs read v
+                T x[2 * PV], o[2 * PV];
+                __builtin_memcpy(x, (PAIR && s == 0) ? &b[j] : &a[j], sizeof(V));
+#pragma unroll
+                for (int e = 0; e < PV; ++e) {
+                    const double2 c = cmul(ch[s], cl[s * PV + e]);
+                    const bool sw = PV == 2 && t.f0;
+                    const T xr = sw ? x[2 * (e ^ (PV - 1))] : x[2 * e], xi = sw ? x[2 * (e ^ (PV - 1)) + 1] : x[2 * e + 1];
+                    const T cr = (T)c.x, ci = (T)c.y;
+                    o[2 * e] = cr * xr - ci * xi;
+                    o[2 * e + 1] = cr * xi + ci * xr;
+                    if constexpr (POPS) {
+                        const double dr = (double)o[2 * e], di = (double)o[2 * e + 1];
+                        acc[s * PV + e] += dr * dr + di * di;
+                    }
+                }
+                __builtin_memcpy(&outv[s], o, sizeof(V));
+            }
+            if (t.store) {
+                sv[iv[j]] = outv[0];
+                if constexpr (PAIR) sv[iv[j] ^ t.fv] = outv[NS - 1];
+            }
+        }
+    }
+    if constexpr (POPS) {
+#pragma unroll
+        for (int i = 0; i < NS * PV; ++i) {
+            acc_l[i * kThreads + threadIdx.x] = acc[i];
+            bin_l[i * kThreads + threadIdx.x] = bl[i];
+        }
+        __syncthreads();
+        // partials[((y << npl | b) * gridDim.x + blockIdx.x) * NS + s]: the w
+        // half of this workgroup lands in the bins of y ^ fph
+        const int nb = 1 << t.npl;
+        for (int q = threadIdx.x; q < nb * NS; q += kThreads) {
+            const int s = q / nb, bb = q % nb;
+            double sum = 0.0;
+            for (int e = 0; e < PV; ++e)
+                for (int i = 0; i < kThreads; ++i) {
+                    const int slot = (s * PV + e) * kThreads + i;
+                    if (bin_l[slot] == bb) sum += acc_l[slot];
+                }
+            const int64_t y = s ? (blockIdx.y ^ t.fph) : blockIdx.y;
+            t.partials[((((int64_t)y << t.npl) | bb) * gridDim.x + blockIdx.x) * NS + s] = sum;
+        }
+    }
+}
+
+template <typename T>
+hipError_t launch_mono_layer(void *sv, const MonoTask &t, int gx, int gy, hipStream_t s) {
+    using V = typename VT<T>::v;
+    const dim3 grid(gx, gy), block(kThreads);
+    V *p = (V *)sv;
+    const bool pops = t.partials != nullptr;
+    if (t.fv != 0) {
+        if (pops) hipLaunchKernelGGL((k_mono_layer<T, true, true>), grid, block, 0, s, p, t);
+        else hipLaunchKernelGGL((k_mono_layer<T, true, false>), grid, block, 0, s, p, t);
+    } else {
+        if (pops) hipLaunchKernelGGL((k_mono_layer<T, false, true>), grid, block, 0, s, p, t);
+        else hipLaunchKernelGGL((k_mono_layer<T, false, false>), grid, block, 0, s, p, t);
+    }
+    return hipGetLastError();
+}
+
 // Complex64 forms of k_norm2 (PROBS = false) and k_qubit_probs (PROBS =
 // true) with 16-byte loads: a float4 holds amplitudes 2f and 2f + 1, and each
 // wave reads 4 contiguous 64-float4 runs per iteration, so every load
@@ -3860,6 +4007,7 @@ hipError_t launch_fill_random(void *sv, int64_t dim, uint64_t seed, hipStream_t 
     template hipError_t launch_sample_chunks<T>(const void *, int64_t, const uint64_t *, const double *,      \
                                                 int64_t, SampleScratch, int64_t *, hipStream_t);               \
     template hipError_t launch_qubit_probs<T>(const void *, int64_t, int, double *, int, hipStream_t);        \
+    template hipError_t launch_mono_layer<T>(void *, const MonoTask &, int, int, hipStream_t);              \
     template hipError_t launch_collapse<T>(void *, int64_t, int, int, double, hipStream_t);                   \
     template hipError_t launch_pauli_expect<T>(const void *, int, const PauliItem *, int, const uint64_t *,   \
                                                double *, int, hipStream_t);                                   \

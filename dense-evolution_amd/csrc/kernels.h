@@ -178,6 +178,39 @@ int64_t cdf_blocks(int64_t dim);
 template <typename T>
 hipError_t launch_apply1q_probs(void *sv, int64_t dim, int bit, const double *m, int pbit, double *partials, int nblk,
                                 hipStream_t s);
+// One layer of single-qubit operators with at most one nonzero entry per row
+// (noise trajectories): out[x] = coef(x) in[x ^ F], in place, and optionally
+// the FP64 populations of the result.  Vector = 16 bytes (VT<T>::v); the lane
+// part of an index is its bits below lb (vector units) = lamp (amplitude
+// units); the rest is uniform per run of 64 vectors.
+constexpr int kMonoLowMax = 8;   // factors on amplitude bits < lamp
+constexpr int kMonoTables = 5;   // coefficient tables over 8 amplitude bits each, from lamp up
+constexpr int kMonoMaxPops = 12;
+struct MonoTask {
+    int64_t nvec;        // vectors in the state
+    int64_t fv;          // flip mask F above the vector's own amplitudes (vector units)
+    int32_t f0;          // complex64: F's bit 0 (swaps the two amplitudes of a vector)
+    int32_t hb;          // highest bit of fv, -1 if fv = 0 (a lane owns v and v ^ fv, bit hb of v clear)
+    int32_t lb;          // vector bits fixed by the lane: 6, or 7 when hb < 6
+    int32_t lamp;        // lb in amplitude bits
+    int32_t store;       // 0: populations only (the identity layer)
+    int32_t nlow;        // factors on amplitude bits < lamp
+    int32_t lbit[kMonoLowMax];
+    double2 lc[kMonoLowMax][2];  // coefficient of row 0, row 1
+    int32_t ntab;        // tables tab[k * 256 + byte] for amplitude bits [lamp + 8 k, lamp + 8 k + 8)
+    int32_t npl;         // population bits of the lane part (plmask)
+    const double2 *tab;
+    uint64_t rmask;      // vector bits walked by the runs (>= lb, not hb, not population bits)
+    uint64_t rstep1;     // lowest bit of rmask
+    int64_t nruns;       // 2^popcount(rmask)
+    uint64_t phmask;     // population bits among the vector bits >= lb other than hb: blockIdx.y
+    uint64_t plmask;     // population bits of the lane part (amplitude units, with hbit_amp)
+    uint64_t hbit_amp;   // bit hb in amplitude units when hb >= lb (set in the partner's bins), else 0
+    uint32_t fph;        // pext(fv, phmask): the partner's blockIdx.y bins are y ^ fph
+    double *partials;    // null: no populations; else [(y << npl | b) * gx + x] * (1 + (fv != 0)) + s
+};
+template <typename T>
+hipError_t launch_mono_layer(void *sv, const MonoTask &t, int gx, int gy, hipStream_t s);
 hipError_t launch_sample_search(const double *cdf, int64_t dim, const double *u, int64_t shots, int64_t *out,
                                 hipStream_t s);
 // Two-level sampling without a CDF array.  launch_sample_totals: blk (nb + 1

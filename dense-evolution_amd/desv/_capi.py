@@ -107,6 +107,7 @@ _SIGS = {
     "desv_pauli_matvec": ([_p, _p, _pu64, _pu64, _pd, _pd, _i64], _i),
     "desv_pauli_sum_expect": ([_p, _pu64, _pu64, _pd, _i64, _pd], _i),
     "desv_apply1q_probs": ([_p, _i, _pd, _i, _pd, _pd], _i),
+    "desv_apply_monomial_layer": ([_p, _i, _pi, _pd, _i, _pi, _pd], _i),
     "desv_dense_matvec": ([_p, _p, _pd], _i),
     "desv_adjoint": ([_p, _pd, _i64, _pi64, _i64, _pu64, _pu64, _pd, _i64, _pd, _pd, _pd], _i),
     "desv_batch_create": ([_i, _i, C.POINTER(_p)], _i),

@@ -5,9 +5,10 @@ with the channels of ``noise/kraus/*.py``) and ``NoiseSpec``
 (``noise/differentiable.py:8-66``).  One trajectory = per target qubit one
 fire/no-fire decision drawn from the caller's NumPy generator, in the
 reference's order (depolarizing: ``r`` then ``ch``; amplitude damping: ``r``
-against the whole-state jump probability), then the chosen 2x2 operator is
-applied to the whole state on the GPU, and the state is divided by
-(norm + 1e-15).  Noise qubits are indexed like the reference: qubit q is
+against the whole-state jump probability, from populations the GPU returns),
+the chosen 2x2 operators are applied to the whole state on the GPU (one pass
+per group of up to 10 qubits for damping, one pass in all for the Pauli
+channels), and the state is divided by (norm + 1e-15).  Noise qubits are indexed like the reference: qubit q is
 basis-index bit q (LSB-first, ``kraus_channels.py:38-58``), i.e. engine qubit
 n-1-q.
 
@@ -113,12 +114,8 @@ class NoiseModel:
             for q in qs:
                 if not 0 <= q < n:
                     raise ValueError(f"noise qubit {q} out of range for {n} qubits")
-            st = _Trajectory(handle)
-            for i, q in enumerate(qs):
-                nxt = n - 1 - qs[i + 1] if i + 1 < len(qs) else None
-                _CHANNELS[model](st, n - 1 - q, float(p), rng, nxt)
-            norm = np.sqrt(handle.norm2())
-            handle.scale(1.0 / (norm + 1e-15))
+            norm2 = _run_trajectory(handle, [n - 1 - q for q in qs], model, float(p), rng)
+            handle.scale(1.0 / (np.sqrt(norm2) + 1e-15))
             if owned:
                 return handle.download()
             return sv
@@ -145,39 +142,85 @@ class NoiseModel:
         return desc.get(model, desc['ideal'])
 
 
-# ---- channels: one decision per qubit, one device pass when it fires --------
+# ---- channels: one decision per qubit, one device pass per group ------------
 
-class _Trajectory:
-    """The state of one trajectory: 2x2 operators are applied in one
-    streaming pass each, and a damping step's pass also returns the
-    populations of the qubit the next damping step draws against, so no
-    separate probability pass is needed after the first.  A Pauli on qubit q
-    leaves every other qubit's populations as they are and swaps (X, Y) or
-    keeps (Z) q's own, so a cached pair survives it."""
+# Distinct qubits per damping layer: the device returns 2^_GROUP populations.
+_GROUP = 10
 
-    def __init__(self, handle: StateHandle):
-        self.h = handle
-        self.q = None    # engine qubit whose populations are cached
-        self.p = None
+
+class _Layer:
+    """The operators drawn for one group of qubits, composed per qubit in
+    draw order, and the FP64 populations over the group's qubits that each
+    draw sees.  Every operator of these channels has at most one nonzero entry
+    per row, so the populations after it are |m_rc|^2-weighted sums of the
+    populations before it (no interference): the host updates them draw by
+    draw, exactly as a pass over the state would, and the whole group is
+    applied in one device pass (``StateHandle.apply_layer``)."""
+
+    def __init__(self, qubits, pops=None):
+        self.pos = {q: j for j, q in enumerate(qubits)}
+        self.h = None if pops is None else np.array(pops, dtype=np.float64)
+        self.ops = {}
+
+    def _axis(self, q):
+        return self.h.reshape(-1, 2, 1 << self.pos[q])
 
     def probs(self, q):
-        return self.p if self.q == q else self.h.qubit_probs(q)
+        h = self._axis(q)
+        return float(h[:, 0, :].sum()), float(h[:, 1, :].sum())
 
-    def pauli(self, m, q, swaps):
-        self.h.apply_matrix(m, [q])
-        if swaps and self.q == q:
-            self.p = (self.p[1], self.p[0])
+    def push(self, m, q):
+        m = np.asarray(m, dtype=np.complex128)
+        self.ops[q] = m @ self.ops[q] if q in self.ops else m
+        if self.h is not None:
+            h = self._axis(q)
+            w = np.abs(m) ** 2
+            h0, h1 = h[:, 0, :].copy(), h[:, 1, :].copy()
+            h[:, 0, :] = w[0, 0] * h0 + w[0, 1] * h1
+            h[:, 1, :] = w[1, 0] * h0 + w[1, 1] * h1
 
-    def kraus(self, m, q, nxt):
-        if nxt is None:
-            self.h.apply_matrix(m, [q])
-            self.q = self.p = None
-        else:
-            self.p = self.h.apply_1q_probs(m, q, nxt)
-            self.q = nxt
+    # the channels' two kinds of step
+    def pauli(self, m, q, swaps=True):
+        self.push(m, q)
+
+    def kraus(self, m, q):
+        self.push(m, q)
 
 
-def _depolarizing(st: _Trajectory, q: int, p: float, rng, nxt=None):
+def _distinct(qs):
+    return list(dict.fromkeys(qs))
+
+
+def _run_trajectory(handle: StateHandle, eqs, model: str, p: float, rng) -> float:
+    """Draw and apply one trajectory over engine qubits `eqs` (in the
+    caller's order); returns the squared norm of the result.  Pauli channels
+    draw independently of the state: one layer over all qubits.  Damping
+    draws against populations: groups of up to _GROUP distinct qubits, each
+    one device pass that also returns the next group's populations."""
+    chan = _CHANNELS[model]
+    if model in ('depolarizing', 'bitflip', 'phaseflip') or not eqs:
+        layer = _Layer(_distinct(eqs))
+        for q in eqs:
+            chan(layer, q, p, rng)
+        return float(handle.apply_layer(layer.ops, [])[0])
+    groups, cur = [], []
+    for q in eqs:
+        if q not in cur and len(_distinct(cur)) == _GROUP:
+            groups.append(cur)
+            cur = []
+        cur.append(q)
+    groups.append(cur)
+    pops = handle.apply_layer({}, _distinct(groups[0]))
+    for gi, g in enumerate(groups):
+        layer = _Layer(_distinct(g), pops)
+        for q in g:
+            chan(layer, q, p, rng)
+        nxt = _distinct(groups[gi + 1]) if gi + 1 < len(groups) else []
+        pops = handle.apply_layer(layer.ops, nxt)
+    return float(pops[0])
+
+
+def _depolarizing(st: _Layer, q: int, p: float, rng):
     r, ch = rng.random(), rng.random()
     if r < p:
         if ch < _THIRD:
@@ -188,17 +231,17 @@ def _depolarizing(st: _Trajectory, q: int, p: float, rng, nxt=None):
             st.pauli(_Z, q, False)
 
 
-def _bitflip(st, q, p, rng, nxt=None):
+def _bitflip(st, q, p, rng):
     if rng.random() < p:
         st.pauli(_X, q, True)
 
 
-def _phaseflip(st, q, p, rng, nxt=None):
+def _phaseflip(st, q, p, rng):
     if rng.random() < p:
         st.pauli(_Z, q, False)
 
 
-def _amplitude_damping(st, q, p, rng, nxt=None):
+def _amplitude_damping(st, q, p, rng):
     gamma = float(np.clip(p, 0.0, 1.0))
     r = rng.random()
     p0, p1 = st.probs(q)
@@ -208,12 +251,12 @@ def _amplitude_damping(st, q, p, rng, nxt=None):
     else:
         nn = np.sqrt(max(1.0 - p1, 1e-15))
         m = np.array([[1.0 / nn, 0], [0, np.sqrt(1.0 - gamma) / nn]], dtype=np.complex128)
-    st.kraus(m, q, nxt)
+    st.kraus(m, q)
 
 
-def _combined(st, q, p, rng, nxt=None):
+def _combined(st, q, p, rng):
     _depolarizing(st, q, p * 0.5, rng)
-    _amplitude_damping(st, q, p * 0.333333, rng, nxt)
+    _amplitude_damping(st, q, p * 0.333333, rng)
 
 
 _CHANNELS = {'depolarizing': _depolarizing, 'bitflip': _bitflip, 'phaseflip': _phaseflip,

@@ -87,6 +87,28 @@ class StateHandle:
                                      C.byref(p0), C.byref(p1)))
         return None if next_qubit is None else (p0.value, p1.value)
 
+    def apply_layer(self, ops, pop_qubits=None):
+        """Apply {qubit: 2x2 matrix} in place in one streaming pass (every
+        matrix with at most one nonzero entry per row: Paulis, diagonal and
+        amplitude-damping forms, their products); with `pop_qubits` (at most
+        12), also return the FP64 populations of the result as an array of
+        2^len(pop_qubits) values, index bit j <-> pop_qubits[j] (an empty list:
+        the squared norm)."""
+        qs = [int(q) for q in ops]
+        mats = np.ascontiguousarray(np.array([np.asarray(ops[q], dtype=np.complex128).reshape(2, 2) for q in ops],
+                                             dtype=np.complex128).reshape(-1))
+        qa = np.ascontiguousarray(np.array(qs, dtype=np.intc))
+        if pop_qubits is None:
+            check(lib.desv_apply_monomial_layer(self.ptr, len(qs), qa.ctypes.data_as(_capi._pi),
+                                                mats.view(np.float64).ctypes.data_as(_capi._pd), 0, None, None))
+            return None
+        pa = np.ascontiguousarray(np.array([int(q) for q in pop_qubits], dtype=np.intc))
+        out = np.empty(1 << len(pa), dtype=np.float64)
+        check(lib.desv_apply_monomial_layer(self.ptr, len(qs), qa.ctypes.data_as(_capi._pi),
+                                            mats.view(np.float64).ctypes.data_as(_capi._pd), len(pa),
+                                            pa.ctypes.data_as(_capi._pi), out.ctypes.data_as(_capi._pd)))
+        return out
+
     def reset(self) -> None:
         check(lib.desv_reset(self.ptr))
 

@@ -133,6 +133,22 @@ int desv_collapse(desv_handle h, int qubit, int outcome, double scale);
  * qubit's |1> population, computed here without a separate pass). */
 int desv_apply1q_probs(desv_handle h, int qubit, const double *m, int next_qubit, double *p0, double *p1);
 
+/* One layer of single-qubit operators on k distinct qubits, applied in place
+ * in one streaming pass: mats holds k row-major 2x2 complex matrices (8
+ * doubles each, interleaved re/im), each with at most one nonzero entry per
+ * row and its nonzero rows reading different columns -- Paulis, diagonal
+ * matrices, the amplitude-damping jump diag/|0><1| forms and their products.
+ * If pops is not null, the same pass returns the FP64 populations of the
+ * result over the 2^n_pop basis states of pop_qubits (index bit j <->
+ * pop_qubits[j]; n_pop <= 12; n_pop = 0: the squared norm); k = 0 only
+ * reads the state.  DESV_EINVAL for any other matrix or a repeated qubit.
+ * Serves the noise trajectories (noise/kraus_channels.py:104-240): the
+ * per-qubit draws of a group of qubits only need the populations over that
+ * group, so one pass applies a whole group's operators and returns the
+ * populations the next group's draws see. */
+int desv_apply_monomial_layer(desv_handle h, int k, const int *qubits, const double *mats, int n_pop,
+                              const int *pop_qubits, double *pops);
+
 /* ---- gate application ----------------------------------------------------
  * desv_run_ops replaces _compile_and_run_circuit_jit(sv, ops) and its
  * donated twin (compiler.py:318-355) as called by run_circuit_jit

@@ -77,6 +77,71 @@ def test_apply_1q_with_next_populations(complex128):
     h.close()
 
 
+def _monomial(rng, kind):
+    a, b = rng.normal(size=2) + 1j * rng.normal(size=2)
+    return np.array([[[a, 0], [0, b]], [[0, a], [0, 0]], [[0, a], [b, 0]], [[0, 0], [0, a]],
+                     [[a, 0], [0, 0]], [[0, 0], [a, 0]]][kind], dtype=np.complex128)
+
+
+@pytest.mark.parametrize("complex128", [True, False])
+@pytest.mark.parametrize("n", [1, 3, 7, 12, 17])
+def test_monomial_layer_and_populations(complex128, n):
+    """One pass applies operators with one nonzero entry per row on several
+    qubits and returns the populations of the result over up to 12 qubits
+    (lane-local and run-uniform index bits, the flipped partner's bins)."""
+    rng = np.random.default_rng(100 + n)
+    h = desv.StateHandle(n, complex128)
+    tol = 1e-11 if complex128 else 2e-5
+    for trial in range(6):
+        sv = _random_state(n, trial)
+        k = int(rng.integers(0, n + 1)) if trial else n
+        qs = [int(q) for q in rng.permutation(n)[:k]]
+        ops = {q: _monomial(rng, int(rng.integers(6))) for q in qs}
+        npop = int(rng.integers(0, min(n, 12) + 1)) if trial != 1 else min(n, 12)
+        pq = [int(q) for q in rng.permutation(n)[:npop]]
+        h.upload(sv)
+        pops = h.apply_layer(ops, pq)
+        want = sv.copy()
+        for q, m in ops.items():
+            want = O.apply_matrix(want, m, [q])
+        assert np.max(np.abs(h.download() - want)) < tol * max(1.0, np.max(np.abs(want))), (n, trial)
+        key = np.zeros(1 << n, dtype=np.int64)
+        for j, q in enumerate(pq):
+            key |= ((np.arange(1 << n) >> (n - 1 - q)) & 1) << j
+        pw = np.bincount(key, weights=np.abs(want) ** 2, minlength=1 << npop)
+        assert np.max(np.abs(pops - pw)) < tol * max(1.0, pw.sum()), (n, trial)
+        # populations only: the state is not written
+        h.upload(sv)
+        p0 = h.apply_layer({}, pq)
+        assert np.array_equal(h.download(), sv.astype(h.download().dtype))
+        assert abs(p0.sum() - 1.0) < tol
+    with pytest.raises(ValueError, match="two nonzero entries"):
+        h.apply_layer({0: np.ones((2, 2))}, [])
+    with pytest.raises(ValueError, match="both rows"):
+        h.apply_layer({0: np.array([[1, 0], [1, 0]])}, [])
+    with pytest.raises(ValueError, match="at most 12"):
+        h.apply_layer({}, list(range(min(n, 13))) + [0] * max(0, 13 - n))
+    h.close()
+
+
+@pytest.mark.parametrize("model", ["amplitude_damping", "combined"])
+@pytest.mark.parametrize("complex128", [True, False])
+def test_damping_groups_with_repeated_qubits(model, complex128):
+    """More than ten distinct qubits (two device passes) and qubits listed
+    twice, inside a group and across the group boundary."""
+    n = 13
+    sv = _random_state(n, 21)
+    qubits = [0, 3, 0, 5, 1, 2, 4, 6, 7, 8, 9, 3, 10, 11, 12, 10, 1]
+    for seed in range(3):
+        h = desv.StateHandle(n, complex128)
+        h.upload(sv)
+        desv.NoiseModel.apply_to_sv(h, n, model, 0.6, rng=np.random.default_rng(seed), qubits=qubits)
+        want = O.noise_apply_to_sv(sv, n, model, 0.6, np.random.default_rng(seed), qubits)
+        tol = 1e-12 if complex128 else 1e-5
+        assert np.max(np.abs(h.download() - want)) < tol, (model, seed)
+        h.close()
+
+
 def test_sequence_of_trajectories_advances_the_generator():
     n = 5
     sv = _random_state(n, 9)
