@@ -962,7 +962,8 @@ int desv_apply_monomial_layer(desv_handle h, int k, const int *qubits, const dou
     const int ns = t.fv ? 2 : 1;
     const int gy = 1 << __builtin_popcountll(t.phmask);
     const int64_t want = std::max<int64_t>(1, 8192 / gy);
-    int64_t gx = std::min<int64_t>(want, std::max<int64_t>(1, (t.nruns + 15) / 16));
+    const int64_t per_wg = 16;  // runs per workgroup iteration (4 waves x 4 runs)
+    int64_t gx = std::min<int64_t>(want, std::max<int64_t>(1, (t.nruns + per_wg - 1) / per_wg));
     if (pops) gx = std::min<int64_t>(gx, std::max<int64_t>(1, (1ll << 22) / ((int64_t)ns << n_pop)));
     if (pops) {
         if (int rc = ensure_part(h, ((size_t)1 << n_pop) * gx * ns)) return rc;
