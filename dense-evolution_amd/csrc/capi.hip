@@ -961,7 +961,10 @@ int desv_apply_monomial_layer(desv_handle h, int k, const int *qubits, const dou
         }
     const int ns = t.fv ? 2 : 1;
     const int gy = 1 << __builtin_popcountll(t.phmask);
-    const int64_t want = std::max<int64_t>(1, 8192 / gy);
+#ifndef DESV_MONO_WGS
+#define DESV_MONO_WGS 8192
+#endif
+    const int64_t want = std::max<int64_t>(1, DESV_MONO_WGS / gy);
     const int64_t per_wg = 16;  // runs per workgroup iteration (4 waves x 4 runs)
     int64_t gx = std::min<int64_t>(want, std::max<int64_t>(1, (t.nruns + per_wg - 1) / per_wg));
     if (pops) gx = std::min<int64_t>(gx, std::max<int64_t>(1, (1ll << 22) / ((int64_t)ns << n_pop)));

@@ -3511,28 +3511,32 @@ __global__ void __launch_bounds__(kThreads) k_mono_layer(typename VT<T>::v *sv, 
     uint64_t hd = pdep64((uint64_t)(wave * GPT), t.rmask);
     for (int64_t r0 = wave * GPT; r0 < t.nruns; r0 += nw * GPT, hd = ((hd | nr) + step) & t.rmask) {
         int64_t iv[GPT];
+        uint64_t hu[GPT];  // the run's bits >= lb: uniform across the wave
         bool on[GPT];
         V a[GPT], b[GPT];
         uint64_t h = hd;
 #pragma unroll
         for (int j = 0; j < GPT; ++j) {
-            iv[j] = (int64_t)(h | hy) | vlo;
+            hu[j] = h | hy;
+            iv[j] = (int64_t)hu[j] | vlo;
             on[j] = r0 + j < t.nruns && iv[j] < t.nvec;
-            if (on[j]) {
-                a[j] = sv[iv[j]];
-                if constexpr (PAIR) b[j] = sv[iv[j] ^ t.fv];
-            }
+            // unconditional loads (an index off the state reads vector 0 and
+            // its partner), so that no branch separates them
+            const int64_t ia = on[j] ? iv[j] : 0;
+            a[j] = sv[ia];
+            if constexpr (PAIR) b[j] = sv[ia ^ t.fv];
             h = ((h | nr) + t.rstep1) & t.rmask;
         }
 #pragma unroll
         for (int j = 0; j < GPT; ++j) {
-            if (!on[j]) continue;
             double2 ch[NS];
 #pragma unroll
             for (int s = 0; s < NS; ++s) {
-                const uint64_t amp = (uint64_t)(s ? (iv[j] ^ t.fv) : iv[j]) << PVB;
+                // bits >= lamp of the amplitude index = bits >= lb of the vector index
+                const uint64_t hv = (s ? (hu[j] ^ (uint64_t)t.fv) : hu[j]) >> t.lb;
                 double2 c = make_double2(1.0, 0.0);
-                for (int k = 0; k < t.ntab; ++k) c = cmul(c, t.tab[k * 256 + ((amp >> (t.lamp + 8 * k)) & 255)]);
+                for (int k = 0; k < t.ntab; ++k)
+                    c = cmul(c, t.tab[k * 256 + __builtin_amdgcn_readfirstlane((int)((hv >> (8 * k)) & 255))]);
                 ch[s] = c;
             }
             V outv[NS];
@@ -3551,12 +3555,12 @@ __global__ void __launch_bounds__(kThreads) k_mono_layer(typename VT<T>::v *sv, 
                     o[2 * e + 1] = cr * xi + ci * xr;
                     if constexpr (POPS) {
                         const double dr = (double)o[2 * e], di = (double)o[2 * e + 1];
-                        acc[s * PV + e] += dr * dr + di * di;
+                        acc[s * PV + e] += on[j] ? dr * dr + di * di : 0.0;
                     }
                 }
                 __builtin_memcpy(&outv[s], o, sizeof(V));
             }
-            if (t.store) {
+            if (t.store && on[j]) {
                 sv[iv[j]] = outv[0];
                 if constexpr (PAIR) sv[iv[j] ^ t.fv] = outv[NS - 1];
             }
