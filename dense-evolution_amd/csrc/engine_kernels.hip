@@ -3378,15 +3378,13 @@ __global__ void __launch_bounds__(kThreads) k_apply1q_probs_c64(float4 *sv, int6
             if constexpr (B0) {
                 ix[j] = un;
                 iy[j] = un;
-                if (on[j]) v[j] = ld<float4, NT>(sv + un);
+                v[j] = ld<float4, NT>(sv + (on[j] ? un : 0));  // unconditional: no per-lane branch
             } else {
                 const int64_t a = 2 * un, x = ((a >> bit) << (bit + 1)) | (a & (tb - 1));
                 ix[j] = x >> 1;
                 iy[j] = (x | tb) >> 1;
-                if (on[j]) {
-                    v[j] = ld<float4, NT>(sv + ix[j]);
-                    u[j] = ld<float4, NT>(sv + iy[j]);
-                }
+                v[j] = ld<float4, NT>(sv + (on[j] ? ix[j] : 0));
+                u[j] = ld<float4, NT>(sv + (on[j] ? iy[j] : 0));
             }
         }
 #pragma unroll
