@@ -62,3 +62,41 @@ def test_ideal_and_zero_probability_return_the_input():
     assert desv.NoiseModel.apply_to_sv(sv, 3, 'depolarizing', 0.0) is sv
     with pytest.raises(ValueError):
         desv.NoiseModel.apply_to_sv(sv, 3, 'thermal', 0.1)
+
+
+class _NumpyLayers:
+    """Stand-in for a device handle: applies a layer with the oracle's
+    matrix routine and returns populations like ``StateHandle.apply_layer``,
+    so the trajectory's host logic (groups, per-draw population updates,
+    operator composition) is checked on the CPU."""
+
+    def __init__(self, sv, n):
+        self.sv, self.n, self.passes = sv.copy(), n, 0
+
+    def apply_layer(self, ops, pop_qubits):
+        self.passes += 1
+        for q, m in ops.items():
+            self.sv = O.apply_matrix(self.sv, m, [q])
+        key = np.zeros(1 << self.n, dtype=np.int64)
+        for j, q in enumerate(pop_qubits):
+            key |= ((np.arange(1 << self.n) >> (self.n - 1 - q)) & 1) << j
+        return np.bincount(key, weights=np.abs(self.sv) ** 2, minlength=1 << len(pop_qubits))
+
+
+@pytest.mark.parametrize("model", ["amplitude_damping", "combined", "depolarizing"])
+@pytest.mark.parametrize("qubits", [None, [0, 3, 0, 5, 1, 2, 4, 6, 7, 8, 9, 3, 10, 11, 12, 10, 1], []])
+def test_grouped_trajectory_host_logic_matches_oracle(model, qubits):
+    from desv.noise import _run_trajectory
+    n = 13
+    sv = _random_state(n, 7)
+    qs = list(range(n)) if qubits is None else qubits
+    for seed in range(3):
+        fake = _NumpyLayers(sv, n)
+        norm2 = _run_trajectory(fake, [n - 1 - q for q in qs], model, 0.6, np.random.default_rng(seed))
+        got = fake.sv / (np.sqrt(norm2) + 1e-15)
+        want = O.noise_apply_to_sv(sv, n, model, 0.6, np.random.default_rng(seed), qs)
+        assert np.max(np.abs(got - want)) < 1e-12, (model, seed)
+        # damping: a populations pass, then one layer per group of at most 10 qubits
+        distinct = len(set(qs))
+        expect = 1 if (model == "depolarizing" or not qs) else 1 + -(-distinct // 10)
+        assert fake.passes == expect
